@@ -1,0 +1,156 @@
+/*
+ * tblaeq.h -- C ABI of the MI355X-native T-BLAEQ query path (gfx950 HIP kernels).
+ *
+ * The reference (Mgepahmge/T-BLAEQ) has no plugin or FFI layer: its operator
+ * interface is the C++ class set QueryHandler / QueryEngine / IQueryStrategy.
+ * This header is the thin C layer under the MI355X-native re-implementation of
+ * those classes; each entry point names the reference code it replaces.
+ *
+ * Conventions
+ *  - Every call returns an int status (TBLAEQ_OK = 0) and never exits the
+ *    process; tblaeq_last_error() gives a message for the calling thread.
+ *  - Device pointers are plain HIP device addresses owned by the caller.
+ *  - Element counts that are produced on the device stay on the device
+ *    (uint64_t* d_count).  Calls that consume such a count also take a host-side
+ *    upper bound (`*_bound`) that sizes the launch; the device count selects the
+ *    work, so no host synchronisation is needed between levels.
+ *  - Work is issued on the context's stream, in order; calls are not
+ *    thread-safe on one context.
+ *  - Coordinates are FP64, ids and counts are 64-bit.
+ */
+#ifndef TBLAEQ_H
+#define TBLAEQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TBLAEQ_OK 0
+#define TBLAEQ_ERR_INVALID 1     /* bad argument (null pointer, zero dimension, ...) */
+#define TBLAEQ_ERR_HIP 2         /* HIP runtime error */
+#define TBLAEQ_ERR_UNSUPPORTED 3 /* shape outside what the kernels handle */
+#define TBLAEQ_ERR_CAPACITY 4    /* an output or scratch buffer is too small */
+#define TBLAEQ_ERR_IO 5          /* index file missing or malformed */
+#define TBLAEQ_ERR_NOMEM 6       /* device allocation failed */
+
+typedef struct tblaeq_ctx tblaeq_ctx;
+
+/* One interval of the hierarchy, resident on the device.  Mirrors the device
+ * half of SparseTensorCscFormat + MaxRadius + Map (Data_Structures.cuh:299-390,
+ * IndexData.cuh:89-243).  child_ids[j] = map[l+1][rowIds[j]] folds the
+ * reference's refactor pass (Refactor.cu:10-16) into the index. */
+typedef struct tblaeq_level {
+    uint64_t D;                  /* dimensionality */
+    uint64_t cols;               /* points of the coarse level l */
+    uint64_t nnz;                /* points of the fine level l+1 */
+    const uint64_t* col_res;     /* [cols+1] CSC column pointers */
+    const uint64_t* nnz_per_col; /* [cols]   children per cluster (KNN STEP weights) */
+    const double* pvals;         /* [nnz*D]  P-tensor ratios, AoS */
+    const double* radius;        /* [cols]   max child distance per cluster */
+    const uint64_t* child_ids;   /* [nnz]    fine-level id of CSC row j */
+    uint64_t empty_cols;         /* clusters with nnz_per_col == 0 (bounds KNN STEP phase 1) */
+} tblaeq_level;
+
+/* --- context --------------------------------------------------------------
+ * Replaces the implicit CUDA default-stream state of the reference: owns a
+ * stream (or adopts `stream` when non-null), the look-back status words and a
+ * growable scratch area. */
+int tblaeq_ctx_create(int device, void* stream, tblaeq_ctx** out);
+int tblaeq_ctx_destroy(tblaeq_ctx* ctx);
+int tblaeq_ctx_sync(tblaeq_ctx* ctx);
+void* tblaeq_ctx_stream(tblaeq_ctx* ctx);
+const char* tblaeq_status_string(int status);
+const char* tblaeq_last_error(void);
+
+/* --- plain per-query level kernels ---------------------------------------- */
+
+/* Range prune + order-preserving compaction, fused.
+ * Replaces rangePruningKernel + countKernel + prefixSumKernel +
+ * prefixSumFinalScanKernel + gridCompactPrefixKernel and the host mask count
+ * (RangePruning.cu:5-48, compact.cuh:34-268, GridCompact.cuh:41-116,
+ * L0Strategy.cu:83-101,160-172).
+ * Keeps grid point i iff boxdist^2(vals_i, [lo,hi]) <= radius[ids_i]^2.
+ * Writes survivors (ids, vals) in grid order, surv_off[s] = exclusive prefix of
+ * the survivors' CSC column lengths (their output offset in the prolongation),
+ * *d_surv_count and *d_out_count (= next level's grid size). */
+int tblaeq_range_prune_compact(tblaeq_ctx* ctx, const tblaeq_level* lv, const uint64_t* ids, const double* vals,
+                               const uint64_t* d_count, uint64_t count_bound, const double* d_lo, const double* d_hi,
+                               uint64_t* surv_ids, double* surv_vals, uint64_t* surv_off, uint64_t* d_surv_count,
+                               uint64_t* d_out_count);
+
+/* KNN lower bound, selection of the closest clusters and STEP, then compaction.
+ * Replaces calculateClusterDistanceKernel + gpuSort + host STEP loop + mask
+ * upload + compaction (KNN.cu:9-110, MergeSort.cuh:628-696,
+ * L0Strategy.cu:103-158).  lb_i = sqrt_rd(|q - c_i|^2) - r_i; clusters are
+ * ordered by (lb, id); STEP takes clusters while the running child count is
+ * below K, raising ub = max(ub, lb + 2r), then keeps every cluster with lb <= ub.
+ * Outputs as tblaeq_range_prune_compact. */
+int tblaeq_knn_lower_bound_sort_step(tblaeq_ctx* ctx, const tblaeq_level* lv, const uint64_t* ids,
+                                     const double* vals, const uint64_t* d_count, uint64_t count_bound,
+                                     const double* d_q, uint64_t K, uint64_t* surv_ids, double* surv_vals,
+                                     uint64_t* surv_off, uint64_t* d_surv_count, uint64_t* d_out_count);
+
+/* SpTSpM prolongation fused with the id refactor.
+ * Replaces SpTSpMMultiplication_v3{,_L0_nb} + SpMSpVKernelAOS_v2 + refactor
+ * (SpMSpV.cu:20-105,200-254, SpMSpV.cuh:47-67, Refactor.cu:10-27).  For every
+ * survivor s (cluster c) and CSC row j of column c, in survivor order then
+ * column order: out_ids[o] = child_ids[j], out_vals[o,d] = pvals[j,d] * surv_vals[s,d]
+ * with o = surv_off[s] + (j - col_res[c]).  No host-built index arrays. */
+int tblaeq_prolong_refactor(tblaeq_ctx* ctx, const tblaeq_level* lv, const uint64_t* surv_ids,
+                            const double* surv_vals, const uint64_t* surv_off, const uint64_t* d_surv_count,
+                            uint64_t surv_bound, uint64_t* out_ids, double* out_vals);
+
+/* Prolongation from P columns packed in survivor order: survivor s's P rows
+ * (and, when packed_child is non-null, its child ids) start at packed row
+ * surv_off[s].  Replaces SpTSpMMultiplication_v3_L2 (SpMSpV.cu:107-198), used by
+ * the streaming policies that do not keep P resident (L2, L3).  lv->pvals is
+ * not read; lv->col_res gives the column lengths. */
+int tblaeq_prolong_refactor_packed(tblaeq_ctx* ctx, const tblaeq_level* lv, const uint64_t* surv_ids,
+                                   const double* surv_vals, const uint64_t* surv_off, const uint64_t* d_surv_count,
+                                   uint64_t surv_bound, const double* packed_pvals, const uint64_t* packed_child,
+                                   uint64_t* out_ids, double* out_vals);
+
+/* KNN lower bounds lb_i = sqrt_rd(|q - c_i|^2) - radius[ids_i] for `count`
+ * grid points (host count), written to out_lb.  Used by the tiled policy (L3)
+ * to run STEP over a grid that does not fit on the device
+ * (calculateClusterDistanceKernelL3, L3Kernels.cu:28-47). */
+int tblaeq_knn_lower_bounds(tblaeq_ctx* ctx, uint64_t D, const uint64_t* ids, const double* vals, uint64_t count,
+                            const double* d_q, const double* radius, double* out_lb);
+
+/* Range prune + compaction over a tile whose element count is host-known and
+ * whose radius is indexed by position (pre-gathered), as
+ * rangePruningKernelL3 (L3Kernels.cu:3-26).  Same outputs as
+ * tblaeq_range_prune_compact, with surv_off relative to the tile. */
+int tblaeq_range_prune_compact_tile(tblaeq_ctx* ctx, const tblaeq_level* lv, const uint64_t* ids,
+                                    const double* vals, uint64_t count, const double* d_lo, const double* d_hi,
+                                    uint64_t* surv_ids, double* surv_vals, uint64_t* surv_off,
+                                    uint64_t* d_surv_count, uint64_t* d_out_count);
+
+/* --- exact refine (no reference counterpart: the reference returns the
+ * candidate superset; these give the exact answer over the reconstructed
+ * coordinates of the candidates) ------------------------------------------ */
+
+/* Ids of candidates inside [lo,hi] (inclusive), ascending.  `count` is the
+ * host-known candidate count.  Returns the hit count in *n_out (host). */
+int tblaeq_refine_range(tblaeq_ctx* ctx, uint64_t D, const uint64_t* ids, const double* vals, uint64_t count,
+                        const double* d_lo, const double* d_hi, uint64_t* out_ids, uint64_t out_cap,
+                        uint64_t* n_out);
+
+/* Top-K candidates by (squared distance, id), ascending; d2 = fma chain of (q-v)^2. */
+int tblaeq_refine_knn_topk(tblaeq_ctx* ctx, uint64_t D, const uint64_t* ids, const double* vals, uint64_t count,
+                           const double* d_q, uint64_t K, uint64_t* out_ids, double* out_d2, uint64_t* n_out);
+
+/* --- sort ------------------------------------------------------------------
+ * Block merge sort of (key, value) pairs, lexicographic, ascending.  LDS
+ * bitonic tiles of 2048 pairs, then merge-path passes.  Replaces
+ * blockMergeSortKernel + hostSerialMergeSort (MergeSort.cuh:240-444,628-696);
+ * also used to build CSC P-tensors from (cluster, point) keys. */
+int tblaeq_sort_pairs(tblaeq_ctx* ctx, uint64_t* keys, uint64_t* vals, uint64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TBLAEQ_H */

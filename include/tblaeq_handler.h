@@ -1,0 +1,67 @@
+/*
+ * tblaeq_handler.h -- C facade over the MI355X-native QueryHandler.
+ *
+ * The reference's public interface is the C++ class QueryHandler
+ * (src/core/QueryHandler.cuh:50-247); these functions expose the same
+ * operations to callers that bind C (ctypes, cgo, JNI).  All return an int
+ * status from tblaeq.h; tblaeq_last_error() describes failures.
+ */
+#ifndef TBLAEQ_HANDLER_H
+#define TBLAEQ_HANDLER_H
+
+#include <stdint.h>
+
+#include "tblaeq.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tblaeq_handler tblaeq_handler;
+typedef struct tblaeq_result tblaeq_result;
+
+/* QueryHandler(N, D, valMin, valMax, isInt, height, ratios, seed, sigmaDivisor)
+ * (QueryHandler.cuh:95-101, QueryHandler.cu:101-124): synthetic hierarchy. */
+int tblaeq_handler_create_random(uint64_t N, uint64_t D, double vmin, double vmax, int isInt, uint64_t height,
+                                 const uint64_t* ratios, uint64_t seed, double sigmaDivisor, tblaeq_handler** out);
+/* QueryHandler(forceCPU, PointCloud, name, height, ratios) (QueryHandler.cu:79-99):
+ * k-means build of an in-memory dataset (data: N*D doubles, AoS). */
+int tblaeq_handler_create_from_data(const double* data, uint64_t N, uint64_t D, uint64_t height,
+                                    const uint64_t* ratios, const char* name, tblaeq_handler** out);
+/* QueryHandler(indexPath, true) (QueryHandler.cu:126-137). */
+int tblaeq_handler_load(const char* path, tblaeq_handler** out);
+/* saveIndex (QueryHandler.cu:139-145), byte-compatible files. */
+int tblaeq_handler_save(tblaeq_handler* h, const char* path);
+int tblaeq_handler_destroy(tblaeq_handler* h);
+/* meta = {D, N, height, intervals}; meshSizes finest first (up to cap). */
+int tblaeq_handler_meta(tblaeq_handler* h, uint64_t* meta, uint64_t* meshSizes, uint64_t cap);
+/* Finest generated data layer (generator-built indexes only). */
+int tblaeq_handler_data(tblaeq_handler* h, double* out, uint64_t cap);
+/* prepareForQuery (QueryHandler.cu:160-182): levels != NULL forces a per-level
+ * policy (0..3 = L0..L3); otherwise the scheduler picks one from the device's
+ * free memory, or from freeMemOverride when it is non-zero. */
+int tblaeq_handler_prepare(tblaeq_handler* h, const uint8_t* levels, uint64_t n, uint64_t freeMemOverride);
+int tblaeq_handler_policy(tblaeq_handler* h, uint8_t* out, uint64_t cap);
+/* PolicyScheduler::recommend with an injected free-memory value (MemoryPolicy.cu:100-177). */
+int tblaeq_handler_recommend(tblaeq_handler* h, uint64_t freeMem, int countWorkBuffers, uint8_t* out, uint64_t cap);
+/* Testing hook: fixed device byte budget for L3 tiles (0 = from free memory). */
+int tblaeq_handler_set_l3_tile_bytes(uint64_t bytes);
+/* performQuery over an in-memory batch (QueryHandler.cu:198-248): type 0 =
+ * range (rows lo|hi, 2*D values), 1 = KNN (rows of D values, K neighbours).
+ * refine adds the exact answer; saveFineMesh keeps host copies of the final
+ * candidate grids. */
+int tblaeq_handler_query(tblaeq_handler* h, int type, const double* queries, uint64_t nq, uint64_t K, int refine,
+                         int saveFineMesh, tblaeq_result** out);
+int tblaeq_result_destroy(tblaeq_result* r);
+int tblaeq_result_info(tblaeq_result* r, uint64_t* nq, uint64_t* levels, double* totalUs);
+int tblaeq_result_query(tblaeq_result* r, uint64_t i, uint64_t* levelSizes, double* us, uint64_t* fineSize,
+                        uint64_t* refinedCount);
+int tblaeq_result_fine_mesh(tblaeq_result* r, uint64_t i, uint64_t* ids, double* vals);
+int tblaeq_result_refined(tblaeq_result* r, uint64_t i, uint64_t* ids, double* d2);
+/* saveQueryResult (QueryEngine.cu:254-305), same CSV columns. */
+int tblaeq_result_save_csv(tblaeq_result* r, const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

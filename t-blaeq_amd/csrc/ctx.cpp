@@ -1,0 +1,134 @@
+// Context management and error reporting for the C ABI.
+#include "ctx.hpp"
+
+#include <cstring>
+#include <string>
+
+namespace {
+thread_local std::string g_lastError;
+}
+
+namespace tblaeq {
+
+void set_error(const std::string& msg) { g_lastError = msg; }
+
+int hip_fail(hipError_t e, const char* what) {
+    g_lastError = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? TBLAEQ_ERR_NOMEM : TBLAEQ_ERR_HIP;
+}
+
+int scan_prepare(tblaeq_ctx* ctx, uint64_t tiles, uint32_t* epoch) {
+    if (tiles > ctx->statusTiles) {
+        uint64_t cap = ctx->statusTiles ? ctx->statusTiles : 1024;
+        while (cap < tiles) cap *= 2;
+        if (ctx->status) TB_HIP(hipFree(ctx->status));
+        ctx->status = nullptr;
+        ctx->statusTiles = 0;
+        TB_HIP(hipMalloc(&ctx->status, cap * 4 * sizeof(uint64_t)));
+        TB_HIP(hipMemsetAsync(ctx->status, 0, cap * 4 * sizeof(uint64_t), ctx->stream));
+        ctx->statusTiles = cap;
+    }
+    // 27-bit epoch; on wrap-around clear the words so no stale word can match.
+    ctx->epoch = (ctx->epoch + 1) & 0x7FFFFFF;
+    if (ctx->epoch == 0) {
+        TB_HIP(hipMemsetAsync(ctx->status, 0, ctx->statusTiles * 4 * sizeof(uint64_t), ctx->stream));
+        ctx->epoch = 1;
+    }
+    *epoch = ctx->epoch;
+    return TBLAEQ_OK;
+}
+
+int ws_reserve(tblaeq_ctx* ctx, size_t bytes) {
+    if (bytes <= ctx->wsBytes) return TBLAEQ_OK;
+    size_t cap = ctx->wsBytes ? ctx->wsBytes : (1u << 20);
+    while (cap < bytes) cap *= 2;
+    if (ctx->ws) {
+        TB_HIP(hipStreamSynchronize(ctx->stream));
+        TB_HIP(hipFree(ctx->ws));
+    }
+    ctx->ws = nullptr;
+    ctx->wsBytes = 0;
+    TB_HIP(hipMalloc(&ctx->ws, cap));
+    ctx->wsBytes = cap;
+    return TBLAEQ_OK;
+}
+
+}  // namespace tblaeq
+
+using namespace tblaeq;
+
+extern "C" int tblaeq_ctx_create(int device, void* stream, tblaeq_ctx** out) {
+    if (!out) {
+        set_error("tblaeq_ctx_create: null output");
+        return TBLAEQ_ERR_INVALID;
+    }
+    *out = nullptr;
+    TB_HIP(hipSetDevice(device));
+    auto* ctx = new tblaeq_ctx();
+    ctx->device = device;
+    int rc = TBLAEQ_OK;
+    do {
+        hipError_t e;
+        if (stream) {
+            ctx->stream = static_cast<hipStream_t>(stream);
+        } else {
+            e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+            if (e != hipSuccess) { rc = hip_fail(e, "hipStreamCreate"); break; }
+            ctx->ownsStream = true;
+        }
+        e = hipDeviceGetAttribute(&ctx->cuCount, hipDeviceAttributeMultiprocessorCount, device);
+        if (e != hipSuccess) { rc = hip_fail(e, "hipDeviceGetAttribute"); break; }
+        e = hipMalloc(&ctx->ctrl, kCtrlSlots * sizeof(ScanCtrl));
+        if (e != hipSuccess) { rc = hip_fail(e, "hipMalloc(ctrl)"); break; }
+        e = hipMemsetAsync(ctx->ctrl, 0, kCtrlSlots * sizeof(ScanCtrl), ctx->stream);
+        if (e != hipSuccess) { rc = hip_fail(e, "hipMemset(ctrl)"); break; }
+        e = hipHostMalloc(reinterpret_cast<void**>(&ctx->hostPinned), 64 * sizeof(uint64_t), hipHostMallocDefault);
+        if (e != hipSuccess) { rc = hip_fail(e, "hipHostMalloc"); break; }
+        uint32_t ep;
+        rc = scan_prepare(ctx, 1024, &ep);
+        if (rc) break;
+        e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) { rc = hip_fail(e, "hipStreamSynchronize"); break; }
+    } while (false);
+    if (rc) {
+        tblaeq_ctx_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return TBLAEQ_OK;
+}
+
+extern "C" int tblaeq_ctx_destroy(tblaeq_ctx* ctx) {
+    if (!ctx) return TBLAEQ_OK;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->status) (void)hipFree(ctx->status);
+    if (ctx->ctrl) (void)hipFree(ctx->ctrl);
+    if (ctx->ws) (void)hipFree(ctx->ws);
+    if (ctx->hostPinned) (void)hipHostFree(ctx->hostPinned);
+    if (ctx->ownsStream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TBLAEQ_OK;
+}
+
+extern "C" int tblaeq_ctx_sync(tblaeq_ctx* ctx) {
+    if (!ctx) return TBLAEQ_ERR_INVALID;
+    TB_HIP(hipStreamSynchronize(ctx->stream));
+    return TBLAEQ_OK;
+}
+
+extern "C" void* tblaeq_ctx_stream(tblaeq_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+extern "C" const char* tblaeq_status_string(int s) {
+    switch (s) {
+        case TBLAEQ_OK: return "ok";
+        case TBLAEQ_ERR_INVALID: return "invalid argument";
+        case TBLAEQ_ERR_HIP: return "HIP runtime error";
+        case TBLAEQ_ERR_UNSUPPORTED: return "unsupported shape";
+        case TBLAEQ_ERR_CAPACITY: return "buffer too small";
+        case TBLAEQ_ERR_IO: return "index file error";
+        case TBLAEQ_ERR_NOMEM: return "out of device memory";
+        default: return "unknown status";
+    }
+}
+
+extern "C" const char* tblaeq_last_error(void) { return g_lastError.c_str(); }

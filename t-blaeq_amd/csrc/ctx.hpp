@@ -1,0 +1,48 @@
+// Internal context shared by the C-ABI entry points (not part of the public header).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/tblaeq.h"
+#include "device_common.hpp"
+
+struct tblaeq_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool ownsStream = false;
+    uint32_t epoch = 0;              // look-back status epoch, bumped per scan launch
+    uint64_t* status = nullptr;      // 4 words per tile (agg0, agg1, inc0, inc1)
+    uint64_t statusTiles = 0;
+    tblaeq::ScanCtrl* ctrl = nullptr;  // [kCtrlSlots]
+    // growable scratch (sort ping-pong buffers, KNN selection lists)
+    void* ws = nullptr;
+    size_t wsBytes = 0;
+    uint64_t* hostPinned = nullptr;  // small pinned staging for counts
+    int cuCount = 0;
+};
+
+namespace tblaeq {
+
+constexpr int kCtrlSlots = 8;
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 1024 elements per look-back tile
+constexpr int kSortTile = 2048;                         // pairs per LDS tile sort
+constexpr int kSortThreads = 256;
+
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+// Ensures the look-back status array can hold `tiles` tiles and returns a fresh epoch.
+int scan_prepare(tblaeq_ctx* ctx, uint64_t tiles, uint32_t* epoch);
+// Grows the context scratch to at least `bytes`.
+int ws_reserve(tblaeq_ctx* ctx, size_t bytes);
+
+}  // namespace tblaeq
+
+#define TB_HIP(call)                                               \
+    do {                                                           \
+        hipError_t e_ = (call);                                    \
+        if (e_ != hipSuccess) return tblaeq::hip_fail(e_, #call);  \
+    } while (0)

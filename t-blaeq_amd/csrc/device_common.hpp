@@ -1,0 +1,126 @@
+// Device-side helpers shared by the T-BLAEQ gfx950 kernels.
+//
+// Arithmetic order is fixed so that the CPU oracle (built with contraction off
+// and explicit fma) reproduces every FP64 result bit-for-bit:
+//   box distance   dist = fma(diff, diff, dist) over d = 0..D-1     (RangePruning.cu:8-21)
+//   KNN bound      lb = sqrt_rd(fma-chain of (q-c)^2) - r           (KNN.cu:14-21)
+//   prolongation   y = P * x                                        (SpMSpV.cuh:64)
+//   point distance fma(diff, diff, acc) with diff = q - v
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tblaeq {
+
+constexpr int kWave = 64;
+
+// Round-toward-minus-infinity square root built from the correctly rounded
+// sqrt: the RN result s is within half an ulp, so RD is s when s*s <= x and the
+// next double below s otherwise; fma gives the exact sign of s*s - x.
+__device__ __forceinline__ double sqrt_rd(double x) {
+    double s = __builtin_sqrt(x);
+    if (!(x > 0.0) || !(x < __builtin_inf())) return s;
+    // The result must be the largest double whose square does not exceed x; fma
+    // gives the exact sign of s*s - x.  Walk at most two ulps either way so the
+    // answer does not depend on the library sqrt being correctly rounded.
+    // s is positive and finite here, so neighbours are the bit pattern -/+ 1.
+    if (__builtin_fma(s, s, -x) > 0.0) {
+        s = __longlong_as_double(__double_as_longlong(s) - 1);
+        if (__builtin_fma(s, s, -x) > 0.0) s = __longlong_as_double(__double_as_longlong(s) - 1);
+    } else {
+        const double n = __longlong_as_double(__double_as_longlong(s) + 1);
+        if (__builtin_fma(n, n, -x) <= 0.0) s = n;
+    }
+    return s;
+}
+
+// Order-preserving map from double to uint64 (total order on non-NaN values,
+// -0.0 folded onto +0.0 so that it compares equal like the double does).
+__device__ __forceinline__ uint64_t okey(double x) {
+    uint64_t b = __double_as_longlong(x + 0.0);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double okey_inv(uint64_t k) {
+    uint64_t b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+    return __longlong_as_double(b);
+}
+
+__device__ __forceinline__ double box_dist2(const double* __restrict__ c, const double* __restrict__ lo,
+                                            const double* __restrict__ hi, int D) {
+    double dist = 0.0;
+    for (int d = 0; d < D; ++d) {
+        const double cv = c[d];
+        double diff = 0.0;
+        if (cv < lo[d]) diff = lo[d] - cv;
+        else if (cv > hi[d]) diff = cv - hi[d];
+        dist = __builtin_fma(diff, diff, dist);
+    }
+    return dist;
+}
+
+__device__ __forceinline__ double point_dist2(const double* __restrict__ v, const double* __restrict__ q, int D) {
+    double acc = 0.0;
+    for (int d = 0; d < D; ++d) {
+        const double diff = q[d] - v[d];
+        acc = __builtin_fma(diff, diff, acc);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ bool in_box(const double* __restrict__ v, const double* __restrict__ lo,
+                                       const double* __restrict__ hi, int D) {
+    bool ok = true;
+    for (int d = 0; d < D; ++d) ok = ok && (v[d] >= lo[d]) && (v[d] <= hi[d]);
+    return ok;
+}
+
+__device__ __forceinline__ double knn_lb(const double* __restrict__ c, const double* __restrict__ q, double r,
+                                         int D) {
+    double dist = 0.0;
+    for (int d = 0; d < D; ++d) {
+        const double diff = q[d] - c[d];
+        dist = __builtin_fma(diff, diff, dist);
+    }
+    return sqrt_rd(dist) - r;
+}
+
+__device__ __forceinline__ uint64_t ld_acquire(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+__device__ __forceinline__ double wave_max_f64(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Per-launch control block for the single-pass scans (decoupled look-back).
+// Status words carry their own epoch so the tile-status arrays never need to be
+// cleared between launches: word = epoch(27b) | valid(1b) | value(36b).
+struct ScanCtrl {
+    uint64_t tileCounter;   // dynamic tile ids; reset by the last workgroup to leave
+    uint64_t exitCounter;
+    uint64_t outCount;      // survivors written by the last launch
+    uint64_t outNnz;        // sum of the survivors' weights (next level's size)
+    double ub;              // KNN STEP bound
+    uint64_t aux;           // spare
+    uint64_t pad[2];
+};
+
+constexpr int kValBits = 36;
+constexpr uint64_t kValMask = (1ull << kValBits) - 1;
+
+__device__ __forceinline__ uint64_t status_word(uint32_t epoch, uint64_t v) {
+    return (static_cast<uint64_t>(epoch & 0x7FFFFFF) << 37) | (1ull << 36) | (v & kValMask);
+}
+__device__ __forceinline__ bool status_valid(uint64_t w, uint32_t epoch) {
+    return (w >> 36) == ((static_cast<uint64_t>(epoch & 0x7FFFFFF) << 1) | 1ull);
+}
+
+}  // namespace tblaeq

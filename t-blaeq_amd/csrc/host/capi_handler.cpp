@@ -1,0 +1,212 @@
+// C facade over QueryHandler for foreign callers (ctypes, cgo, JNI).
+#include <cstring>
+#include <iostream>
+
+#include "../../../include/tblaeq_handler.h"
+#include "../ctx.hpp"
+#include "tblaeq_host.hpp"
+
+using namespace tblaeq_host;
+
+struct tblaeq_handler {
+    std::unique_ptr<QueryHandler> h;
+};
+
+struct tblaeq_result {
+    QueryResult r;
+    std::vector<std::vector<uint64_t>> ids;  // host copies of fine meshes
+};
+
+namespace {
+template <class F>
+int guarded(const char* what, F&& f) {
+    try {
+        f();
+        return TBLAEQ_OK;
+    } catch (const HipError& e) {
+        tblaeq::set_error(std::string(what) + ": " + e.what());
+        return TBLAEQ_ERR_HIP;
+    } catch (const std::invalid_argument& e) {
+        tblaeq::set_error(std::string(what) + ": " + e.what());
+        return TBLAEQ_ERR_INVALID;
+    } catch (const std::exception& e) {
+        tblaeq::set_error(std::string(what) + ": " + e.what());
+        return TBLAEQ_ERR_IO;
+    }
+}
+std::vector<size_t> toVec(const uint64_t* r, uint64_t n) { return std::vector<size_t>(r, r + n); }
+}  // namespace
+
+extern "C" {
+
+int tblaeq_handler_create_random(uint64_t N, uint64_t D, double vmin, double vmax, int isInt, uint64_t height,
+                                 const uint64_t* ratios, uint64_t seed, double sigmaDivisor, tblaeq_handler** out) {
+    if (!out || !ratios) return TBLAEQ_ERR_INVALID;
+    return guarded("tblaeq_handler_create_random", [&] {
+        auto* h = new tblaeq_handler();
+        h->h = std::make_unique<QueryHandler>(N, D, vmin, vmax, isInt != 0, height, toVec(ratios, height - 1), seed,
+                                              sigmaDivisor);
+        *out = h;
+    });
+}
+
+int tblaeq_handler_create_from_data(const double* data, uint64_t N, uint64_t D, uint64_t height,
+                                    const uint64_t* ratios, const char* name, tblaeq_handler** out) {
+    if (!out || !data || !ratios) return TBLAEQ_ERR_INVALID;
+    return guarded("tblaeq_handler_create_from_data", [&] {
+        PointCloud pc;
+        pc.size = static_cast<int64_t>(N);
+        pc.dim = static_cast<int64_t>(D);
+        pc.data.assign(data, data + N * D);
+        auto* h = new tblaeq_handler();
+        h->h = std::make_unique<QueryHandler>(true, pc, name ? name : "in-memory", height, toVec(ratios, height - 1));
+        *out = h;
+    });
+}
+
+int tblaeq_handler_load(const char* path, tblaeq_handler** out) {
+    if (!out || !path) return TBLAEQ_ERR_INVALID;
+    int rc = guarded("tblaeq_handler_load", [&] {
+        auto* h = new tblaeq_handler();
+        h->h = std::make_unique<QueryHandler>(std::string(path), true);
+        *out = h;
+    });
+    return rc == TBLAEQ_ERR_IO || rc == TBLAEQ_ERR_INVALID ? TBLAEQ_ERR_IO : rc;
+}
+
+int tblaeq_handler_save(tblaeq_handler* h, const char* path) {
+    if (!h || !path) return TBLAEQ_ERR_INVALID;
+    return guarded("tblaeq_handler_save", [&] { h->h->saveIndex(path); });
+}
+
+int tblaeq_handler_destroy(tblaeq_handler* h) {
+    delete h;
+    return TBLAEQ_OK;
+}
+
+int tblaeq_handler_meta(tblaeq_handler* h, uint64_t* meta, uint64_t* meshSizes, uint64_t cap) {
+    if (!h || !meta) return TBLAEQ_ERR_INVALID;
+    IndexData& idx = h->h->index();
+    meta[0] = idx.D;
+    meta[1] = idx.N;
+    meta[2] = idx.height;
+    meta[3] = idx.intervals;
+    for (uint64_t i = 0; i < cap && i < idx.meshSizes.size(); ++i) meshSizes[i] = idx.meshSizes[i];
+    return TBLAEQ_OK;
+}
+
+int tblaeq_handler_data(tblaeq_handler* h, double* out, uint64_t cap) {
+    if (!h || !out) return TBLAEQ_ERR_INVALID;
+    const auto& d = h->h->index().finestData;
+    if (d.empty() || cap < d.size()) return TBLAEQ_ERR_CAPACITY;
+    std::memcpy(out, d.data(), d.size() * 8);
+    return TBLAEQ_OK;
+}
+
+int tblaeq_handler_prepare(tblaeq_handler* h, const uint8_t* levels, uint64_t n, uint64_t freeMemOverride) {
+    if (!h) return TBLAEQ_ERR_INVALID;
+    return guarded("tblaeq_handler_prepare", [&] {
+        if (levels) {
+            IndexPolicy p;
+            for (uint64_t i = 0; i < n; ++i) {
+                if (levels[i] > 3) throw std::invalid_argument("policy level out of range");
+                p.levels.push_back(static_cast<LevelPolicy>(levels[i]));
+            }
+            h->h->prepareForQuery(p);
+        } else if (freeMemOverride) {
+            h->h->prepareForQuery(static_cast<size_t>(freeMemOverride));
+        } else {
+            h->h->prepareForQuery();
+        }
+    });
+}
+
+int tblaeq_handler_policy(tblaeq_handler* h, uint8_t* out, uint64_t cap) {
+    if (!h || !out) return TBLAEQ_ERR_INVALID;
+    const auto& p = h->h->getActivePolicy();
+    for (uint64_t i = 0; i < cap && i < p.levels.size(); ++i) out[i] = static_cast<uint8_t>(p.levels[i]);
+    return TBLAEQ_OK;
+}
+
+int tblaeq_handler_recommend(tblaeq_handler* h, uint64_t freeMem, int countWorkBuffers, uint8_t* out, uint64_t cap) {
+    if (!h || !out) return TBLAEQ_ERR_INVALID;
+    return guarded("tblaeq_handler_recommend", [&] {
+        IndexData& idx = h->h->index();
+        idx.computeStats();
+        const IndexPolicy p = PolicyScheduler::recommend(idx, freeMem, countWorkBuffers != 0);
+        for (uint64_t i = 0; i < cap && i < p.levels.size(); ++i) out[i] = static_cast<uint8_t>(p.levels[i]);
+    });
+}
+
+int tblaeq_handler_set_l3_tile_bytes(uint64_t bytes) {
+    L3Strategy::forcedTileBytes = bytes;
+    return TBLAEQ_OK;
+}
+
+int tblaeq_handler_query(tblaeq_handler* h, int type, const double* queries, uint64_t nq, uint64_t K, int refine,
+                         int saveFineMesh, tblaeq_result** out) {
+    if (!h || !out || !queries || nq == 0 || (type != 0 && type != 1)) return TBLAEQ_ERR_INVALID;
+    return guarded("tblaeq_handler_query", [&] {
+        const int D = static_cast<int>(h->h->getDim());
+        Query q(static_cast<int>(nq), D, type == 0 ? QueryType::RANGE : QueryType::POINT);
+        std::memcpy(q.data.data(), queries, q.data.size() * 8);
+        QueryEngine::RunConfig cfg;
+        cfg.K = K;
+        cfg.refine = refine != 0;
+        cfg.saveFineMesh = saveFineMesh != 0;
+        auto* r = new tblaeq_result{h->h->performQuery(q, cfg), {}};
+        if (r->r.errorCode) {
+            delete r;
+            throw std::invalid_argument("query dimension mismatch");
+        }
+        *out = r;
+    });
+}
+
+int tblaeq_result_destroy(tblaeq_result* r) {
+    delete r;
+    return TBLAEQ_OK;
+}
+
+int tblaeq_result_info(tblaeq_result* r, uint64_t* nq, uint64_t* levels, double* totalUs) {
+    if (!r) return TBLAEQ_ERR_INVALID;
+    if (nq) *nq = r->r.fineMeshSize.size();
+    if (levels) *levels = r->r.levelFineMeshSize.empty() ? 0 : r->r.levelFineMeshSize[0].size();
+    if (totalUs) *totalUs = static_cast<double>(r->r.totalTimeUs);
+    return TBLAEQ_OK;
+}
+
+int tblaeq_result_query(tblaeq_result* r, uint64_t i, uint64_t* levelSizes, double* us, uint64_t* fineSize,
+                        uint64_t* refinedCount) {
+    if (!r || i >= r->r.fineMeshSize.size()) return TBLAEQ_ERR_INVALID;
+    if (levelSizes)
+        for (size_t l = 0; l < r->r.levelFineMeshSize[i].size(); ++l) levelSizes[l] = r->r.levelFineMeshSize[i][l];
+    if (us) *us = r->r.queryTimeUsExact[i];
+    if (fineSize) *fineSize = r->r.fineMeshSize[i];
+    if (refinedCount) *refinedCount = i < r->r.refinedIds.size() ? r->r.refinedIds[i].size() : 0;
+    return TBLAEQ_OK;
+}
+
+int tblaeq_result_fine_mesh(tblaeq_result* r, uint64_t i, uint64_t* ids, double* vals) {
+    if (!r || i >= r->r.fineMesh.size()) return TBLAEQ_ERR_INVALID;
+    const SparseGrid* g = r->r.fineMesh[i];
+    const size_t n = g->get_nnz_nums(), D = g->get_dimensions();
+    if (ids) std::memcpy(ids, g->get_ids_(), n * 8);
+    if (vals) std::memcpy(vals, g->get_vals_(), n * D * 8);
+    return TBLAEQ_OK;
+}
+
+int tblaeq_result_refined(tblaeq_result* r, uint64_t i, uint64_t* ids, double* d2) {
+    if (!r || i >= r->r.refinedIds.size()) return TBLAEQ_ERR_INVALID;
+    const auto& v = r->r.refinedIds[i];
+    if (ids) std::memcpy(ids, v.data(), v.size() * 8);
+    if (d2 && i < r->r.refinedDist2.size()) std::memcpy(d2, r->r.refinedDist2[i].data(), r->r.refinedDist2[i].size() * 8);
+    return TBLAEQ_OK;
+}
+
+int tblaeq_result_save_csv(tblaeq_result* r, const char* path) {
+    if (!r || !path) return TBLAEQ_ERR_INVALID;
+    return guarded("tblaeq_result_save_csv", [&] { saveQueryResult(r->r, path); });
+}
+
+}  // extern "C"

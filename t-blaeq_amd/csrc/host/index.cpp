@@ -1,0 +1,636 @@
+// IndexData, on-disk format, synthetic generator, index assembly and text I/O.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <numeric>
+#include <random>
+#include <sstream>
+
+#include "tblaeq_host.hpp"
+
+namespace tblaeq_host {
+
+void hipCheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void tbCheck(int rc, const char* what) {
+    if (rc != TBLAEQ_OK)
+        throw HipError(std::string(what) + ": " + tblaeq_status_string(rc) + " (" + tblaeq_last_error() + ")");
+}
+
+// ---------------------------------------------------------------- IndexData
+
+IndexData::~IndexData() {
+    dLevels.clear();
+    work.clear();
+    dCoarsestIds.free();
+    dCoarsestVals.free();
+    dCounts.free();
+    dQuery.free();
+    if (hQuery) (void)hipHostFree(hQuery);
+    if (ctx) tblaeq_ctx_destroy(ctx);
+}
+
+void IndexData::ensureCtx() {
+    if (!ctx) tbCheck(tblaeq_ctx_create(0, nullptr, &ctx), "tblaeq_ctx_create");
+}
+
+// IndexData.cu:141-194.
+void IndexData::computeStats() {
+    stats = MemoryStats{};
+    for (const auto& t : pTensors) {
+        stats.hostPTensors += t->nnz * t->D * 8 + t->cols * 8 + (t->cols + 1) * 8 + t->nnz * 8;
+        stats.devicePTensorVals += t->nnz * t->D * 8;
+        stats.deviceSpTSpMBufs += 3 * t->rows * 8;
+    }
+    for (size_t i = 1; i < height; ++i) {
+        if (!maps[i].empty()) {
+            stats.hostMaps += meshSizes[intervals - i] * 8;
+            stats.deviceMaps += meshSizes[intervals - i] * 8;
+        }
+    }
+    for (size_t i = 0; i < intervals; ++i) {
+        stats.hostRadius += meshSizes[intervals - i] * 8;
+        stats.deviceRadius += meshSizes[intervals - i] * 8;
+    }
+    stats.hostCoarsestMesh = coarsestIds.size() * 8 + coarsestVals.size() * 8;
+    stats.deviceCoarsestMesh = stats.hostCoarsestMesh;
+    stats.hostTotal = stats.hostPTensors + stats.hostMaps + stats.hostRadius + stats.hostCoarsestMesh;
+    stats.deviceTotal = stats.devicePTensorVals + stats.deviceMaps + stats.deviceRadius + stats.deviceCoarsestMesh;
+    stats.grandTotal = stats.hostTotal + stats.deviceTotal;
+}
+
+static std::string fmtBytes(size_t b) {
+    std::ostringstream o;
+    o << std::fixed << std::setprecision(2);
+    if (b >= (1ull << 30)) o << b / double(1ull << 30) << " GiB";
+    else if (b >= (1ull << 20)) o << b / double(1ull << 20) << " MiB";
+    else if (b >= (1ull << 10)) o << b / double(1ull << 10) << " KiB";
+    else o << b << " B";
+    return o.str();
+}
+
+void MemoryStats::print(std::ostream& os) const {
+    os << "\nIndex Memory Usage\n[HOST]\n"
+       << "    P-tensor arrays : " << fmtBytes(hostPTensors) << "\n"
+       << "    Sort maps : " << fmtBytes(hostMaps) << "\n"
+       << "    Cluster radius : " << fmtBytes(hostRadius) << "\n"
+       << "    Coarsest mesh : " << fmtBytes(hostCoarsestMesh) << "\n"
+       << "    HOST TOTAL : " << fmtBytes(hostTotal) << "\n[DEVICE]\n"
+       << "    P-tensor vals : " << fmtBytes(devicePTensorVals) << "\n"
+       << "    Sort maps : " << fmtBytes(deviceMaps) << "\n"
+       << "    Cluster radius : " << fmtBytes(deviceRadius) << "\n"
+       << "    Coarsest mesh : " << fmtBytes(deviceCoarsestMesh) << "\n"
+       << "    DEVICE TOTAL : " << fmtBytes(deviceTotal) << "\nGRAND TOTAL : " << fmtBytes(grandTotal) << "\n\n";
+}
+
+// Permanent device data: CSC column structure, child ids (= map[l+1] composed
+// with rowIds, folding the refactor into the index), coarsest mesh.
+// IndexData.cu:52-74 uploads maps + coarsest mesh; the column structure is new
+// (the reference keeps it on the host and expands index arrays per query).
+void IndexData::uploadPermanentData() {
+    if (permanentDataOnDevice) return;
+    ensureCtx();
+    dLevels.resize(intervals);
+    for (size_t l = 0; l < intervals; ++l) {
+        const auto& P = *pTensors[l];
+        auto& L = dLevels[l];
+        L.colRes.reset(P.colRes.data(), P.colRes.size());
+        L.nnzPerCol.reset(P.nnzPerCol.data(), P.nnzPerCol.size());
+        const auto& map = maps[l + 1];
+        std::vector<uint64_t> child(P.nnz);
+        for (size_t j = 0; j < P.nnz; ++j) {
+            const uint64_t r = P.rowIds[j];
+            child[j] = map.empty() ? r : map[r];
+        }
+        L.childIds.reset(child.data(), child.size());
+        L.emptyCols = static_cast<uint64_t>(std::count(P.nnzPerCol.begin(), P.nnzPerCol.end(), uint64_t{0}));
+    }
+    dCoarsestIds.reset(coarsestIds.data(), coarsestIds.size());
+    dCoarsestVals.reset(coarsestVals.data(), coarsestVals.size());
+    dCounts.resize(2 * intervals + 2);
+    std::vector<uint64_t> c(2 * intervals + 2, 0);
+    c[2 * intervals] = coarsestIds.size();
+    hipCheck(hipMemcpy(dCounts.data(), c.data(), c.size() * 8, hipMemcpyHostToDevice), "counts");
+    dQuery.resize(2 * D);
+    hipCheck(hipHostMalloc(reinterpret_cast<void**>(&hQuery), 2 * D * sizeof(double), hipHostMallocDefault),
+             "hipHostMalloc");
+    permanentDataOnDevice = true;
+}
+
+void IndexData::uploadLevelValues(size_t l) {
+    auto& L = dLevels[l];
+    if (L.resident) return;
+    L.pvals.reset(pTensors[l]->vals.data(), pTensors[l]->vals.size());
+    L.radius.reset(meshMaxRadius[l].data(), meshMaxRadius[l].size());
+    L.resident = true;
+}
+
+void IndexData::allocLevelWork(size_t l) {
+    if (work.size() != intervals) work.resize(intervals);
+    auto& W = work[l];
+    const size_t cols = pTensors[l]->cols, nnz = pTensors[l]->nnz;
+    if (W.survIds.size() == std::max<size_t>(cols, 1)) return;
+    W.survIds.resize(std::max<size_t>(cols, 1));
+    W.survOff.resize(std::max<size_t>(cols, 1));
+    W.survVals.resize(std::max<size_t>(cols * D, 1));
+    W.outIds.resize(std::max<size_t>(nnz, 1));
+    W.outVals.resize(std::max<size_t>(nnz * D, 1));
+}
+
+tblaeq_level IndexData::levelDesc(size_t l) const {
+    const auto& L = dLevels[l];
+    tblaeq_level v{};
+    v.D = D;
+    v.cols = pTensors[l]->cols;
+    v.nnz = pTensors[l]->nnz;
+    v.col_res = L.colRes.data();
+    v.nnz_per_col = L.nnzPerCol.data();
+    v.pvals = L.pvals.data();
+    v.radius = L.radius.data();
+    v.child_ids = L.childIds.data();
+    v.empty_cols = L.emptyCols;
+    return v;
+}
+
+// ---------------------------------------------------------- IndexSerializer
+
+namespace {
+template <class T>
+void wr(std::ofstream& f, const T* p, size_t n) {
+    f.write(reinterpret_cast<const char*>(p), static_cast<std::streamsize>(n * sizeof(T)));
+}
+template <class T>
+void rd(std::ifstream& f, T* p, size_t n, const std::string& path) {
+    f.read(reinterpret_cast<char*>(p), static_cast<std::streamsize>(n * sizeof(T)));
+    if (!f) throw std::runtime_error("IndexSerializer: short read in " + path);
+}
+}  // namespace
+
+void IndexSerializer::save(const IndexData& d, const std::string& dir) {
+    std::filesystem::create_directories(dir);
+    const uint8_t aos = d.isAosArch ? 1 : 0;
+    {
+        const std::string p = dir + "/metadata.bin";
+        std::ofstream f(p, std::ios::binary);
+        if (!f) throw std::runtime_error("Cannot write: " + p);
+        const uint64_t meta[4] = {d.D, d.N, d.height, d.intervals};
+        wr(f, meta, 4);
+        wr(f, &aos, 1);
+        wr(f, d.ratios.data(), d.intervals);
+        const uint64_t msz = d.meshSizes.size();
+        wr(f, &msz, 1);
+        wr(f, d.meshSizes.data(), msz);
+    }
+    for (size_t i = 0; i < d.intervals; ++i) {
+        const std::string p = dir + "/P_Tensor_" + std::to_string(i) + ".bin";
+        std::ofstream f(p, std::ios::binary);
+        if (!f) throw std::runtime_error("Cannot write: " + p);
+        const auto& t = *d.pTensors[i];
+        const uint64_t hdr[4] = {t.D, t.rows, t.cols, t.nnz};
+        const uint8_t a = t.isAos ? 1 : 0;
+        wr(f, hdr, 4);
+        wr(f, &a, 1);
+        wr(f, t.rowIds.data(), t.nnz);
+        wr(f, t.nnzPerCol.data(), t.cols);
+        wr(f, t.colRes.data(), t.cols + 1);
+        wr(f, t.vals.data(), t.nnz * t.D);
+    }
+    for (size_t i = 0; i < d.intervals; ++i) {
+        const std::string p = dir + "/MaxRadius_" + std::to_string(i) + ".bin";
+        std::ofstream f(p, std::ios::binary);
+        if (!f) throw std::runtime_error("Cannot write: " + p);
+        const uint64_t n = d.meshSizes[d.intervals - i];
+        wr(f, &n, 1);
+        wr(f, d.meshMaxRadius[i].data(), n);
+    }
+    for (size_t i = 1; i < d.height; ++i) {
+        if (d.maps[i].empty()) continue;
+        const std::string p = dir + "/Map_" + std::to_string(i) + ".bin";
+        std::ofstream f(p, std::ios::binary);
+        if (!f) throw std::runtime_error("Cannot write: " + p);
+        const uint64_t n = d.meshSizes[d.intervals - i];
+        wr(f, &n, 1);
+        wr(f, d.maps[i].data(), n);
+    }
+    {
+        const std::string p = dir + "/Coreast_Mesh.bin";
+        std::ofstream f(p, std::ios::binary);
+        if (!f) throw std::runtime_error("Cannot write: " + p);
+        const uint64_t hdr[3] = {d.coarsestRows, d.D, d.coarsestIds.size()};
+        wr(f, hdr, 3);
+        wr(f, &aos, 1);
+        wr(f, d.coarsestIds.data(), d.coarsestIds.size());
+        wr(f, d.coarsestVals.data(), d.coarsestVals.size());
+    }
+}
+
+IndexData* IndexSerializer::load(const std::string& dir) {
+    auto d = std::make_unique<IndexData>();
+    {
+        const std::string p = dir + "/metadata.bin";
+        std::ifstream f(p, std::ios::binary);
+        if (!f) throw std::runtime_error("Cannot read: " + p);
+        uint64_t meta[4];
+        uint8_t aos;
+        rd(f, meta, 4, p);
+        rd(f, &aos, 1, p);
+        d->D = meta[0];
+        d->N = meta[1];
+        d->height = meta[2];
+        d->intervals = meta[3];
+        d->isAosArch = aos != 0;
+        if (d->height != d->intervals + 1 || d->D == 0 || d->intervals == 0)
+            throw std::runtime_error("IndexSerializer: inconsistent metadata in " + p);
+        d->ratios.resize(d->intervals);
+        rd(f, d->ratios.data(), d->intervals, p);
+        uint64_t msz;
+        rd(f, &msz, 1, p);
+        if (msz != d->height) throw std::runtime_error("IndexSerializer: mesh size count mismatch in " + p);
+        d->meshSizes.resize(msz);
+        rd(f, d->meshSizes.data(), msz, p);
+    }
+    d->pTensors.resize(d->intervals);
+    d->meshMaxRadius.resize(d->intervals);
+    d->maps.resize(d->height);
+    for (size_t i = 0; i < d->intervals; ++i) {
+        const std::string p = dir + "/P_Tensor_" + std::to_string(i) + ".bin";
+        std::ifstream f(p, std::ios::binary);
+        if (!f) throw std::runtime_error("Cannot read: " + p);
+        auto t = std::make_unique<SparseTensorCscFormat>();
+        uint64_t hdr[4];
+        uint8_t a;
+        rd(f, hdr, 4, p);
+        rd(f, &a, 1, p);
+        t->D = hdr[0];
+        t->rows = hdr[1];
+        t->cols = hdr[2];
+        t->nnz = hdr[3];
+        t->isAos = a != 0;
+        t->rowIds.resize(t->nnz);
+        t->nnzPerCol.resize(t->cols);
+        t->colRes.resize(t->cols + 1);
+        t->vals.resize(t->nnz * t->D);
+        rd(f, t->rowIds.data(), t->nnz, p);
+        rd(f, t->nnzPerCol.data(), t->cols, p);
+        rd(f, t->colRes.data(), t->cols + 1, p);
+        rd(f, t->vals.data(), t->nnz * t->D, p);
+        if (t->colRes[t->cols] != t->nnz) throw std::runtime_error("IndexSerializer: CSC column pointers in " + p);
+        d->pTensors[i] = std::move(t);
+    }
+    for (size_t i = 0; i < d->intervals; ++i) {
+        const std::string p = dir + "/MaxRadius_" + std::to_string(i) + ".bin";
+        std::ifstream f(p, std::ios::binary);
+        if (!f) throw std::runtime_error("Cannot read: " + p);
+        uint64_t n;
+        rd(f, &n, 1, p);
+        if (n != d->meshSizes[d->intervals - i]) throw std::runtime_error("IndexSerializer: radius count in " + p);
+        d->meshMaxRadius[i].resize(n);
+        rd(f, d->meshMaxRadius[i].data(), n, p);
+    }
+    for (size_t i = 1; i < d->height; ++i) {
+        const std::string p = dir + "/Map_" + std::to_string(i) + ".bin";
+        if (!std::filesystem::exists(p)) continue;
+        std::ifstream f(p, std::ios::binary);
+        uint64_t n;
+        rd(f, &n, 1, p);
+        if (n != d->meshSizes[d->intervals - i]) throw std::runtime_error("IndexSerializer: map size in " + p);
+        d->maps[i].resize(n);
+        rd(f, d->maps[i].data(), n, p);
+    }
+    {
+        const std::string p = dir + "/Coreast_Mesh.bin";
+        std::ifstream f(p, std::ios::binary);
+        if (!f) throw std::runtime_error("Cannot read: " + p);
+        uint64_t hdr[3];
+        uint8_t a;
+        rd(f, hdr, 3, p);
+        rd(f, &a, 1, p);
+        d->coarsestRows = hdr[0];
+        d->coarsestIds.resize(hdr[2]);
+        d->coarsestVals.resize(hdr[2] * hdr[1]);
+        rd(f, d->coarsestIds.data(), hdr[2], p);
+        rd(f, d->coarsestVals.data(), hdr[2] * hdr[1], p);
+    }
+    return d.release();
+}
+
+// ------------------------------------------------------------- IndexBuilder
+
+const std::vector<size_t> IndexBuilder::kDefaultRatios = {100, 50, 20};
+
+size_t IndexBuilder::computeCentroidCount(size_t n, size_t ratio) {
+    if (ratio == 0 || n / ratio == 0) return 1;
+    return n / ratio;
+}
+
+// Setup.cuh:87-171 and func.hpp:124-137,202-213 over flat layers.
+IndexData* IndexBuilder::assemble(size_t N, size_t D, const std::vector<size_t>& ratios,
+                                  std::vector<std::vector<double>>& layers,
+                                  const std::vector<std::vector<uint64_t>>& labels) {
+    auto idx = std::make_unique<IndexData>();
+    idx->D = D;
+    idx->N = N;
+    idx->height = ratios.size() + 1;
+    idx->intervals = ratios.size();
+    idx->ratios = ratios;
+    idx->pTensors.resize(idx->intervals);
+    idx->meshMaxRadius.resize(idx->intervals);
+    idx->maps.resize(idx->height);
+    for (size_t i = 0; i < idx->intervals; ++i) {
+        const size_t fineIdx = idx->intervals - i, pIdx = fineIdx - 1;
+        const auto& fine = layers[i];
+        const auto& coarse = layers[i + 1];
+        const auto& lab = labels[i];
+        const size_t nF = lab.size(), nC = coarse.size() / D;
+        idx->meshSizes.push_back(nF);
+        auto P = std::make_unique<SparseTensorCscFormat>();
+        P->D = D;
+        P->rows = nF;
+        P->cols = nC;
+        P->nnz = nF;
+        P->nnzPerCol.assign(nC, 0);
+        for (uint64_t c : lab) ++P->nnzPerCol[c];
+        P->colRes.assign(nC + 1, 0);
+        for (size_t c = 0; c < nC; ++c) P->colRes[c + 1] = P->colRes[c] + P->nnzPerCol[c];
+        P->rowIds.resize(nF);
+        std::iota(P->rowIds.begin(), P->rowIds.end(), uint64_t{0});
+        // sort-to-original map: argsort of labels with std::sort on the index
+        // vector (the permutation among equal labels is the library's).
+        std::vector<size_t> order(nF);
+        std::iota(order.begin(), order.end(), size_t{0});
+        std::sort(order.begin(), order.end(), [&lab](size_t a, size_t b) { return lab[a] < lab[b]; });
+        auto& map = idx->maps[fineIdx];
+        map.assign(order.begin(), order.end());
+        P->vals.resize(nF * D);
+        auto& rad = idx->meshMaxRadius[pIdx];
+        rad.assign(nC, 0.0);
+        for (size_t c = 0; c < nC; ++c) {
+            const double* cv = &coarse[c * D];
+            double best = 0.0;
+            for (uint64_t pos = P->colRes[c]; pos < P->colRes[c + 1]; ++pos) {
+                const double* fv = &fine[map[pos] * D];
+                double s = 0.0;
+                for (size_t d = 0; d < D; ++d) {
+                    P->vals[pos * D + d] = fv[d] / cv[d];
+                    const double df = cv[d] - fv[d];
+                    const double sq = df * df;
+                    s = s + sq;
+                }
+                best = std::max(best, std::sqrt(s));
+            }
+            rad[c] = best;
+        }
+        idx->pTensors[pIdx] = std::move(P);
+        if (i + 2 == idx->height) {
+            idx->meshSizes.push_back(nC);
+            idx->coarsestRows = nC;
+            idx->coarsestIds.resize(nC);
+            std::iota(idx->coarsestIds.begin(), idx->coarsestIds.end(), uint64_t{0});
+            idx->coarsestVals = coarse;
+        }
+    }
+    return idx.release();
+}
+
+// RandomKmeans.cuh:176-250: coarsest layer uniform in [valMin, valMax]^D, each
+// finer layer assigns child ci to parent ci % nParents and draws it from a
+// normal of sigma = ((valMax-valMin)/nParents^(1/D))/sigmaDivisor, clamped.
+IndexData* IndexBuilder::buildRandom(size_t N, size_t D, double valMin, double valMax, bool isInt,
+                                     const std::string& name, size_t height, const std::vector<size_t>& ratios,
+                                     uint64_t seed, double sigmaDivisor) {
+    if (valMin <= 0.0) throw std::invalid_argument("RandomKmeans: valMin must be > 0");
+    if (valMin >= valMax) throw std::invalid_argument("RandomKmeans: valMin must be < valMax");
+    if (sigmaDivisor <= 0.0) throw std::invalid_argument("RandomKmeans: sigmaDivisor must be > 0");
+    if (height < 2 || ratios.size() != height - 1) throw std::invalid_argument("buildRandom: bad height/ratios");
+    const size_t L = ratios.size();
+    std::vector<size_t> sizes(L + 1);
+    sizes[0] = N;
+    for (size_t i = 0; i < L; ++i) sizes[i + 1] = std::max<size_t>(1, sizes[i] / ratios[i]);
+    std::vector<std::vector<double>> layers(L + 1);
+    std::vector<std::vector<uint64_t>> labels(L);
+    std::mt19937_64 rng(seed);
+    auto round = [isInt](double v) { return isInt ? std::round(v) : v; };
+    std::uniform_real_distribution<double> uni(valMin, valMax);
+    layers[L].resize(sizes[L] * D);
+    for (auto& v : layers[L]) v = round(uni(rng));
+    for (size_t lvl = L; lvl > 0; --lvl) {
+        const size_t nP = sizes[lvl], nC = sizes[lvl - 1];
+        const double spacing = (valMax - valMin) / std::pow(static_cast<double>(nP), 1.0 / static_cast<double>(D));
+        std::normal_distribution<double> noise(0.0, spacing / sigmaDivisor);
+        const auto& par = layers[lvl];
+        auto& ch = layers[lvl - 1];
+        auto& lab = labels[lvl - 1];
+        ch.resize(nC * D);
+        lab.resize(nC);
+        for (size_t ci = 0; ci < nC; ++ci) {
+            const size_t p = ci % nP;
+            lab[ci] = p;
+            for (size_t d = 0; d < D; ++d) ch[ci * D + d] = round(std::clamp(par[p * D + d] + noise(rng), valMin, valMax));
+        }
+    }
+    IndexData* idx = assemble(N, D, ratios, layers, labels);
+    idx->datasetName = name;
+    idx->finestData = std::move(layers[0]);
+    return idx;
+}
+
+// IndexBuilder.cu:44-151 with Lloyd k-means + k-means++ (CPUKmeans.cuh:67-205)
+// on the host.  The MI355X k-means build is a separate entry point.
+IndexData* IndexBuilder::build(const double* data, size_t N, size_t D, const std::string& name, bool /*forceUseCPU*/,
+                               size_t height, const std::vector<size_t>& ratios) {
+    if (height < 2 || ratios.size() != height - 1) throw std::invalid_argument("build: bad height/ratios");
+    constexpr size_t kMaxIter = 7;
+    const size_t L = ratios.size();
+    std::vector<std::vector<double>> layers(L + 1);
+    std::vector<std::vector<uint64_t>> labels(L);
+    layers[0].assign(data, data + N * D);
+    for (size_t i = 0; i < L; ++i) {
+        const auto& x = layers[i];
+        const size_t n = x.size() / D;
+        const size_t k = computeCentroidCount(n, ratios[i]);
+        auto sq = [D](const double* a, const double* b) {
+            double s = 0.0;
+            for (size_t d = 0; d < D; ++d) {
+                const double df = a[d] - b[d];
+                const double p = df * df;
+                s = s + p;
+            }
+            return s;
+        };
+        std::vector<double> c;
+        c.reserve(k * D);
+        std::mt19937_64 rng(42);
+        std::uniform_int_distribution<size_t> uni(0, n - 1);
+        const size_t first = uni(rng);
+        c.insert(c.end(), &x[first * D], &x[first * D] + D);
+        std::vector<double> minD(n, std::numeric_limits<double>::max());
+        for (size_t cc = 1; cc < k; ++cc) {
+            const double* last = &c[(cc - 1) * D];
+            double total = 0.0;
+            for (size_t j = 0; j < n; ++j) {
+                const double dd = sq(&x[j * D], last);
+                if (dd < minD[j]) minD[j] = dd;
+                total += minD[j];
+            }
+            std::uniform_real_distribution<double> dist(0.0, total);
+            const double thr = dist(rng);
+            double cum = 0.0;
+            size_t chosen = n - 1;
+            for (size_t j = 0; j < n; ++j) {
+                cum += minD[j];
+                if (cum >= thr) {
+                    chosen = j;
+                    break;
+                }
+            }
+            c.insert(c.end(), &x[chosen * D], &x[chosen * D] + D);
+        }
+        auto& lab = labels[i];
+        lab.assign(n, 0);
+        for (size_t it = 0; it < kMaxIter; ++it) {
+            bool changed = false;
+            for (size_t j = 0; j < n; ++j) {
+                double best = std::numeric_limits<double>::max();
+                uint64_t bc = 0;
+                for (size_t q = 0; q < k; ++q) {
+                    const double dd = sq(&x[j * D], &c[q * D]);
+                    if (dd < best) {
+                        best = dd;
+                        bc = q;
+                    }
+                }
+                if (lab[j] != bc) {
+                    lab[j] = bc;
+                    changed = true;
+                }
+            }
+            std::vector<double> sums(k * D, 0.0);
+            std::vector<size_t> cnt(k, 0);
+            for (size_t j = 0; j < n; ++j) {
+                ++cnt[lab[j]];
+                for (size_t d = 0; d < D; ++d) sums[lab[j] * D + d] += x[j * D + d];
+            }
+            for (size_t q = 0; q < k; ++q)
+                if (cnt[q])
+                    for (size_t d = 0; d < D; ++d) c[q * D + d] = sums[q * D + d] / static_cast<double>(cnt[q]);
+            if (!changed) break;
+        }
+        layers[i + 1] = std::move(c);
+    }
+    IndexData* idx = assemble(N, D, ratios, layers, labels);
+    idx->datasetName = name;
+    return idx;
+}
+
+// ------------------------------------------------------------------ text I/O
+
+PointCloud loadFromFile(const std::string& path) {
+    std::ifstream f(path);
+    if (!f) throw std::runtime_error("loadFromFile: cannot open '" + path + "'");
+    std::string line;
+    if (!std::getline(f, line)) throw std::runtime_error("loadFromFile: empty file '" + path + "'");
+    PointCloud pc;
+    {
+        std::istringstream iss(line);
+        if (!(iss >> pc.size >> pc.dim) || pc.size <= 0 || pc.dim <= 0)
+            throw std::runtime_error("loadFromFile: invalid header in '" + path + "'");
+    }
+    pc.data.resize(static_cast<size_t>(pc.size * pc.dim));
+    int64_t row = 0;
+    while (row < pc.size && std::getline(f, line)) {
+        std::istringstream iss(line);
+        double v;
+        int64_t col = 0;
+        while (col < pc.dim && iss >> v) pc.data[static_cast<size_t>(row * pc.dim + col++)] = v;
+        if (col != pc.dim) throw std::runtime_error("loadFromFile: dimension mismatch at row " + std::to_string(row + 1));
+        ++row;
+    }
+    if (row != pc.size) throw std::runtime_error("loadFromFile: row count mismatch in '" + path + "'");
+    return pc;
+}
+
+static Query loadQuery(const std::string& path, QueryType t) {
+    std::ifstream f(path);
+    if (!f) throw std::runtime_error("loadQueryFromFile: cannot open '" + path + "'");
+    std::string line;
+    if (!std::getline(f, line)) throw std::runtime_error("loadQueryFromFile: empty file");
+    int len = 0, dim = 0;
+    {
+        std::istringstream iss(line);
+        if (!(iss >> len >> dim) || len <= 0 || dim <= 0) throw std::runtime_error("loadQueryFromFile: invalid header");
+    }
+    Query q(len, dim, t);
+    const int cols = t == QueryType::RANGE ? 2 * dim : dim;
+    int row = 0;
+    while (row < len && std::getline(f, line)) {
+        std::istringstream iss(line);
+        double v;
+        int col = 0;
+        while (col < cols && iss >> v) q.data[static_cast<size_t>(row) * cols + col++] = v;
+        if (col != cols) throw std::runtime_error("loadQueryFromFile: expected " + std::to_string(cols) + " values at row " + std::to_string(row + 1));
+        ++row;
+    }
+    if (row != len) throw std::runtime_error("loadQueryFromFile: row count mismatch");
+    return q;
+}
+
+Query loadQueryPointFromFile(const std::string& path) { return loadQuery(path, QueryType::POINT); }
+Query loadQueryRangeFromFile(const std::string& path) {
+    Query q = loadQuery(path, QueryType::RANGE);
+    q.queryRangeInfo = std::filesystem::path(path).stem().string();
+    return q;
+}
+
+void saveQueryToFile(const Query& q, const std::string& path) {
+    std::ofstream f(path);
+    if (!f) throw std::runtime_error("saveQueryToFile: cannot create '" + path + "'");
+    f << q.length << ' ' << q.dim << '\n';
+    f << std::setprecision(17);
+    const int cols = q.type == QueryType::RANGE ? 2 * q.dim : q.dim;
+    for (int i = 0; i < q.length; ++i) {
+        for (int j = 0; j < cols; ++j) {
+            if (j) f << ' ';
+            f << q.data[static_cast<size_t>(i) * cols + j];
+        }
+        f << '\n';
+    }
+}
+
+std::vector<double> Query::getQueryPoint(int i) const {
+    if (type != QueryType::POINT) throw std::runtime_error("getQueryPoint: query type is not POINT");
+    if (i < 0 || i >= length) throw std::out_of_range("getQueryPoint: index out of range");
+    return std::vector<double>(data.begin() + static_cast<long>(i) * dim, data.begin() + static_cast<long>(i + 1) * dim);
+}
+
+std::pair<std::vector<double>, std::vector<double>> Query::getQueryRange(int i) const {
+    if (type != QueryType::RANGE) throw std::runtime_error("getQueryRange: query type is not RANGE");
+    if (i < 0 || i >= length) throw std::out_of_range("getQueryRange: index out of range");
+    const size_t off = static_cast<size_t>(i) * 2 * dim;
+    return {std::vector<double>(data.begin() + off, data.begin() + off + dim),
+            std::vector<double>(data.begin() + off + dim, data.begin() + off + 2 * dim)};
+}
+
+void Query::setQueryPoint(int i, const std::vector<double>& p) {
+    if (type != QueryType::POINT) throw std::runtime_error("setQueryPoint: query type is not POINT");
+    if (i < 0 || i >= length) throw std::out_of_range("setQueryPoint: index out of range");
+    if (static_cast<int>(p.size()) != dim) throw std::invalid_argument("setQueryPoint: dimension mismatch");
+    std::copy(p.begin(), p.end(), data.begin() + static_cast<long>(i) * dim);
+}
+
+void Query::setQueryRange(int i, const std::vector<double>& lo, const std::vector<double>& hi) {
+    if (type != QueryType::RANGE) throw std::runtime_error("setQueryRange: query type is not RANGE");
+    if (i < 0 || i >= length) throw std::out_of_range("setQueryRange: index out of range");
+    if (static_cast<int>(lo.size()) != dim || static_cast<int>(hi.size()) != dim)
+        throw std::invalid_argument("setQueryRange: dimension mismatch");
+    const size_t off = static_cast<size_t>(i) * 2 * dim;
+    std::copy(lo.begin(), lo.end(), data.begin() + off);
+    std::copy(hi.begin(), hi.end(), data.begin() + off + dim);
+}
+
+}  // namespace tblaeq_host

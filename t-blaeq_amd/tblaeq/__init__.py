@@ -1,0 +1,223 @@
+"""ctypes binding of libtblaeq.so (the gfx950 HIP query path and its C++ host mirror).
+
+The library is built in-tree (``make -C t-blaeq_amd``).  There is no fallback:
+if the shared library is missing, importing any entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libtblaeq.so")
+_LIB = None
+
+u64p = C.POINTER(C.c_uint64)
+f64p = C.POINTER(C.c_double)
+u8p = C.POINTER(C.c_uint8)
+vp = C.c_void_p
+
+# Every function declared in include/tblaeq.h and include/tblaeq_handler.h.
+SIGNATURES = {
+    # tblaeq.h
+    "tblaeq_ctx_create": (C.c_int, [C.c_int, vp, C.POINTER(vp)]),
+    "tblaeq_ctx_destroy": (C.c_int, [vp]),
+    "tblaeq_ctx_sync": (C.c_int, [vp]),
+    "tblaeq_ctx_stream": (vp, [vp]),
+    "tblaeq_status_string": (C.c_char_p, [C.c_int]),
+    "tblaeq_last_error": (C.c_char_p, []),
+    "tblaeq_range_prune_compact": (C.c_int, [vp] * 13),
+    "tblaeq_knn_lower_bound_sort_step": (C.c_int, [vp, vp, vp, vp, vp, C.c_uint64, vp, C.c_uint64, vp, vp, vp, vp,
+                                                   vp]),
+    "tblaeq_prolong_refactor": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_uint64, vp, vp]),
+    "tblaeq_prolong_refactor_packed": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_uint64, vp, vp, vp, vp]),
+    "tblaeq_knn_lower_bounds": (C.c_int, [vp, C.c_uint64, vp, vp, C.c_uint64, vp, vp, vp]),
+    "tblaeq_range_prune_compact_tile": (C.c_int, [vp, vp, vp, vp, C.c_uint64, vp, vp, vp, vp, vp, vp, vp]),
+    "tblaeq_refine_range": (C.c_int, [vp, C.c_uint64, vp, vp, C.c_uint64, vp, vp, vp, C.c_uint64, u64p]),
+    "tblaeq_refine_knn_topk": (C.c_int, [vp, C.c_uint64, vp, vp, C.c_uint64, vp, C.c_uint64, vp, vp, u64p]),
+    "tblaeq_sort_pairs": (C.c_int, [vp, vp, vp, C.c_uint64]),
+    # tblaeq_handler.h
+    "tblaeq_handler_create_random": (C.c_int, [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_int, C.c_uint64,
+                                               u64p, C.c_uint64, C.c_double, C.POINTER(vp)]),
+    "tblaeq_handler_create_from_data": (C.c_int, [f64p, C.c_uint64, C.c_uint64, C.c_uint64, u64p, C.c_char_p,
+                                                  C.POINTER(vp)]),
+    "tblaeq_handler_load": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
+    "tblaeq_handler_save": (C.c_int, [vp, C.c_char_p]),
+    "tblaeq_handler_destroy": (C.c_int, [vp]),
+    "tblaeq_handler_meta": (C.c_int, [vp, u64p, u64p, C.c_uint64]),
+    "tblaeq_handler_data": (C.c_int, [vp, f64p, C.c_uint64]),
+    "tblaeq_handler_prepare": (C.c_int, [vp, u8p, C.c_uint64, C.c_uint64]),
+    "tblaeq_handler_policy": (C.c_int, [vp, u8p, C.c_uint64]),
+    "tblaeq_handler_recommend": (C.c_int, [vp, C.c_uint64, C.c_int, u8p, C.c_uint64]),
+    "tblaeq_handler_set_l3_tile_bytes": (C.c_int, [C.c_uint64]),
+    "tblaeq_handler_query": (C.c_int, [vp, C.c_int, f64p, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.POINTER(vp)]),
+    "tblaeq_result_destroy": (C.c_int, [vp]),
+    "tblaeq_result_info": (C.c_int, [vp, u64p, u64p, f64p]),
+    "tblaeq_result_query": (C.c_int, [vp, C.c_uint64, u64p, f64p, u64p, u64p]),
+    "tblaeq_result_fine_mesh": (C.c_int, [vp, C.c_uint64, u64p, f64p]),
+    "tblaeq_result_refined": (C.c_int, [vp, C.c_uint64, u64p, f64p]),
+    "tblaeq_result_save_csv": (C.c_int, [vp, C.c_char_p]),
+}
+
+
+class TblaeqError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libtblaeq.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise TblaeqError(f"{LIB_PATH} not built: run `make -C t-blaeq_amd` (or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc, what):
+    if rc != 0:
+        L = lib()
+        raise TblaeqError(f"{what}: {L.tblaeq_status_string(rc).decode()} ({L.tblaeq_last_error().decode()})")
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+LEVEL_POLICIES = {"L0": 0, "L1": 1, "L2": 2, "L3": 3}
+
+
+class Handler:
+    """Mirror of the reference QueryHandler (src/core/QueryHandler.cuh:50-247)."""
+
+    def __init__(self, h):
+        self.h = h
+        meta = np.zeros(4, np.uint64)
+        ms = np.zeros(16, np.uint64)
+        check(lib().tblaeq_handler_meta(self.h, _p(meta, u64p), _p(ms, u64p), 16), "meta")
+        self.D, self.N, self.height, self.intervals = (int(x) for x in meta)
+        self.mesh_sizes = [int(x) for x in ms[: self.height]]
+
+    @classmethod
+    def random(cls, N, D, vmin=1.0, vmax=100.0, is_int=False, ratios=(100, 50, 20), seed=12345, sigma_div=3.0):
+        r = np.asarray(ratios, np.uint64)
+        h = vp()
+        check(lib().tblaeq_handler_create_random(N, D, vmin, vmax, int(is_int), len(r) + 1, _p(r, u64p), seed,
+                                                 sigma_div, C.byref(h)), "create_random")
+        return cls(h)
+
+    @classmethod
+    def from_data(cls, data, ratios=(100, 50, 20), name="in-memory"):
+        data = np.ascontiguousarray(data, np.float64)
+        r = np.asarray(ratios, np.uint64)
+        h = vp()
+        check(lib().tblaeq_handler_create_from_data(_p(data, f64p), data.shape[0], data.shape[1], len(r) + 1,
+                                                    _p(r, u64p), name.encode(), C.byref(h)), "create_from_data")
+        return cls(h)
+
+    @classmethod
+    def load(cls, path):
+        h = vp()
+        check(lib().tblaeq_handler_load(path.encode(), C.byref(h)), "load")
+        return cls(h)
+
+    def save(self, path):
+        check(lib().tblaeq_handler_save(self.h, path.encode()), "save")
+
+    def data(self):
+        out = np.zeros((self.N, self.D))
+        check(lib().tblaeq_handler_data(self.h, _p(out, f64p), out.size), "data")
+        return out
+
+    def prepare(self, policy=None, free_mem=0):
+        if policy is None:
+            check(lib().tblaeq_handler_prepare(self.h, None, 0, free_mem), "prepare")
+        else:
+            if isinstance(policy, str):
+                policy = [policy] * self.intervals
+            lv = np.asarray([LEVEL_POLICIES[p] if isinstance(p, str) else p for p in policy], np.uint8)
+            check(lib().tblaeq_handler_prepare(self.h, _p(lv, u8p), lv.size, 0), "prepare")
+
+    def policy(self):
+        out = np.zeros(self.intervals, np.uint8)
+        check(lib().tblaeq_handler_policy(self.h, _p(out, u8p), out.size), "policy")
+        return ["L%d" % x for x in out]
+
+    def recommend(self, free_mem, count_work_buffers=True):
+        out = np.zeros(self.intervals, np.uint8)
+        check(lib().tblaeq_handler_recommend(self.h, free_mem, int(count_work_buffers), _p(out, u8p), out.size),
+              "recommend")
+        return ["L%d" % x for x in out]
+
+    def query(self, qtype, queries, K=0, refine=False, save_fine_mesh=False):
+        """qtype 'range' (rows lo|hi) or 'knn' (rows = points)."""
+        q = np.ascontiguousarray(queries, np.float64)
+        if q.ndim == 1:
+            q = q[None, :]
+        t = 0 if qtype == "range" else 1
+        r = vp()
+        check(lib().tblaeq_handler_query(self.h, t, _p(q, f64p), q.shape[0], K, int(refine), int(save_fine_mesh),
+                                         C.byref(r)), "query")
+        return Result(r, self.D, self.intervals)
+
+    def range(self, lo, hi, refine=True, save_fine_mesh=True):
+        return self.query("range", np.concatenate([np.asarray(lo, float), np.asarray(hi, float)]), 0, refine,
+                          save_fine_mesh)
+
+    def knn(self, q, K, refine=True, save_fine_mesh=True):
+        return self.query("knn", np.asarray(q, float), K, refine, save_fine_mesh)
+
+    def close(self):
+        if self.h:
+            lib().tblaeq_handler_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.tblaeq_handler_destroy(self.h)
+            self.h = None
+
+
+class Result:
+    def __init__(self, h, D, L):
+        self.h = h
+        self.D = D
+        self.L = L
+        nq = C.c_uint64(0)
+        check(lib().tblaeq_result_info(self.h, C.byref(nq), None, None), "result_info")
+        self.nq = nq.value
+
+    def info(self, i):
+        lv = np.zeros(self.L, np.uint64)
+        us = C.c_double(0)
+        fs = C.c_uint64(0)
+        rc = C.c_uint64(0)
+        check(lib().tblaeq_result_query(self.h, i, _p(lv, u64p), C.byref(us), C.byref(fs), C.byref(rc)), "result")
+        return {"levels": lv, "us": us.value, "fine": fs.value, "refined": rc.value}
+
+    def fine_mesh(self, i=0):
+        n = self.info(i)["fine"]
+        ids = np.zeros(n, np.uint64)
+        vals = np.zeros((n, self.D))
+        check(lib().tblaeq_result_fine_mesh(self.h, i, _p(ids, u64p), _p(vals, f64p)), "fine_mesh")
+        return ids, vals
+
+    def refined(self, i=0):
+        n = self.info(i)["refined"]
+        ids = np.zeros(n, np.uint64)
+        d2 = np.zeros(n)
+        check(lib().tblaeq_result_refined(self.h, i, _p(ids, u64p), _p(d2, f64p)), "refined")
+        return ids, d2
+
+    def save_csv(self, path):
+        check(lib().tblaeq_result_save_csv(self.h, path.encode()), "save_csv")
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.tblaeq_result_destroy(self.h)
+            self.h = None
