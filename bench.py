@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""Headline benchmark: range + KNN queries/s on the N=10M, D=8 hierarchy (MI355X).
+
+A step is one pass of the query path over one batch of synthetic queries
+(range boxes with selectivity log-uniform in [1e-5, 1e-2] and KNN points with
+K in {10, 100}), with the index and the queries resident in HBM, producing the
+exact answers (range: sorted id sets; KNN: top-K ids and squared distances).
+
+Reported:
+  value           queries/s of the fastest path measured (whole job, 1 GPU)
+  plain           the per-query GPU traversal (QueryHandler -> QueryEngine -> L0)
+  roofline        dominant kernel: algorithmic bytes per launch / average launch
+                  time, timed with HIP events on the library's stream
+  cpu_baseline    the OpenMP CPU traversal (the oracle) on the host cores
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "t-blaeq_amd"), os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "range+KNN queries/s at N=10M D=8, batched/learned vs plain; p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(D, levels_surv, levels_out, coarsest, hits):
+    """Algorithmic HBM bytes per kernel family for one query (FP64 values, 8-byte ids).
+
+    prune/compact at level l:  p_l (8D + 16) + 8 s_l
+    prolongation at level l:   s_l (8D + 16) + n_l (16D + 16)
+    box filter (range refine): p (8D + 8) + 8 hits
+    """
+    b = {"compact": 0.0, "prolong": 0.0, "filter": 0.0}
+    p = coarsest
+    for s, n in zip(levels_surv, levels_out):
+        b["compact"] += p * (8 * D + 16) + 8 * s
+        b["prolong"] += s * (8 * D + 16) + n * (16 * D + 16)
+        p = n
+    b["filter"] = p * (8 * D + 8) + 8 * hits
+    return b
+
+
+def run_plain(h, boxes, knn_q, Ks, steps, warmup, timing):
+    """Per-query GPU traversal + exact refine, one query at a time."""
+    import tblaeq  # noqa: F401
+
+    D = h.D
+    batches = [("range", boxes, 0)] + [("knn", knn_q[Ks == k], int(k)) for k in np.unique(Ks)]
+    for _ in range(warmup):
+        for t, q, k in batches:
+            h.query(t, q[: min(4, len(q))], K=k, refine=2)
+    timing.reset()
+    us = []
+    totals = {"compact": 0.0, "prolong": 0.0, "filter": 0.0}
+    t0 = time.perf_counter()
+    results = []
+    for _ in range(steps):
+        for t, q, k in batches:
+            results.append((t, h.query(t, q, K=k, refine=2)))
+    wall = time.perf_counter() - t0
+    nq = 0
+    for t, r in results:
+        for i in range(r.nq):
+            info = r.info(i)
+            us.append(info["us"])
+            s, o = r.levels(i)
+            b = kernel_bytes(D, s, o, h.mesh_sizes[-1], info["refined"] if t == "range" else 0)
+            for key in totals:
+                totals[key] += b[key]
+            nq += 1
+    return nq, wall, np.asarray(us), totals
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nq", type=int, default=4096, help="range queries and KNN queries per batch (each)")
+    ap.add_argument("--plain-sample", type=int, default=96, help="queries of each type for the plain path")
+    ap.add_argument("--cpu-range", type=int, default=48)
+    ap.add_argument("--cpu-knn", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--pair-cap", type=int, default=1 << 28)
+    ap.add_argument("--hit-cap", type=int, default=1 << 28)
+    ap.add_argument("--small", action="store_true", help="N=500k D=5 hierarchy (quick check)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (queries would shard across "
+              "replicated indexes; see the sharding notes). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import tblaeq
+    from tblaeq import workload as W
+
+    cfg = dict(W.C1 if args.small else W.C2)
+    log("generating index", cfg)
+    t0 = time.perf_counter()
+    h = tblaeq.Handler.random(cfg["N"], cfg["D"], cfg["vmin"], cfg["vmax"], cfg["is_int"], cfg["ratios"], cfg["seed"],
+                              cfg["sigma_div"])
+    h.prepare("L0")
+    log("index ready in %.1fs, mesh sizes %s" % (time.perf_counter() - t0, h.mesh_sizes))
+    data = h.data()
+    boxes, sel = W.range_boxes(data, args.nq, cfg["seed"] + 1, progress=log)
+    knn_q = W.knn_points(data, args.nq, cfg["seed"] + 2, h.mesh_sizes, cfg["vmin"], cfg["vmax"])
+    Ks = np.where(np.arange(args.nq) < args.nq // 2, 10, 100)
+    log("queries ready")
+
+    timing = tblaeq.Timing(h.ctx())
+    timing.enable(True)
+    P = min(args.plain_sample, args.nq)
+    half = P // 2
+    pk = np.concatenate([knn_q[:half], knn_q[args.nq // 2: args.nq // 2 + (P - half)]])
+    pK = np.concatenate([Ks[:half], Ks[args.nq // 2: args.nq // 2 + (P - half)]])
+    nq, wall, us, bytes_tot = run_plain(h, boxes[:P], pk, pK, args.steps, args.warmup, timing)
+    plain_qps = nq / wall
+    kstats = {}
+    for kind in ("compact_range", "compact_knn", "compact_box", "prolong", "tilesort", "merge", "step"):
+        n, ms = timing.get(kind)
+        kstats[kind] = (n, ms)
+    log("plain: %d queries in %.3fs -> %.1f q/s; kernels %s" % (nq, wall, plain_qps, kstats))
+
+    # roofline of the dominant HBM kernel family over the timed region
+    fam = {"prolong": ("prolong", bytes_tot["prolong"]),
+           "compact": ("compact_range", bytes_tot["compact"]),
+           "filter": ("compact_box", bytes_tot["filter"])}
+    best = max(fam, key=lambda f: kstats[fam[f][0]][1])
+    kname, kbytes = fam[best]
+    if best == "compact":  # range and KNN compaction share the byte model
+        n_l = kstats["compact_range"][0] + kstats["compact_knn"][0]
+        t_ms = kstats["compact_range"][1] + kstats["compact_knn"][1]
+    else:
+        n_l, t_ms = kstats[kname]
+    per_launch_bytes = kbytes / max(n_l, 1)
+    avg_s = t_ms / 1e3 / max(n_l, 1)
+    achieved = per_launch_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": {"prolong": "k_prolong", "compact": "k_compact", "filter": "k_compact<2>"}[best],
+                "launches": n_l, "avg_launch_us": round(avg_s * 1e6, 3), "bytes_per_launch": round(per_launch_bytes)}
+
+    # ---- batched path: the whole batch (nq range + nq KNN) per step
+    batched = None
+    if not args.no_batch:
+        h.batch_init(args.nq, args.pair_cap, args.hit_cap)
+        bK = Ks.astype(np.uint64)
+        t_r, t_k = [], []
+        tot_hits = tot_nn = 0
+        for it in range(args.warmup + args.steps):
+            h.batch_set("range", boxes)
+            timing.reset()
+            t0 = time.perf_counter()
+            tot_hits = h.batch_run()
+            t1 = time.perf_counter()
+            h.batch_set("knn", knn_q, bK)
+            t2 = time.perf_counter()
+            tot_nn = h.batch_run()
+            t3 = time.perf_counter()
+            if it >= args.warmup:
+                t_r.append(t1 - t0)
+                t_k.append(t3 - t2)
+            log("batched step %d: range %.2f ms (%d hits), knn %.2f ms (%d neighbours)"
+                % (it, (t1 - t0) * 1e3, tot_hits, (t3 - t2) * 1e3, tot_nn))
+        bk = {}
+        for name, kind in (("range_prune", 8), ("range_final", 9), ("group_scan", 10), ("group_scatter", 11),
+                           ("radix_sort", 12), ("knn_batch", 13)):
+            bk[name] = timing.get(kind)
+        step_s = float(np.mean(t_r)) + float(np.mean(t_k))
+        batched = {"qps": round(2 * args.nq / step_s, 3), "range_ms": round(float(np.mean(t_r)) * 1e3, 3),
+                   "knn_ms": round(float(np.mean(t_k)) * 1e3, 3), "hits": int(tot_hits), "neighbours": int(tot_nn),
+                   "kernels_ms_last_step": {k: round(v[1], 3) for k, v in bk.items()},
+                   "speedup_vs_plain": round(2 * args.nq / step_s / plain_qps, 2)}
+        log("batched", batched)
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle_py as O
+
+        log("CPU baseline: generating oracle index")
+        o = O.Index.generate(cfg["N"], cfg["D"], cfg["vmin"], cfg["vmax"], cfg["is_int"], cfg["ratios"], cfg["seed"],
+                             cfg["sigma_div"])
+        nr, nk = args.cpu_range, args.cpu_knn
+        w1, us1, c1, th = o.cpu_batch(0, boxes[:nr])
+        w2, us2, c2, _ = o.cpu_batch(1, knn_q[:nk], K=10)
+        w3, us3, c3, _ = o.cpu_batch(1, knn_q[args.nq // 2: args.nq // 2 + nk], K=100)
+        tot_q = nr + 2 * nk
+        cpu = {"value": round(tot_q / (w1 + w2 + w3), 3), "unit": "queries/s", "cores": th, "kind": "port",
+               "sample": "%d range + %d KNN(K=10) + %d KNN(K=100) queries of the same batch, OpenMP over queries, "
+                         "traversal + exact refine" % (nr, nk, nk),
+               "p50_us": round(float(np.percentile(np.concatenate([us1, us2, us3]), 50)), 1),
+               "p99_us": round(float(np.percentile(np.concatenate([us1, us2, us3]), 99)), 1)}
+        log("cpu baseline", cpu)
+
+    value = batched["qps"] if batched else plain_qps
+    ms_per_step = (batched["range_ms"] + batched["knn_ms"]) if batched else wall / args.steps * 1e3
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "hierarchy N=%d D=%d ratios %s sigma/%.1f seed %d; %d range boxes (selectivity "
+                               "log-uniform 1e-5..1e-2) + %d KNN points (K 10|100); exact answers"
+                               % (cfg["N"], cfg["D"], ",".join(map(str, cfg["ratios"])), cfg["sigma_div"],
+                                  cfg["seed"], args.nq, args.nq),
+                   "path": ("batched (all %d queries per step)" % (2 * args.nq)) if batched else
+                   ("plain per-query (sample of %d queries per step)" % nq), "N": cfg["N"], "D": cfg["D"]},
+        "plain": {"qps": round(plain_qps, 3), "p50_us": round(float(np.percentile(us, 50)), 1),
+                  "p99_us": round(float(np.percentile(us, 99)), 1), "queries": nq},
+        "batched": batched,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -63,6 +63,15 @@ int tblaeq_ctx_destroy(tblaeq_ctx* ctx);
 int tblaeq_ctx_sync(tblaeq_ctx* ctx);
 void* tblaeq_ctx_stream(tblaeq_ctx* ctx);
 const char* tblaeq_status_string(int status);
+/* Kernel timing with HIP events recorded on the context stream (no reference
+ * counterpart; the reference times whole queries with steady_clock).  Kinds:
+ * 0 range compaction, 1 KNN compaction, 2 box filter, 3 prolongation, 4 tile
+ * sort, 5 merge pass, 6 STEP, 7 KNN bounds, 8.. batched-path kernels.  `tag`
+ * labels subsequent launches (e.g. the level); tag -1 in a query sums all. */
+int tblaeq_ctx_set_timing(tblaeq_ctx* ctx, int enable);
+int tblaeq_ctx_set_tag(tblaeq_ctx* ctx, int tag);
+int tblaeq_ctx_timing(tblaeq_ctx* ctx, int kind, int tag, uint64_t* launches, double* ms);
+int tblaeq_ctx_timing_reset(tblaeq_ctx* ctx);
 const char* tblaeq_last_error(void);
 
 /* --- plain per-query level kernels ---------------------------------------- */
@@ -149,6 +158,42 @@ int tblaeq_refine_knn_topk(tblaeq_ctx* ctx, uint64_t D, const uint64_t* ids, con
  * blockMergeSortKernel + hostSerialMergeSort (MergeSort.cuh:240-444,628-696);
  * also used to build CSC P-tensors from (cluster, point) keys. */
 int tblaeq_sort_pairs(tblaeq_ctx* ctx, uint64_t* keys, uint64_t* vals, uint64_t n);
+
+/* --- batched engine ----------------------------------------------------------
+ * No reference counterpart (the reference runs one query at a time through
+ * QueryEngine::run, QueryEngine.cu:151-214).  Thousands of queries share one
+ * launch per level; results equal the per-query path's exact refine. */
+typedef struct tblaeq_batch tblaeq_batch;
+
+/* Builds the engine over resident levels (P values, radii, column pointers on
+ * the device): reconstructs every level's coordinates once with the
+ * prolongation kernel.  host_max_radius[l] = largest radius of level l (or
+ * NULL).  D must be <= 32. */
+int tblaeq_batch_create(tblaeq_ctx* ctx, const tblaeq_level* levels, uint64_t nlevels, const uint64_t* coarsest_ids,
+                        const double* coarsest_vals, uint64_t ncoarsest, const double* host_max_radius,
+                        tblaeq_batch** out);
+int tblaeq_batch_destroy(tblaeq_batch* b);
+/* Scratch for up to max_queries per batch, pair_cap (cluster, query) survivor
+ * pairs per level and hit_cap range hits per batch.  Allocates; call before
+ * timing or graph capture. */
+int tblaeq_batch_reserve(tblaeq_batch* b, uint64_t max_queries, uint64_t pair_cap, uint64_t hit_cap);
+/* nq range boxes (device, rows lo|hi).  Per-query hit ids, ascending, in
+ * d_out_ids[d_out_off[q] .. d_out_off[q+1]).  d_status[0] = total hits,
+ * d_status[1] = overflow flags (non-zero: capacity exceeded, results invalid;
+ * split the batch).  Optional d_level_surv[q*L + l] (survivors per level) and
+ * d_fine_count[q] (candidate superset size) must be zeroed by the caller.
+ * Asynchronous on the context stream; no host synchronisation inside. */
+int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64_t nq, uint64_t* d_out_off, uint64_t* d_out_ids,
+                       uint64_t out_cap, uint64_t* d_status, uint64_t* d_level_surv, uint64_t* d_fine_count);
+/* nq KNN points (device, rows of D), K per query in d_K (each <= Kmax).
+ * Top-K ids and squared distances by (distance, id) in rows of Kmax;
+ * d_out_n[q] = number written (min(K, candidates)).  Same traversal (STEP per
+ * level) as the per-query path.  Requires Kmax + empty clusters + 1 <= 256 and
+ * at most 8 levels (else TBLAEQ_ERR_UNSUPPORTED).  d_level_surv / d_fine_count
+ * as for tblaeq_batch_range (written, not accumulated). */
+int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const uint64_t* d_K, uint64_t nq, uint64_t Kmax,
+                     uint64_t* d_out_ids, double* d_out_d2, uint64_t* d_out_n, uint64_t* d_level_surv,
+                     uint64_t* d_fine_count);
 
 #ifdef __cplusplus
 }

@@ -44,11 +44,14 @@ int tblaeq_handler_prepare(tblaeq_handler* h, const uint8_t* levels, uint64_t n,
 int tblaeq_handler_policy(tblaeq_handler* h, uint8_t* out, uint64_t cap);
 /* PolicyScheduler::recommend with an injected free-memory value (MemoryPolicy.cu:100-177). */
 int tblaeq_handler_recommend(tblaeq_handler* h, uint64_t freeMem, int countWorkBuffers, uint8_t* out, uint64_t cap);
+/* The handler's execution context (stream, timing) for the tblaeq.h calls. */
+int tblaeq_handler_ctx(tblaeq_handler* h, tblaeq_ctx** out);
 /* Testing hook: fixed device byte budget for L3 tiles (0 = from free memory). */
 int tblaeq_handler_set_l3_tile_bytes(uint64_t bytes);
 /* performQuery over an in-memory batch (QueryHandler.cu:198-248): type 0 =
  * range (rows lo|hi, 2*D values), 1 = KNN (rows of D values, K neighbours).
- * refine adds the exact answer; saveFineMesh keeps host copies of the final
+ * refine = 1 adds the exact answer (copied to the host), 2 computes it on the
+ * device and reports only its size; saveFineMesh keeps host copies of the final
  * candidate grids. */
 int tblaeq_handler_query(tblaeq_handler* h, int type, const double* queries, uint64_t nq, uint64_t K, int refine,
                          int saveFineMesh, tblaeq_result** out);
@@ -56,10 +59,30 @@ int tblaeq_result_destroy(tblaeq_result* r);
 int tblaeq_result_info(tblaeq_result* r, uint64_t* nq, uint64_t* levels, double* totalUs);
 int tblaeq_result_query(tblaeq_result* r, uint64_t i, uint64_t* levelSizes, double* us, uint64_t* fineSize,
                         uint64_t* refinedCount);
+/* Per level of query i: clusters kept by the prune and points produced by the prolongation. */
+int tblaeq_result_levels(tblaeq_result* r, uint64_t i, uint64_t* survivors, uint64_t* outputs);
 int tblaeq_result_fine_mesh(tblaeq_result* r, uint64_t i, uint64_t* ids, double* vals);
 int tblaeq_result_refined(tblaeq_result* r, uint64_t i, uint64_t* ids, double* d2);
 /* saveQueryResult (QueryEngine.cu:254-305), same CSV columns. */
 int tblaeq_result_save_csv(tblaeq_result* r, const char* path);
+
+/* --- batched engine (tblaeq_batch_* in tblaeq.h) over the handler's index ---
+ * batch_init prepares the index (L0 if not yet prepared), builds the engine and
+ * reserves scratch for maxQueries queries, pairCap survivor pairs per level and
+ * hitCap range hits.  set_queries uploads a batch (type 0 = range rows lo|hi,
+ * 1 = KNN rows with K per query).  batch_run executes it on the device (range
+ * batches that exceed a capacity are split and rerun); *total = number of hits
+ * or neighbours.  Results and per-query statistics stay on the device until
+ * batch_results / batch_stats copy them out. */
+int tblaeq_handler_batch_init(tblaeq_handler* h, uint64_t maxQueries, uint64_t pairCap, uint64_t hitCap);
+int tblaeq_handler_batch_set_queries(tblaeq_handler* h, int type, const double* queries, const uint64_t* Ks,
+                                     uint64_t nq);
+int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total);
+/* range: off[nq+1] and ids[off[nq]]; KNN: n[nq], ids and d2 in rows of Kmax. */
+int tblaeq_handler_batch_results(tblaeq_handler* h, uint64_t* off, uint64_t* ids, double* d2, uint64_t* n);
+/* levelSurv[q*L + l] survivors per level, fine[q] candidate superset size (when run with stats). */
+int tblaeq_handler_batch_stats(tblaeq_handler* h, uint64_t* levelSurv, uint64_t* fine);
+int tblaeq_handler_batch_free(tblaeq_handler* h);
 
 #ifdef __cplusplus
 }

@@ -38,6 +38,8 @@ struct OIndex {
     std::vector<uint64_t> cIds;
     std::vector<double> cVals;
     std::vector<double> finestData;                // generator output (finest layer), if generated
+    std::vector<std::vector<double>> genLayers;    // generator layers, finest first (if generated)
+    std::vector<std::vector<uint64_t>> genLabels;
 };
 
 namespace {
@@ -433,7 +435,9 @@ OIndex* oracle_gen_index(uint64_t N, uint64_t D, double vmin, double vmax, int i
         }
     }
     OIndex* ix = assemble(N, D, ratios, layers, labels);
-    ix->finestData = std::move(layers[0]);
+    ix->finestData = layers[0];
+    ix->genLayers = std::move(layers);
+    ix->genLabels = std::move(labels);
     return ix;
 }
 
@@ -609,6 +613,37 @@ uint64_t oracle_index_data(const OIndex* ix, double* out, uint64_t cap) {
     if (ix->finestData.empty() || cap < ix->finestData.size()) return 0;
     std::memcpy(out, ix->finestData.data(), ix->finestData.size() * sizeof(double));
     return ix->finestData.size();
+}
+
+uint64_t oracle_index_layer(const OIndex* ix, uint64_t i, double* out) {
+    if (i >= ix->genLayers.size()) return 0;
+    if (out) std::memcpy(out, ix->genLayers[i].data(), ix->genLayers[i].size() * sizeof(double));
+    return ix->genLayers[i].size();
+}
+
+uint64_t oracle_index_labels(const OIndex* ix, uint64_t i, uint64_t* out) {
+    if (i >= ix->genLabels.size()) return 0;
+    if (out) std::memcpy(out, ix->genLabels[i].data(), ix->genLabels[i].size() * sizeof(uint64_t));
+    return ix->genLabels[i].size();
+}
+
+// Index arrays of interval l: which = 0 colRes, 1 nnzPerCol, 2 rowIds, 3 map[l+1] (u64);
+// 4 P values, 5 radius (f64).  Returns the element count, copies when out != NULL.
+uint64_t oracle_index_array(const OIndex* ix, uint64_t l, int which, void* out) {
+    const auto& P = ix->P[l];
+    const void* src = nullptr;
+    uint64_t n = 0;
+    switch (which) {
+        case 0: src = P.colRes.data(); n = P.colRes.size(); break;
+        case 1: src = P.nnzPerCol.data(); n = P.nnzPerCol.size(); break;
+        case 2: src = P.rowIds.data(); n = P.rowIds.size(); break;
+        case 3: src = ix->maps[l + 1].data(); n = ix->maps[l + 1].size(); break;
+        case 4: src = P.vals.data(); n = P.vals.size(); break;
+        case 5: src = ix->radius[l].data(); n = ix->radius[l].size(); break;
+        default: return 0;
+    }
+    if (out) std::memcpy(out, src, n * 8);
+    return n;
 }
 
 void oracle_index_scale_radius(OIndex* ix, double factor) {

@@ -69,6 +69,11 @@ void oracle_free_index(OIndex* idx);
 uint64_t oracle_index_meta(const OIndex* idx, uint64_t* meta, uint64_t* ms, uint64_t cap);
 /* Copy the finest generated data layer (only for generator-built indexes). Returns N*D copied or 0. */
 uint64_t oracle_index_data(const OIndex* idx, double* out, uint64_t cap);
+/* Generator layers (finest first) and labels (layer i -> i+1), generator-built indexes only. */
+uint64_t oracle_index_layer(const OIndex* idx, uint64_t i, double* out);
+uint64_t oracle_index_labels(const OIndex* idx, uint64_t i, uint64_t* out);
+/* Arrays of interval l: which = 0 colRes, 1 nnzPerCol, 2 rowIds, 3 map[l+1], 4 P values, 5 radius. */
+uint64_t oracle_index_array(const OIndex* idx, uint64_t l, int which, void* out);
 /* Radius of level l with a scale factor (used by tests to disable pruning with +inf). */
 void oracle_index_scale_radius(OIndex* idx, double factor);
 

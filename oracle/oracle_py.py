@@ -39,6 +39,9 @@ def lib():
             "oracle_index_meta": (C.c_uint64, [vp, u64p, u64p, C.c_uint64]),
             "oracle_index_data": (C.c_uint64, [vp, f64p, C.c_uint64]),
             "oracle_index_scale_radius": (None, [vp, C.c_double]),
+            "oracle_index_layer": (C.c_uint64, [vp, C.c_uint64, f64p]),
+            "oracle_index_labels": (C.c_uint64, [vp, C.c_uint64, u64p]),
+            "oracle_index_array": (C.c_uint64, [vp, C.c_uint64, C.c_int, vp]),
             "oracle_range": (vp, [vp, f64p, f64p]),
             "oracle_knn": (vp, [vp, f64p, C.c_uint64]),
             "oracle_free_result": (None, [vp]),
@@ -116,6 +119,24 @@ class Index:
         out = np.zeros((self.N, self.D))
         if lib().oracle_index_data(self.h, _p(out, f64p), out.size) == 0:
             return None
+        return out
+
+    def layer(self, i):
+        n = lib().oracle_index_layer(self.h, i, None)
+        out = np.zeros(n)
+        lib().oracle_index_layer(self.h, i, _p(out, f64p))
+        return out
+
+    def labels(self, i):
+        n = lib().oracle_index_labels(self.h, i, None)
+        out = np.zeros(n, np.uint64)
+        lib().oracle_index_labels(self.h, i, _p(out, u64p))
+        return out
+
+    def array(self, l, which):
+        n = lib().oracle_index_array(self.h, l, which, None)
+        out = np.zeros(n, np.float64 if which >= 4 else np.uint64)
+        lib().oracle_index_array(self.h, l, which, out.ctypes.data_as(C.c_void_p))
         return out
 
     def scale_radius(self, f):

@@ -1,0 +1,52 @@
+// Device state of the batched query engine (shared by the range and KNN kernels).
+//
+// The engine keeps, for every level l = 0..L, the level's points in "position"
+// order: level 0 in coarsest-mesh order, level l >= 1 in the CSC row order of
+// P[l-1].  Their reconstructed coordinates are produced once, at creation, by
+// running the prolongation over the whole hierarchy (every cluster a survivor),
+// so they are bit-identical to what the per-query prolongation produces.  A
+// cluster c of level l owns the contiguous positions colRes_l[c]..colRes_l[c+1]
+// of level l+1.
+#pragma once
+#include <vector>
+
+#include "ctx.hpp"
+
+struct tblaeq_batch {
+    tblaeq_ctx* ctx = nullptr;
+    int D = 0;
+    int L = 0;                              // intervals
+    std::vector<uint64_t> count;            // [L+1] positions per level
+    std::vector<double*> coords;            // [L+1] count*D, reconstructed
+    std::vector<uint64_t*> ids;             // [L+1] level id of each position
+    std::vector<double*> radiusPos;         // [L]   radius of each position's point
+    std::vector<uint64_t*> parentPos;       // [L]   (l>=1) level-(l-1) id of the parent
+    std::vector<uint64_t*> weightPos;       // [L]   children count of each position's point
+    std::vector<const uint64_t*> colRes;    // [L]   P[l] column pointers (by level-l id)
+    std::vector<const uint64_t*> nnzPerCol; // [L]
+    std::vector<const double*> radiusById;  // [L]
+    std::vector<uint64_t> emptyCols;        // [L]
+    std::vector<double> maxRadius;          // [L]   host: largest radius of the level
+    std::vector<void*> owned;               // device allocations to free
+
+    // range scratch
+    uint64_t maxQ = 0, pairCap = 0, hitCap = 0, itemCap = 0, maxMesh = 0;
+    uint64_t* pairs = nullptr;   // (id << 24 | q)
+    uint32_t* qlist = nullptr;   // grouped query lists
+    uint64_t* cnt = nullptr;     // [maxMesh]  per-cluster pair counts (kept zeroed between steps)
+    uint64_t* off = nullptr;     // [maxMesh+1]
+    uint64_t* items = nullptr;   // (id << 20 | slice)
+    uint64_t* hits = nullptr;    // (q << idBits | id)
+    uint64_t* hitsTmp = nullptr;
+    uint64_t* hist = nullptr;    // radix histograms
+    uint64_t histCap = 0;
+    uint64_t* ctl = nullptr;     // [0] pairs [1] items [2] hits [3] overflow [4..] spare
+    // knn scratch
+    uint8_t* flags = nullptr;    // per query survivor flags of a middle level
+    uint64_t flagCap = 0;
+};
+
+namespace tblaeq {
+constexpr int kSliceBits = 20;
+constexpr int kQueryBits = 24;
+}  // namespace tblaeq

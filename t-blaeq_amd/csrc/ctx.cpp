@@ -53,9 +53,94 @@ int ws_reserve(tblaeq_ctx* ctx, size_t bytes) {
     return TBLAEQ_OK;
 }
 
+static hipEvent_t take_event(tblaeq_ctx* ctx) {
+    if (!ctx->eventPool.empty()) {
+        hipEvent_t e = ctx->eventPool.back();
+        ctx->eventPool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+TimedScope::TimedScope(tblaeq_ctx* c, int kind) : ctx(c) {
+    if (!ctx->timing) return;
+    TimedLaunch t{kind, ctx->tag, take_event(ctx), take_event(ctx)};
+    if (!t.a || !t.b) return;
+    (void)hipEventRecord(t.a, ctx->stream);
+    ctx->pending.push_back(t);
+    idx = static_cast<int>(ctx->pending.size()) - 1;
+}
+
+TimedScope::~TimedScope() {
+    if (idx >= 0) (void)hipEventRecord(ctx->pending[idx].b, ctx->stream);
+}
+
+static int collect_timing(tblaeq_ctx* ctx) {
+    if (ctx->pending.empty()) return TBLAEQ_OK;
+    TB_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto& t : ctx->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess && t.kind >= 0 && t.kind < kKindCount) {
+            const int tg = t.tag < 0 ? 0 : (t.tag >= kTagCount ? kTagCount - 1 : t.tag);
+            ctx->accMs[t.kind][tg] += ms;
+            ctx->accN[t.kind][tg] += 1;
+        }
+        ctx->eventPool.push_back(t.a);
+        ctx->eventPool.push_back(t.b);
+    }
+    ctx->pending.clear();
+    return TBLAEQ_OK;
+}
+
 }  // namespace tblaeq
 
 using namespace tblaeq;
+
+extern "C" int tblaeq_ctx_set_timing(tblaeq_ctx* ctx, int enable) {
+    if (!ctx) return TBLAEQ_ERR_INVALID;
+    if (!enable) {
+        int rc = collect_timing(ctx);
+        if (rc) return rc;
+    }
+    ctx->timing = enable != 0;
+    return TBLAEQ_OK;
+}
+
+extern "C" int tblaeq_ctx_set_tag(tblaeq_ctx* ctx, int tag) {
+    if (!ctx) return TBLAEQ_ERR_INVALID;
+    ctx->tag = tag;
+    return TBLAEQ_OK;
+}
+
+extern "C" int tblaeq_ctx_timing(tblaeq_ctx* ctx, int kind, int tag, uint64_t* launches, double* ms) {
+    if (!ctx || kind < 0 || kind >= kKindCount) return TBLAEQ_ERR_INVALID;
+    int rc = collect_timing(ctx);
+    if (rc) return rc;
+    uint64_t n = 0;
+    double t = 0;
+    for (int g = 0; g < kTagCount; ++g) {
+        if (tag >= 0 && g != tag) continue;
+        n += ctx->accN[kind][g];
+        t += ctx->accMs[kind][g];
+    }
+    if (launches) *launches = n;
+    if (ms) *ms = t;
+    return TBLAEQ_OK;
+}
+
+extern "C" int tblaeq_ctx_timing_reset(tblaeq_ctx* ctx) {
+    if (!ctx) return TBLAEQ_ERR_INVALID;
+    int rc = collect_timing(ctx);
+    if (rc) return rc;
+    for (int k = 0; k < kKindCount; ++k)
+        for (int g = 0; g < kTagCount; ++g) {
+            ctx->accMs[k][g] = 0;
+            ctx->accN[k][g] = 0;
+        }
+    return TBLAEQ_OK;
+}
 
 extern "C" int tblaeq_ctx_create(int device, void* stream, tblaeq_ctx** out) {
     if (!out) {
@@ -101,6 +186,11 @@ extern "C" int tblaeq_ctx_create(int device, void* stream, tblaeq_ctx** out) {
 extern "C" int tblaeq_ctx_destroy(tblaeq_ctx* ctx) {
     if (!ctx) return TBLAEQ_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& t : ctx->pending) {
+        ctx->eventPool.push_back(t.a);
+        ctx->eventPool.push_back(t.b);
+    }
+    for (auto e : ctx->eventPool) (void)hipEventDestroy(e);
     if (ctx->status) (void)hipFree(ctx->status);
     if (ctx->ctrl) (void)hipFree(ctx->ctrl);
     if (ctx->ws) (void)hipFree(ctx->ws);

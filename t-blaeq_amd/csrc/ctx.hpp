@@ -4,9 +4,31 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/tblaeq.h"
 #include "device_common.hpp"
+
+namespace tblaeq {
+// Kernel families timed by HIP events when timing is enabled on a context.
+enum KernelKind {
+    kKCompactRange = 0,
+    kKCompactKnn = 1,
+    kKCompactBox = 2,
+    kKProlong = 3,
+    kKTileSort = 4,
+    kKMerge = 5,
+    kKStep = 6,
+    kKKnnLb = 7,
+    kKBatch0 = 8,  // batched-path kernels use kKBatch0 + i
+    kKindCount = 32
+};
+constexpr int kTagCount = 16;
+struct TimedLaunch {
+    int kind, tag;
+    hipEvent_t a, b;
+};
+}  // namespace tblaeq
 
 struct tblaeq_ctx {
     int device = 0;
@@ -21,6 +43,13 @@ struct tblaeq_ctx {
     size_t wsBytes = 0;
     uint64_t* hostPinned = nullptr;  // small pinned staging for counts
     int cuCount = 0;
+    // HIP-event timing of kernel launches on `stream` (off unless enabled)
+    bool timing = false;
+    int tag = 0;
+    std::vector<tblaeq::TimedLaunch> pending;
+    std::vector<hipEvent_t> eventPool;
+    double accMs[tblaeq::kKindCount][tblaeq::kTagCount] = {};
+    uint64_t accN[tblaeq::kKindCount][tblaeq::kTagCount] = {};
 };
 
 namespace tblaeq {
@@ -38,6 +67,14 @@ int hip_fail(hipError_t e, const char* what);
 int scan_prepare(tblaeq_ctx* ctx, uint64_t tiles, uint32_t* epoch);
 // Grows the context scratch to at least `bytes`.
 int ws_reserve(tblaeq_ctx* ctx, size_t bytes);
+
+// Brackets one kernel launch with HIP events on the context stream when timing is on.
+struct TimedScope {
+    TimedScope(tblaeq_ctx* c, int kind);
+    ~TimedScope();
+    tblaeq_ctx* ctx;
+    int idx = -1;
+};
 
 }  // namespace tblaeq
 

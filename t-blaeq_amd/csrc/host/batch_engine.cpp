@@ -1,0 +1,208 @@
+// Handler-level driver of the batched engine: device-resident queries and
+// results, capacity-overflow recovery by splitting the batch.
+#include <algorithm>
+#include <cstring>
+
+#include "../../../include/tblaeq_handler.h"
+#include "../ctx.hpp"
+#include "handler_internal.hpp"
+
+using namespace tblaeq_host;
+
+namespace {
+
+hipStream_t streamOf(IndexData& idx) { return static_cast<hipStream_t>(tblaeq_ctx_stream(idx.ctx)); }
+
+template <class F>
+int guarded(const char* what, F&& f) {
+    try {
+        return f();
+    } catch (const std::exception& e) {
+        tblaeq::set_error(std::string(what) + ": " + e.what());
+        return TBLAEQ_ERR_HIP;
+    }
+}
+
+BatchState* stateOf(tblaeq_handler* h) { return h ? h->batch.get() : nullptr; }
+
+}  // namespace
+
+extern "C" {
+
+int tblaeq_handler_batch_init(tblaeq_handler* h, uint64_t maxQueries, uint64_t pairCap, uint64_t hitCap) {
+    if (!h || maxQueries == 0) return TBLAEQ_ERR_INVALID;
+    return guarded("tblaeq_handler_batch_init", [&]() -> int {
+        IndexData& idx = h->h->index();
+        if (!h->h->prepared()) h->h->prepareForQuery(LevelPolicy::L0);
+        for (size_t l = 0; l < idx.intervals; ++l) idx.uploadLevelValues(l);
+        h->batch.reset();
+        auto st = std::make_unique<BatchState>();
+        std::vector<tblaeq_level> lv(idx.intervals);
+        std::vector<double> maxR(idx.intervals, 0.0);
+        for (size_t l = 0; l < idx.intervals; ++l) {
+            lv[l] = idx.levelDesc(l);
+            for (double r : idx.meshMaxRadius[l]) maxR[l] = std::max(maxR[l], r);
+        }
+        int rc = tblaeq_batch_create(idx.ctx, lv.data(), lv.size(), idx.dCoarsestIds.data(), idx.dCoarsestVals.data(),
+                                     idx.coarsestIds.size(), maxR.data(), &st->b);
+        if (rc) return rc;
+        rc = tblaeq_batch_reserve(st->b, maxQueries, pairCap, hitCap);
+        if (rc) return rc;
+        st->maxQ = maxQueries;
+        st->pairCap = pairCap;
+        st->hitCap = hitCap;
+        st->dQ.resize(maxQueries * 2 * idx.D);
+        st->dK.resize(maxQueries);
+        st->dOff.resize(maxQueries + 1);
+        st->dOffTmp.resize(maxQueries + 1);
+        st->dStatus.resize(4);
+        st->dSurv.resize(maxQueries * idx.intervals);
+        st->dFine.resize(maxQueries);
+        st->dN.resize(maxQueries);
+        st->dIds.resize(std::max<uint64_t>(hitCap, 1));
+        h->batch = std::move(st);
+        return TBLAEQ_OK;
+    });
+}
+
+int tblaeq_handler_batch_set_queries(tblaeq_handler* h, int type, const double* queries, const uint64_t* Ks,
+                                     uint64_t nq) {
+    BatchState* st = stateOf(h);
+    if (!st || !queries || nq == 0 || nq > st->maxQ || (type != 0 && type != 1) || (type == 1 && !Ks))
+        return TBLAEQ_ERR_INVALID;
+    return guarded("tblaeq_handler_batch_set_queries", [&]() -> int {
+        IndexData& idx = h->h->index();
+        const size_t D = idx.D;
+        st->type = type;
+        st->nq = nq;
+        hipCheck(hipMemcpy(st->dQ.data(), queries, nq * (type == 0 ? 2 * D : D) * 8, hipMemcpyHostToDevice), "H2D");
+        if (type == 1) {
+            hipCheck(hipMemcpy(st->dK.data(), Ks, nq * 8, hipMemcpyHostToDevice), "H2D");
+            st->Kmax = *std::max_element(Ks, Ks + nq);
+            if (st->dIds.size() < nq * st->Kmax) st->dIds.resize(nq * st->Kmax);
+            if (st->dD2.size() < nq * st->Kmax) st->dD2.resize(std::max<uint64_t>(nq * st->Kmax, 1));
+        }
+        return TBLAEQ_OK;
+    });
+}
+
+// Runs the uploaded batch.  Range: splits the batch in halves while a capacity
+// is exceeded.  *total = hits (range) or results (KNN).
+int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total) {
+    BatchState* st = stateOf(h);
+    if (!st || st->nq == 0) return TBLAEQ_ERR_INVALID;
+    return guarded("tblaeq_handler_batch_run", [&]() -> int {
+        IndexData& idx = h->h->index();
+        const size_t D = idx.D, L = idx.intervals;
+        hipStream_t s = streamOf(idx);
+        if (stats) {
+            hipCheck(hipMemsetAsync(st->dSurv.data(), 0, st->nq * L * 8, s), "memset");
+            hipCheck(hipMemsetAsync(st->dFine.data(), 0, st->nq * 8, s), "memset");
+        }
+        if (st->type == 1) {
+            int rc = tblaeq_batch_knn(st->b, st->dQ.data(), st->dK.data(), st->nq, st->Kmax, st->dIds.data(),
+                                      st->dD2.data(), st->dN.data(), stats ? st->dSurv.data() : nullptr,
+                                      stats ? st->dFine.data() : nullptr);
+            if (rc) return rc;
+            hipCheck(hipStreamSynchronize(s), "sync");
+            if (total) {
+                std::vector<uint64_t> n(st->nq);
+                hipCheck(hipMemcpy(n.data(), st->dN.data(), st->nq * 8, hipMemcpyDeviceToHost), "D2H");
+                uint64_t t = 0;
+                for (auto v : n) t += v;
+                *total = t;
+            }
+            return TBLAEQ_OK;
+        }
+        // range: [a, a+n) sub-batches
+        std::vector<std::pair<uint64_t, uint64_t>> todo{{0, st->nq}};
+        uint64_t base = 0;
+        std::vector<uint64_t> hostOff(st->nq + 1, 0);
+        bool split = false;
+        while (!todo.empty()) {
+            auto [a, n] = todo.front();
+            todo.erase(todo.begin());
+            const uint64_t cap = st->dIds.size() - base;
+            int rc = tblaeq_batch_range(st->b, st->dQ.data() + a * 2 * D, n, split ? st->dOffTmp.data() : st->dOff.data(),
+                                        st->dIds.data() + base, cap, st->dStatus.data(),
+                                        stats ? st->dSurv.data() + a * L : nullptr,
+                                        stats ? st->dFine.data() + a : nullptr);
+            if (rc) return rc;
+            uint64_t status[2];
+            hipCheck(hipMemcpyAsync(status, st->dStatus.data(), 16, hipMemcpyDeviceToHost, s), "D2H");
+            hipCheck(hipStreamSynchronize(s), "sync");
+            if (status[1] != 0) {
+                if (n == 1) {
+                    tblaeq::set_error("batched range: one query exceeds the engine capacities");
+                    return TBLAEQ_ERR_CAPACITY;
+                }
+                if (stats) {
+                    hipCheck(hipMemsetAsync(st->dSurv.data() + a * L, 0, n * L * 8, s), "memset");
+                    hipCheck(hipMemsetAsync(st->dFine.data() + a, 0, n * 8, s), "memset");
+                }
+                if (!split) {
+                    // restart from scratch in split mode
+                    split = true;
+                    todo.clear();
+                    base = 0;
+                    todo.push_back({0, n / 2});
+                    todo.push_back({n / 2, n - n / 2});
+                    if (stats) {
+                        hipCheck(hipMemsetAsync(st->dSurv.data(), 0, st->nq * L * 8, s), "memset");
+                        hipCheck(hipMemsetAsync(st->dFine.data(), 0, st->nq * 8, s), "memset");
+                    }
+                    continue;
+                }
+                todo.insert(todo.begin(), {a + n / 2, n - n / 2});
+                todo.insert(todo.begin(), {a, n / 2});
+                continue;
+            }
+            if (split) {
+                std::vector<uint64_t> off(n + 1);
+                hipCheck(hipMemcpy(off.data(), st->dOffTmp.data(), (n + 1) * 8, hipMemcpyDeviceToHost), "D2H");
+                for (uint64_t i = 0; i <= n; ++i) hostOff[a + i] = base + off[i];
+            }
+            base += status[0];
+        }
+        if (split) hipCheck(hipMemcpy(st->dOff.data(), hostOff.data(), (st->nq + 1) * 8, hipMemcpyHostToDevice), "H2D");
+        st->lastTotal = base;
+        if (total) *total = base;
+        return TBLAEQ_OK;
+    });
+}
+
+int tblaeq_handler_batch_results(tblaeq_handler* h, uint64_t* off, uint64_t* ids, double* d2, uint64_t* n) {
+    BatchState* st = stateOf(h);
+    if (!st || st->nq == 0) return TBLAEQ_ERR_INVALID;
+    return guarded("tblaeq_handler_batch_results", [&]() -> int {
+        if (st->type == 0) {
+            std::vector<uint64_t> o(st->nq + 1);
+            hipCheck(hipMemcpy(o.data(), st->dOff.data(), (st->nq + 1) * 8, hipMemcpyDeviceToHost), "D2H");
+            if (off) std::memcpy(off, o.data(), (st->nq + 1) * 8);
+            if (ids) hipCheck(hipMemcpy(ids, st->dIds.data(), o[st->nq] * 8, hipMemcpyDeviceToHost), "D2H");
+        } else {
+            if (n) hipCheck(hipMemcpy(n, st->dN.data(), st->nq * 8, hipMemcpyDeviceToHost), "D2H");
+            if (ids) hipCheck(hipMemcpy(ids, st->dIds.data(), st->nq * st->Kmax * 8, hipMemcpyDeviceToHost), "D2H");
+            if (d2) hipCheck(hipMemcpy(d2, st->dD2.data(), st->nq * st->Kmax * 8, hipMemcpyDeviceToHost), "D2H");
+        }
+        return TBLAEQ_OK;
+    });
+}
+
+int tblaeq_handler_batch_stats(tblaeq_handler* h, uint64_t* levelSurv, uint64_t* fine) {
+    BatchState* st = stateOf(h);
+    if (!st || st->nq == 0) return TBLAEQ_ERR_INVALID;
+    return guarded("tblaeq_handler_batch_stats", [&]() -> int {
+        const size_t L = h->h->index().intervals;
+        if (levelSurv) hipCheck(hipMemcpy(levelSurv, st->dSurv.data(), st->nq * L * 8, hipMemcpyDeviceToHost), "D2H");
+        if (fine) hipCheck(hipMemcpy(fine, st->dFine.data(), st->nq * 8, hipMemcpyDeviceToHost), "D2H");
+        return TBLAEQ_OK;
+    });
+}
+
+int tblaeq_handler_batch_free(tblaeq_handler* h) {
+    if (h) h->batch.reset();
+    return TBLAEQ_OK;
+}
+
+}  // extern "C"

@@ -4,13 +4,9 @@
 
 #include "../../../include/tblaeq_handler.h"
 #include "../ctx.hpp"
-#include "tblaeq_host.hpp"
+#include "handler_internal.hpp"
 
 using namespace tblaeq_host;
-
-struct tblaeq_handler {
-    std::unique_ptr<QueryHandler> h;
-};
 
 struct tblaeq_result {
     QueryResult r;
@@ -138,6 +134,14 @@ int tblaeq_handler_recommend(tblaeq_handler* h, uint64_t freeMem, int countWorkB
     });
 }
 
+int tblaeq_handler_ctx(tblaeq_handler* h, tblaeq_ctx** out) {
+    if (!h || !out) return TBLAEQ_ERR_INVALID;
+    return guarded("tblaeq_handler_ctx", [&] {
+        h->h->index().ensureCtx();
+        *out = h->h->index().ctx;
+    });
+}
+
 int tblaeq_handler_set_l3_tile_bytes(uint64_t bytes) {
     L3Strategy::forcedTileBytes = bytes;
     return TBLAEQ_OK;
@@ -153,6 +157,7 @@ int tblaeq_handler_query(tblaeq_handler* h, int type, const double* queries, uin
         QueryEngine::RunConfig cfg;
         cfg.K = K;
         cfg.refine = refine != 0;
+        cfg.copyRefined = refine != 2;
         cfg.saveFineMesh = saveFineMesh != 0;
         auto* r = new tblaeq_result{h->h->performQuery(q, cfg), {}};
         if (r->r.errorCode) {
@@ -183,7 +188,16 @@ int tblaeq_result_query(tblaeq_result* r, uint64_t i, uint64_t* levelSizes, doub
         for (size_t l = 0; l < r->r.levelFineMeshSize[i].size(); ++l) levelSizes[l] = r->r.levelFineMeshSize[i][l];
     if (us) *us = r->r.queryTimeUsExact[i];
     if (fineSize) *fineSize = r->r.fineMeshSize[i];
-    if (refinedCount) *refinedCount = i < r->r.refinedIds.size() ? r->r.refinedIds[i].size() : 0;
+    if (refinedCount) *refinedCount = i < r->r.refinedCount.size() ? r->r.refinedCount[i] : 0;
+    return TBLAEQ_OK;
+}
+
+int tblaeq_result_levels(tblaeq_result* r, uint64_t i, uint64_t* survivors, uint64_t* outputs) {
+    if (!r || i >= r->r.fineMeshSize.size()) return TBLAEQ_ERR_INVALID;
+    for (size_t l = 0; l < r->r.levelFineMeshSize[i].size(); ++l) {
+        if (survivors) survivors[l] = r->r.levelSurvivors[i][l];
+        if (outputs) outputs[l] = r->r.levelFineMeshSize[i][l];
+    }
     return TBLAEQ_OK;
 }
 

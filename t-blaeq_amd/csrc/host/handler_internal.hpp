@@ -1,0 +1,29 @@
+// Internal definition of the C handle behind tblaeq_handler.h.
+#pragma once
+#include <memory>
+
+#include "tblaeq_host.hpp"
+
+namespace tblaeq_host {
+
+// Device-resident batched engine state of a handler.
+struct BatchState {
+    tblaeq_batch* b = nullptr;
+    uint64_t maxQ = 0, pairCap = 0, hitCap = 0;
+    int type = 0;
+    uint64_t nq = 0, Kmax = 0;
+    DeviceBuffer<double> dQ;
+    DeviceBuffer<uint64_t> dK, dOff, dOffTmp, dIds, dStatus, dSurv, dFine, dN;
+    DeviceBuffer<double> dD2;
+    uint64_t lastTotal = 0;
+    ~BatchState() {
+        if (b) tblaeq_batch_destroy(b);
+    }
+};
+
+}  // namespace tblaeq_host
+
+struct tblaeq_handler {
+    std::unique_ptr<tblaeq_host::QueryHandler> h;
+    std::unique_ptr<tblaeq_host::BatchState> batch;  // destroyed before the index it uses
+};

@@ -329,6 +329,7 @@ LevelResult L1Strategy::runLevel(size_t l, SparseGrid* cur, QueryType t, const d
     DeviceBuffer<double> sVals;
     size_t ns = 0, nout = 0;
     pruneDevice(idx, l, lv, g, t, lo, hi, q, K, sIds, sVals, sOff, ns, nout);
+    if (idx.hostSurv.size() > l) idx.hostSurv[l] = ns;
     uint64_t* oIds = nullptr;
     double* oVals = nullptr;
     hipCheck(hipMalloc(&oIds, std::max<size_t>(nout, 1) * 8), "hipMalloc");
@@ -361,6 +362,7 @@ LevelResult L2Strategy::runLevel(size_t l, SparseGrid* cur, QueryType t, const d
     DeviceBuffer<double> sVals;
     size_t ns = 0, nout = 0;
     pruneDevice(idx, l, lv, g, t, lo, hi, q, K, sIds, sVals, sOff, ns, nout);
+    if (idx.hostSurv.size() > l) idx.hostSurv[l] = ns;
     radius.free();
     // Pack the survivors' P columns in output order (survivor order, then column order).
     std::vector<uint64_t> hIds(ns);
@@ -500,6 +502,7 @@ LevelResult L3Strategy::runLevel(size_t l, SparseGrid* cur, QueryType t, const d
         nout += P.colRes[hIds[i] + 1] - P.colRes[hIds[i]];
     }
     const size_t ns = sId.size();
+    if (idx.hostSurv.size() > l) idx.hostSurv[l] = ns;
     auto* oIds = new uint64_t[std::max<size_t>(nout, 1)];
     auto* oVals = new double[std::max<size_t>(nout * D, 1)];
     size_t s0 = 0;

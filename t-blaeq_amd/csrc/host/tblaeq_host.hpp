@@ -240,6 +240,7 @@ struct IndexData {
     IndexPolicy activePolicy;
     bool permanentDataOnDevice = false;
     bool workReady = false;
+    std::vector<size_t> hostSurv;  // survivors of the last query per level (host-synchronising strategies)
     MemoryStats stats;
 
     IndexData() = default;
@@ -364,10 +365,12 @@ struct QueryResult {
     std::vector<size_t> fineMeshSize;
     std::vector<size_t> levelOriginalSize;
     std::vector<std::vector<size_t>> levelFineMeshSize;
+    std::vector<std::vector<size_t>> levelSurvivors;  // clusters kept by each level's prune
     std::vector<long> queryTimeUs;
     std::vector<double> queryTimeUsExact;  // sub-microsecond timings (same clock)
     std::vector<SparseGrid*> fineMesh;     // host copies when saveFineMesh
     // exact refine (new): per query sorted hit ids (range) or top-K ids + squared distances (KNN)
+    std::vector<size_t> refinedCount;
     std::vector<std::vector<uint64_t>> refinedIds;
     std::vector<std::vector<double>> refinedDist2;
 };
@@ -379,12 +382,14 @@ public:
         int maxQueryCount = std::numeric_limits<int>::max();
         size_t K = 0;
         bool refine = false;   // run the exact refine after the traversal
+        bool copyRefined = true;  // copy refined ids/distances to the host (else counts only)
         bool quiet = true;     // no console output inside the query loop
     };
     static QueryResult run(IndexData& idx, IQueryStrategy& strategy, const Query& q, const RunConfig& cfg);
     // One query through the level loop; fills levelSizes and returns the final grid.
     static LevelResult runSingleQuery(IndexData& idx, QueryType t, const double* lo, const double* hi,
-                                      const double* qp, size_t K, size_t* levelFineMeshSize);
+                                      const double* qp, size_t K, size_t* levelFineMeshSize,
+                                      size_t* levelSurvivors = nullptr);
 };
 
 void saveQueryResult(const QueryResult& r, const std::string& path);  // QueryEngine.cu:254-305

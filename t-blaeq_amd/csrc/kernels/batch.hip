@@ -1,0 +1,755 @@
+// Batched range queries: thousands of boxes per launch, cluster-major.
+//
+// Per level, surviving (cluster, query) pairs are grouped by cluster with a
+// counting sort, so that one 64-wide wave processes a slice of 64 queries that
+// all selected the same cluster: each lane holds one query box in registers and
+// the wave walks the cluster's children, whose reconstructed coordinates,
+// radius and id are wave-uniform loads.  A child is read once per slice instead
+// of once per query.  Every work item is a (cluster, slice) pair taken from a
+// device-resident list; counts stay on the device, so a batch runs without host
+// synchronisation until its hits are sorted into per-query id sets.
+//
+// Pruning at level l uses the same test and operation order as the per-query
+// path (RangePruning.cu:5-25: fma chain of the clamped differences against
+// r*r); the last step is the exact point-in-box filter of the candidates.
+#include <algorithm>
+
+#include "../batch_internal.hpp"
+#include "../scan_common.hpp"
+
+namespace tblaeq {
+
+// ------------------------------------------------------------------ creation
+
+__global__ void k_gather_off(const uint64_t* __restrict__ ids, uint64_t n, const uint64_t* __restrict__ colRes,
+                             uint64_t* __restrict__ off) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i < n) off[i] = colRes[ids[i]];
+}
+
+__global__ void k_gather_f64(const uint64_t* __restrict__ ids, uint64_t n, const double* __restrict__ src,
+                             double* __restrict__ dst) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i < n) dst[i] = src[ids[i]];
+}
+
+__global__ void k_gather_u64(const uint64_t* __restrict__ ids, uint64_t n, const uint64_t* __restrict__ src,
+                             uint64_t* __restrict__ dst) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i < n) dst[i] = src[ids[i]];
+}
+
+// parent[j] = c for every CSC row j of column c.
+__global__ void k_fill_parent(const uint64_t* __restrict__ colRes, uint64_t cols, uint64_t* __restrict__ parent) {
+    const uint64_t c = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (c < cols)
+        for (uint64_t j = colRes[c]; j < colRes[c + 1]; ++j) parent[j] = c;
+}
+
+__global__ void k_set_u64(uint64_t* p, uint64_t v, int n) {
+    if (static_cast<int>(threadIdx.x) < n) p[threadIdx.x] = v;
+}
+
+// ------------------------------------------------------------- range steps
+
+struct RangeStepArgs {
+    int D;
+    int level0;        // candidates are all level-0 positions, items are query slices
+    uint64_t nq;
+    const double* boxes;  // nq x 2D (lo | hi)
+    // groups of the previous level
+    const uint64_t* items;
+    const uint64_t* nItems;
+    uint64_t itemCap;
+    const uint64_t* off;
+    const uint32_t* qlist;
+    const uint64_t* colResPrev;
+    // candidates
+    uint64_t count0;
+    const double* coords;
+    const double* radiusPos;
+    const uint64_t* ids;
+    // output
+    uint64_t* out;      // pairs (prune) or hits (final)
+    uint64_t cap;
+    uint64_t* outCount;
+    uint64_t* cnt;
+    uint64_t* overflow;
+    int idBits;
+    // optional per-query statistics
+    uint64_t* survStat;  // [nq * L]
+    uint64_t* fineStat;  // [nq]
+    int level, L;
+};
+
+template <int DM, bool FINAL>
+__global__ __launch_bounds__(256) void k_range_step(RangeStepArgs A) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t wpb = blockDim.x / kWave;
+    const uint64_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint64_t gw = blockIdx.x * wpb + wave;
+    const uint64_t nw = static_cast<uint64_t>(gridDim.x) * wpb;
+    const int D = A.D;
+    uint64_t nItems = A.level0 ? (A.nq + kWave - 1) / kWave : *A.nItems;
+    if (!A.level0 && nItems > A.itemCap) nItems = A.itemCap;
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    for (uint64_t item = gw; item < nItems; item += nw) {
+        uint64_t jb, je, q = 0;
+        bool valid;
+        if (A.level0) {
+            const uint64_t qq = item * kWave + lane;
+            valid = qq < A.nq;
+            q = valid ? qq : 0;
+            jb = 0;
+            je = A.count0;
+        } else {
+            const uint64_t it = A.items[item];
+            const uint64_t c = it >> kSliceBits;
+            const uint64_t start = (it & ((1ull << kSliceBits) - 1)) * kWave;
+            const uint64_t g0 = A.off[c];
+            const uint64_t gn = A.off[c + 1] - g0;
+            valid = start + lane < gn;
+            q = valid ? A.qlist[g0 + start + lane] : 0;
+            jb = A.colResPrev[c];
+            je = A.colResPrev[c + 1];
+        }
+        double lo[DM], hi[DM];
+#pragma unroll
+        for (int d = 0; d < DM; ++d) {
+            lo[d] = 0.0;
+            hi[d] = 0.0;
+            if (d < D) {
+                lo[d] = A.boxes[q * 2 * D + d];
+                hi[d] = A.boxes[q * 2 * D + D + d];
+            }
+        }
+        uint64_t mine = 0;
+        for (uint64_t j0 = jb; j0 < je; j0 += kWave) {
+            const int nc = static_cast<int>(je - j0 < static_cast<uint64_t>(kWave) ? je - j0 : kWave);
+            uint64_t myMask = 0, total = 0;
+            for (int k = 0; k < nc; ++k) {
+                const uint64_t j = j0 + k;
+                const double* x = A.coords + j * D;
+                bool hit;
+                if (FINAL) {
+                    bool in = true;
+#pragma unroll
+                    for (int d = 0; d < DM; ++d)
+                        if (d < D) in = in && (x[d] >= lo[d]) && (x[d] <= hi[d]);
+                    hit = valid && in;
+                } else {
+                    double dist = 0.0;
+#pragma unroll
+                    for (int d = 0; d < DM; ++d) {
+                        if (d < D) {
+                            const double cv = x[d];
+                            double diff = 0.0;
+                            if (cv < lo[d]) diff = lo[d] - cv;
+                            else if (cv > hi[d]) diff = cv - hi[d];
+                            dist = __builtin_fma(diff, diff, dist);
+                        }
+                    }
+                    const double r = A.radiusPos[j];
+                    hit = valid && dist <= r * r;
+                }
+                const uint64_t m = __ballot(hit);
+                if (lane == k) myMask = m;
+                total += __popcll(m);
+            }
+            if (total == 0) continue;
+            uint64_t base = 0;
+            if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(A.outCount), total);
+            base = __shfl(base, 0, kWave);
+            if (base + total > A.cap) {
+                // mark the unwritten part of the allocation so no consumer reads stale entries
+                for (uint64_t o = base + lane; o < A.cap; o += kWave) A.out[o] = ~0ull;
+                if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), FINAL ? 4ull : 1ull);
+                continue;
+            }
+            const uint64_t myId = lane < nc ? A.ids[j0 + lane] : 0;
+            uint64_t pos = base;
+            for (int k = 0; k < nc; ++k) {
+                const uint64_t m = __shfl(myMask, k, kWave);
+                if (m == 0) continue;
+                const uint64_t id = __shfl(myId, k, kWave);
+                if ((m >> lane) & 1ull) {
+                    const uint64_t o = pos + __popcll(m & ltmask);
+                    A.out[o] = FINAL ? ((q << A.idBits) | id) : ((id << kQueryBits) | q);
+                    ++mine;
+                }
+                if (!FINAL && lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[id]),
+                                                   static_cast<unsigned long long>(__popcll(m)));
+                pos += __popcll(m);
+            }
+        }
+        if (valid) {
+            if (A.survStat && !FINAL && mine)
+                atomicAdd(reinterpret_cast<unsigned long long*>(&A.survStat[q * A.L + A.level]), mine);
+            if (A.fineStat && FINAL)
+                atomicAdd(reinterpret_cast<unsigned long long*>(&A.fineStat[q]), je - jb);
+        }
+    }
+}
+
+// Group scan over the clusters of one level: off[i] = exclusive prefix of
+// cnt[i], items for every 64-query slice of every cluster, totals.
+__global__ __launch_bounds__(kScanThreads) void k_group_scan(const uint64_t* __restrict__ cnt, uint64_t n,
+                                                             uint64_t* __restrict__ off, uint64_t* __restrict__ items,
+                                                             uint64_t itemCap, uint64_t* nItems, uint64_t* overflow,
+                                                             uint64_t* status, ScanCtrl* ctrl, uint32_t epoch) {
+    __shared__ uint64_t s_tile;
+    __shared__ uint64_t s_c[kScanThreads / kWave], s_s[kScanThreads / kWave];
+    __shared__ uint64_t s_ex0, s_ex1;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const uint64_t ntiles = n == 0 ? 1 : (n + kScanThreads - 1) / kScanThreads;
+    for (;;) {
+        if (tid == 0) s_tile = atomicAdd(reinterpret_cast<unsigned long long*>(&ctrl->tileCounter), 1ull);
+        __syncthreads();
+        const uint64_t tile = s_tile;
+        if (tile >= ntiles) break;
+        const uint64_t i = tile * kScanThreads + tid;
+        const uint64_t c = i < n ? cnt[i] : 0;
+        const uint64_t sl = (c + kWave - 1) / kWave;
+        uint64_t xc = c, xs = sl;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint64_t yc = __shfl_up(xc, o, kWave), ys = __shfl_up(xs, o, kWave);
+            if (lane >= o) {
+                xc += yc;
+                xs += ys;
+            }
+        }
+        if (lane == kWave - 1) {
+            s_c[wid] = xc;
+            s_s[wid] = xs;
+        }
+        __syncthreads();
+        if (wid == 0) {
+            uint64_t tc = lane < kScanThreads / kWave ? s_c[lane] : 0;
+            uint64_t ts = lane < kScanThreads / kWave ? s_s[lane] : 0;
+            tc = wave_sum_u64(tc);
+            ts = wave_sum_u64(ts);
+            uint64_t e0, e1;
+            tile_prefix2(status, epoch, tile, lane, tc, ts, e0, e1);
+            if (lane == 0) {
+                s_ex0 = e0;
+                s_ex1 = e1;
+                if (tile == ntiles - 1) {
+                    off[n] = e0 + tc;
+                    *nItems = e1 + ts;
+                    if (e1 + ts > itemCap) atomicOr(reinterpret_cast<unsigned long long*>(overflow), 2ull);
+                }
+            }
+        }
+        __syncthreads();
+        uint64_t pc = s_ex0 + xc - c, ps = s_ex1 + xs - sl;
+        for (int w = 0; w < wid; ++w) {
+            pc += s_c[w];
+            ps += s_s[w];
+        }
+        if (i < n) {
+            off[i] = pc;
+            for (uint64_t k = 0; k < sl; ++k)
+                if (ps + k < itemCap) items[ps + k] = (i << kSliceBits) | k;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long*>(&ctrl->exitCounter), 1ull);
+        if (old == gridDim.x - 1) {
+            st_release(&ctrl->tileCounter, 0);
+            st_release(&ctrl->exitCounter, 0);
+        }
+    }
+}
+
+// Counting-sort scatter of (id, q) pairs into grouped query lists; leaves cnt zeroed.
+__global__ void k_group_scatter(const uint64_t* __restrict__ pairs, const uint64_t* dn, uint64_t cap, uint64_t mesh,
+                                uint64_t* __restrict__ cnt, const uint64_t* __restrict__ off,
+                                uint32_t* __restrict__ qlist) {
+    uint64_t n = *dn;
+    if (n > cap) n = cap;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t p = pairs[i];
+        const uint64_t id = p >> kQueryBits;
+        if (id >= mesh) continue;  // sentinel of an overflowed allocation
+        const uint32_t q = static_cast<uint32_t>(p & ((1ull << kQueryBits) - 1));
+        const uint64_t pos = atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[id]), ~0ull) - 1;
+        qlist[off[id] + pos] = q;
+    }
+}
+
+// ------------------------------------------------------------- radix sort
+// LSD radix sort of u64 keys, 8-bit digits, tiles of 4096 keys.  The key count
+// lives on the device; a persistent grid walks the tiles.
+
+constexpr int kRsThreads = 256;
+constexpr int kRsPer = 16;
+constexpr int kRsTile = kRsThreads * kRsPer;
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint64_t* __restrict__ keys, const uint64_t* dn,
+                                                        uint64_t cap, int shift, uint64_t* __restrict__ hist) {
+    __shared__ uint32_t lc[256];
+    uint64_t n = *dn;
+    if (n > cap) n = cap;
+    const uint64_t ntiles = (n + kRsTile - 1) / kRsTile;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        lc[threadIdx.x] = 0;
+        __syncthreads();
+        for (int r = 0; r < kRsPer; ++r) {
+            const uint64_t e = t * kRsTile + static_cast<uint64_t>(r) * kRsThreads + threadIdx.x;
+            if (e < n) atomicAdd(&lc[(keys[e] >> shift) & 0xFF], 1u);
+        }
+        __syncthreads();
+        hist[static_cast<uint64_t>(threadIdx.x) * ntiles + t] = lc[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// In-place exclusive scan of a u64 array whose length is 256 * ntiles(dn).
+__global__ __launch_bounds__(kScanThreads) void k_scan_hist(uint64_t* __restrict__ a, const uint64_t* dn,
+                                                            uint64_t cap, uint64_t* status, ScanCtrl* ctrl,
+                                                            uint32_t epoch) {
+    __shared__ uint64_t s_tile;
+    __shared__ uint64_t s_w[kScanThreads / kWave];
+    __shared__ uint64_t s_ex;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    uint64_t nk = *dn;
+    if (nk > cap) nk = cap;
+    const uint64_t n = 256 * ((nk + kRsTile - 1) / kRsTile);
+    const uint64_t ntiles = n == 0 ? 1 : (n + kScanThreads - 1) / kScanThreads;
+    for (;;) {
+        if (tid == 0) s_tile = atomicAdd(reinterpret_cast<unsigned long long*>(&ctrl->tileCounter), 1ull);
+        __syncthreads();
+        const uint64_t tile = s_tile;
+        if (tile >= ntiles) break;
+        const uint64_t i = tile * kScanThreads + tid;
+        const uint64_t v = i < n ? a[i] : 0;
+        uint64_t x = v;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint64_t y = __shfl_up(x, o, kWave);
+            if (lane >= o) x += y;
+        }
+        if (lane == kWave - 1) s_w[wid] = x;
+        __syncthreads();
+        if (wid == 0) {
+            uint64_t t = lane < kScanThreads / kWave ? s_w[lane] : 0;
+            t = wave_sum_u64(t);
+            uint64_t e0, e1;
+            tile_prefix2(status, epoch, tile, lane, t, 0, e0, e1);
+            if (lane == 0) s_ex = e0;
+        }
+        __syncthreads();
+        uint64_t p = s_ex + x - v;
+        for (int w = 0; w < wid; ++w) p += s_w[w];
+        if (i < n) a[i] = p;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long*>(&ctrl->exitCounter), 1ull);
+        if (old == gridDim.x - 1) {
+            st_release(&ctrl->tileCounter, 0);
+            st_release(&ctrl->exitCounter, 0);
+        }
+    }
+}
+
+// Stable scatter: within a tile, keys keep their order per digit (round, then
+// wave, then lane); tiles keep theirs through the digit-major scanned histogram.
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                                                           const uint64_t* dn, uint64_t cap, int shift,
+                                                           const uint64_t* __restrict__ hist, uint64_t idMask,
+                                                           int extract) {
+    __shared__ uint32_t running[256];
+    __shared__ uint32_t wc[kRsThreads / kWave][256];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    uint64_t n = *dn;
+    if (n > cap) n = cap;
+    const uint64_t ntiles = (n + kRsTile - 1) / kRsTile;
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        running[tid] = 0;
+        __syncthreads();
+        for (int r = 0; r < kRsPer; ++r) {
+            for (int w = 0; w < kRsThreads / kWave; ++w) wc[w][tid] = 0;
+            __syncthreads();
+            const uint64_t e = t * kRsTile + static_cast<uint64_t>(r) * kRsThreads + tid;
+            const bool ok = e < n;
+            const uint64_t key = ok ? in[e] : 0;
+            const uint32_t dig = ok ? static_cast<uint32_t>((key >> shift) & 0xFF) : 0;
+            uint64_t peers = __ballot(ok);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const uint64_t bal = __ballot(((dig >> b) & 1u) != 0);
+                peers &= ((dig >> b) & 1u) ? bal : ~bal;
+            }
+            const uint32_t rankW = static_cast<uint32_t>(__popcll(peers & ltmask));
+            if (ok && rankW == 0) wc[wid][dig] = static_cast<uint32_t>(__popcll(peers));
+            __syncthreads();
+            if (ok) {
+                uint32_t before = running[dig];
+                for (int w = 0; w < wid; ++w) before += wc[w][dig];
+                const uint64_t dst = hist[static_cast<uint64_t>(dig) * ntiles + t] + before + rankW;
+                out[dst] = extract ? (key & idMask) : key;
+            }
+            __syncthreads();
+            uint32_t add = 0;
+            for (int w = 0; w < kRsThreads / kWave; ++w) add += wc[w][tid];
+            running[tid] += add;
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void k_query_offsets(const uint64_t* __restrict__ keys, const uint64_t* dn, uint64_t cap, uint64_t nq,
+                                int idBits, uint64_t* __restrict__ outOff) {
+    const uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (q > nq) return;
+    uint64_t n = *dn;
+    if (n > cap) n = cap;
+    if (q == nq) {
+        outOff[q] = n;
+        return;
+    }
+    const uint64_t target = q << idBits;
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < target) lo = mid + 1;
+        else hi = mid;
+    }
+    outOff[q] = lo;
+}
+
+__global__ void k_extract_ids(const uint64_t* __restrict__ keys, const uint64_t* dn, uint64_t cap, uint64_t mask,
+                              uint64_t* __restrict__ out) {
+    uint64_t n = *dn;
+    if (n > cap) n = cap;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        out[i] = keys[i] & mask;
+}
+
+// status[0] = hits, status[1] = overflow flags (1 pairs, 2 items, 4 hits)
+__global__ void k_copy_status(const uint64_t* ctl, uint64_t* status) {
+    if (threadIdx.x == 0) {
+        status[0] = ctl[2];
+        status[1] = ctl[3];
+    }
+}
+
+int scan_launch_grid(tblaeq_ctx* ctx, uint64_t tiles) {
+    const uint64_t cap = static_cast<uint64_t>(ctx->cuCount) * 4;
+    return static_cast<int>(tiles < cap ? (tiles ? tiles : 1) : cap);
+}
+
+// LSD radix sort of `dn` keys (device count, at most cap) on bits [0, nbits).
+// Ping-pongs between a and b; returns the buffer holding the result.  The last
+// pass can write key & idMask instead of the key.
+int radix_sort(tblaeq_ctx* ctx, uint64_t* a, uint64_t* b, const uint64_t* dn, uint64_t cap, int nbits,
+               uint64_t* hist, uint64_t** result, bool extractLast, uint64_t idMask) {
+    const uint64_t capTiles = (cap + kRsTile - 1) / kRsTile;
+    const int grid = scan_launch_grid(ctx, capTiles);
+    const uint64_t histTiles = (256 * (capTiles ? capTiles : 1) + kScanThreads - 1) / kScanThreads;
+    const int passes = (nbits + 7) / 8;
+    uint64_t* src = a;
+    uint64_t* dst = b;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        {
+            TimedScope ts(ctx, kKBatch0 + 4);
+            hipLaunchKernelGGL(k_rs_hist, dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dn, cap, shift, hist);
+        }
+        uint32_t epoch;
+        int rc = scan_prepare(ctx, histTiles, &epoch);
+        if (rc) return rc;
+        {
+            TimedScope ts(ctx, kKBatch0 + 4);
+            hipLaunchKernelGGL(k_scan_hist, dim3(scan_launch_grid(ctx, histTiles)), dim3(kScanThreads), 0,
+                               ctx->stream, hist, dn, cap, ctx->status, ctx->ctrl, epoch);
+        }
+        {
+            TimedScope ts(ctx, kKBatch0 + 4);
+            hipLaunchKernelGGL(k_rs_scatter, dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dst, dn, cap, shift,
+                               hist, idMask, (extractLast && p == passes - 1) ? 1 : 0);
+        }
+        TB_HIP(hipGetLastError());
+        std::swap(src, dst);
+    }
+    *result = src;
+    return TBLAEQ_OK;
+}
+
+int bits_for(uint64_t v) {
+    int b = 1;
+    while (b < 64 && (v >> b) != 0) ++b;
+    return b;
+}
+
+template <bool FINAL>
+int launch_range_step(tblaeq_ctx* ctx, const RangeStepArgs& A, uint64_t itemBound) {
+    uint64_t blocks = (itemBound + 3) / 4;
+    const uint64_t cap = static_cast<uint64_t>(ctx->cuCount) * 16;
+    if (blocks > cap) blocks = cap;
+    if (blocks == 0) blocks = 1;
+    const dim3 g(static_cast<unsigned>(blocks)), b(256);
+    TimedScope ts(ctx, FINAL ? kKBatch0 + 1 : kKBatch0 + 0);
+    if (A.D <= 4) hipLaunchKernelGGL((k_range_step<4, FINAL>), g, b, 0, ctx->stream, A);
+    else if (A.D <= 8) hipLaunchKernelGGL((k_range_step<8, FINAL>), g, b, 0, ctx->stream, A);
+    else if (A.D <= 16) hipLaunchKernelGGL((k_range_step<16, FINAL>), g, b, 0, ctx->stream, A);
+    else hipLaunchKernelGGL((k_range_step<32, FINAL>), g, b, 0, ctx->stream, A);
+    TB_HIP(hipGetLastError());
+    return TBLAEQ_OK;
+}
+
+template <class T>
+T* balloc(tblaeq_batch* b, size_t n, int* rc) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, (n ? n : 1) * sizeof(T));
+    if (e != hipSuccess) {
+        *rc = hip_fail(e, "hipMalloc(batch)");
+        return nullptr;
+    }
+    b->owned.push_back(p);
+    return static_cast<T*>(p);
+}
+
+}  // namespace tblaeq
+
+using namespace tblaeq;
+
+extern "C" int tblaeq_batch_create(tblaeq_ctx* ctx, const tblaeq_level* levels, uint64_t nlevels,
+                                   const uint64_t* coarsest_ids, const double* coarsest_vals, uint64_t ncoarsest,
+                                   const double* host_max_radius, tblaeq_batch** out) {
+    if (!ctx || !levels || nlevels == 0 || !coarsest_ids || !coarsest_vals || !out || ncoarsest == 0) {
+        set_error("tblaeq_batch_create: invalid argument");
+        return TBLAEQ_ERR_INVALID;
+    }
+    const uint64_t D = levels[0].D;
+    if (D == 0 || D > 32) {
+        set_error("tblaeq_batch_create: the batched path handles 1 <= D <= 32");
+        return TBLAEQ_ERR_UNSUPPORTED;
+    }
+    for (uint64_t l = 0; l < nlevels; ++l)
+        if (!levels[l].pvals || !levels[l].radius || !levels[l].col_res || levels[l].D != D) {
+            set_error("tblaeq_batch_create: every level needs resident P values, radii and column pointers");
+            return TBLAEQ_ERR_INVALID;
+        }
+    auto* b = new tblaeq_batch();
+    b->ctx = ctx;
+    b->D = static_cast<int>(D);
+    b->L = static_cast<int>(nlevels);
+    const int L = b->L;
+    b->count.resize(L + 1);
+    b->coords.resize(L + 1);
+    b->ids.resize(L + 1);
+    b->radiusPos.resize(L, nullptr);
+    b->parentPos.resize(L, nullptr);
+    b->weightPos.resize(L, nullptr);
+    b->colRes.resize(L);
+    b->nnzPerCol.resize(L);
+    b->radiusById.resize(L);
+    b->emptyCols.resize(L);
+    b->maxRadius.assign(host_max_radius ? host_max_radius : nullptr, host_max_radius ? host_max_radius + L : nullptr);
+    int rc = TBLAEQ_OK;
+    auto fail = [&](int code) {
+        tblaeq_batch_destroy(b);
+        return code;
+    };
+    b->count[0] = ncoarsest;
+    b->coords[0] = const_cast<double*>(coarsest_vals);
+    b->ids[0] = const_cast<uint64_t*>(coarsest_ids);
+    for (int l = 0; l < L; ++l) {
+        b->colRes[l] = levels[l].col_res;
+        b->nnzPerCol[l] = levels[l].nnz_per_col;
+        b->radiusById[l] = levels[l].radius;
+        b->emptyCols[l] = levels[l].empty_cols;
+        b->count[l + 1] = levels[l].nnz;
+        b->maxMesh = std::max<uint64_t>(b->maxMesh, levels[l].cols);
+    }
+    // Whole-hierarchy prolongation: level l+1 positions from all level-l positions.
+    uint64_t* dn = balloc<uint64_t>(b, 1, &rc);
+    if (rc) return fail(rc);
+    for (int l = 0; l < L; ++l) {
+        const uint64_t n = b->count[l];
+        uint64_t* offp = balloc<uint64_t>(b, n, &rc);
+        if (rc) return fail(rc);
+        b->coords[l + 1] = balloc<double>(b, b->count[l + 1] * D, &rc);
+        if (rc) return fail(rc);
+        b->ids[l + 1] = balloc<uint64_t>(b, b->count[l + 1], &rc);
+        if (rc) return fail(rc);
+        b->radiusPos[l] = balloc<double>(b, n, &rc);
+        if (rc) return fail(rc);
+        b->weightPos[l] = balloc<uint64_t>(b, n, &rc);
+        if (rc) return fail(rc);
+        const unsigned g = static_cast<unsigned>((n + 255) / 256);
+        hipLaunchKernelGGL(k_gather_off, dim3(g), dim3(256), 0, ctx->stream, b->ids[l], n, levels[l].col_res, offp);
+        hipLaunchKernelGGL(k_gather_f64, dim3(g), dim3(256), 0, ctx->stream, b->ids[l], n, levels[l].radius,
+                           b->radiusPos[l]);
+        hipLaunchKernelGGL(k_gather_u64, dim3(g), dim3(256), 0, ctx->stream, b->ids[l], n, levels[l].nnz_per_col,
+                           b->weightPos[l]);
+        if (hipMemcpyAsync(dn, &n, 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) return fail(TBLAEQ_ERR_HIP);
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(TBLAEQ_ERR_HIP);
+        rc = tblaeq_prolong_refactor(ctx, &levels[l], b->ids[l], b->coords[l], offp, dn, n, b->ids[l + 1],
+                                     b->coords[l + 1]);
+        if (rc) return fail(rc);
+        if (l >= 1) {
+            b->parentPos[l] = balloc<uint64_t>(b, n, &rc);
+            if (rc) return fail(rc);
+            hipLaunchKernelGGL(k_fill_parent, dim3(static_cast<unsigned>((levels[l - 1].cols + 255) / 256)), dim3(256),
+                               0, ctx->stream, levels[l - 1].col_res, levels[l - 1].cols, b->parentPos[l]);
+        }
+    }
+    b->ctl = balloc<uint64_t>(b, 16, &rc);
+    if (rc) return fail(rc);
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, ctx->stream, b->ctl, 0ull, 16);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(hip_fail(hipGetLastError(), "batch create"));
+    *out = b;
+    return TBLAEQ_OK;
+}
+
+extern "C" int tblaeq_batch_destroy(tblaeq_batch* b) {
+    if (!b) return TBLAEQ_OK;
+    if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+    for (void* p : b->owned) (void)hipFree(p);
+    delete b;
+    return TBLAEQ_OK;
+}
+
+extern "C" int tblaeq_batch_reserve(tblaeq_batch* b, uint64_t max_queries, uint64_t pair_cap, uint64_t hit_cap) {
+    if (!b || max_queries == 0 || max_queries >= (1ull << kQueryBits)) {
+        set_error("tblaeq_batch_reserve: invalid argument (queries per batch must be below 2^24)");
+        return TBLAEQ_ERR_INVALID;
+    }
+    int rc = TBLAEQ_OK;
+    if (pair_cap > b->pairCap) {
+        b->pairs = balloc<uint64_t>(b, pair_cap, &rc);
+        if (rc) return rc;
+        b->qlist = balloc<uint32_t>(b, pair_cap, &rc);
+        if (rc) return rc;
+        b->itemCap = pair_cap / kWave + b->maxMesh + 1;
+        b->items = balloc<uint64_t>(b, b->itemCap, &rc);
+        if (rc) return rc;
+        b->pairCap = pair_cap;
+    }
+    if (!b->cnt) {
+        b->cnt = balloc<uint64_t>(b, b->maxMesh, &rc);
+        if (rc) return rc;
+        b->off = balloc<uint64_t>(b, b->maxMesh + 1, &rc);
+        if (rc) return rc;
+        TB_HIP(hipMemsetAsync(b->cnt, 0, b->maxMesh * 8, b->ctx->stream));
+    }
+    if (hit_cap > b->hitCap) {
+        b->hits = balloc<uint64_t>(b, hit_cap, &rc);
+        if (rc) return rc;
+        b->hitsTmp = balloc<uint64_t>(b, hit_cap, &rc);
+        if (rc) return rc;
+        const uint64_t tiles = (hit_cap + kRsTile - 1) / kRsTile;
+        b->histCap = 256 * tiles;
+        b->hist = balloc<uint64_t>(b, b->histCap, &rc);
+        if (rc) return rc;
+        b->hitCap = hit_cap;
+    }
+    b->maxQ = std::max(b->maxQ, max_queries);
+    TB_HIP(hipStreamSynchronize(b->ctx->stream));
+    return TBLAEQ_OK;
+}
+
+extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64_t nq, uint64_t* d_out_off,
+                                  uint64_t* d_out_ids, uint64_t out_cap, uint64_t* d_status,
+                                  uint64_t* d_level_surv, uint64_t* d_fine_count) {
+    if (!b || !d_boxes || nq == 0 || !d_out_off || !d_out_ids || !d_status || nq > b->maxQ || !b->pairs ||
+        !b->hits) {
+        set_error("tblaeq_batch_range: invalid argument or engine not reserved");
+        return TBLAEQ_ERR_INVALID;
+    }
+    tblaeq_ctx* ctx = b->ctx;
+    const int L = b->L;
+    uint64_t* ctl = b->ctl;
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, ctx->stream, ctl, 0ull, 8);
+    RangeStepArgs A{};
+    A.D = b->D;
+    A.nq = nq;
+    A.boxes = d_boxes;
+    A.cnt = b->cnt;
+    A.overflow = ctl + 3;
+    A.survStat = d_level_surv;
+    A.fineStat = d_fine_count;
+    A.L = L;
+    int rc;
+    for (int l = 0; l < L; ++l) {
+        // prune the candidates of level l
+        A.level0 = l == 0;
+        A.level = l;
+        A.items = b->items;
+        A.nItems = ctl + 1;
+        A.itemCap = b->itemCap;
+        A.off = b->off;
+        A.qlist = b->qlist;
+        A.colResPrev = l > 0 ? b->colRes[l - 1] : nullptr;
+        A.count0 = b->count[0];
+        A.coords = b->coords[l];
+        A.radiusPos = b->radiusPos[l];
+        A.ids = b->ids[l];
+        A.out = b->pairs;
+        A.cap = b->pairCap;
+        A.outCount = ctl + 0;
+        const uint64_t bound = l == 0 ? (nq + kWave - 1) / kWave : b->itemCap;
+        if ((rc = launch_range_step<false>(ctx, A, bound))) return rc;
+        // group the survivors of level l by cluster
+        const uint64_t mesh = l == 0 ? b->count[0] : b->count[l];
+        const uint64_t tiles = (mesh + kScanThreads - 1) / kScanThreads;
+        uint32_t epoch;
+        if ((rc = scan_prepare(ctx, tiles, &epoch))) return rc;
+        {
+            TimedScope ts(ctx, kKBatch0 + 2);
+            hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream,
+                               b->cnt, mesh, b->off, b->items, b->itemCap, ctl + 1, ctl + 3, ctx->status, ctx->ctrl,
+                               epoch);
+        }
+        {
+            TimedScope ts(ctx, kKBatch0 + 3);
+            const uint64_t g = std::min<uint64_t>((b->pairCap + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 8);
+            hipLaunchKernelGGL(k_group_scatter, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream,
+                               b->pairs, ctl + 0, b->pairCap, mesh, b->cnt, b->off, b->qlist);
+        }
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, ctx->stream, ctl, 0ull, 1);
+        TB_HIP(hipGetLastError());
+    }
+    // final step: exact filter of the finest candidates
+    const uint64_t maxId = b->count[L];
+    const int idBits = bits_for(maxId);
+    const int qBits = bits_for(nq);
+    if (idBits + qBits > 64) {
+        set_error("tblaeq_batch_range: id and query bits exceed 64");
+        return TBLAEQ_ERR_UNSUPPORTED;
+    }
+    A.level0 = 0;
+    A.level = L;
+    A.items = b->items;
+    A.nItems = ctl + 1;
+    A.colResPrev = b->colRes[L - 1];
+    A.coords = b->coords[L];
+    A.radiusPos = nullptr;
+    A.ids = b->ids[L];
+    A.out = b->hits;
+    A.cap = std::min(b->hitCap, out_cap);
+    A.outCount = ctl + 2;
+    A.idBits = idBits;
+    if ((rc = launch_range_step<true>(ctx, A, b->itemCap))) return rc;
+    uint64_t* sorted = nullptr;
+    if ((rc = radix_sort(ctx, b->hits, b->hitsTmp, ctl + 2, A.cap, idBits + qBits, b->hist, &sorted, false, 0)))
+        return rc;
+    hipLaunchKernelGGL(k_query_offsets, dim3(static_cast<unsigned>((nq + 1 + 255) / 256)), dim3(256), 0, ctx->stream,
+                       sorted, ctl + 2, A.cap, nq, idBits, d_out_off);
+    // ids = key & mask, written to the caller's buffer
+    {
+        const uint64_t mask = (idBits >= 64) ? ~0ull : ((1ull << idBits) - 1);
+        const uint64_t g = std::min<uint64_t>((A.cap + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 8);
+        hipLaunchKernelGGL(k_extract_ids, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream, sorted,
+                           ctl + 2, A.cap, mask, d_out_ids);
+    }
+    hipLaunchKernelGGL(k_copy_status, dim3(1), dim3(64), 0, ctx->stream, ctl, d_status);
+    TB_HIP(hipGetLastError());
+    return TBLAEQ_OK;
+}

@@ -9,6 +9,7 @@
 // look-back, so survivors land in grid order together with their prolongation
 // offsets, and the element count is read from device memory (no host sync).
 #include "../ctx.hpp"
+#include "../scan_common.hpp"
 
 namespace tblaeq {
 
@@ -37,50 +38,6 @@ struct CompactArgs {
     uint64_t maxTiles;  // capacity of the status array (never exceeded, whatever the device count says)
     int radiusByPos;  // radius[i] belongs to grid position i (pre-gathered tile)
 };
-
-// Exclusive prefix of (count, weight) over all tiles before `tile`, read from
-// the predecessors' status words by one wave (64 predecessors per probe).
-__device__ void lookback2(const uint64_t* status, uint32_t epoch, uint64_t tile, int lane, uint64_t& ex0,
-                          uint64_t& ex1) {
-    uint64_t acc0 = 0, acc1 = 0;
-    int64_t base = static_cast<int64_t>(tile) - 1;
-    for (;;) {
-        const int64_t t = base - lane;
-        int s = 2;
-        uint64_t v0 = 0, v1 = 0;
-        if (t >= 0) {
-            const uint64_t* st = status + static_cast<uint64_t>(t) * 4;
-            const uint64_t i0 = ld_acquire(st + 2), i1 = ld_acquire(st + 3);
-            if (status_valid(i0, epoch) && status_valid(i1, epoch)) {
-                v0 = i0 & kValMask;
-                v1 = i1 & kValMask;
-            } else {
-                const uint64_t a0 = ld_acquire(st + 0), a1 = ld_acquire(st + 1);
-                if (status_valid(a0, epoch) && status_valid(a1, epoch)) {
-                    s = 1;
-                    v0 = a0 & kValMask;
-                    v1 = a1 & kValMask;
-                } else {
-                    s = 0;
-                }
-            }
-        }
-        const uint64_t m2 = __ballot(s == 2);
-        const uint64_t m0 = __ballot(s == 0);
-        const int f2 = m2 ? __ffsll(static_cast<unsigned long long>(m2)) - 1 : 64;
-        const int f0 = m0 ? __ffsll(static_cast<unsigned long long>(m0)) - 1 : 64;
-        if (f0 < f2) {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        acc0 += wave_sum_u64(lane <= f2 ? v0 : 0);
-        acc1 += wave_sum_u64(lane <= f2 ? v1 : 0);
-        if (f2 < 64) break;
-        base -= 64;
-    }
-    ex0 = acc0;
-    ex1 = acc1;
-}
 
 template <int MODE>
 __global__ __launch_bounds__(kScanThreads) void k_compact(CompactArgs A) {
@@ -233,6 +190,7 @@ int launch_compact(tblaeq_ctx* ctx, int mode, CompactArgs A, uint64_t bound) {
     A.maxTiles = tiles;
     const uint64_t cap = static_cast<uint64_t>(ctx->cuCount) * 8;
     const unsigned grid = static_cast<unsigned>(tiles < cap ? tiles : cap);
+    TimedScope ts(ctx, mode == kModeRange ? kKCompactRange : (mode == kModeKnn ? kKCompactKnn : kKCompactBox));
     switch (mode) {
         case kModeRange: hipLaunchKernelGGL(k_compact<kModeRange>, dim3(grid), dim3(kScanThreads), 0, ctx->stream, A); break;
         case kModeKnn: hipLaunchKernelGGL(k_compact<kModeKnn>, dim3(grid), dim3(kScanThreads), 0, ctx->stream, A); break;
@@ -363,6 +321,7 @@ extern "C" int tblaeq_knn_lower_bounds(tblaeq_ctx* ctx, uint64_t D, const uint64
         return TBLAEQ_ERR_INVALID;
     }
     if (count == 0) return TBLAEQ_OK;
+    TimedScope ts(ctx, kKKnnLb);
     hipLaunchKernelGGL(k_knn_lb, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, ctx->stream,
                        static_cast<int>(D), ids, vals, count, d_q, radius, out_lb);
     TB_HIP(hipGetLastError());

@@ -82,6 +82,7 @@ extern "C" int tblaeq_prolong_refactor(tblaeq_ctx* ctx, const tblaeq_level* lv, 
     uint64_t blocks = (waves + 3) / 4;
     const uint64_t cap = static_cast<uint64_t>(ctx->cuCount) * 16;
     if (blocks > cap) blocks = cap;
+    TimedScope ts(ctx, kKProlong);
     hipLaunchKernelGGL(k_prolong, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, ctx->stream,
                        static_cast<int>(lv->D), surv_ids, surv_vals, surv_off, d_surv_count, lv->col_res, lv->pvals,
                        lv->child_ids, static_cast<const double*>(nullptr), static_cast<const uint64_t*>(nullptr),
@@ -104,6 +105,7 @@ extern "C" int tblaeq_prolong_refactor_packed(tblaeq_ctx* ctx, const tblaeq_leve
     uint64_t blocks = (waves + 3) / 4;
     const uint64_t cap = static_cast<uint64_t>(ctx->cuCount) * 16;
     if (blocks > cap) blocks = cap;
+    TimedScope ts(ctx, kKProlong);
     hipLaunchKernelGGL(k_prolong, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, ctx->stream,
                        static_cast<int>(lv->D), surv_ids, surv_vals, surv_off, d_surv_count, lv->col_res,
                        static_cast<const double*>(nullptr), lv->child_ids, packed_pvals, packed_child, out_ids,

@@ -245,6 +245,7 @@ __global__ __launch_bounds__(kSortThreads) void k_step(const uint64_t* __restric
 int launch_tilesort(tblaeq_ctx* ctx, int mode, const TileArgs& A, uint64_t bound) {
     const uint64_t tiles = bound == 0 ? 1 : (bound + kSortTile - 1) / kSortTile;
     const dim3 grid(static_cast<unsigned>(tiles)), block(kSortThreads);
+    TimedScope ts(ctx, kKTileSort);
     switch (mode) {
         case kTilePairs: hipLaunchKernelGGL(k_tilesort<kTilePairs>, grid, block, 0, ctx->stream, A); break;
         case kTileKnnLb: hipLaunchKernelGGL(k_tilesort<kTileKnnLb>, grid, block, 0, ctx->stream, A); break;
@@ -263,7 +264,10 @@ int merge_passes(tblaeq_ctx* ctx, uint64_t* k, uint64_t* v, uint64_t* tk, uint64
     uint64_t* dv = tv;
     const unsigned blocks = static_cast<unsigned>((n + kSortTile - 1) / kSortTile);
     for (uint64_t width = kSortTile; width < n; width <<= 1) {
-        hipLaunchKernelGGL(k_merge, dim3(blocks), dim3(kSortThreads), 0, ctx->stream, sk, svv, dk, dv, n, width);
+        {
+            TimedScope ts(ctx, kKMerge);
+            hipLaunchKernelGGL(k_merge, dim3(blocks), dim3(kSortThreads), 0, ctx->stream, sk, svv, dk, dv, n, width);
+        }
         TB_HIP(hipGetLastError());
         std::swap(sk, dk);
         std::swap(svv, dv);
@@ -382,6 +386,7 @@ extern "C" int tblaeq_knn_lower_bound_sort_step(tblaeq_ctx* ctx, const tblaeq_le
         A.radius = lv->radius;
         rc = select_smallest(ctx, kTileKnnLb, A, count_bound, keep, outK, outV);
         if (rc) return rc;
+        TimedScope ts(ctx, kKStep);
         hipLaunchKernelGGL(k_step, dim3(1), dim3(kSortThreads), 0, ctx->stream, outK, outV, keep, K,
                            lv->nnz_per_col, lv->radius, d_ub);
         TB_HIP(hipGetLastError());

@@ -26,6 +26,10 @@ SIGNATURES = {
     "tblaeq_ctx_stream": (vp, [vp]),
     "tblaeq_status_string": (C.c_char_p, [C.c_int]),
     "tblaeq_last_error": (C.c_char_p, []),
+    "tblaeq_ctx_set_timing": (C.c_int, [vp, C.c_int]),
+    "tblaeq_ctx_set_tag": (C.c_int, [vp, C.c_int]),
+    "tblaeq_ctx_timing": (C.c_int, [vp, C.c_int, C.c_int, u64p, f64p]),
+    "tblaeq_ctx_timing_reset": (C.c_int, [vp]),
     "tblaeq_range_prune_compact": (C.c_int, [vp] * 13),
     "tblaeq_knn_lower_bound_sort_step": (C.c_int, [vp, vp, vp, vp, vp, C.c_uint64, vp, C.c_uint64, vp, vp, vp, vp,
                                                    vp]),
@@ -49,14 +53,27 @@ SIGNATURES = {
     "tblaeq_handler_prepare": (C.c_int, [vp, u8p, C.c_uint64, C.c_uint64]),
     "tblaeq_handler_policy": (C.c_int, [vp, u8p, C.c_uint64]),
     "tblaeq_handler_recommend": (C.c_int, [vp, C.c_uint64, C.c_int, u8p, C.c_uint64]),
+    "tblaeq_handler_ctx": (C.c_int, [vp, C.POINTER(vp)]),
     "tblaeq_handler_set_l3_tile_bytes": (C.c_int, [C.c_uint64]),
     "tblaeq_handler_query": (C.c_int, [vp, C.c_int, f64p, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.POINTER(vp)]),
     "tblaeq_result_destroy": (C.c_int, [vp]),
     "tblaeq_result_info": (C.c_int, [vp, u64p, u64p, f64p]),
     "tblaeq_result_query": (C.c_int, [vp, C.c_uint64, u64p, f64p, u64p, u64p]),
+    "tblaeq_result_levels": (C.c_int, [vp, C.c_uint64, u64p, u64p]),
     "tblaeq_result_fine_mesh": (C.c_int, [vp, C.c_uint64, u64p, f64p]),
     "tblaeq_result_refined": (C.c_int, [vp, C.c_uint64, u64p, f64p]),
     "tblaeq_result_save_csv": (C.c_int, [vp, C.c_char_p]),
+    "tblaeq_batch_create": (C.c_int, [vp, vp, C.c_uint64, vp, vp, C.c_uint64, vp, C.POINTER(vp)]),
+    "tblaeq_batch_destroy": (C.c_int, [vp]),
+    "tblaeq_batch_reserve": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint64]),
+    "tblaeq_batch_range": (C.c_int, [vp, vp, C.c_uint64, vp, vp, C.c_uint64, vp, vp, vp]),
+    "tblaeq_batch_knn": (C.c_int, [vp, vp, vp, C.c_uint64, C.c_uint64, vp, vp, vp, vp, vp]),
+    "tblaeq_handler_batch_init": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint64]),
+    "tblaeq_handler_batch_set_queries": (C.c_int, [vp, C.c_int, f64p, u64p, C.c_uint64]),
+    "tblaeq_handler_batch_run": (C.c_int, [vp, C.c_int, u64p]),
+    "tblaeq_handler_batch_results": (C.c_int, [vp, u64p, u64p, f64p, u64p]),
+    "tblaeq_handler_batch_stats": (C.c_int, [vp, u64p, u64p]),
+    "tblaeq_handler_batch_free": (C.c_int, [vp]),
 }
 
 
@@ -172,6 +189,55 @@ class Handler:
     def knn(self, q, K, refine=True, save_fine_mesh=True):
         return self.query("knn", np.asarray(q, float), K, refine, save_fine_mesh)
 
+    # --- batched engine -------------------------------------------------------
+    def batch_init(self, max_queries, pair_cap, hit_cap):
+        check(lib().tblaeq_handler_batch_init(self.h, max_queries, pair_cap, hit_cap), "batch_init")
+
+    def batch_set(self, qtype, queries, Ks=None):
+        q = np.ascontiguousarray(queries, np.float64)
+        self._bq = (qtype, q.shape[0])
+        if qtype == "range":
+            check(lib().tblaeq_handler_batch_set_queries(self.h, 0, _p(q, f64p), None, q.shape[0]), "batch_set")
+        else:
+            k = np.ascontiguousarray(np.broadcast_to(np.asarray(Ks, np.uint64), (q.shape[0],)))
+            self._bK = int(k.max())
+            check(lib().tblaeq_handler_batch_set_queries(self.h, 1, _p(q, f64p), _p(k, u64p), q.shape[0]),
+                  "batch_set")
+
+    def batch_run(self, stats=False):
+        t = C.c_uint64(0)
+        check(lib().tblaeq_handler_batch_run(self.h, int(stats), C.byref(t)), "batch_run")
+        return t.value
+
+    def batch_results(self):
+        qtype, nq = self._bq
+        if qtype == "range":
+            off = np.zeros(nq + 1, np.uint64)
+            check(lib().tblaeq_handler_batch_results(self.h, _p(off, u64p), None, None, None), "batch_results")
+            ids = np.zeros(int(off[-1]), np.uint64)
+            check(lib().tblaeq_handler_batch_results(self.h, _p(off, u64p), _p(ids, u64p), None, None),
+                  "batch_results")
+            return [ids[int(off[i]):int(off[i + 1])] for i in range(nq)]
+        K = self._bK
+        n = np.zeros(nq, np.uint64)
+        ids = np.zeros(nq * K, np.uint64)
+        d2 = np.zeros(nq * K)
+        check(lib().tblaeq_handler_batch_results(self.h, None, _p(ids, u64p), _p(d2, f64p), _p(n, u64p)),
+              "batch_results")
+        return [(ids[i * K:i * K + int(n[i])], d2[i * K:i * K + int(n[i])]) for i in range(nq)]
+
+    def batch_stats(self):
+        nq = self._bq[1]
+        s = np.zeros(nq * self.intervals, np.uint64)
+        f = np.zeros(nq, np.uint64)
+        check(lib().tblaeq_handler_batch_stats(self.h, _p(s, u64p), _p(f, u64p)), "batch_stats")
+        return s.reshape(nq, self.intervals), f
+
+    def ctx(self):
+        c = vp()
+        check(lib().tblaeq_handler_ctx(self.h, C.byref(c)), "handler_ctx")
+        return c
+
     def close(self):
         if self.h:
             lib().tblaeq_handler_destroy(self.h)
@@ -200,6 +266,12 @@ class Result:
         check(lib().tblaeq_result_query(self.h, i, _p(lv, u64p), C.byref(us), C.byref(fs), C.byref(rc)), "result")
         return {"levels": lv, "us": us.value, "fine": fs.value, "refined": rc.value}
 
+    def levels(self, i=0):
+        s = np.zeros(self.L, np.uint64)
+        o = np.zeros(self.L, np.uint64)
+        check(lib().tblaeq_result_levels(self.h, i, _p(s, u64p), _p(o, u64p)), "result_levels")
+        return s, o
+
     def fine_mesh(self, i=0):
         n = self.info(i)["fine"]
         ids = np.zeros(n, np.uint64)
@@ -221,3 +293,30 @@ class Result:
         if getattr(self, "h", None) and _LIB is not None:
             _LIB.tblaeq_result_destroy(self.h)
             self.h = None
+
+
+KERNEL_KINDS = {"compact_range": 0, "compact_knn": 1, "compact_box": 2, "prolong": 3, "tilesort": 4, "merge": 5,
+                "step": 6, "knn_lb": 7}
+
+
+class Timing:
+    """HIP-event timing of the kernels a context launches (on its own stream)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def enable(self, on=True):
+        check(lib().tblaeq_ctx_set_timing(self.ctx, int(on)), "set_timing")
+
+    def reset(self):
+        check(lib().tblaeq_ctx_timing_reset(self.ctx), "timing_reset")
+
+    def tag(self, t):
+        check(lib().tblaeq_ctx_set_tag(self.ctx, t), "set_tag")
+
+    def get(self, kind, tag=-1):
+        k = KERNEL_KINDS.get(kind, kind)
+        n = C.c_uint64(0)
+        ms = C.c_double(0)
+        check(lib().tblaeq_ctx_timing(self.ctx, k, tag, C.byref(n), C.byref(ms)), "timing")
+        return n.value, ms.value

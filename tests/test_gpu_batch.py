@@ -1,0 +1,146 @@
+"""GPU parity of the batched engine: per-query results equal the oracle's exact
+refine of the reference traversal (range: sorted id sets; KNN: top-K ids and
+squared distances bit-exact), and the per-level survivor counts and candidate
+superset sizes equal the oracle's."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+import tblaeq
+from tblaeq import workload as W
+
+pytestmark = pytest.mark.gpu
+
+CFG = dict(N=60000, D=6, vmin=1.0, vmax=100.0, is_int=False, ratios=(30, 10, 8), seed=21, sigma_div=2.5)
+
+
+def _make(cfg, max_q=512, pair_cap=1 << 22, hit_cap=1 << 22):
+    h = tblaeq.Handler.random(cfg["N"], cfg["D"], cfg["vmin"], cfg["vmax"], cfg["is_int"], cfg["ratios"], cfg["seed"],
+                              cfg["sigma_div"])
+    o = O.Index.generate(cfg["N"], cfg["D"], cfg["vmin"], cfg["vmax"], cfg["is_int"], cfg["ratios"], cfg["seed"],
+                         cfg["sigma_div"])
+    h.prepare("L0")
+    h.batch_init(max_q, pair_cap, hit_cap)
+    return h, o
+
+
+@pytest.fixture(scope="module")
+def idx():
+    return _make(CFG)
+
+
+def _check_range_batch(h, o, boxes):
+    h.batch_set("range", boxes)
+    h.batch_run(stats=True)
+    res = h.batch_results()
+    surv, fine = h.batch_stats()
+    D = h.D
+    for i, row in enumerate(boxes):
+        lo, hi = row[:D], row[D:]
+        r = o.range(lo, hi)
+        assert np.array_equal(res[i], r.refine_range(lo, hi)), "query %d hits differ" % i
+        _, s, out = r.levels()
+        assert np.array_equal(surv[i], s), "query %d survivors per level differ" % i
+        assert fine[i] == out[-1]
+    return res
+
+
+def test_batch_range_matches_oracle(idx):
+    h, o = idx
+    data = h.data()
+    boxes, _ = W.range_boxes(data, 300, 5, 1e-4, 5e-2)
+    res = _check_range_batch(h, o, boxes)
+    assert sum(len(x) for x in res) > 0
+
+
+def test_batch_range_edges(idx):
+    h, o = idx
+    D = h.D
+    boxes = np.array([
+        np.r_[np.full(D, -1e6), np.full(D, -1e6 + 1)],   # nothing
+        np.r_[np.full(D, -1e9), np.full(D, 1e9)],         # everything
+        np.r_[np.full(D, 60.0), np.full(D, 40.0)],        # inverted
+        np.r_[h.data()[7], h.data()[7]],                  # degenerate on a point
+    ])
+    res = _check_range_batch(h, o, boxes)
+    assert len(res[1]) == h.N and len(res[0]) == 0
+
+
+def test_batch_range_single_and_uneven(idx):
+    h, o = idx
+    data = h.data()
+    boxes, _ = W.range_boxes(data, 1, 9, 1e-2, 1e-2)
+    _check_range_batch(h, o, boxes)
+    # very uneven selectivities in one batch
+    b1, _ = W.range_boxes(data, 64, 10, 1e-5, 1e-5)
+    b2, _ = W.range_boxes(data, 64, 11, 0.2, 0.5)
+    _check_range_batch(h, o, np.concatenate([b1, b2]))
+
+
+def test_batch_range_overflow_splits():
+    """Capacities far below the batch's needs: the driver splits and still matches."""
+    h, o = _make(dict(CFG, N=20000, seed=3), max_q=128, pair_cap=3000, hit_cap=40000)
+    data = h.data()
+    boxes, _ = W.range_boxes(data, 96, 12, 1e-3, 2e-2)
+    _check_range_batch(h, o, boxes)
+
+
+def _check_knn_batch(h, o, pts, Ks):
+    h.batch_set("knn", pts, Ks)
+    h.batch_run(stats=True)
+    res = h.batch_results()
+    surv, fine = h.batch_stats()
+    for i, q in enumerate(pts):
+        K = int(Ks[i])
+        r = o.knn(q, K)
+        eid, ed2 = r.refine_knn(q, K)
+        gid, gd2 = res[i]
+        assert np.array_equal(gid, eid), "query %d ids differ" % i
+        assert np.array_equal(gd2.view(np.uint64), ed2.view(np.uint64)), "query %d distances differ" % i
+        _, s, out = r.levels()
+        assert np.array_equal(surv[i], s), "query %d survivors per level differ" % i
+        assert fine[i] == out[-1]
+
+
+@pytest.mark.parametrize("K", [1, 10, 100])
+def test_batch_knn_matches_oracle(idx, K):
+    h, o = idx
+    pts = W.knn_points(h.data(), 64, 20 + K, h.mesh_sizes, CFG["vmin"], CFG["vmax"])
+    _check_knn_batch(h, o, pts, np.full(64, K))
+
+
+def test_batch_knn_mixed_k(idx):
+    h, o = idx
+    pts = W.knn_points(h.data(), 97, 31, h.mesh_sizes, CFG["vmin"], CFG["vmax"], noise=0.5)
+    Ks = np.random.default_rng(4).choice([0, 1, 5, 10, 37, 100, 200], 97)
+    _check_knn_batch(h, o, pts, Ks)
+
+
+def test_batch_knn_far_and_ties():
+    """Integer grid (many equal lower bounds and distances) and far-away queries."""
+    h, o = _make(dict(CFG, N=20000, is_int=True, seed=8), max_q=64)
+    data = h.data()
+    pts = np.concatenate([data[:8], data[100:108] + 0.5, np.full((4, h.D), 1e4), np.full((4, h.D), -50.0)])
+    _check_knn_batch(h, o, pts, np.full(len(pts), 25))
+
+
+def test_batch_matches_plain_c1():
+    cfg = W.C1
+    h, o = _make(cfg, max_q=256, pair_cap=1 << 24, hit_cap=1 << 24)
+    data = h.data()
+    boxes, _ = W.range_boxes(data, 100, 12345, 1e-3, 1e-3)
+    h.batch_set("range", boxes)
+    h.batch_run()
+    res = h.batch_results()
+    plain = h.query("range", boxes, refine=True)
+    for i in range(len(boxes)):
+        assert np.array_equal(res[i], plain.refined(i)[0])
+    pts = W.knn_points(data, 100, 12346, h.mesh_sizes, cfg["vmin"], cfg["vmax"])
+    h.batch_set("knn", pts, np.full(100, 10))
+    h.batch_run()
+    kres = h.batch_results()
+    kplain = h.query("knn", pts, K=10, refine=True)
+    for i in range(len(pts)):
+        pid, pd2 = kplain.refined(i)
+        assert np.array_equal(kres[i][0], pid)
+        assert np.array_equal(kres[i][1].view(np.uint64), pd2.view(np.uint64))

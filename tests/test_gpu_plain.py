@@ -55,8 +55,9 @@ def _check_range(h, o, lo, hi):
     oids, ovals = ro.grid()
     assert np.array_equal(ids, oids), "candidate ids/order differ"
     assert _same_bits(vals, ovals), "candidate coordinates differ"
-    _, _, out = ro.levels()
+    _, surv, out = ro.levels()
     assert np.array_equal(r.info(0)["levels"], out)
+    assert np.array_equal(r.levels(0)[0], surv)
     assert np.array_equal(r.refined()[0], ro.refine_range(lo, hi))
     return ids.size
 
@@ -68,8 +69,9 @@ def _check_knn(h, o, q, K):
     oids, ovals = ro.grid()
     assert np.array_equal(ids, oids), "candidate ids/order differ"
     assert _same_bits(vals, ovals), "candidate coordinates differ"
-    _, _, out = ro.levels()
+    _, surv, out = ro.levels()
     assert np.array_equal(r.info(0)["levels"], out)
+    assert np.array_equal(r.levels(0)[0], surv)
     rid, rd2 = r.refined()
     eid, ed2 = ro.refine_knn(q, K)
     assert np.array_equal(rid, eid)

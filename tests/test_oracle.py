@@ -1,0 +1,306 @@
+"""CPU checks that pin the oracle (no GPU).
+
+Each test checks the oracle against something other than its own formula: the
+reference's own code (golden vectors written from the reference headers, and the
+reference library itself when it is built here), IEEE directed rounding done by
+the FPU, exact rational arithmetic, a literal restatement of the reference's
+sequential STEP loop, brute force, and closed-form invariants.
+"""
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden", "ref_vectors.npz")
+
+
+# ---------------------------------------------------------------- sqrt_rd
+
+def test_sqrt_rd_matches_fpu_round_down():
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.uniform(0, 1e4, 20000), rng.uniform(0, 1e-3, 2000), 10.0 ** rng.uniform(-300, 300, 4000),
+                         np.arange(0, 2000, dtype=float), [0.0, 1.0, 2.0, 4.0, 1e308, 5e-324, np.inf]])
+    for x in xs:
+        assert O.sqrt_rd(x) == O.sqrt_rd_fenv(x), x
+
+
+def test_sqrt_rd_exact_rational():
+    """s*s <= x < next(s)^2 in exact arithmetic."""
+    rng = np.random.default_rng(2)
+    for x in rng.uniform(1e-6, 1e6, 3000):
+        s = O.sqrt_rd(x)
+        n = np.nextafter(s, np.inf)
+        assert Fraction(s) ** 2 <= Fraction(x) < Fraction(n) ** 2
+
+
+# ---------------------------------------------------------- pruning tests
+
+def test_range_dist2_special_cases():
+    rng = np.random.default_rng(3)
+    for _ in range(500):
+        D = rng.integers(1, 9)
+        lo = rng.uniform(0, 50, D)
+        hi = lo + rng.uniform(0, 50, D)
+        inside = rng.uniform(lo, hi)
+        assert O.range_dist2(inside, lo, hi) == 0.0
+        c = rng.uniform(-20, 120, D)
+        d2 = O.range_dist2(c, lo, hi)
+        # exact: sum of squared clamped distances, rounded by an fma chain (error << 1e-12 relative)
+        diff = np.where(c < lo, lo - c, np.where(c > hi, c - hi, 0.0))
+        exact = sum(Fraction(v) ** 2 for v in diff)
+        assert abs(Fraction(d2) - exact) <= Fraction(1e-12) * (exact + 1)
+        # with r = 0 the prune test is point-in-box
+        assert (d2 <= 0.0) == bool(np.all((c >= lo) & (c <= hi)))
+
+
+def test_knn_lb_closed_form():
+    rng = np.random.default_rng(4)
+    for _ in range(500):
+        D = rng.integers(1, 9)
+        c = rng.uniform(0, 100, D)
+        q = rng.uniform(0, 100, D)
+        r = rng.uniform(0, 30)
+        lb = O.knn_lb(c, q, r)
+        d = np.sqrt(np.sum((q - c) ** 2))
+        assert abs(lb - (d - r)) <= 1e-12 * (d + r + 1)
+        # lower bound property against every point of a ball of radius r around c
+        u = rng.normal(size=D)
+        x = c + u / np.linalg.norm(u) * r * rng.uniform(0, 1)
+        assert lb <= np.sqrt(np.sum((q - x) ** 2)) + 1e-12
+
+
+def _step_closed(lb, w, r, K):
+    """Selected set as used by the oracle/GPU: {lb <= ub}, ub from phase 1."""
+    order = np.lexsort((np.arange(len(lb)), lb))
+    cnt, ub, i = 0, 0.0, 0
+    while i < len(order) and cnt < K:
+        cnt += w[order[i]]
+        ub = max(ub, lb[order[i]] + 2.0 * r[order[i]])
+        i += 1
+    sel = lb <= ub
+    sel[order[:i]] = True
+    return sel
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_step_closed_form_equals_reference_loop(seed):
+    """KNN.cu:75-96 walked literally over the sorted list == the closed form."""
+    rng = np.random.default_rng(seed)
+    p = int(rng.integers(1, 300))
+    lb = np.round(rng.normal(10, 5, p), int(rng.integers(0, 3)))  # rounding makes ties
+    w = rng.integers(0 if seed % 3 == 0 else 1, 50, p).astype(np.uint64)
+    r = rng.uniform(0, 5, p)
+    for K in (0, 1, 7, 50, 10 ** 6):
+        order = np.lexsort((np.arange(p), lb))
+        sel_seq = O.step_sequential(lb[order], w[order], r[order], K)
+        expect = np.zeros(p, bool)
+        expect[order] = sel_seq
+        assert np.array_equal(expect, _step_closed(lb, w, r, K))
+
+
+def test_step_knn_traversal_uses_sequential_step():
+    """Oracle KNN survivors per level == a Python traversal that calls the literal STEP loop."""
+    ix = O.Index.generate(4000, 3, 1.0, 100.0, False, (10, 8), 5, 2.5)
+    D = ix.D
+    rng = np.random.default_rng(9)
+    recon_by_level = []
+    # reconstructed coordinates per level (position-free): walk the CSC arrays
+    coarse = ix.layer(ix.intervals).reshape(-1, D)
+    vals = {int(i): coarse[i] for i in range(coarse.shape[0])}
+    recon_by_level.append(vals)
+    for l in range(ix.intervals):
+        colres = ix.array(l, 0)
+        mp = ix.array(l, 3)
+        P = ix.array(l, 4).reshape(-1, D)
+        nxt = {}
+        for c, x in recon_by_level[l].items():
+            for j in range(int(colres[c]), int(colres[c + 1])):
+                nxt[int(mp[j])] = P[j] * x
+        recon_by_level.append(nxt)
+    for K in (1, 10, 60):
+        q = ix.layer(0).reshape(-1, D)[rng.integers(4000)] + 0.3
+        res = ix.knn(q, K)
+        grid = list(range(len(recon_by_level[0])))
+        for l in range(ix.intervals):
+            rad = ix.array(l, 5)
+            w = ix.array(l, 1)
+            colres = ix.array(l, 0)
+            mp = ix.array(l, 3)
+            lb = np.array([O.knn_lb(recon_by_level[l][g], q, rad[g]) for g in grid])
+            gid = np.array(grid, np.uint64)
+            order = np.lexsort((gid, lb))
+            sel = O.step_sequential(lb[order], w[gid[order]], rad[gid[order]], K)
+            surv = sorted(int(gid[order[i]]) for i in range(len(order)) if sel[i])
+            assert surv == sorted(res.survivors(l).tolist()), (K, l)
+            grid = [int(mp[j]) for c in surv for j in range(int(colres[c]), int(colres[c + 1]))]
+
+
+# ---------------------------------------------------------- generator / build
+
+@pytest.mark.skipif(not os.path.exists(GOLD), reason="golden vectors not generated")
+@pytest.mark.parametrize("name", ["g1", "g2"])
+def test_generator_matches_reference_golden(name):
+    g = np.load(GOLD)
+    a = g[name + "_args"]
+    ratios = tuple(int(x) for x in g[name + "_ratios"])
+    N, D = int(a[0]), int(a[1])
+    ix = O.Index.generate(N, D, a[2], a[3], bool(a[4]), ratios, int(a[5]), a[6])
+    for i in range(len(ratios) + 1):
+        assert np.array_equal(ix.layer(i), g["%s_layer%d" % (name, i)]), "layer %d" % i
+    for i in range(len(ratios)):
+        lab = g["%s_labels%d" % (name, i)]
+        assert np.array_equal(ix.labels(i), lab)
+        # map[fineIdx] of interval i (P index = intervals-1-i) is the reference argsort
+        pidx = len(ratios) - 1 - i
+        assert np.array_equal(ix.array(pidx, 3), g["%s_argsort%d" % (name, i)])
+
+
+@pytest.mark.skipif(not os.path.exists(GOLD), reason="golden vectors not generated")
+def test_radius_and_ratios_follow_reference_formulas():
+    g = np.load(GOLD)
+    # Python restatement of dist::euclidean, pinned by the reference's own values
+    a, b = g["euclid_a"], g["euclid_b"]
+    for i in range(a.shape[0]):
+        s = 0.0
+        for d in range(a.shape[1]):
+            t = a[i, d] - b[i, d]
+            s = s + t * t
+        assert float(np.sqrt(s)) == g["euclid"][i]
+    ix = O.Index.generate(3000, 3, 1.0, 100.0, False, (10, 5), 12345, 3.0)
+    D = 3
+    for i in range(2):
+        fine = ix.layer(i).reshape(-1, D)
+        coarse = ix.layer(i + 1).reshape(-1, D)
+        pidx = 1 - i
+        colres, mp = ix.array(pidx, 0), ix.array(pidx, 3)
+        P, rad = ix.array(pidx, 4).reshape(-1, D), ix.array(pidx, 5)
+        for c in range(coarse.shape[0]):
+            best = 0.0
+            for j in range(int(colres[c]), int(colres[c + 1])):
+                f = fine[int(mp[j])]
+                assert np.array_equal(P[j], f / coarse[c])
+                s = 0.0
+                for d in range(D):
+                    t = coarse[c, d] - f[d]
+                    s = s + t * t
+                best = max(best, float(np.sqrt(s)))
+            assert rad[c] == best
+
+
+@pytest.mark.skipif(not os.path.exists(GOLD), reason="golden vectors not generated")
+def test_cpu_kmeans_matches_reference_golden():
+    g = np.load(GOLD)
+    lab, cen, _ = O.kmeans(g["km_data"], 30, 7)
+    assert np.array_equal(lab, g["km_labels"])
+    assert np.array_equal(cen, g["km_centroids"])
+
+
+def test_kmeans_inertia_non_increasing():
+    rng = np.random.default_rng(5)
+    data = rng.uniform(1, 100, (2000, 3))
+    prev = None
+    for it in (1, 2, 4, 8):
+        lab, cen, _ = O.kmeans(data, 25, it)
+        lab2, inertia = O.kmeans_assign(data, cen)
+        if prev is not None:
+            assert inertia <= prev * (1 + 1e-12)
+        prev = inertia
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(HERE), "oracle", "_ref", "libref.so")),
+                    reason="reference headers not built here")
+def test_generator_matches_reference_library_live():
+    import ctypes as C
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(HERE, "golden", "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    L = mg.load()
+    layers, labels = mg.generate(L, 20000, 4, (20, 10, 5), 1.0, 100.0, False, 7, 3.0)
+    ix = O.Index.generate(20000, 4, 1.0, 100.0, False, (20, 10, 5), 7, 3.0)
+    for i, a in enumerate(layers):
+        assert np.array_equal(ix.layer(i), a)
+    for i, a in enumerate(labels):
+        s = np.zeros(a.size, np.uint64)
+        L.ref_argsort(a.ctypes.data_as(C.POINTER(C.c_uint64)), a.size, s.ctypes.data_as(C.POINTER(C.c_uint64)))
+        assert np.array_equal(ix.array(2 - i, 3), s)
+
+
+# ---------------------------------------------------------- traversal invariants
+
+def test_infinite_radius_gives_whole_dataset_and_layout_invariant():
+    ix = O.Index.generate(5000, 4, 1.0, 100.0, False, (10, 5), 11, 3.0)
+    data = ix.data()
+    ix.scale_radius(np.inf)
+    r = ix.range(np.full(4, 50.0), np.full(4, 50.0))
+    ids, vals = r.grid()
+    assert sorted(ids.tolist()) == list(range(5000))
+    # LayoutConsistencyTest.cu:33-37,142-185: reconstructed vals match the dataset within 1e-6 abs or rel
+    err = np.abs(vals - data[ids.astype(np.int64)])
+    assert np.all((err <= 1e-6) | (err <= 1e-6 * np.abs(data[ids.astype(np.int64)])))
+    # and the refine over all candidates equals brute force over the reconstructed points
+    recon = ix.recon()
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        c = recon[rng.integers(5000)]
+        lo, hi = c - 8, c + 8
+        rr = ix.range(lo, hi)
+        assert np.array_equal(rr.refine_range(lo, hi), O.brute_range(recon, lo, hi))
+        q = c + 0.7
+        kk = ix.knn(q, 17)
+        a_ids, a_d2 = kk.refine_knn(q, 17)
+        b_ids, b_d2 = O.brute_knn(recon, q, 17)
+        assert np.array_equal(a_ids, b_ids) and np.array_equal(a_d2, b_d2)
+
+
+def test_range_candidates_are_superset_of_hits_with_surviving_ancestors():
+    ix = O.Index.generate(20000, 3, 1.0, 100.0, False, (20, 10), 3, 3.0)
+    recon = ix.recon()
+    rng = np.random.default_rng(2)
+    for _ in range(5):
+        c = recon[rng.integers(20000)]
+        lo, hi = c - 6, c + 6
+        r = ix.range(lo, hi)
+        cand = set(r.grid()[0].tolist())
+        hits = set(r.refine_range(lo, hi).tolist())
+        assert hits <= cand
+        assert hits <= set(O.brute_range(recon, lo, hi).tolist())
+
+
+def test_index_save_load_roundtrip(tmp_path):
+    ix = O.Index.generate(3000, 3, 1.0, 100.0, False, (10, 5), 4, 3.0)
+    a, b = str(tmp_path / "a"), str(tmp_path / "b")
+    ix.save(a)
+    O.Index.load(a).save(b)
+    for f in sorted(os.listdir(a)):
+        with open(os.path.join(a, f), "rb") as fa, open(os.path.join(b, f), "rb") as fb:
+            assert fa.read() == fb.read(), f
+    # metadata layout: D, N, height, intervals (u64), isAos (1 byte), ratios, msz, meshSizes
+    raw = open(os.path.join(a, "metadata.bin"), "rb").read()
+    head = np.frombuffer(raw[:32], np.uint64)
+    assert head.tolist() == [3, 3000, 3, 2] and raw[32] == 1
+    rest = np.frombuffer(raw[33:], np.uint64)
+    assert rest.tolist() == [10, 5, 3, 3000, 300, 60]
+
+
+# ---------------------------------------------------------- scheduler
+
+def test_policy_recommend_hand_computed():
+    # two levels, D = 2: pv = nnz*16, rv = cols*8, sp = nnz*24
+    nnz, cols, D = [100, 1000], [10, 100], 2
+    c0 = [100 * 16 * 2 + 80 + 2400, 1000 * 16 * 2 + 800 + 24000]   # L0 = L1 cost: 2 pv + rv + sp
+    c2 = [2 * 1600 + 2400, 2 * 16000 + 24000]
+    perm = 500
+    # budget for exactly level 0 in L0 and level 1 in L2
+    free = int((perm + c0[0] + c2[1]) / 0.85) + 2
+    assert O.policy_recommend(nnz, cols, D, 200, 300, free).tolist() == [0, 2]
+    # everything fits
+    free = int((perm + c0[0] + c0[1]) / 0.85) + 2
+    assert O.policy_recommend(nnz, cols, D, 200, 300, free).tolist() == [0, 0]
+    # not even the permanent data
+    assert O.policy_recommend(nnz, cols, D, 200, 300, 100).tolist() == [3, 3]
