@@ -185,6 +185,12 @@ int tblaeq_batch_reserve(tblaeq_batch* b, uint64_t max_queries, uint64_t pair_ca
  * Asynchronous on the context stream; no host synchronisation inside. */
 int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64_t nq, uint64_t* d_out_off, uint64_t* d_out_ids,
                        uint64_t out_cap, uint64_t* d_status, uint64_t* d_level_surv, uint64_t* d_fine_count);
+/* tblaeq_batch_range as one hipGraph launch: the first call with a given set
+ * of arguments captures the whole batch (level loop, grouping, hit sort) on
+ * the context stream; later calls with the same arguments replay it (the
+ * buffers' contents may change between calls).  No statistics outputs. */
+int tblaeq_batch_range_graph(tblaeq_batch* b, const double* d_boxes, uint64_t nq, uint64_t* d_out_off,
+                             uint64_t* d_out_ids, uint64_t out_cap, uint64_t* d_status);
 /* nq KNN points (device, rows of D), K per query in d_K (each <= Kmax).
  * Top-K ids and squared distances by (distance, id) in rows of Kmax;
  * d_out_n[q] = number written (min(K, candidates)).  Same traversal (STEP per

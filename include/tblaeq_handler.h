@@ -82,6 +82,8 @@ int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total);
 int tblaeq_handler_batch_results(tblaeq_handler* h, uint64_t* off, uint64_t* ids, double* d2, uint64_t* n);
 /* levelSurv[q*L + l] survivors per level, fine[q] candidate superset size (when run with stats). */
 int tblaeq_handler_batch_stats(tblaeq_handler* h, uint64_t* levelSurv, uint64_t* fine);
+/* Range batches run as a captured hipGraph (default on) or as plain launches. */
+int tblaeq_handler_batch_use_graph(tblaeq_handler* h, int on);
 int tblaeq_handler_batch_free(tblaeq_handler* h);
 
 #ifdef __cplusplus

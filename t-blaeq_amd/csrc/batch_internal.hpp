@@ -41,6 +41,15 @@ struct tblaeq_batch {
     uint64_t* hist = nullptr;    // radix histograms
     uint64_t histCap = 0;
     uint64_t* ctl = nullptr;     // [0] pairs [1] items [2] hits [3] overflow [4..] spare
+    // captured range batch (replayed while the arguments are unchanged)
+    hipGraph_t rangeGraph = nullptr;
+    hipGraphExec_t rangeExec = nullptr;
+    const void* gBoxes = nullptr;
+    uint64_t gNq = 0, gCap = 0;
+    void* gOff = nullptr;
+    void* gIds = nullptr;
+    void* gStatus = nullptr;
+    uint64_t gScanLaunches = 0;
     // knn scratch
     uint8_t* flags = nullptr;    // per query survivor flags of a middle level
     uint64_t flagCap = 0;

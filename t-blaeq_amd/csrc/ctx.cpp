@@ -17,24 +17,28 @@ int hip_fail(hipError_t e, const char* what) {
     return e == hipErrorOutOfMemory ? TBLAEQ_ERR_NOMEM : TBLAEQ_ERR_HIP;
 }
 
-int scan_prepare(tblaeq_ctx* ctx, uint64_t tiles, uint32_t* epoch) {
+int scan_prepare(tblaeq_ctx* ctx, uint64_t tiles) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(ctx->stream, &cap);
     if (tiles > ctx->statusTiles) {
-        uint64_t cap = ctx->statusTiles ? ctx->statusTiles : 1024;
-        while (cap < tiles) cap *= 2;
+        if (cap != hipStreamCaptureStatusNone) {
+            set_error("look-back status array too small during graph capture (reserve first)");
+            return TBLAEQ_ERR_CAPACITY;
+        }
+        uint64_t n = ctx->statusTiles ? ctx->statusTiles : 1024;
+        while (n < tiles) n *= 2;
+        TB_HIP(hipStreamSynchronize(ctx->stream));
         if (ctx->status) TB_HIP(hipFree(ctx->status));
         ctx->status = nullptr;
         ctx->statusTiles = 0;
-        TB_HIP(hipMalloc(&ctx->status, cap * 4 * sizeof(uint64_t)));
-        TB_HIP(hipMemsetAsync(ctx->status, 0, cap * 4 * sizeof(uint64_t), ctx->stream));
-        ctx->statusTiles = cap;
+        TB_HIP(hipMalloc(&ctx->status, n * 4 * sizeof(uint64_t)));
+        TB_HIP(hipMemsetAsync(ctx->status, 0, n * 4 * sizeof(uint64_t), ctx->stream));
+        ctx->statusTiles = n;
     }
-    // 27-bit epoch; on wrap-around clear the words so no stale word can match.
-    ctx->epoch = (ctx->epoch + 1) & 0x7FFFFFF;
-    if (ctx->epoch == 0) {
+    // The device epoch has 27 bits; clear every status word well before it can
+    // wrap onto a value still present in the array (outside graph capture).
+    if (++ctx->scanLaunches % (1ull << 26) == 0 && cap == hipStreamCaptureStatusNone)
         TB_HIP(hipMemsetAsync(ctx->status, 0, ctx->statusTiles * 4 * sizeof(uint64_t), ctx->stream));
-        ctx->epoch = 1;
-    }
-    *epoch = ctx->epoch;
     return TBLAEQ_OK;
 }
 
@@ -169,8 +173,7 @@ extern "C" int tblaeq_ctx_create(int device, void* stream, tblaeq_ctx** out) {
         if (e != hipSuccess) { rc = hip_fail(e, "hipMemset(ctrl)"); break; }
         e = hipHostMalloc(reinterpret_cast<void**>(&ctx->hostPinned), 64 * sizeof(uint64_t), hipHostMallocDefault);
         if (e != hipSuccess) { rc = hip_fail(e, "hipHostMalloc"); break; }
-        uint32_t ep;
-        rc = scan_prepare(ctx, 1024, &ep);
+        rc = scan_prepare(ctx, 1024);
         if (rc) break;
         e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) { rc = hip_fail(e, "hipStreamSynchronize"); break; }

@@ -34,7 +34,7 @@ struct tblaeq_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool ownsStream = false;
-    uint32_t epoch = 0;              // look-back status epoch, bumped per scan launch
+    uint64_t scanLaunches = 0;       // scan launches issued (status words are cleared before the epoch wraps)
     uint64_t* status = nullptr;      // 4 words per tile (agg0, agg1, inc0, inc1)
     uint64_t statusTiles = 0;
     tblaeq::ScanCtrl* ctrl = nullptr;  // [kCtrlSlots]
@@ -63,8 +63,8 @@ constexpr int kSortThreads = 256;
 
 void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
-// Ensures the look-back status array can hold `tiles` tiles and returns a fresh epoch.
-int scan_prepare(tblaeq_ctx* ctx, uint64_t tiles, uint32_t* epoch);
+// Ensures the look-back status array can hold `tiles` tiles (epochs advance on the device).
+int scan_prepare(tblaeq_ctx* ctx, uint64_t tiles);
 // Grows the context scratch to at least `bytes`.
 int ws_reserve(tblaeq_ctx* ctx, size_t bytes);
 

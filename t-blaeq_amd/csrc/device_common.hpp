@@ -110,7 +110,8 @@ struct ScanCtrl {
     uint64_t outNnz;        // sum of the survivors' weights (next level's size)
     double ub;              // KNN STEP bound
     uint64_t aux;           // spare
-    uint64_t pad[2];
+    uint64_t seq;           // epoch of the next scan launch (advanced on the device)
+    uint64_t pad;
 };
 
 constexpr int kValBits = 36;

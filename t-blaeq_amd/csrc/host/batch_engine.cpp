@@ -86,8 +86,9 @@ int tblaeq_handler_batch_set_queries(tblaeq_handler* h, int type, const double* 
     });
 }
 
-// Runs the uploaded batch.  Range: splits the batch in halves while a capacity
-// is exceeded.  *total = hits (range) or results (KNN).
+// Runs the uploaded batch.  Range: one graph launch when no statistics are
+// asked for; splits the batch in halves while a capacity is exceeded.
+// *total = hits (range) or results (KNN).
 int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total) {
     BatchState* st = stateOf(h);
     if (!st || st->nq == 0) return TBLAEQ_ERR_INVALID;
@@ -113,6 +114,20 @@ int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total) {
                 *total = t;
             }
             return TBLAEQ_OK;
+        }
+        if (!stats && st->useGraph) {
+            int rc = tblaeq_batch_range_graph(st->b, st->dQ.data(), st->nq, st->dOff.data(), st->dIds.data(),
+                                              st->dIds.size(), st->dStatus.data());
+            if (rc) return rc;
+            uint64_t status[2];
+            hipCheck(hipMemcpyAsync(status, st->dStatus.data(), 16, hipMemcpyDeviceToHost, s), "D2H");
+            hipCheck(hipStreamSynchronize(s), "sync");
+            if (status[1] == 0) {
+                st->lastTotal = status[0];
+                if (total) *total = status[0];
+                return TBLAEQ_OK;
+            }
+            // capacity exceeded: fall through to the splitting path
         }
         // range: [a, a+n) sub-batches
         std::vector<std::pair<uint64_t, uint64_t>> todo{{0, st->nq}};
@@ -198,6 +213,13 @@ int tblaeq_handler_batch_stats(tblaeq_handler* h, uint64_t* levelSurv, uint64_t*
         if (fine) hipCheck(hipMemcpy(fine, st->dFine.data(), st->nq * 8, hipMemcpyDeviceToHost), "D2H");
         return TBLAEQ_OK;
     });
+}
+
+int tblaeq_handler_batch_use_graph(tblaeq_handler* h, int on) {
+    BatchState* st = stateOf(h);
+    if (!st) return TBLAEQ_ERR_INVALID;
+    st->useGraph = on != 0;
+    return TBLAEQ_OK;
 }
 
 int tblaeq_handler_batch_free(tblaeq_handler* h) {

@@ -16,6 +16,7 @@ struct BatchState {
     DeviceBuffer<uint64_t> dK, dOff, dOffTmp, dIds, dStatus, dSurv, dFine, dN;
     DeviceBuffer<double> dD2;
     uint64_t lastTotal = 0;
+    bool useGraph = true;
     ~BatchState() {
         if (b) tblaeq_batch_destroy(b);
     }

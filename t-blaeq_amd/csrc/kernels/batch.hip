@@ -196,9 +196,10 @@ __global__ __launch_bounds__(256) void k_range_step(RangeStepArgs A) {
 __global__ __launch_bounds__(kScanThreads) void k_group_scan(const uint64_t* __restrict__ cnt, uint64_t n,
                                                              uint64_t* __restrict__ off, uint64_t* __restrict__ items,
                                                              uint64_t itemCap, uint64_t* nItems, uint64_t* overflow,
-                                                             uint64_t* status, ScanCtrl* ctrl, uint32_t epoch) {
+                                                             uint64_t* status, ScanCtrl* ctrl) {
     __shared__ uint64_t s_tile;
     __shared__ uint64_t s_c[kScanThreads / kWave], s_s[kScanThreads / kWave];
+    const uint32_t epoch = launch_epoch(ctrl);
     __shared__ uint64_t s_ex0, s_ex1;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const uint64_t ntiles = n == 0 ? 1 : (n + kScanThreads - 1) / kScanThreads;
@@ -253,13 +254,7 @@ __global__ __launch_bounds__(kScanThreads) void k_group_scan(const uint64_t* __r
         }
         __syncthreads();
     }
-    if (tid == 0) {
-        const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long*>(&ctrl->exitCounter), 1ull);
-        if (old == gridDim.x - 1) {
-            st_release(&ctrl->tileCounter, 0);
-            st_release(&ctrl->exitCounter, 0);
-        }
-    }
+    if (tid == 0) scan_exit(ctrl);
 }
 
 // Counting-sort scatter of (id, q) pairs into grouped query lists; leaves cnt zeroed.
@@ -308,8 +303,8 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint64_t* __restri
 
 // In-place exclusive scan of a u64 array whose length is 256 * ntiles(dn).
 __global__ __launch_bounds__(kScanThreads) void k_scan_hist(uint64_t* __restrict__ a, const uint64_t* dn,
-                                                            uint64_t cap, uint64_t* status, ScanCtrl* ctrl,
-                                                            uint32_t epoch) {
+                                                            uint64_t cap, uint64_t* status, ScanCtrl* ctrl) {
+    const uint32_t epoch = launch_epoch(ctrl);
     __shared__ uint64_t s_tile;
     __shared__ uint64_t s_w[kScanThreads / kWave];
     __shared__ uint64_t s_ex;
@@ -345,13 +340,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_hist(uint64_t* __restrict
         if (i < n) a[i] = p;
         __syncthreads();
     }
-    if (tid == 0) {
-        const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long*>(&ctrl->exitCounter), 1ull);
-        if (old == gridDim.x - 1) {
-            st_release(&ctrl->tileCounter, 0);
-            st_release(&ctrl->exitCounter, 0);
-        }
-    }
+    if (tid == 0) scan_exit(ctrl);
 }
 
 // Stable scatter: within a tile, keys keep their order per digit (round, then
@@ -460,13 +449,12 @@ int radix_sort(tblaeq_ctx* ctx, uint64_t* a, uint64_t* b, const uint64_t* dn, ui
             TimedScope ts(ctx, kKBatch0 + 4);
             hipLaunchKernelGGL(k_rs_hist, dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dn, cap, shift, hist);
         }
-        uint32_t epoch;
-        int rc = scan_prepare(ctx, histTiles, &epoch);
+        int rc = scan_prepare(ctx, histTiles);
         if (rc) return rc;
         {
             TimedScope ts(ctx, kKBatch0 + 4);
             hipLaunchKernelGGL(k_scan_hist, dim3(scan_launch_grid(ctx, histTiles)), dim3(kScanThreads), 0,
-                               ctx->stream, hist, dn, cap, ctx->status, ctx->ctrl, epoch);
+                               ctx->stream, hist, dn, cap, ctx->status, ctx->ctrl);
         }
         {
             TimedScope ts(ctx, kKBatch0 + 4);
@@ -611,6 +599,8 @@ extern "C" int tblaeq_batch_create(tblaeq_ctx* ctx, const tblaeq_level* levels, 
 extern "C" int tblaeq_batch_destroy(tblaeq_batch* b) {
     if (!b) return TBLAEQ_OK;
     if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+    if (b->rangeExec) (void)hipGraphExecDestroy(b->rangeExec);
+    if (b->rangeGraph) (void)hipGraphDestroy(b->rangeGraph);
     for (void* p : b->owned) (void)hipFree(p);
     delete b;
     return TBLAEQ_OK;
@@ -651,6 +641,11 @@ extern "C" int tblaeq_batch_reserve(tblaeq_batch* b, uint64_t max_queries, uint6
         b->hitCap = hit_cap;
     }
     b->maxQ = std::max(b->maxQ, max_queries);
+    // size the look-back status array for every scan a batch runs, so that a
+    // captured batch never allocates
+    const uint64_t histTiles = (b->histCap + kScanThreads - 1) / kScanThreads;
+    const uint64_t meshTiles = (b->maxMesh + kScanThreads - 1) / kScanThreads;
+    if ((rc = scan_prepare(b->ctx, std::max(histTiles, meshTiles)))) return rc;
     TB_HIP(hipStreamSynchronize(b->ctx->stream));
     return TBLAEQ_OK;
 }
@@ -699,13 +694,11 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
         // group the survivors of level l by cluster
         const uint64_t mesh = l == 0 ? b->count[0] : b->count[l];
         const uint64_t tiles = (mesh + kScanThreads - 1) / kScanThreads;
-        uint32_t epoch;
-        if ((rc = scan_prepare(ctx, tiles, &epoch))) return rc;
+        if ((rc = scan_prepare(ctx, tiles))) return rc;
         {
             TimedScope ts(ctx, kKBatch0 + 2);
             hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream,
-                               b->cnt, mesh, b->off, b->items, b->itemCap, ctl + 1, ctl + 3, ctx->status, ctx->ctrl,
-                               epoch);
+                               b->cnt, mesh, b->off, b->items, b->itemCap, ctl + 1, ctl + 3, ctx->status, ctx->ctrl);
         }
         {
             TimedScope ts(ctx, kKBatch0 + 3);
@@ -751,5 +744,50 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     }
     hipLaunchKernelGGL(k_copy_status, dim3(1), dim3(64), 0, ctx->stream, ctl, d_status);
     TB_HIP(hipGetLastError());
+    return TBLAEQ_OK;
+}
+
+extern "C" int tblaeq_batch_range_graph(tblaeq_batch* b, const double* d_boxes, uint64_t nq, uint64_t* d_out_off,
+                                        uint64_t* d_out_ids, uint64_t out_cap, uint64_t* d_status) {
+    if (!b) return TBLAEQ_ERR_INVALID;
+    tblaeq_ctx* ctx = b->ctx;
+    const bool same = b->rangeExec && b->gBoxes == d_boxes && b->gNq == nq && b->gOff == d_out_off &&
+                      b->gIds == d_out_ids && b->gCap == out_cap && b->gStatus == d_status;
+    if (!same) {
+        if (b->rangeExec) TB_HIP(hipGraphExecDestroy(b->rangeExec));
+        if (b->rangeGraph) TB_HIP(hipGraphDestroy(b->rangeGraph));
+        b->rangeExec = nullptr;
+        b->rangeGraph = nullptr;
+        const bool timing = ctx->timing;
+        ctx->timing = false;  // no event records inside the graph
+        const uint64_t before = ctx->scanLaunches;
+        TB_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int rc = tblaeq_batch_range(b, d_boxes, nq, d_out_off, d_out_ids, out_cap, d_status, nullptr, nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        ctx->timing = timing;
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        TB_HIP(e);
+        b->rangeGraph = g;
+        TB_HIP(hipGraphInstantiate(&b->rangeExec, g, nullptr, nullptr, 0));
+        b->gScanLaunches = ctx->scanLaunches - before;
+        ctx->scanLaunches = before;
+        b->gBoxes = d_boxes;
+        b->gNq = nq;
+        b->gOff = d_out_off;
+        b->gIds = d_out_ids;
+        b->gCap = out_cap;
+        b->gStatus = d_status;
+    }
+    // keep the host count of scan launches (epoch wrap protection) in step with replays
+    for (uint64_t i = 0; i < b->gScanLaunches; ++i) {
+        int rc = scan_prepare(ctx, 1);
+        if (rc) return rc;
+    }
+    TimedScope ts(ctx, kKBatch0 + 6);
+    TB_HIP(hipGraphLaunch(b->rangeExec, ctx->stream));
     return TBLAEQ_OK;
 }

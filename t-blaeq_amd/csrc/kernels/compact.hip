@@ -34,7 +34,6 @@ struct CompactArgs {
     uint64_t* dOutNnz;
     uint64_t* status;
     ScanCtrl* ctrl;
-    uint32_t epoch;
     uint64_t maxTiles;  // capacity of the status array (never exceeded, whatever the device count says)
     int radiusByPos;  // radius[i] belongs to grid position i (pre-gathered tile)
 };
@@ -62,6 +61,7 @@ __global__ __launch_bounds__(kScanThreads) void k_compact(CompactArgs A) {
     if (ntiles > A.maxTiles) ntiles = A.maxTiles;
     const double ub = (MODE == kModeKnn) ? *A.ub : 0.0;
     const uint64_t ltmask = (1ull << lane) - 1ull;
+    const uint32_t epoch = launch_epoch(A.ctrl);
 
     for (;;) {
         if (tid == 0) s_tile = atomicAdd(reinterpret_cast<unsigned long long*>(&A.ctrl->tileCounter), 1ull);
@@ -116,18 +116,18 @@ __global__ __launch_bounds__(kScanThreads) void k_compact(CompactArgs A) {
             uint64_t ex0 = 0, ex1 = 0;
             if (tile == 0) {
                 if (lane == 0) {
-                    st_release(st + 2, status_word(A.epoch, c));
-                    st_release(st + 3, status_word(A.epoch, s));
+                    st_release(st + 2, status_word(epoch, c));
+                    st_release(st + 3, status_word(epoch, s));
                 }
             } else {
                 if (lane == 0) {
-                    st_release(st + 0, status_word(A.epoch, c));
-                    st_release(st + 1, status_word(A.epoch, s));
+                    st_release(st + 0, status_word(epoch, c));
+                    st_release(st + 1, status_word(epoch, s));
                 }
-                lookback2(A.status, A.epoch, tile, lane, ex0, ex1);
+                lookback2(A.status, epoch, tile, lane, ex0, ex1);
                 if (lane == 0) {
-                    st_release(st + 2, status_word(A.epoch, ex0 + c));
-                    st_release(st + 3, status_word(A.epoch, ex1 + s));
+                    st_release(st + 2, status_word(epoch, ex0 + c));
+                    st_release(st + 3, status_word(epoch, ex1 + s));
                 }
             }
             if (lane == 0) {
@@ -168,25 +168,17 @@ __global__ __launch_bounds__(kScanThreads) void k_compact(CompactArgs A) {
         __syncthreads();
     }
 
-    if (tid == 0) {
-        const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long*>(&A.ctrl->exitCounter), 1ull);
-        if (old == gridDim.x - 1) {
-            st_release(&A.ctrl->tileCounter, 0);
-            st_release(&A.ctrl->exitCounter, 0);
-        }
-    }
+    if (tid == 0) scan_exit(A.ctrl);
 }
 
 // Launch helper shared by the C entry points.  `bound` sizes the grid; the
 // element count comes from dCount (device) or hostCount.
 int launch_compact(tblaeq_ctx* ctx, int mode, CompactArgs A, uint64_t bound) {
     const uint64_t tiles = bound == 0 ? 1 : (bound + kScanTile - 1) / kScanTile;
-    uint32_t epoch = 0;
-    int rc = scan_prepare(ctx, tiles, &epoch);
+    int rc = scan_prepare(ctx, tiles);
     if (rc) return rc;
     A.status = ctx->status;
     A.ctrl = ctx->ctrl;
-    A.epoch = epoch;
     A.maxTiles = tiles;
     const uint64_t cap = static_cast<uint64_t>(ctx->cuCount) * 8;
     const unsigned grid = static_cast<unsigned>(tiles < cap ? tiles : cap);

@@ -75,4 +75,23 @@ __device__ __forceinline__ void tile_prefix2(uint64_t* status, uint32_t epoch, u
     }
 }
 
+// Epoch of the current scan launch.  Every workgroup reads it when it starts;
+// the last workgroup to leave advances it, so launches need no host-provided
+// epoch and can be captured in a graph and replayed.
+__device__ __forceinline__ uint32_t launch_epoch(ScanCtrl* ctrl) {
+    return static_cast<uint32_t>(ld_acquire(&ctrl->seq) & 0x7FFFFFF);
+}
+
+// Called by thread 0 of every workgroup after its last tile: the last one to
+// leave resets the ticket counters and advances the epoch.
+__device__ __forceinline__ void scan_exit(ScanCtrl* ctrl) {
+    const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long*>(&ctrl->exitCounter), 1ull);
+    if (old == gridDim.x - 1) {
+        const uint64_t next = (ld_acquire(&ctrl->seq) + 1) & 0x7FFFFFF;
+        st_release(&ctrl->seq, next);
+        st_release(&ctrl->tileCounter, 0);
+        st_release(&ctrl->exitCounter, 0);
+    }
+}
+
 }  // namespace tblaeq

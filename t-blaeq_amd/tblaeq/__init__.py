@@ -74,6 +74,8 @@ SIGNATURES = {
     "tblaeq_handler_batch_results": (C.c_int, [vp, u64p, u64p, f64p, u64p]),
     "tblaeq_handler_batch_stats": (C.c_int, [vp, u64p, u64p]),
     "tblaeq_handler_batch_free": (C.c_int, [vp]),
+    "tblaeq_handler_batch_use_graph": (C.c_int, [vp, C.c_int]),
+    "tblaeq_batch_range_graph": (C.c_int, [vp, vp, C.c_uint64, vp, vp, C.c_uint64, vp]),
 }
 
 
@@ -203,6 +205,9 @@ class Handler:
             self._bK = int(k.max())
             check(lib().tblaeq_handler_batch_set_queries(self.h, 1, _p(q, f64p), _p(k, u64p), q.shape[0]),
                   "batch_set")
+
+    def batch_use_graph(self, on=True):
+        check(lib().tblaeq_handler_batch_use_graph(self.h, int(on)), "batch_use_graph")
 
     def batch_run(self, stats=False):
         t = C.c_uint64(0)
