@@ -174,7 +174,8 @@ int tblaeq_batch_create(tblaeq_ctx* ctx, const tblaeq_level* levels, uint64_t nl
                         tblaeq_batch** out);
 int tblaeq_batch_destroy(tblaeq_batch* b);
 /* Scratch for up to max_queries per batch, pair_cap (cluster, query) survivor
- * pairs per level and hit_cap range hits per batch.  Allocates; call before
+ * pairs per level and hit_cap range hits per batch (KNN batches use
+ * min(pair_cap, hit_cap) candidate records).  Allocates; call before
  * timing or graph capture. */
 int tblaeq_batch_reserve(tblaeq_batch* b, uint64_t max_queries, uint64_t pair_cap, uint64_t hit_cap);
 /* nq range boxes (device, rows lo|hi).  Per-query hit ids, ascending, in
@@ -194,12 +195,27 @@ int tblaeq_batch_range_graph(tblaeq_batch* b, const double* d_boxes, uint64_t nq
 /* nq KNN points (device, rows of D), K per query in d_K (each <= Kmax).
  * Top-K ids and squared distances by (distance, id) in rows of Kmax;
  * d_out_n[q] = number written (min(K, candidates)).  Same traversal (STEP per
- * level) as the per-query path.  Requires Kmax + empty clusters + 1 <= 256 and
- * at most 8 levels (else TBLAEQ_ERR_UNSUPPORTED).  d_level_surv / d_fine_count
- * as for tblaeq_batch_range (written, not accumulated). */
+ * level) as the per-query path.  d_status[0] = candidate records refined
+ * cluster-major, d_status[1] = overflow flags (non-zero: a capacity of
+ * tblaeq_batch_reserve was exceeded, results invalid; split the batch).
+ * Requires Kmax + empty clusters + 1 <= 256 and at most 8 levels (else
+ * TBLAEQ_ERR_UNSUPPORTED).  d_level_surv / d_fine_count as for
+ * tblaeq_batch_range (written, not accumulated). */
 int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const uint64_t* d_K, uint64_t nq, uint64_t Kmax,
-                     uint64_t* d_out_ids, double* d_out_d2, uint64_t* d_out_n, uint64_t* d_level_surv,
-                     uint64_t* d_fine_count);
+                     uint64_t* d_out_ids, double* d_out_d2, uint64_t* d_out_n, uint64_t* d_status,
+                     uint64_t* d_level_surv, uint64_t* d_fine_count);
+
+/* Launch parameters of the batched engine (no effect on results, only on
+ * speed); tblaeq_batch_get_param reads the current value.
+ *   TBLAEQ_PARAM_KNN_SEED       last-level clusters expanded first to fix the
+ *                               certified radius, 1..256 (default 64)
+ *   TBLAEQ_PARAM_KNN_WG_PER_CU  KNN workgroups per CU, 1..16 (default 4)
+ *   TBLAEQ_PARAM_KNN_MODE       0: the last level's surviving clusters are
+ *                               grouped and refined cluster-major (default);
+ *                               1: each workgroup refines its own queries */
+enum { TBLAEQ_PARAM_KNN_SEED = 1, TBLAEQ_PARAM_KNN_WG_PER_CU = 2, TBLAEQ_PARAM_KNN_MODE = 3 };
+int tblaeq_batch_set_param(tblaeq_batch* b, int param, uint64_t value);
+int tblaeq_batch_get_param(const tblaeq_batch* b, int param, uint64_t* value);
 
 #ifdef __cplusplus
 }

@@ -38,6 +38,7 @@ struct tblaeq_batch {
     uint64_t* items = nullptr;   // (id << 20 | slice)
     uint64_t* hits = nullptr;    // (q << idBits | id)
     uint64_t* hitsTmp = nullptr;
+    uint64_t* recAux = nullptr;  // [hitCap] grouped KNN record ids
     uint64_t* hist = nullptr;    // radix histograms
     uint64_t histCap = 0;
     uint64_t* ctl = nullptr;     // [0] pairs [1] items [2] hits [3] overflow [4..] spare
@@ -53,9 +54,29 @@ struct tblaeq_batch {
     // knn scratch
     uint8_t* flags = nullptr;    // per query survivor flags of a middle level
     uint64_t flagCap = 0;
+    // launch parameters (tblaeq_batch_set_param)
+    uint64_t knnSeed = 64;       // clusters expanded before the certified radius is taken
+    uint64_t knnWgPerCu = 4;     // KNN workgroups per CU (grid = min(groups, CUs * this))
+    uint64_t knnMode = 0;        // 0 cluster-major refine, 1 query-major refine
 };
 
 namespace tblaeq {
 constexpr int kSliceBits = 20;
 constexpr int kQueryBits = 24;
+constexpr int kStepWaves = 4;  // waves per workgroup of the cluster-major kernels
+
+// ctl slots: [0] pairs [1] items [2] hits / records [3] overflow flags
+// (1 pairs, 2 items, 4 hits / records) [6], [7] scratch of count-only scans.
+
+// Groups the (cluster id << kQueryBits | q) pairs counted in b->cnt (mesh
+// clusters) by cluster: b->off, work items (cluster, 64-query slice) in
+// b->items with their count in ctl[1], query lists in b->qlist.  Leaves
+// b->cnt zeroed and resets the pair count ctl[0].
+int launch_grouping(tblaeq_batch* b, uint64_t mesh);
+// Exclusive scan of cnt[0..n) into off[0..n] (off[n] = total).
+int launch_count_scan(tblaeq_batch* b, const uint64_t* cnt, uint64_t n, uint64_t* off);
+// status[0] = ctl[2], status[1] = ctl[3].
+void launch_copy_status(tblaeq_batch* b, uint64_t* status);
+// Fills n words with v (a kernel, so that captured batches stay replayable).
+void launch_fill(tblaeq_batch* b, uint64_t* p, uint64_t v, uint64_t n);
 }  // namespace tblaeq

@@ -100,6 +100,15 @@ __device__ __forceinline__ double wave_max_f64(double v) {
     return v;
 }
 
+// Orders LDS writes of one wave before its later LDS reads by other lanes
+// (a wave's LDS operations execute in order; this keeps the compiler from
+// moving accesses across the point).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Per-launch control block for the single-pass scans (decoupled look-back).
 // Status words carry their own epoch so the tile-status arrays never need to be
 // cleared between launches: word = epoch(27b) | valid(1b) | value(36b).

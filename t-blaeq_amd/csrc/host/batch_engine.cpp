@@ -101,11 +101,28 @@ int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total) {
             hipCheck(hipMemsetAsync(st->dFine.data(), 0, st->nq * 8, s), "memset");
         }
         if (st->type == 1) {
-            int rc = tblaeq_batch_knn(st->b, st->dQ.data(), st->dK.data(), st->nq, st->Kmax, st->dIds.data(),
-                                      st->dD2.data(), st->dN.data(), stats ? st->dSurv.data() : nullptr,
-                                      stats ? st->dFine.data() : nullptr);
-            if (rc) return rc;
-            hipCheck(hipStreamSynchronize(s), "sync");
+            // split in halves while a capacity is exceeded; KNN rows are fixed-size
+            std::vector<std::pair<uint64_t, uint64_t>> todo{{0, st->nq}};
+            while (!todo.empty()) {
+                auto [a, n] = todo.back();
+                todo.pop_back();
+                int rc = tblaeq_batch_knn(st->b, st->dQ.data() + a * D, st->dK.data() + a, n, st->Kmax,
+                                          st->dIds.data() + a * st->Kmax, st->dD2.data() + a * st->Kmax,
+                                          st->dN.data() + a, st->dStatus.data(), stats ? st->dSurv.data() + a * L : nullptr,
+                                          stats ? st->dFine.data() + a : nullptr);
+                if (rc) return rc;
+                uint64_t status[2];
+                hipCheck(hipMemcpyAsync(status, st->dStatus.data(), 16, hipMemcpyDeviceToHost, s), "D2H");
+                hipCheck(hipStreamSynchronize(s), "sync");
+                if (status[1] != 0) {
+                    if (n == 1) {
+                        tblaeq::set_error("batched KNN: one query exceeds the engine capacities");
+                        return TBLAEQ_ERR_CAPACITY;
+                    }
+                    todo.push_back({a + n / 2, n - n / 2});
+                    todo.push_back({a, n / 2});
+                }
+            }
             if (total) {
                 std::vector<uint64_t> n(st->nq);
                 hipCheck(hipMemcpy(n.data(), st->dN.data(), st->nq * 8, hipMemcpyDeviceToHost), "D2H");
@@ -220,6 +237,15 @@ int tblaeq_handler_batch_use_graph(tblaeq_handler* h, int on) {
     if (!st) return TBLAEQ_ERR_INVALID;
     st->useGraph = on != 0;
     return TBLAEQ_OK;
+}
+
+int tblaeq_handler_batch_set_param(tblaeq_handler* h, int param, uint64_t value) {
+    BatchState* st = stateOf(h);
+    if (!st) {
+        tblaeq::set_error("tblaeq_handler_batch_set_param: call tblaeq_handler_batch_init first");
+        return TBLAEQ_ERR_INVALID;
+    }
+    return tblaeq_batch_set_param(st->b, param, value);
 }
 
 int tblaeq_handler_batch_free(tblaeq_handler* h) {

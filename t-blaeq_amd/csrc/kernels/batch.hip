@@ -82,10 +82,16 @@ struct RangeStepArgs {
     int level, L;
 };
 
+// Children are staged per wave in LDS, 64 at a time, with coalesced vector
+// loads (row stride DM); the per-child loop then reads them as wave-uniform
+// LDS broadcasts, so no lane waits on a scalar-cache miss per child.
+
 template <int DM, bool FINAL>
-__global__ __launch_bounds__(256) void k_range_step(RangeStepArgs A) {
+__global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs A) {
+    __shared__ double s_x[kStepWaves][kWave * DM];
+    __shared__ double s_r[kStepWaves][kWave];
     const int lane = threadIdx.x & (kWave - 1);
-    const uint64_t wpb = blockDim.x / kWave;
+    const uint64_t wpb = kStepWaves;
     const uint64_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint64_t gw = blockIdx.x * wpb + wave;
     const uint64_t nw = static_cast<uint64_t>(gridDim.x) * wpb;
@@ -93,6 +99,8 @@ __global__ __launch_bounds__(256) void k_range_step(RangeStepArgs A) {
     uint64_t nItems = A.level0 ? (A.nq + kWave - 1) / kWave : *A.nItems;
     if (!A.level0 && nItems > A.itemCap) nItems = A.itemCap;
     const uint64_t ltmask = (1ull << lane) - 1ull;
+    if (D != DM)
+        for (int t = lane; t < kWave * DM; t += kWave) s_x[wave][t] = 0.0;
     for (uint64_t item = gw; item < nItems; item += nw) {
         uint64_t jb, je, q = 0;
         bool valid;
@@ -124,38 +132,50 @@ __global__ __launch_bounds__(256) void k_range_step(RangeStepArgs A) {
             }
         }
         uint64_t mine = 0;
+        double* sx = s_x[wave];
+        double* sr = s_r[wave];
         for (uint64_t j0 = jb; j0 < je; j0 += kWave) {
             const int nc = static_cast<int>(je - j0 < static_cast<uint64_t>(kWave) ? je - j0 : kWave);
+            {
+                const double* src = A.coords + j0 * D;
+                const int tot = nc * D;
+                if (D == DM) {
+                    for (int t = lane; t < tot; t += kWave) sx[t] = src[t];
+                } else {
+                    for (int t = lane; t < tot; t += kWave) sx[(t / D) * DM + t % D] = src[t];
+                }
+                if (!FINAL && lane < nc) sr[lane] = A.radiusPos[j0 + lane];
+            }
+            const uint64_t myId = lane < nc ? A.ids[j0 + lane] : 0;
+            wave_lds_sync();
             uint64_t myMask = 0, total = 0;
             for (int k = 0; k < nc; ++k) {
-                const uint64_t j = j0 + k;
-                const double* x = A.coords + j * D;
-                bool hit;
-                if (FINAL) {
-                    bool in = true;
+                // branch-free: padding dimensions hold 0 in the children and
+                // in the boxes, which passes the box test and adds fma(0,0,d) = d
+                const double* x = sx + k * DM;
+                double xv[DM];
 #pragma unroll
-                    for (int d = 0; d < DM; ++d)
-                        if (d < D) in = in && (x[d] >= lo[d]) && (x[d] <= hi[d]);
-                    hit = valid && in;
+                for (int d = 0; d < DM; ++d) xv[d] = x[d];
+                bool hit = valid;
+                if (FINAL) {
+#pragma unroll
+                    for (int d = 0; d < DM; ++d) hit = hit & (xv[d] >= lo[d]) & (xv[d] <= hi[d]);
                 } else {
                     double dist = 0.0;
 #pragma unroll
                     for (int d = 0; d < DM; ++d) {
-                        if (d < D) {
-                            const double cv = x[d];
-                            double diff = 0.0;
-                            if (cv < lo[d]) diff = lo[d] - cv;
-                            else if (cv > hi[d]) diff = cv - hi[d];
-                            dist = __builtin_fma(diff, diff, dist);
-                        }
+                        const double below = lo[d] - xv[d], above = xv[d] - hi[d];
+                        const double diff = xv[d] < lo[d] ? below : (xv[d] > hi[d] ? above : 0.0);
+                        dist = __builtin_fma(diff, diff, dist);
                     }
-                    const double r = A.radiusPos[j];
-                    hit = valid && dist <= r * r;
+                    const double r = sr[k];
+                    hit = hit & (dist <= r * r);
                 }
                 const uint64_t m = __ballot(hit);
                 if (lane == k) myMask = m;
                 total += __popcll(m);
             }
+            wave_lds_sync();
             if (total == 0) continue;
             uint64_t base = 0;
             if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(A.outCount), total);
@@ -166,7 +186,6 @@ __global__ __launch_bounds__(256) void k_range_step(RangeStepArgs A) {
                 if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), FINAL ? 4ull : 1ull);
                 continue;
             }
-            const uint64_t myId = lane < nc ? A.ids[j0 + lane] : 0;
             uint64_t pos = base;
             for (int k = 0; k < nc; ++k) {
                 const uint64_t m = __shfl(myMask, k, kWave);
@@ -476,11 +495,11 @@ int bits_for(uint64_t v) {
 
 template <bool FINAL>
 int launch_range_step(tblaeq_ctx* ctx, const RangeStepArgs& A, uint64_t itemBound) {
-    uint64_t blocks = (itemBound + 3) / 4;
+    uint64_t blocks = (itemBound + kStepWaves - 1) / kStepWaves;
     const uint64_t cap = static_cast<uint64_t>(ctx->cuCount) * 16;
     if (blocks > cap) blocks = cap;
     if (blocks == 0) blocks = 1;
-    const dim3 g(static_cast<unsigned>(blocks)), b(256);
+    const dim3 g(static_cast<unsigned>(blocks)), b(kStepWaves * kWave);
     TimedScope ts(ctx, FINAL ? kKBatch0 + 1 : kKBatch0 + 0);
     if (A.D <= 4) hipLaunchKernelGGL((k_range_step<4, FINAL>), g, b, 0, ctx->stream, A);
     else if (A.D <= 8) hipLaunchKernelGGL((k_range_step<8, FINAL>), g, b, 0, ctx->stream, A);
@@ -488,6 +507,55 @@ int launch_range_step(tblaeq_ctx* ctx, const RangeStepArgs& A, uint64_t itemBoun
     else hipLaunchKernelGGL((k_range_step<32, FINAL>), g, b, 0, ctx->stream, A);
     TB_HIP(hipGetLastError());
     return TBLAEQ_OK;
+}
+
+int launch_grouping(tblaeq_batch* b, uint64_t mesh) {
+    tblaeq_ctx* ctx = b->ctx;
+    uint64_t* ctl = b->ctl;
+    const uint64_t tiles = (mesh + kScanThreads - 1) / kScanThreads;
+    int rc = scan_prepare(ctx, tiles);
+    if (rc) return rc;
+    {
+        TimedScope ts(ctx, kKBatch0 + 2);
+        hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream,
+                           b->cnt, mesh, b->off, b->items, b->itemCap, ctl + 1, ctl + 3, ctx->status, ctx->ctrl);
+    }
+    {
+        TimedScope ts(ctx, kKBatch0 + 3);
+        const uint64_t g = std::min<uint64_t>((b->pairCap + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 8);
+        hipLaunchKernelGGL(k_group_scatter, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream,
+                           b->pairs, ctl + 0, b->pairCap, mesh, b->cnt, b->off, b->qlist);
+    }
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, ctx->stream, ctl, 0ull, 1);
+    TB_HIP(hipGetLastError());
+    return TBLAEQ_OK;
+}
+
+int launch_count_scan(tblaeq_batch* b, const uint64_t* cnt, uint64_t n, uint64_t* off) {
+    tblaeq_ctx* ctx = b->ctx;
+    const uint64_t tiles = (n + kScanThreads - 1) / kScanThreads;
+    int rc = scan_prepare(ctx, tiles);
+    if (rc) return rc;
+    TimedScope ts(ctx, kKBatch0 + 2);
+    hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream, cnt, n,
+                       off, nullptr, 0ull, b->ctl + 6, b->ctl + 7, ctx->status, ctx->ctrl);
+    TB_HIP(hipGetLastError());
+    return TBLAEQ_OK;
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+void launch_copy_status(tblaeq_batch* b, uint64_t* status) {
+    hipLaunchKernelGGL(k_copy_status, dim3(1), dim3(64), 0, b->ctx->stream, b->ctl, status);
+}
+
+void launch_fill(tblaeq_batch* b, uint64_t* p, uint64_t v, uint64_t n) {
+    const uint64_t g = std::min<uint64_t>((n + 255) / 256, static_cast<uint64_t>(b->ctx->cuCount) * 8);
+    hipLaunchKernelGGL(k_fill_u64, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, b->ctx->stream, p, v, n);
 }
 
 template <class T>
@@ -634,6 +702,8 @@ extern "C" int tblaeq_batch_reserve(tblaeq_batch* b, uint64_t max_queries, uint6
         if (rc) return rc;
         b->hitsTmp = balloc<uint64_t>(b, hit_cap, &rc);
         if (rc) return rc;
+        b->recAux = balloc<uint64_t>(b, hit_cap, &rc);
+        if (rc) return rc;
         const uint64_t tiles = (hit_cap + kRsTile - 1) / kRsTile;
         b->histCap = 256 * tiles;
         b->hist = balloc<uint64_t>(b, b->histCap, &rc);
@@ -692,21 +762,7 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
         const uint64_t bound = l == 0 ? (nq + kWave - 1) / kWave : b->itemCap;
         if ((rc = launch_range_step<false>(ctx, A, bound))) return rc;
         // group the survivors of level l by cluster
-        const uint64_t mesh = l == 0 ? b->count[0] : b->count[l];
-        const uint64_t tiles = (mesh + kScanThreads - 1) / kScanThreads;
-        if ((rc = scan_prepare(ctx, tiles))) return rc;
-        {
-            TimedScope ts(ctx, kKBatch0 + 2);
-            hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream,
-                               b->cnt, mesh, b->off, b->items, b->itemCap, ctl + 1, ctl + 3, ctx->status, ctx->ctrl);
-        }
-        {
-            TimedScope ts(ctx, kKBatch0 + 3);
-            const uint64_t g = std::min<uint64_t>((b->pairCap + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 8);
-            hipLaunchKernelGGL(k_group_scatter, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream,
-                               b->pairs, ctl + 0, b->pairCap, mesh, b->cnt, b->off, b->qlist);
-        }
-        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, ctx->stream, ctl, 0ull, 1);
+        if ((rc = launch_grouping(b, l == 0 ? b->count[0] : b->count[l]))) return rc;
         TB_HIP(hipGetLastError());
     }
     // final step: exact filter of the finest candidates

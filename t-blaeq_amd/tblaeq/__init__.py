@@ -67,7 +67,7 @@ SIGNATURES = {
     "tblaeq_batch_destroy": (C.c_int, [vp]),
     "tblaeq_batch_reserve": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint64]),
     "tblaeq_batch_range": (C.c_int, [vp, vp, C.c_uint64, vp, vp, C.c_uint64, vp, vp, vp]),
-    "tblaeq_batch_knn": (C.c_int, [vp, vp, vp, C.c_uint64, C.c_uint64, vp, vp, vp, vp, vp]),
+    "tblaeq_batch_knn": (C.c_int, [vp, vp, vp, C.c_uint64, C.c_uint64, vp, vp, vp, vp, vp, vp]),
     "tblaeq_handler_batch_init": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint64]),
     "tblaeq_handler_batch_set_queries": (C.c_int, [vp, C.c_int, f64p, u64p, C.c_uint64]),
     "tblaeq_handler_batch_run": (C.c_int, [vp, C.c_int, u64p]),
@@ -76,7 +76,14 @@ SIGNATURES = {
     "tblaeq_handler_batch_free": (C.c_int, [vp]),
     "tblaeq_handler_batch_use_graph": (C.c_int, [vp, C.c_int]),
     "tblaeq_batch_range_graph": (C.c_int, [vp, vp, C.c_uint64, vp, vp, C.c_uint64, vp]),
+    "tblaeq_batch_set_param": (C.c_int, [vp, C.c_int, C.c_uint64]),
+    "tblaeq_batch_get_param": (C.c_int, [vp, C.c_int, u64p]),
+    "tblaeq_handler_batch_set_param": (C.c_int, [vp, C.c_int, C.c_uint64]),
 }
+
+PARAM_KNN_SEED = 1
+PARAM_KNN_WG_PER_CU = 2
+PARAM_KNN_MODE = 3
 
 
 class TblaeqError(RuntimeError):
@@ -208,6 +215,10 @@ class Handler:
 
     def batch_use_graph(self, on=True):
         check(lib().tblaeq_handler_batch_use_graph(self.h, int(on)), "batch_use_graph")
+
+    def batch_set_param(self, param, value):
+        """Launch parameter of the batched engine (PARAM_*); results do not depend on it."""
+        check(lib().tblaeq_handler_batch_set_param(self.h, int(param), int(value)), "batch_set_param")
 
     def batch_run(self, stats=False):
         t = C.c_uint64(0)

@@ -144,3 +144,43 @@ def test_batch_matches_plain_c1():
         pid, pd2 = kplain.refined(i)
         assert np.array_equal(kres[i][0], pid)
         assert np.array_equal(kres[i][1].view(np.uint64), pd2.view(np.uint64))
+
+
+@pytest.mark.parametrize("seed,wg", [(1, 1), (7, 16), (256, 2)])
+def test_batch_knn_launch_params(seed, wg):
+    """Seed cluster count and grid shape change only the work done, never the answer."""
+    h, o = _make(dict(CFG, N=30000, is_int=True, seed=9), max_q=128)
+    h.batch_set_param(tblaeq.PARAM_KNN_SEED, seed)
+    h.batch_set_param(tblaeq.PARAM_KNN_WG_PER_CU, wg)
+    data = h.data()
+    pts = np.concatenate([data[:40] + 0.5, W.knn_points(data, 40, 77, h.mesh_sizes, CFG["vmin"], CFG["vmax"])])
+    Ks = np.random.default_rng(seed).choice([1, 10, 64, 150], len(pts))
+    _check_knn_batch(h, o, pts, Ks)
+
+
+def test_batch_param_rejects_bad_values(idx):
+    h, _ = idx
+    with pytest.raises(tblaeq.TblaeqError):
+        h.batch_set_param(tblaeq.PARAM_KNN_SEED, 0)
+    with pytest.raises(tblaeq.TblaeqError):
+        h.batch_set_param(tblaeq.PARAM_KNN_WG_PER_CU, 17)
+    with pytest.raises(tblaeq.TblaeqError):
+        h.batch_set_param(99, 1)
+
+
+def test_batch_knn_query_major_mode():
+    """The query-major refine (every workgroup expands its own clusters) gives the same answers."""
+    h, o = _make(dict(CFG, N=30000, seed=12), max_q=128)
+    h.batch_set_param(tblaeq.PARAM_KNN_MODE, 1)
+    data = h.data()
+    pts = W.knn_points(data, 70, 5, h.mesh_sizes, CFG["vmin"], CFG["vmax"], noise=0.3)
+    Ks = np.random.default_rng(6).choice([0, 1, 10, 100], len(pts))
+    _check_knn_batch(h, o, pts, Ks)
+
+
+def test_batch_knn_overflow_splits():
+    """Pair and record capacities far below the batch's needs: the driver splits and still matches."""
+    h, o = _make(dict(CFG, N=20000, seed=4), max_q=128, pair_cap=600, hit_cap=300)
+    data = h.data()
+    pts = W.knn_points(data, 48, 8, h.mesh_sizes, CFG["vmin"], CFG["vmax"], noise=0.2)
+    _check_knn_batch(h, o, pts, np.full(len(pts), 20))
