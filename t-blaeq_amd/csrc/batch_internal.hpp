@@ -33,6 +33,10 @@ struct tblaeq_batch {
     uint64_t maxQ = 0, pairCap = 0, hitCap = 0, itemCap = 0, maxMesh = 0;
     uint64_t* pairs = nullptr;   // (id << 24 | q)
     uint32_t* qlist = nullptr;   // grouped query lists
+    uint32_t* qlist2 = nullptr;  // second set of groups (range levels alternate)
+    uint64_t* off2 = nullptr;
+    uint64_t* items2 = nullptr;
+    float* aabb = nullptr;       // [count[L-1] * 2D] bounding box of each last-level position's points
     uint64_t* cnt = nullptr;     // [maxMesh]  per-cluster pair counts (kept zeroed between steps)
     uint64_t* off = nullptr;     // [maxMesh+1]
     uint64_t* items = nullptr;   // (id << 20 | slice)
@@ -66,7 +70,8 @@ constexpr int kQueryBits = 24;
 constexpr int kStepWaves = 4;  // waves per workgroup of the cluster-major kernels
 
 // ctl slots: [0] pairs [1] items [2] hits / records [3] overflow flags
-// (1 pairs, 2 items, 4 hits / records) [6], [7] scratch of count-only scans.
+// (1 pairs, 2 items, 4 hits / records) [4] items of the second group set
+// [6], [7] scratch of count-only scans.
 
 // Groups the (cluster id << kQueryBits | q) pairs counted in b->cnt (mesh
 // clusters) by cluster: b->off, work items (cluster, 64-query slice) in

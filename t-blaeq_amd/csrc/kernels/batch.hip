@@ -51,6 +51,23 @@ __global__ void k_set_u64(uint64_t* p, uint64_t v, int n) {
 }
 
 // ------------------------------------------------------------- range steps
+//
+// Work item = (cluster c of the previous level, slice of <= 64 queries that kept
+// c), or at level 0 a slice of queries over all coarsest points.  Lane = query,
+// box in registers.  The cluster's children are staged per wave in LDS, 64 at a
+// time, with coalesced vector loads (row stride DM; padding dimensions hold 0
+// in the children and in the boxes, which passes the box test and adds
+// fma(0,0,d) = d to the distance), then tested branch-free against every lane.
+//
+// Survivors of a prune are grouped by cluster without materialising pairs: the
+// count pass adds each child's survivor mask popcount to cnt[id]; after the
+// group scan the write pass recomputes the masks and writes the queries
+// straight into the next level's per-cluster query lists (cursor = cnt counted
+// back down to zero).  The last prune additionally drops (cluster, query) pairs
+// whose box misses the bounding box of the cluster's points: such a cluster
+// holds no hit.  Survivor statistics still count every ball survivor.
+
+enum RangeMode { kCount = 0, kWrite = 1, kFinal = 2 };
 
 struct RangeStepArgs {
     int D;
@@ -63,44 +80,72 @@ struct RangeStepArgs {
     uint64_t itemCap;
     const uint64_t* off;
     const uint32_t* qlist;
+    uint64_t qcap;        // entries of qlist
     const uint64_t* colResPrev;
     // candidates
     uint64_t count0;
     const double* coords;
     const double* radiusPos;
+    const uint64_t* weightPos;   // child counts (last prune: candidate superset size)
+    const float* aabb;           // last prune: per position lo[D] | hi[D], rounded outward
     const uint64_t* ids;
-    // output
-    uint64_t* out;      // pairs (prune) or hits (final)
+    // next level's groups (count / write)
+    uint64_t* cnt;
+    const uint64_t* offNext;
+    uint32_t* qlistNext;
+    // hits (final)
+    uint64_t* out;
     uint64_t cap;
     uint64_t* outCount;
-    uint64_t* cnt;
     uint64_t* overflow;
     int idBits;
-    // optional per-query statistics
+    // optional per-query statistics (count pass)
     uint64_t* survStat;  // [nq * L]
     uint64_t* fineStat;  // [nq]
     int level, L;
 };
 
-// Children are staged per wave in LDS, 64 at a time, with coalesced vector
-// loads (row stride DM); the per-child loop then reads them as wave-uniform
-// LDS broadcasts, so no lane waits on a scalar-cache miss per child.
+constexpr int kHitStage = 512;
 
-template <int DM, bool FINAL>
+// Appends a wave's staged hits to the global hit list (entries past the
+// capacity are dropped and flagged).
+__device__ __forceinline__ void flush_hits(const RangeStepArgs& A, const uint64_t* stage, uint64_t n, int lane) {
+    if (n == 0) return;
+    wave_lds_sync();
+    uint64_t base = 0;
+    if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(A.outCount), n);
+    base = __shfl(base, 0, kWave);
+    for (uint64_t o = lane; o < n; o += kWave)
+        if (base + o < A.cap) A.out[base + o] = stage[o];
+    if (lane == 0 && base + n > A.cap) atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 4ull);
+    wave_lds_sync();
+}
+
+template <int DM, int MODE>
 __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs A) {
+    constexpr bool FINAL = MODE == kFinal;
+    __shared__ uint64_t s_hit[kStepWaves][FINAL ? kHitStage : 1];
+    uint64_t staged = 0;
     __shared__ double s_x[kStepWaves][kWave * DM];
-    __shared__ double s_r[kStepWaves][kWave];
+    __shared__ double s_r[kStepWaves][FINAL ? 1 : kWave];
+    __shared__ float s_ab[kStepWaves][FINAL ? 1 : kWave * 2 * DM];
     const int lane = threadIdx.x & (kWave - 1);
-    const uint64_t wpb = kStepWaves;
     const uint64_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const uint64_t gw = blockIdx.x * wpb + wave;
-    const uint64_t nw = static_cast<uint64_t>(gridDim.x) * wpb;
+    const uint64_t gw = blockIdx.x * static_cast<uint64_t>(kStepWaves) + wave;
+    const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kStepWaves;
     const int D = A.D;
     uint64_t nItems = A.level0 ? (A.nq + kWave - 1) / kWave : *A.nItems;
     if (!A.level0 && nItems > A.itemCap) nItems = A.itemCap;
     const uint64_t ltmask = (1ull << lane) - 1ull;
-    if (D != DM)
-        for (int t = lane; t < kWave * DM; t += kWave) s_x[wave][t] = 0.0;
+    double* sx = s_x[wave];
+    double* sr = s_r[wave];
+    float* sab = s_ab[wave];
+    const bool useBox = !FINAL && A.aabb != nullptr;
+    if (D != DM) {
+        for (int t = lane; t < kWave * DM; t += kWave) sx[t] = 0.0;
+        if (!FINAL)
+            for (int t = lane; t < kWave * 2 * DM; t += kWave) sab[t] = 0.0f;
+    }
     for (uint64_t item = gw; item < nItems; item += nw) {
         uint64_t jb, je, q = 0;
         bool valid;
@@ -116,12 +161,13 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
             const uint64_t start = (it & ((1ull << kSliceBits) - 1)) * kWave;
             const uint64_t g0 = A.off[c];
             const uint64_t gn = A.off[c + 1] - g0;
-            valid = start + lane < gn;
+            valid = start + lane < gn && g0 + start + lane < A.qcap;
             q = valid ? A.qlist[g0 + start + lane] : 0;
             jb = A.colResPrev[c];
             je = A.colResPrev[c + 1];
         }
         double lo[DM], hi[DM];
+        float flo[DM], fhi[DM];
 #pragma unroll
         for (int d = 0; d < DM; ++d) {
             lo[d] = 0.0;
@@ -130,10 +176,10 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
                 lo[d] = A.boxes[q * 2 * D + d];
                 hi[d] = A.boxes[q * 2 * D + D + d];
             }
+            flo[d] = __double2float_rd(lo[d]);
+            fhi[d] = __double2float_ru(hi[d]);
         }
-        uint64_t mine = 0;
-        double* sx = s_x[wave];
-        double* sr = s_r[wave];
+        uint64_t mine = 0, fine = 0;
         for (uint64_t j0 = jb; j0 < je; j0 += kWave) {
             const int nc = static_cast<int>(je - j0 < static_cast<uint64_t>(kWave) ? je - j0 : kWave);
             {
@@ -145,21 +191,28 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
                     for (int t = lane; t < tot; t += kWave) sx[(t / D) * DM + t % D] = src[t];
                 }
                 if (!FINAL && lane < nc) sr[lane] = A.radiusPos[j0 + lane];
+                if (useBox) {
+                    const float* bsrc = A.aabb + j0 * 2 * D;
+                    const int btot = nc * 2 * D;
+                    for (int t = lane; t < btot; t += kWave) {
+                        const int row = t / (2 * D), rem = t % (2 * D);
+                        sab[row * 2 * DM + (rem < D ? rem : DM + rem - D)] = bsrc[t];
+                    }
+                }
             }
             const uint64_t myId = lane < nc ? A.ids[j0 + lane] : 0;
             wave_lds_sync();
-            uint64_t myMask = 0, total = 0;
+            uint64_t myMask = 0, myBall = 0;
             for (int k = 0; k < nc; ++k) {
-                // branch-free: padding dimensions hold 0 in the children and
-                // in the boxes, which passes the box test and adds fma(0,0,d) = d
                 const double* x = sx + k * DM;
                 double xv[DM];
 #pragma unroll
                 for (int d = 0; d < DM; ++d) xv[d] = x[d];
-                bool hit = valid;
                 if (FINAL) {
+                    uint64_t m = __ballot(valid);
 #pragma unroll
-                    for (int d = 0; d < DM; ++d) hit = hit & (xv[d] >= lo[d]) & (xv[d] <= hi[d]);
+                    for (int d = 0; d < DM; ++d) m &= __ballot(xv[d] >= lo[d]) & __ballot(xv[d] <= hi[d]);
+                    if (lane == k) myMask = m;
                 } else {
                     double dist = 0.0;
 #pragma unroll
@@ -169,45 +222,104 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
                         dist = __builtin_fma(diff, diff, dist);
                     }
                     const double r = sr[k];
-                    hit = hit & (dist <= r * r);
+                    const bool ball = valid && dist <= r * r;
+                    bool keep = ball;
+                    if (useBox) {
+                        const float* ab = sab + k * 2 * DM;
+#pragma unroll
+                        for (int d = 0; d < DM; ++d) keep = keep & (ab[DM + d] >= flo[d]) & (ab[d] <= fhi[d]);
+                    }
+                    const uint64_t m = __ballot(keep);
+                    if (lane == k) myMask = m;
+                    if (MODE == kCount) {
+                        const uint64_t mb = useBox ? __ballot(ball) : m;
+                        if (lane == k) myBall = mb;
+                    }
                 }
-                const uint64_t m = __ballot(hit);
-                if (lane == k) myMask = m;
-                total += __popcll(m);
             }
             wave_lds_sync();
-            if (total == 0) continue;
-            uint64_t base = 0;
-            if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(A.outCount), total);
-            base = __shfl(base, 0, kWave);
-            if (base + total > A.cap) {
-                // mark the unwritten part of the allocation so no consumer reads stale entries
-                for (uint64_t o = base + lane; o < A.cap; o += kWave) A.out[o] = ~0ull;
-                if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), FINAL ? 4ull : 1ull);
+            if (MODE == kCount) {
+                // ball survivors per lane for the statistics; kept pairs per cluster
+                if (A.survStat || A.fineStat) {
+                    const uint64_t myW = (A.weightPos && lane < nc) ? A.weightPos[j0 + lane] : 0;
+                    for (int k = 0; k < nc; ++k) {
+                        const uint64_t mb = __shfl(myBall, k, kWave);
+                        const uint64_t w = __shfl(myW, k, kWave);
+                        if ((mb >> lane) & 1ull) {
+                            ++mine;
+                            fine += w;
+                        }
+                    }
+                }
+                if (lane < nc && myMask)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[myId]),
+                              static_cast<unsigned long long>(__popcll(myMask)));
                 continue;
             }
-            uint64_t pos = base;
+            if (MODE == kWrite) {
+                uint64_t myBase = 0, myOff = 0;
+                if (lane < nc && myMask) {
+                    const uint64_t pc = __popcll(myMask);
+                    myBase = atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[myId]), 0ull - pc) - pc;
+                    myOff = A.offNext[myId];
+                }
+                for (int k = 0; k < nc; ++k) {
+                    const uint64_t m = __shfl(myMask, k, kWave);
+                    const uint64_t base = __shfl(myOff, k, kWave) + __shfl(myBase, k, kWave);
+                    if (m == 0) continue;
+                    if ((m >> lane) & 1ull) {
+                        const uint64_t o = base + __popcll(m & ltmask);
+                        if (o < A.cap) A.qlistNext[o] = static_cast<uint32_t>(q);
+                        else atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 1ull);
+                    }
+                }
+                continue;
+            }
+            // final: hits are staged per wave in LDS and appended to the global
+            // list in runs of up to kHitStage (one atomic per run)
+            uint64_t total = lane < nc ? __popcll(myMask) : 0;
+            for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o, kWave);
+            if (total == 0) continue;
+            if (staged + total > static_cast<uint64_t>(kHitStage)) {
+                flush_hits(A, s_hit[wave], staged, lane);
+                staged = 0;
+            }
+            if (total > static_cast<uint64_t>(kHitStage)) {
+                uint64_t base = 0;
+                if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(A.outCount), total);
+                base = __shfl(base, 0, kWave);
+                uint64_t pos = base;
+                for (int k = 0; k < nc; ++k) {
+                    const uint64_t m = __shfl(myMask, k, kWave);
+                    const uint64_t id = __shfl(myId, k, kWave);
+                    if (m == 0) continue;
+                    if ((m >> lane) & 1ull) {
+                        const uint64_t o = pos + __popcll(m & ltmask);
+                        if (o < A.cap) A.out[o] = (q << A.idBits) | id;
+                    }
+                    pos += __popcll(m);
+                }
+                if (lane == 0 && base + total > A.cap) atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 4ull);
+                continue;
+            }
+            uint64_t pos = staged;
             for (int k = 0; k < nc; ++k) {
                 const uint64_t m = __shfl(myMask, k, kWave);
-                if (m == 0) continue;
                 const uint64_t id = __shfl(myId, k, kWave);
-                if ((m >> lane) & 1ull) {
-                    const uint64_t o = pos + __popcll(m & ltmask);
-                    A.out[o] = FINAL ? ((q << A.idBits) | id) : ((id << kQueryBits) | q);
-                    ++mine;
-                }
-                if (!FINAL && lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[id]),
-                                                   static_cast<unsigned long long>(__popcll(m)));
+                if (m == 0) continue;
+                if ((m >> lane) & 1ull) s_hit[wave][pos + __popcll(m & ltmask)] = (q << A.idBits) | id;
                 pos += __popcll(m);
             }
+            staged += total;
         }
-        if (valid) {
-            if (A.survStat && !FINAL && mine)
+        if (MODE == kCount && valid) {
+            if (A.survStat && mine)
                 atomicAdd(reinterpret_cast<unsigned long long*>(&A.survStat[q * A.L + A.level]), mine);
-            if (A.fineStat && FINAL)
-                atomicAdd(reinterpret_cast<unsigned long long*>(&A.fineStat[q]), je - jb);
+            if (A.fineStat && A.level == A.L - 1 && fine)
+                atomicAdd(reinterpret_cast<unsigned long long*>(&A.fineStat[q]), fine);
         }
     }
+    if (FINAL) flush_hits(A, s_hit[wave], staged, lane);
 }
 
 // Group scan over the clusters of one level: off[i] = exclusive prefix of
@@ -303,24 +415,36 @@ constexpr int kRsTile = kRsThreads * kRsPer;
 
 __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint64_t* __restrict__ keys, const uint64_t* dn,
                                                         uint64_t cap, int shift, uint64_t* __restrict__ hist) {
-    __shared__ uint32_t lc[256];
+    __shared__ uint32_t lc[kRsThreads / kWave][256];
+    const int tid = threadIdx.x, wid = tid / kWave;
     uint64_t n = *dn;
     if (n > cap) n = cap;
     const uint64_t ntiles = (n + kRsTile - 1) / kRsTile;
     for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        lc[threadIdx.x] = 0;
+        for (int w = 0; w < kRsThreads / kWave; ++w) lc[w][tid] = 0;
         __syncthreads();
+        uint64_t k[kRsPer];
+#pragma unroll
         for (int r = 0; r < kRsPer; ++r) {
-            const uint64_t e = t * kRsTile + static_cast<uint64_t>(r) * kRsThreads + threadIdx.x;
-            if (e < n) atomicAdd(&lc[(keys[e] >> shift) & 0xFF], 1u);
+            const uint64_t e = t * kRsTile + static_cast<uint64_t>(r) * kRsThreads + tid;
+            k[r] = e < n ? keys[e] : ~0ull;
+        }
+#pragma unroll
+        for (int r = 0; r < kRsPer; ++r) {
+            const uint64_t e = t * kRsTile + static_cast<uint64_t>(r) * kRsThreads + tid;
+            if (e < n) atomicAdd(&lc[wid][(k[r] >> shift) & 0xFF], 1u);
         }
         __syncthreads();
-        hist[static_cast<uint64_t>(threadIdx.x) * ntiles + t] = lc[threadIdx.x];
+        uint32_t c = 0;
+        for (int w = 0; w < kRsThreads / kWave; ++w) c += lc[w][tid];
+        hist[static_cast<uint64_t>(tid) * ntiles + t] = c;
         __syncthreads();
     }
 }
 
-// In-place exclusive scan of a u64 array whose length is 256 * ntiles(dn).
+// In-place exclusive scan of a u64 array whose length is 256 * ntiles(dn);
+// each thread scans kRsPer consecutive entries.
+constexpr int kHsPer = 16;
 __global__ __launch_bounds__(kScanThreads) void k_scan_hist(uint64_t* __restrict__ a, const uint64_t* dn,
                                                             uint64_t cap, uint64_t* status, ScanCtrl* ctrl) {
     const uint32_t epoch = launch_epoch(ctrl);
@@ -331,15 +455,22 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_hist(uint64_t* __restrict
     uint64_t nk = *dn;
     if (nk > cap) nk = cap;
     const uint64_t n = 256 * ((nk + kRsTile - 1) / kRsTile);
-    const uint64_t ntiles = n == 0 ? 1 : (n + kScanThreads - 1) / kScanThreads;
+    constexpr uint64_t kTile = static_cast<uint64_t>(kScanThreads) * kHsPer;
+    const uint64_t ntiles = n == 0 ? 1 : (n + kTile - 1) / kTile;
     for (;;) {
         if (tid == 0) s_tile = atomicAdd(reinterpret_cast<unsigned long long*>(&ctrl->tileCounter), 1ull);
         __syncthreads();
         const uint64_t tile = s_tile;
         if (tile >= ntiles) break;
-        const uint64_t i = tile * kScanThreads + tid;
-        const uint64_t v = i < n ? a[i] : 0;
-        uint64_t x = v;
+        const uint64_t i0 = tile * kTile + static_cast<uint64_t>(tid) * kHsPer;
+        uint64_t v[kHsPer];
+        uint64_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < kHsPer; ++k) {
+            v[k] = i0 + k < n ? a[i0 + k] : 0;
+            sum += v[k];
+        }
+        uint64_t x = sum;
         for (int o = 1; o < kWave; o <<= 1) {
             const uint64_t y = __shfl_up(x, o, kWave);
             if (lane >= o) x += y;
@@ -354,37 +485,54 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_hist(uint64_t* __restrict
             if (lane == 0) s_ex = e0;
         }
         __syncthreads();
-        uint64_t p = s_ex + x - v;
+        uint64_t p = s_ex + x - sum;
         for (int w = 0; w < wid; ++w) p += s_w[w];
-        if (i < n) a[i] = p;
+#pragma unroll
+        for (int k = 0; k < kHsPer; ++k) {
+            if (i0 + k < n) a[i0 + k] = p;
+            p += v[k];
+        }
         __syncthreads();
     }
     if (tid == 0) scan_exit(ctrl);
 }
 
-// Stable scatter: within a tile, keys keep their order per digit (round, then
-// wave, then lane); tiles keep theirs through the digit-major scanned histogram.
+// Stable scatter.  Each key's rank among the tile's keys of its digit is found
+// in order (round, then wave, then lane); the tile is then laid out sorted by
+// digit in LDS and written from there, so consecutive threads write
+// consecutive addresses within each digit's run.  Tiles keep their order
+// through the digit-major scanned histogram.
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
                                                            const uint64_t* dn, uint64_t cap, int shift,
                                                            const uint64_t* __restrict__ hist, uint64_t idMask,
                                                            int extract) {
     __shared__ uint32_t running[256];
+    __shared__ uint32_t lstart[256];
     __shared__ uint32_t wc[kRsThreads / kWave][256];
+    __shared__ uint64_t sk[kRsTile];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     uint64_t n = *dn;
     if (n > cap) n = cap;
     const uint64_t ntiles = (n + kRsTile - 1) / kRsTile;
     const uint64_t ltmask = (1ull << lane) - 1ull;
     for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint64_t t0 = t * kRsTile;
+        const uint32_t cnt = static_cast<uint32_t>(n - t0 < static_cast<uint64_t>(kRsTile) ? n - t0 : kRsTile);
+        uint64_t key[kRsPer];
+        uint32_t rank[kRsPer];
+#pragma unroll
+        for (int r = 0; r < kRsPer; ++r) {
+            const uint32_t e = static_cast<uint32_t>(r) * kRsThreads + tid;
+            key[r] = e < cnt ? in[t0 + e] : 0;
+        }
         running[tid] = 0;
         __syncthreads();
+#pragma unroll
         for (int r = 0; r < kRsPer; ++r) {
             for (int w = 0; w < kRsThreads / kWave; ++w) wc[w][tid] = 0;
             __syncthreads();
-            const uint64_t e = t * kRsTile + static_cast<uint64_t>(r) * kRsThreads + tid;
-            const bool ok = e < n;
-            const uint64_t key = ok ? in[e] : 0;
-            const uint32_t dig = ok ? static_cast<uint32_t>((key >> shift) & 0xFF) : 0;
+            const bool ok = static_cast<uint32_t>(r) * kRsThreads + tid < cnt;
+            const uint32_t dig = ok ? static_cast<uint32_t>((key[r] >> shift) & 0xFF) : 0;
             uint64_t peers = __ballot(ok);
 #pragma unroll
             for (int b = 0; b < 8; ++b) {
@@ -394,18 +542,46 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* __res
             const uint32_t rankW = static_cast<uint32_t>(__popcll(peers & ltmask));
             if (ok && rankW == 0) wc[wid][dig] = static_cast<uint32_t>(__popcll(peers));
             __syncthreads();
+            uint32_t before = 0;
             if (ok) {
-                uint32_t before = running[dig];
+                before = running[dig];
                 for (int w = 0; w < wid; ++w) before += wc[w][dig];
-                const uint64_t dst = hist[static_cast<uint64_t>(dig) * ntiles + t] + before + rankW;
-                out[dst] = extract ? (key & idMask) : key;
             }
+            rank[r] = before + rankW;
             __syncthreads();
             uint32_t add = 0;
             for (int w = 0; w < kRsThreads / kWave; ++w) add += wc[w][tid];
             running[tid] += add;
             __syncthreads();
         }
+        // exclusive scan of the tile's digit counts (256 entries, 4 waves)
+        {
+            const uint32_t v = running[tid];
+            uint32_t x = v;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o, kWave);
+                if (lane >= o) x += y;
+            }
+            if (lane == kWave - 1) wc[0][wid] = x;
+            __syncthreads();
+            uint32_t p = x - v;
+            for (int w = 0; w < wid; ++w) p += wc[0][w];
+            lstart[tid] = p;
+            __syncthreads();
+        }
+#pragma unroll
+        for (int r = 0; r < kRsPer; ++r) {
+            const uint32_t e = static_cast<uint32_t>(r) * kRsThreads + tid;
+            if (e < cnt) sk[lstart[(key[r] >> shift) & 0xFF] + rank[r]] = key[r];
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < cnt; i += kRsThreads) {
+            const uint64_t k = sk[i];
+            const uint32_t dig = static_cast<uint32_t>((k >> shift) & 0xFF);
+            const uint64_t dst = hist[static_cast<uint64_t>(dig) * ntiles + t] + (i - lstart[dig]);
+            out[dst] = extract ? (k & idMask) : k;
+        }
+        __syncthreads();
     }
 }
 
@@ -458,7 +634,7 @@ int radix_sort(tblaeq_ctx* ctx, uint64_t* a, uint64_t* b, const uint64_t* dn, ui
                uint64_t* hist, uint64_t** result, bool extractLast, uint64_t idMask) {
     const uint64_t capTiles = (cap + kRsTile - 1) / kRsTile;
     const int grid = scan_launch_grid(ctx, capTiles);
-    const uint64_t histTiles = (256 * (capTiles ? capTiles : 1) + kScanThreads - 1) / kScanThreads;
+    const uint64_t histTiles = (256 * (capTiles ? capTiles : 1) + kScanThreads * kHsPer - 1) / (kScanThreads * kHsPer);
     const int passes = (nbits + 7) / 8;
     uint64_t* src = a;
     uint64_t* dst = b;
@@ -493,20 +669,50 @@ int bits_for(uint64_t v) {
     return b;
 }
 
-template <bool FINAL>
+template <int MODE>
 int launch_range_step(tblaeq_ctx* ctx, const RangeStepArgs& A, uint64_t itemBound) {
     uint64_t blocks = (itemBound + kStepWaves - 1) / kStepWaves;
     const uint64_t cap = static_cast<uint64_t>(ctx->cuCount) * 16;
     if (blocks > cap) blocks = cap;
     if (blocks == 0) blocks = 1;
     const dim3 g(static_cast<unsigned>(blocks)), b(kStepWaves * kWave);
-    TimedScope ts(ctx, FINAL ? kKBatch0 + 1 : kKBatch0 + 0);
-    if (A.D <= 4) hipLaunchKernelGGL((k_range_step<4, FINAL>), g, b, 0, ctx->stream, A);
-    else if (A.D <= 8) hipLaunchKernelGGL((k_range_step<8, FINAL>), g, b, 0, ctx->stream, A);
-    else if (A.D <= 16) hipLaunchKernelGGL((k_range_step<16, FINAL>), g, b, 0, ctx->stream, A);
-    else hipLaunchKernelGGL((k_range_step<32, FINAL>), g, b, 0, ctx->stream, A);
+    TimedScope ts(ctx, MODE == kFinal ? kKBatch0 + 1 : kKBatch0 + 0);
+    if (A.D <= 4) hipLaunchKernelGGL((k_range_step<4, MODE>), g, b, 0, ctx->stream, A);
+    else if (A.D <= 8) hipLaunchKernelGGL((k_range_step<8, MODE>), g, b, 0, ctx->stream, A);
+    else if (A.D <= 16) hipLaunchKernelGGL((k_range_step<16, MODE>), g, b, 0, ctx->stream, A);
+    else hipLaunchKernelGGL((k_range_step<32, MODE>), g, b, 0, ctx->stream, A);
     TB_HIP(hipGetLastError());
     return TBLAEQ_OK;
+}
+
+// Bounding box of each last-level position's points (its CSC column at the
+// finest level), rounded outward to float: lo[D] | hi[D] per position.  An
+// empty cluster gets an empty box (+inf | -inf).
+__global__ void k_aabb(const uint64_t* __restrict__ ids, uint64_t n, const uint64_t* __restrict__ colRes,
+                       const double* __restrict__ coords, int D, float* __restrict__ out) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t w0 = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / kWave;
+    const uint64_t nw = static_cast<uint64_t>(gridDim.x) * blockDim.x / kWave;
+    for (uint64_t j = w0; j < n; j += nw) {
+        const uint64_t c = ids[j];
+        const uint64_t cb = colRes[c], ce = colRes[c + 1];
+        for (int d = 0; d < D; ++d) {
+            double mn = __builtin_inf(), mx = -__builtin_inf();
+            for (uint64_t e = cb + lane; e < ce; e += kWave) {
+                const double v = coords[e * D + d];
+                mn = fmin(mn, v);
+                mx = fmax(mx, v);
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                mn = fmin(mn, __shfl_xor(mn, o, kWave));
+                mx = fmax(mx, __shfl_xor(mx, o, kWave));
+            }
+            if (lane == 0) {
+                out[j * 2 * D + d] = __double2float_rd(mn);
+                out[j * 2 * D + D + d] = __double2float_ru(mx);
+            }
+        }
+    }
 }
 
 int launch_grouping(tblaeq_batch* b, uint64_t mesh) {
@@ -656,6 +862,14 @@ extern "C" int tblaeq_batch_create(tblaeq_ctx* ctx, const tblaeq_level* levels, 
                                0, ctx->stream, levels[l - 1].col_res, levels[l - 1].cols, b->parentPos[l]);
         }
     }
+    {
+        const uint64_t n = b->count[L - 1];
+        b->aabb = balloc<float>(b, n * 2 * D, &rc);
+        if (rc) return fail(rc);
+        const uint64_t g = std::min<uint64_t>((n * kWave + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 16);
+        hipLaunchKernelGGL(k_aabb, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream, b->ids[L - 1], n,
+                           levels[L - 1].col_res, b->coords[L], static_cast<int>(D), b->aabb);
+    }
     b->ctl = balloc<uint64_t>(b, 16, &rc);
     if (rc) return fail(rc);
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, ctx->stream, b->ctl, 0ull, 16);
@@ -685,8 +899,12 @@ extern "C" int tblaeq_batch_reserve(tblaeq_batch* b, uint64_t max_queries, uint6
         if (rc) return rc;
         b->qlist = balloc<uint32_t>(b, pair_cap, &rc);
         if (rc) return rc;
+        b->qlist2 = balloc<uint32_t>(b, pair_cap, &rc);
+        if (rc) return rc;
         b->itemCap = pair_cap / kWave + b->maxMesh + 1;
         b->items = balloc<uint64_t>(b, b->itemCap, &rc);
+        if (rc) return rc;
+        b->items2 = balloc<uint64_t>(b, b->itemCap, &rc);
         if (rc) return rc;
         b->pairCap = pair_cap;
     }
@@ -694,6 +912,8 @@ extern "C" int tblaeq_batch_reserve(tblaeq_batch* b, uint64_t max_queries, uint6
         b->cnt = balloc<uint64_t>(b, b->maxMesh, &rc);
         if (rc) return rc;
         b->off = balloc<uint64_t>(b, b->maxMesh + 1, &rc);
+        if (rc) return rc;
+        b->off2 = balloc<uint64_t>(b, b->maxMesh + 1, &rc);
         if (rc) return rc;
         TB_HIP(hipMemsetAsync(b->cnt, 0, b->maxMesh * 8, b->ctx->stream));
     }
@@ -738,32 +958,52 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     A.boxes = d_boxes;
     A.cnt = b->cnt;
     A.overflow = ctl + 3;
-    A.survStat = d_level_surv;
-    A.fineStat = d_fine_count;
     A.L = L;
+    A.itemCap = b->itemCap;
+    A.qcap = b->pairCap;
+    // two group sets: the prune of level l reads one and writes the other
+    uint64_t* offS[2] = {b->off, b->off2};
+    uint64_t* itemS[2] = {b->items, b->items2};
+    uint32_t* qS[2] = {b->qlist, b->qlist2};
+    uint64_t* nS[2] = {ctl + 1, ctl + 4};
+    int cur = 0;
     int rc;
     for (int l = 0; l < L; ++l) {
-        // prune the candidates of level l
+        const int nxt = cur ^ 1;
         A.level0 = l == 0;
         A.level = l;
-        A.items = b->items;
-        A.nItems = ctl + 1;
-        A.itemCap = b->itemCap;
-        A.off = b->off;
-        A.qlist = b->qlist;
+        A.items = itemS[cur];
+        A.nItems = nS[cur];
+        A.off = offS[cur];
+        A.qlist = qS[cur];
         A.colResPrev = l > 0 ? b->colRes[l - 1] : nullptr;
         A.count0 = b->count[0];
         A.coords = b->coords[l];
         A.radiusPos = b->radiusPos[l];
         A.ids = b->ids[l];
-        A.out = b->pairs;
+        A.aabb = l == L - 1 ? b->aabb : nullptr;
+        A.weightPos = (l == L - 1 && d_fine_count) ? b->weightPos[l] : nullptr;
+        A.offNext = offS[nxt];
+        A.qlistNext = qS[nxt];
         A.cap = b->pairCap;
-        A.outCount = ctl + 0;
+        A.survStat = d_level_surv;
+        A.fineStat = d_fine_count;
         const uint64_t bound = l == 0 ? (nq + kWave - 1) / kWave : b->itemCap;
-        if ((rc = launch_range_step<false>(ctx, A, bound))) return rc;
-        // group the survivors of level l by cluster
-        if ((rc = launch_grouping(b, l == 0 ? b->count[0] : b->count[l]))) return rc;
-        TB_HIP(hipGetLastError());
+        if ((rc = launch_range_step<kCount>(ctx, A, bound))) return rc;
+        // group the kept (cluster, query) pairs of level l by cluster
+        const uint64_t mesh = b->count[l];
+        const uint64_t tiles = (mesh + kScanThreads - 1) / kScanThreads;
+        if ((rc = scan_prepare(ctx, tiles))) return rc;
+        {
+            TimedScope ts(ctx, kKBatch0 + 2);
+            hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream,
+                               b->cnt, mesh, offS[nxt], itemS[nxt], b->itemCap, nS[nxt], ctl + 3, ctx->status,
+                               ctx->ctrl);
+        }
+        A.survStat = nullptr;
+        A.fineStat = nullptr;
+        if ((rc = launch_range_step<kWrite>(ctx, A, bound))) return rc;
+        cur = nxt;
     }
     // final step: exact filter of the finest candidates
     const uint64_t maxId = b->count[L];
@@ -775,17 +1015,21 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     }
     A.level0 = 0;
     A.level = L;
-    A.items = b->items;
-    A.nItems = ctl + 1;
+    A.items = itemS[cur];
+    A.nItems = nS[cur];
+    A.off = offS[cur];
+    A.qlist = qS[cur];
     A.colResPrev = b->colRes[L - 1];
     A.coords = b->coords[L];
     A.radiusPos = nullptr;
+    A.aabb = nullptr;
+    A.weightPos = nullptr;
     A.ids = b->ids[L];
     A.out = b->hits;
     A.cap = std::min(b->hitCap, out_cap);
     A.outCount = ctl + 2;
     A.idBits = idBits;
-    if ((rc = launch_range_step<true>(ctx, A, b->itemCap))) return rc;
+    if ((rc = launch_range_step<kFinal>(ctx, A, b->itemCap))) return rc;
     uint64_t* sorted = nullptr;
     if ((rc = radix_sort(ctx, b->hits, b->hitsTmp, ctl + 2, A.cap, idBits + qBits, b->hist, &sorted, false, 0)))
         return rc;
