@@ -78,6 +78,8 @@ constexpr int kStepWaves = 4;  // waves per workgroup of the cluster-major kerne
 // b->items with their count in ctl[1], query lists in b->qlist.  Leaves
 // b->cnt zeroed and resets the pair count ctl[0].
 int launch_grouping(tblaeq_batch* b, uint64_t mesh);
+// Group scan of b->cnt over mesh clusters into off / items (count in *nItems).
+int launch_group_scan(tblaeq_batch* b, uint64_t mesh, uint64_t* off, uint64_t* items, uint64_t* nItems);
 // Exclusive scan of cnt[0..n) into off[0..n] (off[n] = total).
 int launch_count_scan(tblaeq_batch* b, const uint64_t* cnt, uint64_t n, uint64_t* off);
 // status[0] = ctl[2], status[1] = ctl[3].

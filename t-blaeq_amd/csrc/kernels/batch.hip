@@ -737,6 +737,18 @@ int launch_grouping(tblaeq_batch* b, uint64_t mesh) {
     return TBLAEQ_OK;
 }
 
+int launch_group_scan(tblaeq_batch* b, uint64_t mesh, uint64_t* off, uint64_t* items, uint64_t* nItems) {
+    tblaeq_ctx* ctx = b->ctx;
+    const uint64_t tiles = (mesh + kScanThreads - 1) / kScanThreads;
+    int rc = scan_prepare(ctx, tiles);
+    if (rc) return rc;
+    TimedScope ts(ctx, kKBatch0 + 2);
+    hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream, b->cnt,
+                       mesh, off, items, b->itemCap, nItems, b->ctl + 3, ctx->status, ctx->ctrl);
+    TB_HIP(hipGetLastError());
+    return TBLAEQ_OK;
+}
+
 int launch_count_scan(tblaeq_batch* b, const uint64_t* cnt, uint64_t n, uint64_t* off) {
     tblaeq_ctx* ctx = b->ctx;
     const uint64_t tiles = (n + kScanThreads - 1) / kScanThreads;

@@ -65,6 +65,11 @@ struct KnnArgs {
     uint64_t* cnt;          // pairs per last-level cluster
     uint64_t* overflow;
     uint64_t* thr;          // nq x 2: (key, id) of the K-th seed child, ~0 if fewer than K
+    // split mode: levels 0..L-2 here, the last level cluster-major elsewhere.
+    // Survivors of level L-2 become (cluster, query) pairs and tau[q] bounds
+    // the best M (lb, id) keys of the last level.
+    int split;
+    uint64_t* tau;          // nq x 2
     uint64_t* outIds;       // nq x Kmax
     double* outD2;
     uint64_t* outN;         // nq
@@ -183,22 +188,24 @@ __device__ __forceinline__ double lb_reg(const double (&x)[DM], const double* q,
     return sqrt_rd(d2_reg<DM>(x, q)) - r;
 }
 
-// Queues the children of last-level cluster `cid` for query slot i (caller
-// guarantees a free slot).
-__device__ __forceinline__ void knn_queue(const KnnArgs& A, KnnShared& S, int i, uint64_t cid) {
-    const uint64_t cs = A.colRes[A.L - 1][cid], ce = A.colRes[A.L - 1][cid + 1];
+// Queues `count` points starting at position `start` for query slot i
+// (caller guarantees a free slot).
+__device__ __forceinline__ void knn_queue(KnnShared& S, int i, uint64_t start, uint64_t count) {
     const uint32_t p = atomicAdd(&S.listN, 1u);
-    S.lstart[p] = cs;
-    S.lpre[p] = ce - cs;
+    S.lstart[p] = start;
+    S.lpre[p] = count;
     S.listQ[p] = static_cast<uint8_t>(i);
 }
 
-// Scores every queued child against its query and inserts it into that query's
-// top-K buffer; the queue is empty afterwards.  Entry e's children are the flat
-// indices [lpre[e], lpre[e+1]) once the counts are scanned in place.
+// Scores every queued point against its query and inserts it into that
+// query's buffer; the queue is empty afterwards.  Key = okey(lb) with
+// lb = sqrt_rd(d2) - r when `radius` is given, okey(d2) otherwise.  Entry e's
+// points are the flat indices [lpre[e], lpre[e+1]) once the counts are
+// scanned in place.
 template <int DM>
-__device__ void knn_flush(const KnnArgs& A, KnnShared& S, int tid) {
-    const int D = A.D, L = A.L, lane = tid & (kWave - 1), wid = tid / kWave;
+__device__ void knn_flush(KnnShared& S, int tid, int D, const double* coords, const uint64_t* ids,
+                          const double* radius) {
+    const int lane = tid & (kWave - 1), wid = tid / kWave;
     const uint32_t n = S.listN;
     if (n == 0) return;
     constexpr uint32_t kPer = (kKnnList + kKnnThreads - 1) / kKnnThreads;
@@ -238,9 +245,10 @@ __device__ void knn_flush(const KnnArgs& A, KnnShared& S, int tid) {
             const uint64_t j = S.lstart[lo] + (f - S.lpre[lo]);
             const int i = S.listQ[lo];
             double xv[DM];
-            load_row<DM>(A.coords[L] + j * D, D, xv);
+            load_row<DM>(coords + j * D, D, xv);
             const double d2 = d2_reg<DM>(xv, S.qp[i]);
-            knn_insert(S, i, okey(d2), A.ids[L][j]);
+            const uint64_t key = radius ? okey(sqrt_rd(d2) - radius[j]) : okey(d2);
+            knn_insert(S, i, key, ids[j]);
         }
         __syncthreads();
     }
@@ -248,7 +256,7 @@ __device__ void knn_flush(const KnnArgs& A, KnnShared& S, int tid) {
     __syncthreads();
 }
 
-// Queues the (cluster, query slot) pair of a surviving last-level cluster.
+// Queues the (cluster, query slot) pair of a surviving cluster.
 __device__ __forceinline__ void knn_pair_queue(KnnShared& S, int i, uint64_t cid) {
     const uint32_t p = atomicAdd(&S.listN, 1u);
     S.lstart[p] = cid;
@@ -277,11 +285,129 @@ __device__ void knn_emit_pairs(const KnnArgs& A, KnnShared& S, int tid, uint64_t
     __syncthreads();
 }
 
+// STEP (KNN.cu:75-96) over each query's sorted buffer, wave i for query i:
+// phase 1 takes clusters while the exclusive child count is below K, ub is
+// the largest lb + 2r of phase 1 (at least 0).  Sets S.ub and S.i0 (first
+// entry after phase 1).
+__device__ void knn_step(KnnShared& S, int tid, const uint64_t* weightById, const double* radiusById) {
+    const int lane = tid & (kWave - 1), wid = tid / kWave;
+    if (wid < kKnnQB) {
+        const int i = wid;
+        const uint32_t m = S.cnt[i];
+        const uint64_t K = S.K[i];
+        uint64_t carry = 0;
+        double best = 0.0;
+        uint32_t i0 = m;
+        bool done = false;
+        for (uint32_t b = 0; b < m && !done; b += kWave) {
+            const uint32_t e = b + lane;
+            uint64_t w = 0;
+            double c = -__builtin_inf();
+            if (e < m) {
+                const uint64_t id = S.bv[i][e];
+                w = weightById[id];
+                c = okey_inv(S.bk[i][e]) + 2.0 * radiusById[id];
+            }
+            uint64_t x = w;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint64_t y = __shfl_up(x, o, kWave);
+                if (lane >= o) x += y;
+            }
+            const uint64_t ex = carry + x - w;
+            const bool in1 = e < m && ex < K;
+            best = fmax(best, wave_max_f64(in1 ? c : -__builtin_inf()));
+            const uint64_t firstOut = __ballot(e < m && ex >= K);
+            if (firstOut) {
+                i0 = b + (__ffsll(static_cast<unsigned long long>(firstOut)) - 1);
+                done = true;
+            }
+            carry += __shfl(x, kWave - 1, kWave);
+        }
+        if (lane == 0) {
+            S.ub[i] = best;
+            S.i0[i] = i0;
+        }
+    }
+    __syncthreads();
+}
+
+// Seed clusters of the last level: the surviving (lb <= ub) prefix of the
+// sorted buffer, never shorter than phase 1, up to seedMin entries.  Sets
+// S.seedN and the last seed's (key, id) in S.bndK / S.bndV.
+__device__ void knn_pick_seeds(KnnShared& S, int tid, uint64_t q0, uint64_t nq, uint32_t seedMin) {
+    if (tid < kKnnQB) {
+        const int i = tid;
+        uint32_t ns = 0;
+        if (q0 + i < nq && S.K[i] > 0) {
+            const uint32_t m = S.cnt[i];
+            uint32_t lim = S.i0[i] > seedMin ? S.i0[i] : seedMin;
+            if (lim > m) lim = m;
+            const uint64_t ubk = okey(S.ub[i]);
+            uint32_t lo = S.i0[i] < lim ? S.i0[i] : lim, hi = lim;  // first entry above ub
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (S.bk[i][mid] <= ubk) lo = mid + 1;
+                else hi = mid;
+            }
+            ns = lo;
+        }
+        S.seedN[i] = ns;
+        S.bndK[i] = ns > 0 ? S.bk[i][ns - 1] : 0;
+        S.bndV[i] = ns > 0 ? S.bv[i][ns - 1] : 0;
+        if (i == 0) S.listN = 0;
+    }
+    __syncthreads();
+}
+
+// Expands the seeds (their finest children) into a fresh top-K buffer; the
+// compacted buffer's threshold is the certified radius.
+template <int DM>
+__device__ void knn_expand_seeds(KnnShared& S, int tid, uint64_t q0, uint64_t nq, int D, const uint64_t* colRes,
+                                 const double* coords, const uint64_t* ids) {
+    for (int i = 0; i < kKnnQB; ++i)
+        for (uint32_t e = tid; e < S.seedN[i]; e += kKnnThreads) {
+            const uint64_t cid = S.bv[i][e];
+            knn_queue(S, i, colRes[cid], colRes[cid + 1] - colRes[cid]);
+        }
+    __syncthreads();
+    if (tid < kKnnQB) {
+        S.M[tid] = static_cast<uint32_t>(q0 + tid < nq ? S.K[tid] : 0);
+        S.cnt[tid] = 0;
+        S.thrK[tid] = ~0ull;
+        S.thrV[tid] = ~0ull;
+    }
+    __syncthreads();
+    knn_flush<DM>(S, tid, D, coords, ids, nullptr);
+    knn_compact(S, tid);
+}
+
+// Writes each query's sorted buffer as its (provisional) top-K.
+__device__ void knn_write_topk(const KnnShared& S, int tid, uint64_t q0, uint64_t nq, uint64_t Kmax,
+                               uint64_t* outIds, double* outD2, uint64_t* outN, uint64_t* thr) {
+    for (int i = 0; i < kKnnQB; ++i) {
+        const uint64_t q = q0 + i;
+        if (q >= nq) continue;
+        const uint32_t k = S.cnt[i];
+        for (uint32_t e = tid; e < k; e += kKnnThreads) {
+            outIds[q * Kmax + e] = S.bv[i][e];
+            outD2[q * Kmax + e] = okey_inv(S.bk[i][e]);
+        }
+        if (tid == 0) {
+            outN[q] = k;
+            if (thr) {
+                thr[2 * q] = S.thrK[i];
+                thr[2 * q + 1] = S.thrV[i];
+            }
+        }
+    }
+}
+
 template <int DM>
 __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
     __shared__ KnnShared S;
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const int tid = threadIdx.x;
     const int D = A.D, L = A.L;
+    const int Lend = A.split ? L - 1 : L;
     const uint64_t ngroups = (A.nq + kKnnQB - 1) / kKnnQB;
     for (uint64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
         const uint64_t q0 = g * kKnnQB;
@@ -291,13 +417,14 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
             S.surv[tid] = 0;
             S.fine[tid] = 0;
         }
+        if (tid == 0) S.listN = 0;
         for (int t = tid; t < kKnnQB * 32; t += kKnnThreads) {
             const int i = t / 32, d = t % 32;
             const uint64_t q = q0 + i;
             S.qp[i][d] = q < A.nq && d < D ? A.points[q * D + d] : 0.0;
         }
         __syncthreads();
-        for (int l = 0; l < L; ++l) {
+        for (int l = 0; l < Lend; ++l) {
             const uint64_t n = A.count[l];
             const double* X = A.coords[l];
             const double* R = A.radiusPos[l];
@@ -315,7 +442,6 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
                 S.cnt[tid] = 0;
                 S.thrK[tid] = ~0ull;
                 S.thrV[tid] = ~0ull;
-                S.tot[tid] = 0;
             }
             __syncthreads();
             for (uint64_t j0 = 0; j0 < n; j0 += kKnnThreads) {
@@ -336,52 +462,69 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
                 __syncthreads();
             }
             knn_compact(S, tid);
-            // ---- STEP: phase 1 over the sorted best-M list (wave i handles query i)
-            if (wid < kKnnQB) {
-                const int i = wid;
-                const uint32_t m = S.cnt[i];
-                const uint64_t K = S.K[i];
-                uint64_t carry = 0;
-                double best = 0.0;
-                uint32_t i0 = m;
-                bool done = false;
-                for (uint32_t b = 0; b < m && !done; b += kWave) {
-                    const uint32_t e = b + lane;
-                    uint64_t w = 0;
-                    double c = -__builtin_inf();
-                    if (e < m) {
-                        const uint64_t id = S.bv[i][e];
-                        w = A.weightById[l][id];
-                        c = okey_inv(S.bk[i][e]) + 2.0 * A.radiusById[l][id];
-                    }
-                    uint64_t x = w;
-                    for (int o = 1; o < kWave; o <<= 1) {
-                        const uint64_t y = __shfl_up(x, o, kWave);
-                        if (lane >= o) x += y;
-                    }
-                    const uint64_t ex = carry + x - w;
-                    const bool in1 = e < m && ex < K;
-                    best = fmax(best, wave_max_f64(in1 ? c : -__builtin_inf()));
-                    const uint64_t firstOut = __ballot(e < m && ex >= K);
-                    if (firstOut) {
-                        i0 = b + (__ffsll(static_cast<unsigned long long>(firstOut)) - 1);
-                        done = true;
-                    }
-                    carry += __shfl(x, kWave - 1, kWave);
+            knn_step(S, tid, A.weightById[l], A.radiusById[l]);
+            if (A.split && l == L - 2) {
+                // ---- tau: best M (lb, id) keys among the last-level children of
+                // the first surviving clusters (in (lb, id) order) that hold at
+                // least M of them -- an upper bound on the last level's M-th key
+                if (tid < kKnnQB) {
+                    const int i = tid;
+                    const uint64_t q = q0 + i;
+                    uint64_t m = q < A.nq && S.K[i] > 0 ? S.K[i] + A.emptyCols[L - 1] + 1 : 0;
+                    if (m > 0 && m < A.seedMin) m = A.seedMin;
+                    if (m > static_cast<uint64_t>(kKnnMaxM)) m = kKnnMaxM;
+                    const uint64_t ubk = okey(S.ub[i]);
+                    uint32_t e = 0;
+                    uint64_t got = 0;
+                    if (m > 0)
+                        for (; e < S.cnt[i] && got < m && S.bk[i][e] <= ubk; ++e) got += A.weightById[l][S.bv[i][e]];
+                    S.seedN[i] = e;  // clusters whose children are scored
+                    S.M[i] = static_cast<uint32_t>(m);
                 }
-                if (lane == 0) {
-                    S.ub[i] = best;
-                    S.i0[i] = i0;
-                    if (i0 < m) {
-                        S.bndK[i] = S.bk[i][i0];
-                        S.bndV[i] = S.bv[i][i0];
-                    } else {
-                        S.bndK[i] = ~0ull;
-                        S.bndV[i] = ~0ull;
+                __syncthreads();
+                for (int i = 0; i < kKnnQB; ++i)
+                    for (uint32_t e = tid; e < S.seedN[i]; e += kKnnThreads) {
+                        const uint64_t cid = S.bv[i][e];
+                        knn_queue(S, i, A.colRes[l][cid], A.colRes[l][cid + 1] - A.colRes[l][cid]);
                     }
+                __syncthreads();
+                if (tid < kKnnQB) {
+                    S.cnt[tid] = 0;
+                    S.thrK[tid] = ~0ull;
+                    S.thrV[tid] = ~0ull;
                 }
+                __syncthreads();
+                knn_flush<DM>(S, tid, D, A.coords[L - 1], A.ids[L - 1], A.radiusPos[L - 1]);
+                knn_compact(S, tid);
+                if (tid < kKnnQB && q0 + tid < A.nq) {
+                    A.tau[2 * (q0 + tid)] = S.thrK[tid];
+                    A.tau[2 * (q0 + tid) + 1] = S.thrV[tid];
+                }
+                // ---- pass 2: survivors of level L-2 become (cluster, query) pairs
+                for (uint64_t j0 = 0; j0 < n; j0 += kKnnThreads) {
+                    const uint64_t j = j0 + tid;
+                    if (j < n) {
+                        double xv[DM];
+                        load_row<DM>(X + j * D, D, xv);
+                        const double r = R[j];
+                        const uint64_t id = I[j];
+                        for (int i = 0; i < kKnnQB; ++i) {
+                            const uint64_t q = q0 + i;
+                            if (q >= A.nq) continue;
+                            if (l > 0 && !A.flags[q * A.flagStride + A.flagOff[l - 1] + Pp[j]]) continue;
+                            if (!(lb_reg<DM>(xv, S.qp[i], r) <= S.ub[i])) continue;
+                            atomicAdd(reinterpret_cast<unsigned long long*>(&S.surv[i]), 1ull);
+                            knn_pair_queue(S, i, id);
+                        }
+                    }
+                    __syncthreads();
+                    if (S.listN >= static_cast<uint32_t>(kKnnFlush)) knn_emit_pairs(A, S, tid, q0);
+                }
+                knn_emit_pairs(A, S, tid, q0);
+                if (tid < kKnnQB && A.survStat && q0 + tid < A.nq) A.survStat[(q0 + tid) * L + l] = S.surv[tid];
+                __syncthreads();
+                continue;
             }
-            __syncthreads();
             if (l < L - 1) {
                 // ---- pass 2: flag survivors (lb <= ub) of a middle level
                 for (uint64_t j0 = 0; j0 < n; j0 += kKnnThreads) {
@@ -408,44 +551,9 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
                 __syncthreads();
                 continue;
             }
-            // ---- last level: seed clusters = the surviving prefix of the sorted
-            // list, at least phase 1 and up to seedMin clusters
-            if (tid < kKnnQB) {
-                const int i = tid;
-                uint32_t ns = 0;
-                if (q0 + i < A.nq && S.K[i] > 0) {
-                    const uint32_t m = S.cnt[i];
-                    uint32_t lim = S.i0[i] > A.seedMin ? S.i0[i] : A.seedMin;
-                    if (lim > m) lim = m;
-                    const uint64_t ubk = okey(S.ub[i]);
-                    uint32_t lo = S.i0[i] < lim ? S.i0[i] : lim, hi = lim;  // first entry above ub
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (S.bk[i][mid] <= ubk) lo = mid + 1;
-                        else hi = mid;
-                    }
-                    ns = lo;
-                }
-                S.seedN[i] = ns;
-                if (ns > 0) {
-                    S.bndK[i] = S.bk[i][ns - 1];
-                    S.bndV[i] = S.bv[i][ns - 1];
-                }
-                if (i == 0) S.listN = 0;
-            }
-            __syncthreads();
-            for (int i = 0; i < kKnnQB; ++i)
-                for (uint32_t e = tid; e < S.seedN[i]; e += kKnnThreads) knn_queue(A, S, i, S.bv[i][e]);
-            __syncthreads();
-            if (tid < kKnnQB) {
-                S.M[tid] = static_cast<uint32_t>(q0 + tid < A.nq ? S.K[tid] : 0);
-                S.cnt[tid] = 0;
-                S.thrK[tid] = ~0ull;
-                S.thrV[tid] = ~0ull;
-            }
-            __syncthreads();
-            knn_flush<DM>(A, S, tid);
-            knn_compact(S, tid);  // thr = K-th seed child: the certified radius
+            // ---- last level (whole level in this kernel): seeds, certified radius
+            knn_pick_seeds(S, tid, q0, A.nq, A.seedMin);
+            knn_expand_seeds<DM>(S, tid, q0, A.nq, D, A.colRes[L - 1], A.coords[L], A.ids[L]);
             // ---- pass 2 of the last level: other survivors within the certified radius
             for (uint64_t j0 = 0; j0 < n; j0 += kKnnThreads) {
                 const uint64_t j = j0 + tid;
@@ -470,38 +578,341 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
                             if (lb > rho + 1e-9 * (1.0 + rho + r)) continue;
                         }
                         if (A.pairs) knn_pair_queue(S, i, id);
-                        else knn_queue(A, S, i, id);
+                        else knn_queue(S, i, A.colRes[L - 1][id], A.colRes[L - 1][id + 1] - A.colRes[L - 1][id]);
                     }
                 }
                 __syncthreads();
                 if (S.listN >= static_cast<uint32_t>(kKnnFlush)) {
                     if (A.pairs) knn_emit_pairs(A, S, tid, q0);
-                    else knn_flush<DM>(A, S, tid);
+                    else knn_flush<DM>(S, tid, D, A.coords[L], A.ids[L], nullptr);
                 }
             }
             if (A.pairs) knn_emit_pairs(A, S, tid, q0);
-            else knn_flush<DM>(A, S, tid);
+            else knn_flush<DM>(S, tid, D, A.coords[L], A.ids[L], nullptr);
             knn_compact(S, tid);  // cluster-major: the seeds' top-K, refined later
+            knn_write_topk(S, tid, q0, A.nq, A.Kmax, A.outIds, A.outD2, A.outN, A.thr);
             for (int i = 0; i < kKnnQB; ++i) {
                 const uint64_t q = q0 + i;
-                if (q >= A.nq) continue;
-                const uint32_t k = S.cnt[i];
-                for (uint32_t e = tid; e < k; e += kKnnThreads) {
-                    A.outIds[q * A.Kmax + e] = S.bv[i][e];
-                    A.outD2[q * A.Kmax + e] = okey_inv(S.bk[i][e]);
-                }
-                if (tid == 0) {
-                    A.outN[q] = k;
-                    if (A.thr) {
-                        A.thr[2 * q] = S.thrK[i];
-                        A.thr[2 * q + 1] = S.thrV[i];
-                    }
-                    if (A.survStat) A.survStat[q * L + l] = S.surv[i];
-                    if (A.fineStat) A.fineStat[q] = S.fine[i];
-                }
+                if (q >= A.nq || tid != 0) continue;
+                if (A.survStat) A.survStat[q * L + l] = S.surv[i];
+                if (A.fineStat) A.fineStat[q] = S.fine[i];
             }
             __syncthreads();
         }
+    }
+}
+
+// --------------------------------------------- cluster-major last level (split)
+// Work item = (cluster p of level L-2, slice of <= 64 queries that kept p);
+// lane = query, p's children (last-level clusters) staged in LDS.  Three
+// passes share the structure:
+//   kCand   records (okey(lb), q, id) with (lb, id) <= tau[q]: a superset of
+//           each query's best M clusters, selected exactly afterwards;
+//   kCount  survivor statistics (lb <= ub) and, per cluster, the number of
+//           queries that must refine it (survivor, not a seed, lb within the
+//           certified radius);
+//   kWrite  the same test, writing those queries into per-cluster lists.
+// lb is computed exactly (sqrt_rd, as everywhere) only where the squared
+// distance does not already decide the test: with t the threshold and
+// B = t + r + 1e-9(|t| + r), d2 > B^2 implies sqrt_rd(d2) - r > t by far more
+// than the rounding of the bounds.
+enum LastPass { kCand = 0, kLCount = 1, kLWrite = 2 };
+
+struct KnnLastArgs {
+    int D, L;
+    uint64_t nq;
+    const uint64_t* items;
+    const uint64_t* nItems;
+    uint64_t itemCap;
+    const uint64_t* off;
+    const uint32_t* qlist;
+    uint64_t qcap;
+    const uint64_t* colResPrev;  // P[L-2] column pointers
+    const double* coords;        // level L-1 positions
+    const double* radius;
+    const uint64_t* ids;
+    const uint64_t* weight;
+    const double* points;
+    const uint64_t* Ks;
+    const uint64_t* tau;     // kCand
+    const double* ub;        // kLCount / kLWrite
+    const uint64_t* bnd;     // last seed (key, id); (0, 0) = none
+    const uint64_t* thr;     // seed top-K threshold (key of rho^2, id)
+    // kCand
+    uint64_t* recK;
+    uint64_t* recV;
+    uint64_t recCap;
+    uint64_t* recCount;
+    uint64_t* cntQ;
+    // kLCount / kLWrite
+    uint64_t* cnt;
+    const uint64_t* offNext;
+    uint32_t* qlistNext;
+    uint64_t cap;
+    uint64_t* overflow;
+    uint64_t* survStat;
+    uint64_t* fineStat;
+};
+
+constexpr int kRecIdBits = 40;
+constexpr int kRecStage = 256;
+
+__device__ __forceinline__ double guard_bound(double t, double r) {
+    return t + r + 1e-9 * (__builtin_fabs(t) + r);
+}
+
+template <int DM, int PASS>
+__global__ __launch_bounds__(kStepWaves * kWave) void k_knn_last(KnnLastArgs A) {
+    __shared__ double s_x[kStepWaves][kWave * DM];
+    __shared__ double s_r[kStepWaves][kWave];
+    __shared__ uint64_t s_id[kStepWaves][kWave];
+    __shared__ uint64_t s_w[kStepWaves][kWave];
+    __shared__ uint64_t s_rk[kStepWaves][PASS == kCand ? kRecStage : 1];
+    __shared__ uint64_t s_rv[kStepWaves][PASS == kCand ? kRecStage : 1];
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint64_t gw = blockIdx.x * static_cast<uint64_t>(kStepWaves) + wave;
+    const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kStepWaves;
+    const int D = A.D;
+    uint64_t nItems = *A.nItems;
+    if (nItems > A.itemCap) nItems = A.itemCap;
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    double* sx = s_x[wave];
+    if (D != DM)
+        for (int t = lane; t < kWave * DM; t += kWave) sx[t] = 0.0;
+    uint64_t staged = 0;
+    auto flushRec = [&]() {
+        if (staged == 0) return;
+        wave_lds_sync();
+        uint64_t base = 0;
+        if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(A.recCount), staged);
+        base = __shfl(base, 0, kWave);
+        for (uint64_t o = lane; o < staged; o += kWave)
+            if (base + o < A.recCap) {
+                A.recK[base + o] = s_rk[wave][o];
+                A.recV[base + o] = s_rv[wave][o];
+                atomicAdd(reinterpret_cast<unsigned long long*>(&A.cntQ[s_rv[wave][o] >> kRecIdBits]), 1ull);
+            }
+        if (lane == 0 && base + staged > A.recCap) atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 4ull);
+        wave_lds_sync();
+        staged = 0;
+    };
+    for (uint64_t item = gw; item < nItems; item += nw) {
+        const uint64_t it = A.items[item];
+        const uint64_t p = it >> kSliceBits;
+        const uint64_t start = (it & ((1ull << kSliceBits) - 1)) * kWave;
+        const uint64_t g0 = A.off[p];
+        const uint64_t gn = A.off[p + 1] - g0;
+        const bool valid = start + lane < gn && g0 + start + lane < A.qcap;
+        const uint64_t q = valid ? A.qlist[g0 + start + lane] : 0;
+        double qp[DM];
+#pragma unroll
+        for (int d = 0; d < DM; ++d) qp[d] = (d < D) ? A.points[q * D + d] : 0.0;
+        const uint64_t K = valid ? A.Ks[q] : 0;
+        // per-lane thresholds
+        uint64_t tK = 0, tV = 0, bK = 0, bV = 0;
+        double t1 = -__builtin_inf(), ub = -__builtin_inf(), rho = __builtin_inf();
+        bool rhoInf = true;
+        if (PASS == kCand) {
+            if (valid && K > 0) {
+                tK = A.tau[2 * q];
+                tV = A.tau[2 * q + 1];
+            }
+        } else if (valid) {
+            ub = A.ub[q];
+            bK = A.bnd[2 * q];
+            bV = A.bnd[2 * q + 1];
+            const uint64_t rk = A.thr[2 * q];
+            rhoInf = rk == ~0ull;
+            if (!rhoInf) rho = __builtin_sqrt(okey_inv(rk));
+        }
+        if (PASS == kCand && tK != ~0ull) t1 = okey_inv(tK);
+        const uint64_t jb = A.colResPrev[p], je = A.colResPrev[p + 1];
+        uint64_t mine = 0, fine = 0;
+        for (uint64_t j0 = jb; j0 < je; j0 += kWave) {
+            const int nc = static_cast<int>(je - j0 < static_cast<uint64_t>(kWave) ? je - j0 : kWave);
+            {
+                const double* src = A.coords + j0 * D;
+                const int tot = nc * D;
+                if (D == DM) {
+                    for (int t = lane; t < tot; t += kWave) sx[t] = src[t];
+                } else {
+                    for (int t = lane; t < tot; t += kWave) sx[(t / D) * DM + t % D] = src[t];
+                }
+                if (lane < nc) {
+                    s_r[wave][lane] = A.radius[j0 + lane];
+                    s_id[wave][lane] = A.ids[j0 + lane];
+                    s_w[wave][lane] = A.weight[j0 + lane];
+                }
+            }
+            wave_lds_sync();
+            uint64_t myMask = 0;
+            for (int k = 0; k < nc; ++k) {
+                const double* x = sx + k * DM;
+                double xv[DM];
+#pragma unroll
+                for (int d = 0; d < DM; ++d) xv[d] = x[d];
+                const double d2 = d2_reg<DM>(xv, qp);
+                const double r = s_r[wave][k];
+                const uint64_t id = s_id[wave][k];
+                if (PASS == kCand) {
+                    bool emit = false;
+                    uint64_t key = 0;
+                    if (valid && K > 0) {
+                        const double B = guard_bound(t1, r);
+                        if (tK == ~0ull || (B >= 0.0 && d2 <= B * B)) {
+                            key = okey(sqrt_rd(d2) - r);
+                            emit = key < tK || (key == tK && id <= tV);
+                        }
+                    }
+                    const uint64_t m = __ballot(emit);
+                    if (m) {
+                        const uint64_t c = __popcll(m);
+                        if (staged + c > static_cast<uint64_t>(kRecStage)) flushRec();
+                        if (emit) {
+                            const uint64_t o = staged + __popcll(m & ltmask);
+                            s_rk[wave][o] = key;
+                            s_rv[wave][o] = (q << kRecIdBits) | id;
+                        }
+                        staged += c;
+                    }
+                } else {
+                    bool surv = false, pair = false;
+                    if (valid) {
+                        const double tp = rhoInf ? ub : fmin(ub, rho + 1e-9 * (1.0 + rho + r));
+                        const double t = (PASS == kLCount && A.survStat) ? ub : tp;
+                        const double B = guard_bound(t, r);
+                        if (B >= 0.0 && d2 <= B * B) {
+                            const double lb = sqrt_rd(d2) - r;
+                            surv = lb <= ub;
+                            const bool seed = !lt_pair(bK, bV, okey(lb), id);
+                            pair = surv && K > 0 && !seed && (rhoInf || lb <= rho + 1e-9 * (1.0 + rho + r));
+                        }
+                    }
+                    if (PASS == kLCount && surv) {
+                        ++mine;
+                        fine += s_w[wave][k];
+                    }
+                    const uint64_t m = __ballot(pair);
+                    if (lane == k) myMask = m;
+                }
+            }
+            if (PASS == kLCount) {
+                if (lane < nc && myMask)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[s_id[wave][lane]]),
+                              static_cast<unsigned long long>(__popcll(myMask)));
+            } else if (PASS == kLWrite) {
+                uint64_t myBase = 0;
+                if (lane < nc && myMask) {
+                    const uint64_t cid = s_id[wave][lane];
+                    const uint64_t pc = __popcll(myMask);
+                    myBase = A.offNext[cid] +
+                             (atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[cid]), 0ull - pc) - pc);
+                }
+                for (int k = 0; k < nc; ++k) {
+                    const uint64_t m = __shfl(myMask, k, kWave);
+                    const uint64_t base = __shfl(myBase, k, kWave);
+                    if (m == 0) continue;
+                    if ((m >> lane) & 1ull) {
+                        const uint64_t o = base + __popcll(m & ltmask);
+                        if (o < A.cap) A.qlistNext[o] = static_cast<uint32_t>(q);
+                        else atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 1ull);
+                    }
+                }
+            }
+            wave_lds_sync();
+        }
+        if (PASS == kLCount && valid) {
+            if (A.survStat && mine)
+                atomicAdd(reinterpret_cast<unsigned long long*>(&A.survStat[q * A.L + A.L - 1]), mine);
+            if (A.fineStat && fine) atomicAdd(reinterpret_cast<unsigned long long*>(&A.fineStat[q]), fine);
+        }
+    }
+    if (PASS == kCand) flushRec();
+}
+
+// Last level, query-major: exact best M from the grouped candidate records,
+// STEP, seeds and their expansion (the seed top-K and certified radius).
+struct KnnSelectArgs {
+    int D, L;
+    uint64_t nq;
+    const double* points;
+    const uint64_t* Ks;
+    uint64_t Kmax;
+    uint64_t emptyLast;
+    uint32_t seedMin;
+    const uint64_t* offQ;
+    const uint64_t* gK;
+    const uint64_t* gV;
+    const uint64_t* weightById;  // level L-1
+    const double* radiusById;
+    const uint64_t* colRes;      // P[L-1]
+    const double* coordsFine;
+    const uint64_t* idsFine;
+    double* ub;
+    uint64_t* bnd;
+    uint64_t* thr;
+    uint64_t* outIds;
+    double* outD2;
+    uint64_t* outN;
+    uint64_t* survStat;
+    uint64_t* fineStat;
+};
+
+template <int DM>
+__global__ __launch_bounds__(kKnnThreads) void k_knn_select(KnnSelectArgs A) {
+    __shared__ KnnShared S;
+    const int tid = threadIdx.x;
+    const int D = A.D;
+    const uint64_t ngroups = (A.nq + kKnnQB - 1) / kKnnQB;
+    for (uint64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        const uint64_t q0 = g * kKnnQB;
+        if (tid < kKnnQB) {
+            const uint64_t q = q0 + tid;
+            const uint64_t K = q < A.nq ? A.Ks[q] : 0;
+            S.K[tid] = K;
+            uint64_t m = K > 0 ? K + A.emptyLast + 1 : 0;
+            if (m > 0 && m < A.seedMin) m = A.seedMin;
+            if (m > static_cast<uint64_t>(kKnnMaxM)) m = kKnnMaxM;
+            S.M[tid] = static_cast<uint32_t>(m);
+            S.cnt[tid] = 0;
+            S.thrK[tid] = ~0ull;
+            S.thrV[tid] = ~0ull;
+            if (q < A.nq) {
+                if (A.survStat) A.survStat[q * A.L + A.L - 1] = 0;
+                if (A.fineStat) A.fineStat[q] = 0;
+            }
+        }
+        if (tid == 0) S.listN = 0;
+        for (int t = tid; t < kKnnQB * 32; t += kKnnThreads) {
+            const int i = t / 32, d = t % 32;
+            const uint64_t q = q0 + i;
+            S.qp[i][d] = q < A.nq && d < D ? A.points[q * D + d] : 0.0;
+        }
+        __syncthreads();
+        for (int i = 0; i < kKnnQB; ++i) {
+            const uint64_t q = q0 + i;
+            if (q >= A.nq || S.M[i] == 0) continue;
+            const uint64_t r0 = A.offQ[q], rn = A.offQ[q + 1] - r0;
+            for (uint64_t b = 0; b < rn; b += kKnnThreads) {
+                if (knn_needs_compact(S)) knn_compact(S, tid);
+                const uint64_t r = b + tid;
+                if (r < rn) knn_insert(S, i, A.gK[r0 + r], A.gV[r0 + r]);
+                __syncthreads();
+            }
+        }
+        knn_compact(S, tid);
+        knn_step(S, tid, A.weightById, A.radiusById);
+        knn_pick_seeds(S, tid, q0, A.nq, A.seedMin);
+        if (tid < kKnnQB && q0 + tid < A.nq) {
+            const uint64_t q = q0 + tid;
+            A.ub[q] = S.ub[tid];
+            A.bnd[2 * q] = S.bndK[tid];
+            A.bnd[2 * q + 1] = S.bndV[tid];
+        }
+        knn_expand_seeds<DM>(S, tid, q0, A.nq, D, A.colRes, A.coordsFine, A.idsFine);
+        knn_write_topk(S, tid, q0, A.nq, A.Kmax, A.outIds, A.outD2, A.outN, A.thr);
+        __syncthreads();
     }
 }
 
@@ -519,6 +930,7 @@ struct KnnClusterArgs {
     uint64_t itemCap;
     const uint64_t* off;
     const uint32_t* qlist;
+    uint64_t qcap;
     const uint64_t* colRes;   // P[L-1] column pointers
     const double* coords;     // finest positions
     const uint64_t* ids;
@@ -531,8 +943,6 @@ struct KnnClusterArgs {
     uint64_t* cntQ;           // records per query
     uint64_t* overflow;
 };
-
-constexpr int kRecIdBits = 40;
 
 template <int DM>
 __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_cluster(KnnClusterArgs A) {
@@ -555,7 +965,7 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_cluster(KnnClusterAr
         const uint64_t start = (it & ((1ull << kSliceBits) - 1)) * kWave;
         const uint64_t g0 = A.off[c];
         const uint64_t gn = A.off[c + 1] - g0;
-        const bool valid = start + lane < gn;
+        const bool valid = start + lane < gn && g0 + start + lane < A.qcap;
         const uint64_t q = valid ? A.qlist[g0 + start + lane] : 0;
         double qp[DM];
 #pragma unroll
@@ -788,12 +1198,17 @@ extern "C" int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const u
         stride += b->count[l];
     }
     if (stride == 0) stride = 1;
-    // workspace: thr [2nq] | cntQ [nq] | offQ [nq+1] | flags [nq*stride bytes]
-    const uint64_t words = 2 * nq + nq + nq + 1;
+    // split: levels 0..L-2 query-major, the last level cluster-major
+    const bool split = clusterMajor && L >= 2;
+    // workspace: thr | tau | bnd [2nq each] | ub [nq] | cntQ [nq] | offQ [nq+1] | flags [nq*stride bytes]
+    const uint64_t words = 6 * nq + nq + nq + nq + 1;
     int rc = ws_reserve(ctx, 8 * words + nq * stride + 64);
     if (rc) return rc;
     uint64_t* thr = static_cast<uint64_t*>(ctx->ws);
-    uint64_t* cntQ = thr + 2 * nq;
+    uint64_t* tau = thr + 2 * nq;
+    uint64_t* bnd = tau + 2 * nq;
+    double* ubq = reinterpret_cast<double*>(bnd + 2 * nq);
+    uint64_t* cntQ = reinterpret_cast<uint64_t*>(ubq + nq);
     uint64_t* offQ = cntQ + nq;
     uint8_t* flags = reinterpret_cast<uint8_t*>(offQ + nq + 1);
     uint64_t* ctl = b->ctl;
@@ -826,6 +1241,8 @@ extern "C" int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const u
     A.outN = d_out_n;
     A.survStat = d_level_surv;
     A.fineStat = d_fine_count;
+    A.split = split ? 1 : 0;
+    A.tau = tau;
     if (clusterMajor) {
         A.pairs = b->pairs;
         A.pairCap = b->pairCap;
@@ -836,25 +1253,144 @@ extern "C" int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const u
         launch_fill(b, cntQ, 0, nq);
     }
     const uint64_t groups = (nq + kKnnQB - 1) / kKnnQB;
+    const uint64_t wgGrid = std::min<uint64_t>(groups, static_cast<uint64_t>(ctx->cuCount) * b->knnWgPerCu);
+    const uint64_t recCap = std::min(b->hitCap, b->pairCap);
+    const dim3 tK(kKnnThreads);
     {
-        const uint64_t grid = std::min<uint64_t>(groups, static_cast<uint64_t>(ctx->cuCount) * b->knnWgPerCu);
         TimedScope ts(ctx, kKBatch0 + 5);
-        const dim3 g(static_cast<unsigned>(grid)), t(kKnnThreads);
-        if (A.D <= 4) hipLaunchKernelGGL(k_knn_batch<4>, g, t, 0, ctx->stream, A);
-        else if (A.D <= 8) hipLaunchKernelGGL(k_knn_batch<8>, g, t, 0, ctx->stream, A);
-        else if (A.D <= 16) hipLaunchKernelGGL(k_knn_batch<16>, g, t, 0, ctx->stream, A);
-        else hipLaunchKernelGGL(k_knn_batch<32>, g, t, 0, ctx->stream, A);
+        const dim3 g(static_cast<unsigned>(wgGrid));
+        if (A.D <= 4) hipLaunchKernelGGL(k_knn_batch<4>, g, tK, 0, ctx->stream, A);
+        else if (A.D <= 8) hipLaunchKernelGGL(k_knn_batch<8>, g, tK, 0, ctx->stream, A);
+        else if (A.D <= 16) hipLaunchKernelGGL(k_knn_batch<16>, g, tK, 0, ctx->stream, A);
+        else hipLaunchKernelGGL(k_knn_batch<32>, g, tK, 0, ctx->stream, A);
         TB_HIP(hipGetLastError());
     }
-    if (clusterMajor) {
+    const uint64_t stepBlocks = std::min<uint64_t>((b->itemCap + kStepWaves - 1) / kStepWaves,
+                                                   static_cast<uint64_t>(ctx->cuCount) * 16);
+    const dim3 sg(static_cast<unsigned>(stepBlocks ? stepBlocks : 1)), st(kStepWaves * kWave);
+    // refine groups: set 1 after a split last level, set 0 otherwise
+    const uint64_t* rItems = b->items;
+    const uint64_t* rN = ctl + 1;
+    const uint64_t* rOff = b->off;
+    const uint32_t* rQ = b->qlist;
+    if (split) {
+        if ((rc = launch_grouping(b, b->count[L - 2]))) return rc;
+        KnnLastArgs LA{};
+        LA.D = b->D;
+        LA.L = L;
+        LA.nq = nq;
+        LA.items = b->items;
+        LA.nItems = ctl + 1;
+        LA.itemCap = b->itemCap;
+        LA.off = b->off;
+        LA.qlist = b->qlist;
+        LA.qcap = b->pairCap;
+        LA.colResPrev = b->colRes[L - 2];
+        LA.coords = b->coords[L - 1];
+        LA.radius = b->radiusPos[L - 1];
+        LA.ids = b->ids[L - 1];
+        LA.weight = b->weightPos[L - 1];
+        LA.points = d_points;
+        LA.Ks = d_K;
+        LA.tau = tau;
+        LA.ub = ubq;
+        LA.bnd = bnd;
+        LA.thr = thr;
+        LA.recK = b->hits;
+        LA.recV = b->hitsTmp;
+        LA.recCap = recCap;
+        LA.recCount = ctl + 2;
+        LA.cntQ = cntQ;
+        LA.cnt = b->cnt;
+        LA.offNext = b->off2;
+        LA.qlistNext = b->qlist2;
+        LA.cap = b->pairCap;
+        LA.overflow = ctl + 3;
+        auto launchLast = [&](int pass) {
+            TimedScope ts(ctx, kKBatch0 + 9);
+#define TB_LAST(DMV)                                                                             \
+    do {                                                                                         \
+        if (pass == kCand) hipLaunchKernelGGL((k_knn_last<DMV, kCand>), sg, st, 0, ctx->stream, LA);     \
+        else if (pass == kLCount)                                                                \
+            hipLaunchKernelGGL((k_knn_last<DMV, kLCount>), sg, st, 0, ctx->stream, LA);          \
+        else hipLaunchKernelGGL((k_knn_last<DMV, kLWrite>), sg, st, 0, ctx->stream, LA);         \
+    } while (0)
+            if (LA.D <= 4) TB_LAST(4);
+            else if (LA.D <= 8) TB_LAST(8);
+            else if (LA.D <= 16) TB_LAST(16);
+            else TB_LAST(32);
+#undef TB_LAST
+        };
+        // candidates for each query's best M last-level clusters, grouped per query
+        launchLast(kCand);
+        TB_HIP(hipGetLastError());
+        if ((rc = launch_count_scan(b, cntQ, nq, offQ))) return rc;
+        {
+            const uint64_t g = std::min<uint64_t>((recCap + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 8);
+            hipLaunchKernelGGL(k_rec_scatter, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream,
+                               b->hits, b->hitsTmp, ctl + 2, recCap, cntQ, offQ, b->pairs, b->recAux);
+        }
+        launch_fill(b, ctl + 2, 0, 1);
+        KnnSelectArgs SA{};
+        SA.D = b->D;
+        SA.L = L;
+        SA.nq = nq;
+        SA.points = d_points;
+        SA.Ks = d_K;
+        SA.Kmax = Kmax;
+        SA.emptyLast = b->emptyCols[L - 1];
+        SA.seedMin = static_cast<uint32_t>(b->knnSeed);
+        SA.offQ = offQ;
+        SA.gK = b->pairs;
+        SA.gV = b->recAux;
+        SA.weightById = b->nnzPerCol[L - 1];
+        SA.radiusById = b->radiusById[L - 1];
+        SA.colRes = b->colRes[L - 1];
+        SA.coordsFine = b->coords[L];
+        SA.idsFine = b->ids[L];
+        SA.ub = ubq;
+        SA.bnd = bnd;
+        SA.thr = thr;
+        SA.outIds = d_out_ids;
+        SA.outD2 = d_out_d2;
+        SA.outN = d_out_n;
+        SA.survStat = d_level_surv;
+        SA.fineStat = d_fine_count;
+        {
+            TimedScope ts(ctx, kKBatch0 + 10);
+            const dim3 g(static_cast<unsigned>(wgGrid));
+            if (SA.D <= 4) hipLaunchKernelGGL(k_knn_select<4>, g, tK, 0, ctx->stream, SA);
+            else if (SA.D <= 8) hipLaunchKernelGGL(k_knn_select<8>, g, tK, 0, ctx->stream, SA);
+            else if (SA.D <= 16) hipLaunchKernelGGL(k_knn_select<16>, g, tK, 0, ctx->stream, SA);
+            else hipLaunchKernelGGL(k_knn_select<32>, g, tK, 0, ctx->stream, SA);
+            TB_HIP(hipGetLastError());
+        }
+        // survivors of the last level: statistics and the pairs to refine
+        LA.survStat = d_level_surv;
+        LA.fineStat = d_fine_count;
+        launchLast(kLCount);
+        TB_HIP(hipGetLastError());
+        if ((rc = launch_group_scan(b, b->count[L - 1], b->off2, b->items2, ctl + 4))) return rc;
+        LA.survStat = nullptr;
+        LA.fineStat = nullptr;
+        launchLast(kLWrite);
+        TB_HIP(hipGetLastError());
+        rItems = b->items2;
+        rN = ctl + 4;
+        rOff = b->off2;
+        rQ = b->qlist2;
+    } else if (clusterMajor) {
         if ((rc = launch_grouping(b, b->count[L - 1]))) return rc;
+    }
+    if (clusterMajor) {
         KnnClusterArgs C{};
         C.D = b->D;
-        C.items = b->items;
-        C.nItems = ctl + 1;
+        C.items = rItems;
+        C.nItems = rN;
         C.itemCap = b->itemCap;
-        C.off = b->off;
-        C.qlist = b->qlist;
+        C.off = rOff;
+        C.qlist = rQ;
+        C.qcap = b->pairCap;
         C.colRes = b->colRes[L - 1];
         C.coords = b->coords[L];
         C.ids = b->ids[L];
@@ -862,27 +1398,24 @@ extern "C" int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const u
         C.thr = thr;
         C.recK = b->hits;
         C.recV = b->hitsTmp;
-        C.recCap = std::min(b->hitCap, b->pairCap);
+        C.recCap = recCap;
         C.recCount = ctl + 2;
         C.cntQ = cntQ;
         C.overflow = ctl + 3;
         {
-            uint64_t blocks = std::min<uint64_t>((b->itemCap + kStepWaves - 1) / kStepWaves,
-                                                 static_cast<uint64_t>(ctx->cuCount) * 16);
-            const dim3 g(static_cast<unsigned>(blocks ? blocks : 1)), t(kStepWaves * kWave);
             TimedScope ts(ctx, kKBatch0 + 7);
-            if (C.D <= 4) hipLaunchKernelGGL(k_knn_cluster<4>, g, t, 0, ctx->stream, C);
-            else if (C.D <= 8) hipLaunchKernelGGL(k_knn_cluster<8>, g, t, 0, ctx->stream, C);
-            else if (C.D <= 16) hipLaunchKernelGGL(k_knn_cluster<16>, g, t, 0, ctx->stream, C);
-            else hipLaunchKernelGGL(k_knn_cluster<32>, g, t, 0, ctx->stream, C);
+            if (C.D <= 4) hipLaunchKernelGGL(k_knn_cluster<4>, sg, st, 0, ctx->stream, C);
+            else if (C.D <= 8) hipLaunchKernelGGL(k_knn_cluster<8>, sg, st, 0, ctx->stream, C);
+            else if (C.D <= 16) hipLaunchKernelGGL(k_knn_cluster<16>, sg, st, 0, ctx->stream, C);
+            else hipLaunchKernelGGL(k_knn_cluster<32>, sg, st, 0, ctx->stream, C);
             TB_HIP(hipGetLastError());
         }
         if ((rc = launch_count_scan(b, cntQ, nq, offQ))) return rc;
         {
             TimedScope ts(ctx, kKBatch0 + 8);
-            const uint64_t g = std::min<uint64_t>((C.recCap + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 8);
+            const uint64_t g = std::min<uint64_t>((recCap + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 8);
             hipLaunchKernelGGL(k_rec_scatter, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream,
-                               b->hits, b->hitsTmp, ctl + 2, C.recCap, cntQ, offQ, b->pairs, b->recAux);
+                               b->hits, b->hitsTmp, ctl + 2, recCap, cntQ, offQ, b->pairs, b->recAux);
             KnnMergeArgs M{};
             M.nq = nq;
             M.Ks = d_K;

@@ -180,7 +180,18 @@ def test_batch_knn_query_major_mode():
 
 def test_batch_knn_overflow_splits():
     """Pair and record capacities far below the batch's needs: the driver splits and still matches."""
-    h, o = _make(dict(CFG, N=20000, seed=4), max_q=128, pair_cap=600, hit_cap=300)
+    h, o = _make(dict(CFG, N=20000, seed=4), max_q=128, pair_cap=1500, hit_cap=1500)
     data = h.data()
     pts = W.knn_points(data, 48, 8, h.mesh_sizes, CFG["vmin"], CFG["vmax"], noise=0.2)
     _check_knn_batch(h, o, pts, np.full(len(pts), 20))
+
+
+@pytest.mark.parametrize("ratios", [(40,), (30, 10), (20, 8, 4, 3)])
+def test_batch_knn_hierarchy_heights(ratios):
+    """One level (no parent level to split on), two levels (the split parent is the coarsest level)
+    and four levels."""
+    h, o = _make(dict(CFG, N=24000, ratios=ratios, seed=13), max_q=96)
+    data = h.data()
+    pts = W.knn_points(data, 60, 17, h.mesh_sizes, CFG["vmin"], CFG["vmax"], noise=0.2)
+    Ks = np.random.default_rng(3).choice([0, 1, 7, 50, 120], len(pts))
+    _check_knn_batch(h, o, pts, Ks)
