@@ -9,8 +9,10 @@ exact answers (range: sorted id sets; KNN: top-K ids and squared distances).
 Reported:
   value           queries/s of the fastest path measured (whole job, 1 GPU)
   plain           the per-query GPU traversal (QueryHandler -> QueryEngine -> L0)
-  roofline        dominant kernel: algorithmic bytes per launch / average launch
-                  time, timed with HIP events on the library's stream
+  roofline        dominant batched kernel (the exact box filter): FP64 compare
+                  operations per launch / average launch time (HIP events on
+                  the library's stream) against the FP64 peak; the plain
+                  path's dominant HBM kernel is reported beside it
   cpu_baseline    the OpenMP CPU traversal (the oracle) on the host cores
 """
 import argparse
@@ -28,6 +30,21 @@ import numpy as np  # noqa: E402
 
 METRIC = "range+KNN queries/s at N=10M D=8, batched/learned vs plain; p99 latency"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def final_traffic(cfg):
+    """HBM bytes per launch of the exact box filter, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary for this workload (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "c2_final_filter_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("N") != cfg["N"] or t.get("D") != cfg["D"]:
+        return None
+    return t.get("bytes_per_launch")
 
 
 def log(*a):
@@ -96,6 +113,9 @@ def main():
     ap.add_argument("--pair-cap", type=int, default=1 << 28)
     ap.add_argument("--hit-cap", type=int, default=1 << 28)
     ap.add_argument("--small", action="store_true", help="N=500k D=5 hierarchy (quick check)")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB once on the GPU (a kernel of known "
+                         "traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (queries would shard across "
@@ -146,43 +166,85 @@ def main():
     per_launch_bytes = kbytes / max(n_l, 1)
     avg_s = t_ms / 1e3 / max(n_l, 1)
     achieved = per_launch_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    plain_roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": {"prolong": "k_prolong", "compact": "k_compact", "filter": "k_compact<2>"}[best],
                 "launches": n_l, "avg_launch_us": round(avg_s * 1e6, 3), "bytes_per_launch": round(per_launch_bytes)}
 
     # ---- batched path: the whole batch (nq range + nq KNN) per step
     batched = None
+    roofline = plain_roofline
     if not args.no_batch:
         h.batch_init(args.nq, args.pair_cap, args.hit_cap)
         bK = Ks.astype(np.uint64)
+        # plain launches (not the captured graph) in the timed steps, so that
+        # every kernel is bracketed by HIP events on the library's stream
+        h.batch_use_graph(False)
         t_r, t_k = [], []
         tot_hits = tot_nn = 0
+        kinds = (("range_prune", 8), ("range_final", 9), ("group_scan", 10), ("group_scatter", 11),
+                 ("radix_sort", 12), ("knn_levels", 13), ("knn_last_passes", 17), ("knn_select", 18),
+                 ("knn_refine", 15), ("knn_merge", 16))
+        kacc = {n: [0, 0.0] for n, _ in kinds}
+        work_r = work_k = (0, 0)
         for it in range(args.warmup + args.steps):
             h.batch_set("range", boxes)
             timing.reset()
             t0 = time.perf_counter()
             tot_hits = h.batch_run()
             t1 = time.perf_counter()
+            work_r = h.batch_work()
             h.batch_set("knn", knn_q, bK)
             t2 = time.perf_counter()
             tot_nn = h.batch_run()
             t3 = time.perf_counter()
+            work_k = h.batch_work()
             if it >= args.warmup:
                 t_r.append(t1 - t0)
                 t_k.append(t3 - t2)
+                for n, k in kinds:
+                    c, ms = timing.get(k)
+                    kacc[n][0] += c
+                    kacc[n][1] += ms
             log("batched step %d: range %.2f ms (%d hits), knn %.2f ms (%d neighbours)"
                 % (it, (t1 - t0) * 1e3, tot_hits, (t3 - t2) * 1e3, tot_nn))
-        bk = {}
-        for name, kind in (("range_prune", 8), ("range_final", 9), ("group_scan", 10), ("group_scatter", 11),
-                           ("radix_sort", 12), ("knn_batch", 13)):
-            bk[name] = timing.get(kind)
+        # the same range batch as one captured hipGraph launch (untimed in the step)
+        h.batch_use_graph(True)
+        h.batch_set("range", boxes)
+        h.batch_run()
+        tg = []
+        for _ in range(max(args.steps, 2)):
+            t0 = time.perf_counter()
+            h.batch_run()
+            tg.append(time.perf_counter() - t0)
+        h.batch_use_graph(False)
         step_s = float(np.mean(t_r)) + float(np.mean(t_k))
+        per_step = {n: round(v[1] / args.steps, 3) for n, v in kacc.items()}
         batched = {"qps": round(2 * args.nq / step_s, 3), "range_ms": round(float(np.mean(t_r)) * 1e3, 3),
                    "knn_ms": round(float(np.mean(t_k)) * 1e3, 3), "hits": int(tot_hits), "neighbours": int(tot_nn),
-                   "kernels_ms_last_step": {k: round(v[1], 3) for k, v in bk.items()},
+                   "range_graph_ms": round(float(np.min(tg)) * 1e3, 3),
+                   "kernels_ms_per_step": per_step,
+                   "final_filter_tests": work_r[0], "final_filter_rows": work_r[1],
+                   "knn_refine_evals": work_k[0], "knn_refine_rows": work_k[1],
                    "speedup_vs_plain": round(2 * args.nq / step_s / plain_qps, 2)}
         log("batched", batched)
+        # roofline of the dominant batched kernel: the exact box filter (one
+        # launch per range batch).  Its work is 2D FP64 compares per candidate
+        # row x box (work_r[0] tests); it is bounded by FP64 vector issue, not
+        # by HBM (each child row is read once per 64-box slice).
+        n_f, ms_f = kacc["range_final"]
+        if n_f:
+            avg_s = ms_f / 1e3 / n_f
+            ops = 2.0 * cfg["D"] * work_r[0]
+            ach = ops / avg_s / 1e12
+            roofline = {"bound": "mfma", "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": final_traffic(cfg),
+                        "kernel": "k_range_step<8,kFinal> (exact box filter of the batched range path)",
+                        "launches": n_f, "avg_launch_us": round(avg_s * 1e6, 3), "ops_per_launch": ops,
+                        "note": "FP64 vector compares (2D per candidate row x box), counted against MI355X's FP64 "
+                                "peak (vector and matrix both 78.6 TFLOP/s); the kernel reads each child row once "
+                                "per 64-box slice, so HBM is not its bound",
+                        "plain": plain_roofline}
 
     cpu = None
     if not args.no_cpu:
@@ -222,6 +284,23 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.calib_copy_mb:
+        # device-to-device hipMemcpy: the runtime's copy kernel, n bytes read and n written
+        import ctypes
+
+        hip = ctypes.CDLL("libamdhip64.so")
+        n = args.calib_copy_mb * (1 << 20)
+        src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+        if hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(n)) or hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(n)):
+            raise RuntimeError("hipMalloc failed for the calibration copy")
+        hip.hipMemset(src, 1, ctypes.c_size_t(n))
+        hip.hipDeviceSynchronize()
+        if hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3):
+            raise RuntimeError("hipMemcpy failed for the calibration copy")
+        hip.hipDeviceSynchronize()
+        hip.hipFree(src)
+        hip.hipFree(dst)
+        log("calibration copy: %d bytes read, %d bytes written" % (n, n))
     return 0
 
 

@@ -181,7 +181,9 @@ int tblaeq_batch_reserve(tblaeq_batch* b, uint64_t max_queries, uint64_t pair_ca
 /* nq range boxes (device, rows lo|hi).  Per-query hit ids, ascending, in
  * d_out_ids[d_out_off[q] .. d_out_off[q+1]).  d_status[0] = total hits,
  * d_status[1] = overflow flags (non-zero: capacity exceeded, results invalid;
- * split the batch).  Optional d_level_surv[q*L + l] (survivors per level) and
+ * split the batch), d_status[2] = box tests of the final filter (candidate
+ * rows x queries), d_status[3] = candidate rows it scanned (d_status holds 4
+ * words).  Optional d_level_surv[q*L + l] (survivors per level) and
  * d_fine_count[q] (candidate superset size) must be zeroed by the caller.
  * Asynchronous on the context stream; no host synchronisation inside. */
 int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64_t nq, uint64_t* d_out_off, uint64_t* d_out_ids,
@@ -197,7 +199,9 @@ int tblaeq_batch_range_graph(tblaeq_batch* b, const double* d_boxes, uint64_t nq
  * d_out_n[q] = number written (min(K, candidates)).  Same traversal (STEP per
  * level) as the per-query path.  d_status[0] = candidate records refined
  * cluster-major, d_status[1] = overflow flags (non-zero: a capacity of
- * tblaeq_batch_reserve was exceeded, results invalid; split the batch).
+ * tblaeq_batch_reserve was exceeded, results invalid; split the batch),
+ * d_status[2] / [3] = distance evaluations / candidate rows of the
+ * cluster-major refine (d_status holds 4 words).
  * Requires Kmax + empty clusters + 1 <= 256 and at most 8 levels (else
  * TBLAEQ_ERR_UNSUPPORTED).  d_level_surv / d_fine_count as for
  * tblaeq_batch_range (written, not accumulated). */

@@ -84,6 +84,11 @@ int tblaeq_handler_batch_results(tblaeq_handler* h, uint64_t* off, uint64_t* ids
 int tblaeq_handler_batch_stats(tblaeq_handler* h, uint64_t* levelSurv, uint64_t* fine);
 /* Range batches run as a captured hipGraph (default on) or as plain launches. */
 int tblaeq_handler_batch_use_graph(tblaeq_handler* h, int on);
+/* Work of the last batch_run's final stage (summed over split sub-batches):
+ * work[0] = tests (range: candidate rows x boxes of the exact filter; KNN:
+ * distance evaluations of the cluster-major refine), work[1] = candidate rows
+ * scanned. */
+int tblaeq_handler_batch_work(tblaeq_handler* h, uint64_t* work);
 /* tblaeq_batch_set_param on the handler's engine (TBLAEQ_PARAM_* of tblaeq.h). */
 int tblaeq_handler_batch_set_param(tblaeq_handler* h, int param, uint64_t value);
 int tblaeq_handler_batch_free(tblaeq_handler* h);

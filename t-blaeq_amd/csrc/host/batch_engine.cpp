@@ -96,6 +96,7 @@ int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total) {
         IndexData& idx = h->h->index();
         const size_t D = idx.D, L = idx.intervals;
         hipStream_t s = streamOf(idx);
+        st->work[0] = st->work[1] = 0;
         if (stats) {
             hipCheck(hipMemsetAsync(st->dSurv.data(), 0, st->nq * L * 8, s), "memset");
             hipCheck(hipMemsetAsync(st->dFine.data(), 0, st->nq * 8, s), "memset");
@@ -111,8 +112,8 @@ int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total) {
                                           st->dN.data() + a, st->dStatus.data(), stats ? st->dSurv.data() + a * L : nullptr,
                                           stats ? st->dFine.data() + a : nullptr);
                 if (rc) return rc;
-                uint64_t status[2];
-                hipCheck(hipMemcpyAsync(status, st->dStatus.data(), 16, hipMemcpyDeviceToHost, s), "D2H");
+                uint64_t status[4];
+                hipCheck(hipMemcpyAsync(status, st->dStatus.data(), 32, hipMemcpyDeviceToHost, s), "D2H");
                 hipCheck(hipStreamSynchronize(s), "sync");
                 if (status[1] != 0) {
                     if (n == 1) {
@@ -121,7 +122,10 @@ int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total) {
                     }
                     todo.push_back({a + n / 2, n - n / 2});
                     todo.push_back({a, n / 2});
+                    continue;
                 }
+                st->work[0] += status[2];
+                st->work[1] += status[3];
             }
             if (total) {
                 std::vector<uint64_t> n(st->nq);
@@ -136,10 +140,12 @@ int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total) {
             int rc = tblaeq_batch_range_graph(st->b, st->dQ.data(), st->nq, st->dOff.data(), st->dIds.data(),
                                               st->dIds.size(), st->dStatus.data());
             if (rc) return rc;
-            uint64_t status[2];
-            hipCheck(hipMemcpyAsync(status, st->dStatus.data(), 16, hipMemcpyDeviceToHost, s), "D2H");
+            uint64_t status[4];
+            hipCheck(hipMemcpyAsync(status, st->dStatus.data(), 32, hipMemcpyDeviceToHost, s), "D2H");
             hipCheck(hipStreamSynchronize(s), "sync");
             if (status[1] == 0) {
+                st->work[0] = status[2];
+                st->work[1] = status[3];
                 st->lastTotal = status[0];
                 if (total) *total = status[0];
                 return TBLAEQ_OK;
@@ -160,8 +166,8 @@ int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total) {
                                         stats ? st->dSurv.data() + a * L : nullptr,
                                         stats ? st->dFine.data() + a : nullptr);
             if (rc) return rc;
-            uint64_t status[2];
-            hipCheck(hipMemcpyAsync(status, st->dStatus.data(), 16, hipMemcpyDeviceToHost, s), "D2H");
+            uint64_t status[4];
+            hipCheck(hipMemcpyAsync(status, st->dStatus.data(), 32, hipMemcpyDeviceToHost, s), "D2H");
             hipCheck(hipStreamSynchronize(s), "sync");
             if (status[1] != 0) {
                 if (n == 1) {
@@ -195,6 +201,8 @@ int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total) {
                 for (uint64_t i = 0; i <= n; ++i) hostOff[a + i] = base + off[i];
             }
             base += status[0];
+            st->work[0] += status[2];
+            st->work[1] += status[3];
         }
         if (split) hipCheck(hipMemcpy(st->dOff.data(), hostOff.data(), (st->nq + 1) * 8, hipMemcpyHostToDevice), "H2D");
         st->lastTotal = base;
@@ -246,6 +254,14 @@ int tblaeq_handler_batch_set_param(tblaeq_handler* h, int param, uint64_t value)
         return TBLAEQ_ERR_INVALID;
     }
     return tblaeq_batch_set_param(st->b, param, value);
+}
+
+int tblaeq_handler_batch_work(tblaeq_handler* h, uint64_t* work) {
+    BatchState* st = stateOf(h);
+    if (!st || !work) return TBLAEQ_ERR_INVALID;
+    work[0] = st->work[0];
+    work[1] = st->work[1];
+    return TBLAEQ_OK;
 }
 
 int tblaeq_handler_batch_free(tblaeq_handler* h) {

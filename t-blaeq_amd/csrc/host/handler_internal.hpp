@@ -16,6 +16,7 @@ struct BatchState {
     DeviceBuffer<uint64_t> dK, dOff, dOffTmp, dIds, dStatus, dSurv, dFine, dN;
     DeviceBuffer<double> dD2;
     uint64_t lastTotal = 0;
+    uint64_t work[2] = {0, 0};  // last run: final-stage tests and candidate rows (summed over sub-batches)
     bool useGraph = true;
     ~BatchState() {
         if (b) tblaeq_batch_destroy(b);

@@ -103,6 +103,7 @@ struct RangeStepArgs {
     uint64_t* survStat;  // [nq * L]
     uint64_t* fineStat;  // [nq]
     int level, L;
+    uint64_t* work;      // final: [0] box tests, [1] child rows scanned
 };
 
 constexpr int kHitStage = 512;
@@ -125,7 +126,7 @@ template <int DM, int MODE>
 __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs A) {
     constexpr bool FINAL = MODE == kFinal;
     __shared__ uint64_t s_hit[kStepWaves][FINAL ? kHitStage : 1];
-    uint64_t staged = 0;
+    uint64_t staged = 0, wTests = 0, wRows = 0;
     __shared__ double s_x[kStepWaves][kWave * DM];
     __shared__ double s_r[kStepWaves][FINAL ? 1 : kWave];
     __shared__ float s_ab[kStepWaves][FINAL ? 1 : kWave * 2 * DM];
@@ -165,6 +166,10 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
             q = valid ? A.qlist[g0 + start + lane] : 0;
             jb = A.colResPrev[c];
             je = A.colResPrev[c + 1];
+        }
+        if (FINAL) {
+            wRows += je - jb;
+            wTests += (je - jb) * static_cast<uint64_t>(__popcll(__ballot(valid)));
         }
         double lo[DM], hi[DM];
         float flo[DM], fhi[DM];
@@ -319,7 +324,14 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
                 atomicAdd(reinterpret_cast<unsigned long long*>(&A.fineStat[q]), fine);
         }
     }
-    if (FINAL) flush_hits(A, s_hit[wave], staged, lane);
+    if (FINAL) {
+        flush_hits(A, s_hit[wave], staged, lane);
+        // work done by this wave: box tests (child rows x queries) and child rows
+        if (lane == 0 && A.work) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(A.work), wTests);
+            atomicAdd(reinterpret_cast<unsigned long long*>(A.work + 1), wRows);
+        }
+    }
 }
 
 // Group scan over the clusters of one level: off[i] = exclusive prefix of
@@ -619,6 +631,8 @@ __global__ void k_copy_status(const uint64_t* ctl, uint64_t* status) {
     if (threadIdx.x == 0) {
         status[0] = ctl[2];
         status[1] = ctl[3];
+        status[2] = ctl[8];
+        status[3] = ctl[9];
     }
 }
 
@@ -963,7 +977,7 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     tblaeq_ctx* ctx = b->ctx;
     const int L = b->L;
     uint64_t* ctl = b->ctl;
-    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, ctx->stream, ctl, 0ull, 8);
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, ctx->stream, ctl, 0ull, 16);
     RangeStepArgs A{};
     A.D = b->D;
     A.nq = nq;
@@ -1041,6 +1055,7 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     A.cap = std::min(b->hitCap, out_cap);
     A.outCount = ctl + 2;
     A.idBits = idBits;
+    A.work = ctl + 8;
     if ((rc = launch_range_step<kFinal>(ctx, A, b->itemCap))) return rc;
     uint64_t* sorted = nullptr;
     if ((rc = radix_sort(ctx, b->hits, b->hitsTmp, ctl + 2, A.cap, idBits + qBits, b->hist, &sorted, false, 0)))

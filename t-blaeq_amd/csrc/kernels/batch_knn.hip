@@ -942,6 +942,7 @@ struct KnnClusterArgs {
     uint64_t* recCount;
     uint64_t* cntQ;           // records per query
     uint64_t* overflow;
+    uint64_t* work;           // [0] distance evaluations (child rows x queries), [1] child rows
 };
 
 template <int DM>
@@ -957,6 +958,7 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_cluster(KnnClusterAr
     if (nItems > A.itemCap) nItems = A.itemCap;
     const uint64_t ltmask = (1ull << lane) - 1ull;
     double* sx = s_x[wave];
+    uint64_t wTests = 0, wRows = 0;
     if (D != DM)
         for (int t = lane; t < kWave * DM; t += kWave) sx[t] = 0.0;
     for (uint64_t item = gw; item < nItems; item += nw) {
@@ -1017,6 +1019,12 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_cluster(KnnClusterAr
             wave_lds_sync();
         }
         if (valid && mine) atomicAdd(reinterpret_cast<unsigned long long*>(&A.cntQ[q]), mine);
+        wRows += je - jb;
+        wTests += (je - jb) * static_cast<uint64_t>(__popcll(__ballot(valid)));
+    }
+    if (lane == 0 && A.work) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.work), wTests);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.work + 1), wRows);
     }
 }
 
@@ -1212,7 +1220,7 @@ extern "C" int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const u
     uint64_t* offQ = cntQ + nq;
     uint8_t* flags = reinterpret_cast<uint8_t*>(offQ + nq + 1);
     uint64_t* ctl = b->ctl;
-    launch_fill(b, ctl, 0, 8);
+    launch_fill(b, ctl, 0, 16);
     A.D = b->D;
     A.L = L;
     A.nq = nq;
@@ -1402,6 +1410,7 @@ extern "C" int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const u
         C.recCount = ctl + 2;
         C.cntQ = cntQ;
         C.overflow = ctl + 3;
+        C.work = ctl + 8;
         {
             TimedScope ts(ctx, kKBatch0 + 7);
             if (C.D <= 4) hipLaunchKernelGGL(k_knn_cluster<4>, sg, st, 0, ctx->stream, C);

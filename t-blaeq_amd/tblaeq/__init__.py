@@ -79,6 +79,7 @@ SIGNATURES = {
     "tblaeq_batch_set_param": (C.c_int, [vp, C.c_int, C.c_uint64]),
     "tblaeq_batch_get_param": (C.c_int, [vp, C.c_int, u64p]),
     "tblaeq_handler_batch_set_param": (C.c_int, [vp, C.c_int, C.c_uint64]),
+    "tblaeq_handler_batch_work": (C.c_int, [vp, u64p]),
 }
 
 PARAM_KNN_SEED = 1
@@ -215,6 +216,12 @@ class Handler:
 
     def batch_use_graph(self, on=True):
         check(lib().tblaeq_handler_batch_use_graph(self.h, int(on)), "batch_use_graph")
+
+    def batch_work(self):
+        """(tests, candidate rows) of the last batch_run's final stage."""
+        w = np.zeros(2, np.uint64)
+        check(lib().tblaeq_handler_batch_work(self.h, _p(w, u64p)), "batch_work")
+        return int(w[0]), int(w[1])
 
     def batch_set_param(self, param, value):
         """Launch parameter of the batched engine (PARAM_*); results do not depend on it."""
