@@ -177,6 +177,12 @@ def main():
     if not args.no_batch:
         h.batch_init(args.nq, args.pair_cap, args.hit_cap)
         bK = Ks.astype(np.uint64)
+        # launch parameters from the autotuner on this batch (untimed set-up)
+        tuned = h.batch_autotune(boxes, knn_q, bK, reps=3)
+        tuned_params = {name: h.batch_get_param(pid) for name, pid in (
+            ("knn_seed", 1), ("knn_wg_per_cu", 2), ("knn_mode", 3), ("range_blocks_per_cu", 4),
+            ("step_blocks_per_cu", 5), ("radix_blocks_per_cu", 6), ("hit_stage", 7))}
+        log("autotune", tuned, tuned_params)
         # plain launches (not the captured graph) in the timed steps, so that
         # every kernel is bracketed by HIP events on the library's stream
         h.batch_use_graph(False)
@@ -226,7 +232,10 @@ def main():
                    "kernels_ms_per_step": per_step,
                    "final_filter_tests": work_r[0], "final_filter_rows": work_r[1],
                    "knn_refine_evals": work_k[0], "knn_refine_rows": work_k[1],
-                   "speedup_vs_plain": round(2 * args.nq / step_s / plain_qps, 2)}
+                   "speedup_vs_plain": round(2 * args.nq / step_s / plain_qps, 2),
+                   "autotune": {"range_ms_before_after": [round(x, 3) for x in tuned["range_ms"]],
+                                "knn_ms_before_after": [round(x, 3) for x in tuned["knn_ms"]],
+                                "params": tuned_params}}
         log("batched", batched)
         # roofline of the dominant batched kernel: the exact box filter (one
         # launch per range batch).  Its work is 2D FP64 compares per candidate

@@ -211,15 +211,43 @@ int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const uint64_t* d_
 
 /* Launch parameters of the batched engine (no effect on results, only on
  * speed); tblaeq_batch_get_param reads the current value.
- *   TBLAEQ_PARAM_KNN_SEED       last-level clusters expanded first to fix the
- *                               certified radius, 1..256 (default 64)
- *   TBLAEQ_PARAM_KNN_WG_PER_CU  KNN workgroups per CU, 1..16 (default 4)
- *   TBLAEQ_PARAM_KNN_MODE       0: the last level's surviving clusters are
- *                               grouped and refined cluster-major (default);
- *                               1: each workgroup refines its own queries */
-enum { TBLAEQ_PARAM_KNN_SEED = 1, TBLAEQ_PARAM_KNN_WG_PER_CU = 2, TBLAEQ_PARAM_KNN_MODE = 3 };
+ *   TBLAEQ_PARAM_KNN_SEED           last-level clusters expanded first to fix
+ *                                   the certified radius, 1..256 (default 64)
+ *   TBLAEQ_PARAM_KNN_WG_PER_CU      query-major KNN workgroups per CU, 1..16 (4)
+ *   TBLAEQ_PARAM_KNN_MODE           0: the last level's surviving clusters are
+ *                                   grouped and refined cluster-major (default);
+ *                                   1: each workgroup refines its own queries
+ *   TBLAEQ_PARAM_RANGE_BLOCKS_PER_CU  range level kernels, 1..64 (16)
+ *   TBLAEQ_PARAM_STEP_BLOCKS_PER_CU   cluster-major KNN kernels, 1..64 (16)
+ *   TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU  hit radix sort, 1..32 (4)
+ *   TBLAEQ_PARAM_HIT_STAGE          LDS hit stage per wave of the final range
+ *                                   filter: 256, 512 or 1024 (512) */
+enum {
+    TBLAEQ_PARAM_KNN_SEED = 1,
+    TBLAEQ_PARAM_KNN_WG_PER_CU = 2,
+    TBLAEQ_PARAM_KNN_MODE = 3,
+    TBLAEQ_PARAM_RANGE_BLOCKS_PER_CU = 4,
+    TBLAEQ_PARAM_STEP_BLOCKS_PER_CU = 5,
+    TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU = 6,
+    TBLAEQ_PARAM_HIT_STAGE = 7
+};
 int tblaeq_batch_set_param(tblaeq_batch* b, int param, uint64_t value);
 int tblaeq_batch_get_param(const tblaeq_batch* b, int param, uint64_t* value);
+
+/* Autotuner: times candidate values of the launch parameters on a sample
+ * batch (n_boxes range boxes and / or n_points KNN points with K in d_K,
+ * device memory) with HIP events, one parameter at a time, and keeps the
+ * fastest; every candidate's results are hashed on the device and must equal
+ * the starting configuration's (else TBLAEQ_ERR_INVALID).  Allocates.
+ * out_ms (optional, 4 doubles): range ms before / after, KNN ms before / after
+ * (median of reps runs).  The sample must fit the reserved capacities. */
+int tblaeq_batch_autotune(tblaeq_batch* b, const double* d_boxes, uint64_t n_boxes, const double* d_points,
+                          const uint64_t* d_K, uint64_t n_points, uint64_t Kmax, int reps, double* out_ms);
+/* Tuning table: one "name value" line per launch parameter, with a comment
+ * header naming the index shape; load applies each line through
+ * tblaeq_batch_set_param. */
+int tblaeq_batch_save_params(const tblaeq_batch* b, const char* path);
+int tblaeq_batch_load_params(tblaeq_batch* b, const char* path);
 
 #ifdef __cplusplus
 }

@@ -89,8 +89,15 @@ int tblaeq_handler_batch_use_graph(tblaeq_handler* h, int on);
  * distance evaluations of the cluster-major refine), work[1] = candidate rows
  * scanned. */
 int tblaeq_handler_batch_work(tblaeq_handler* h, uint64_t* work);
+/* tblaeq_batch_autotune on the handler's engine with host sample queries
+ * (boxes rows lo|hi, KNN points and K per point); out_ms as there. */
+int tblaeq_handler_batch_autotune(tblaeq_handler* h, const double* boxes, uint64_t nBoxes, const double* points,
+                                  const uint64_t* Ks, uint64_t nPoints, int reps, double* outMs);
+int tblaeq_handler_batch_save_params(tblaeq_handler* h, const char* path);
+int tblaeq_handler_batch_load_params(tblaeq_handler* h, const char* path);
 /* tblaeq_batch_set_param on the handler's engine (TBLAEQ_PARAM_* of tblaeq.h). */
 int tblaeq_handler_batch_set_param(tblaeq_handler* h, int param, uint64_t value);
+int tblaeq_handler_batch_get_param(tblaeq_handler* h, int param, uint64_t* value);
 int tblaeq_handler_batch_free(tblaeq_handler* h);
 
 #ifdef __cplusplus

@@ -62,6 +62,12 @@ struct tblaeq_batch {
     uint64_t knnSeed = 64;       // clusters expanded before the certified radius is taken
     uint64_t knnWgPerCu = 4;     // KNN workgroups per CU (grid = min(groups, CUs * this))
     uint64_t knnMode = 0;        // 0 cluster-major refine, 1 query-major refine
+    uint64_t rangeBlocksPerCu = 16;   // range step kernels (4 waves per block)
+    uint64_t stepBlocksPerCu = 16;    // cluster-major KNN passes and refine
+    uint64_t radixBlocksPerCu = 4;    // hit radix sort
+    uint64_t hitStage = 512;          // LDS hits staged per wave by the final filter (256, 512, 1024)
+    uint64_t paramVersion = 0;        // bumped by every parameter change (captured graphs re-capture)
+    uint64_t gParamVersion = 0;
 };
 
 namespace tblaeq {

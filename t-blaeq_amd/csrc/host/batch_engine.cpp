@@ -264,6 +264,50 @@ int tblaeq_handler_batch_work(tblaeq_handler* h, uint64_t* work) {
     return TBLAEQ_OK;
 }
 
+int tblaeq_handler_batch_autotune(tblaeq_handler* h, const double* boxes, uint64_t nBoxes, const double* points,
+                                  const uint64_t* Ks, uint64_t nPoints, int reps, double* outMs) {
+    BatchState* st = stateOf(h);
+    if (!st || (nBoxes && !boxes) || (nPoints && (!points || !Ks))) {
+        tblaeq::set_error("tblaeq_handler_batch_autotune: invalid argument or batch_init not called");
+        return TBLAEQ_ERR_INVALID;
+    }
+    return guarded("tblaeq_handler_batch_autotune", [&]() -> int {
+        const size_t D = h->h->index().D;
+        DeviceBuffer<double> dB, dP;
+        DeviceBuffer<uint64_t> dK;
+        uint64_t Kmax = 0;
+        if (nBoxes) dB.reset(boxes, nBoxes * 2 * D);
+        if (nPoints) {
+            dP.reset(points, nPoints * D);
+            dK.reset(Ks, nPoints);
+            Kmax = *std::max_element(Ks, Ks + nPoints);
+        }
+        return tblaeq_batch_autotune(st->b, nBoxes ? dB.data() : nullptr, nBoxes, nPoints ? dP.data() : nullptr,
+                                     nPoints ? dK.data() : nullptr, nPoints, Kmax, reps, outMs);
+    });
+}
+
+int tblaeq_handler_batch_get_param(tblaeq_handler* h, int param, uint64_t* value) {
+    BatchState* st = stateOf(h);
+    if (!st) {
+        tblaeq::set_error("tblaeq_handler_batch_get_param: call tblaeq_handler_batch_init first");
+        return TBLAEQ_ERR_INVALID;
+    }
+    return tblaeq_batch_get_param(st->b, param, value);
+}
+
+int tblaeq_handler_batch_save_params(tblaeq_handler* h, const char* path) {
+    BatchState* st = stateOf(h);
+    if (!st) return TBLAEQ_ERR_INVALID;
+    return tblaeq_batch_save_params(st->b, path);
+}
+
+int tblaeq_handler_batch_load_params(tblaeq_handler* h, const char* path) {
+    BatchState* st = stateOf(h);
+    if (!st) return TBLAEQ_ERR_INVALID;
+    return tblaeq_batch_load_params(st->b, path);
+}
+
 int tblaeq_handler_batch_free(tblaeq_handler* h) {
     if (h) h->batch.reset();
     return TBLAEQ_OK;

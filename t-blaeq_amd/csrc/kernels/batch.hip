@@ -106,8 +106,6 @@ struct RangeStepArgs {
     uint64_t* work;      // final: [0] box tests, [1] child rows scanned
 };
 
-constexpr int kHitStage = 512;
-
 // Appends a wave's staged hits to the global hit list (entries past the
 // capacity are dropped and flagged).
 __device__ __forceinline__ void flush_hits(const RangeStepArgs& A, const uint64_t* stage, uint64_t n, int lane) {
@@ -122,9 +120,10 @@ __device__ __forceinline__ void flush_hits(const RangeStepArgs& A, const uint64_
     wave_lds_sync();
 }
 
-template <int DM, int MODE>
+template <int DM, int MODE, int STAGE>
 __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs A) {
     constexpr bool FINAL = MODE == kFinal;
+    constexpr int kHitStage = STAGE;
     __shared__ uint64_t s_hit[kStepWaves][FINAL ? kHitStage : 1];
     uint64_t staged = 0, wTests = 0, wRows = 0;
     __shared__ double s_x[kStepWaves][kWave * DM];
@@ -645,9 +644,10 @@ int scan_launch_grid(tblaeq_ctx* ctx, uint64_t tiles) {
 // Ping-pongs between a and b; returns the buffer holding the result.  The last
 // pass can write key & idMask instead of the key.
 int radix_sort(tblaeq_ctx* ctx, uint64_t* a, uint64_t* b, const uint64_t* dn, uint64_t cap, int nbits,
-               uint64_t* hist, uint64_t** result, bool extractLast, uint64_t idMask) {
+               uint64_t* hist, uint64_t** result, bool extractLast, uint64_t idMask, uint64_t blocksPerCu) {
     const uint64_t capTiles = (cap + kRsTile - 1) / kRsTile;
-    const int grid = scan_launch_grid(ctx, capTiles);
+    const uint64_t gcap = static_cast<uint64_t>(ctx->cuCount) * blocksPerCu;
+    const int grid = static_cast<int>(capTiles < gcap ? (capTiles ? capTiles : 1) : gcap);
     const uint64_t histTiles = (256 * (capTiles ? capTiles : 1) + kScanThreads * kHsPer - 1) / (kScanThreads * kHsPer);
     const int passes = (nbits + 7) / 8;
     uint64_t* src = a;
@@ -683,18 +683,27 @@ int bits_for(uint64_t v) {
     return b;
 }
 
+template <int MODE, int STAGE>
+void launch_range_dm(tblaeq_ctx* ctx, const RangeStepArgs& A, dim3 g, dim3 b) {
+    if (A.D <= 4) hipLaunchKernelGGL((k_range_step<4, MODE, STAGE>), g, b, 0, ctx->stream, A);
+    else if (A.D <= 8) hipLaunchKernelGGL((k_range_step<8, MODE, STAGE>), g, b, 0, ctx->stream, A);
+    else if (A.D <= 16) hipLaunchKernelGGL((k_range_step<16, MODE, STAGE>), g, b, 0, ctx->stream, A);
+    else hipLaunchKernelGGL((k_range_step<32, MODE, STAGE>), g, b, 0, ctx->stream, A);
+}
+
 template <int MODE>
-int launch_range_step(tblaeq_ctx* ctx, const RangeStepArgs& A, uint64_t itemBound) {
+int launch_range_step(tblaeq_batch* bt, const RangeStepArgs& A, uint64_t itemBound) {
+    tblaeq_ctx* ctx = bt->ctx;
     uint64_t blocks = (itemBound + kStepWaves - 1) / kStepWaves;
-    const uint64_t cap = static_cast<uint64_t>(ctx->cuCount) * 16;
+    const uint64_t cap = static_cast<uint64_t>(ctx->cuCount) * bt->rangeBlocksPerCu;
     if (blocks > cap) blocks = cap;
     if (blocks == 0) blocks = 1;
     const dim3 g(static_cast<unsigned>(blocks)), b(kStepWaves * kWave);
     TimedScope ts(ctx, MODE == kFinal ? kKBatch0 + 1 : kKBatch0 + 0);
-    if (A.D <= 4) hipLaunchKernelGGL((k_range_step<4, MODE>), g, b, 0, ctx->stream, A);
-    else if (A.D <= 8) hipLaunchKernelGGL((k_range_step<8, MODE>), g, b, 0, ctx->stream, A);
-    else if (A.D <= 16) hipLaunchKernelGGL((k_range_step<16, MODE>), g, b, 0, ctx->stream, A);
-    else hipLaunchKernelGGL((k_range_step<32, MODE>), g, b, 0, ctx->stream, A);
+    if (MODE != kFinal) launch_range_dm<MODE, 1>(ctx, A, g, b);
+    else if (bt->hitStage == 256) launch_range_dm<MODE, 256>(ctx, A, g, b);
+    else if (bt->hitStage == 1024) launch_range_dm<MODE, 1024>(ctx, A, g, b);
+    else launch_range_dm<MODE, 512>(ctx, A, g, b);
     TB_HIP(hipGetLastError());
     return TBLAEQ_OK;
 }
@@ -1015,7 +1024,7 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
         A.survStat = d_level_surv;
         A.fineStat = d_fine_count;
         const uint64_t bound = l == 0 ? (nq + kWave - 1) / kWave : b->itemCap;
-        if ((rc = launch_range_step<kCount>(ctx, A, bound))) return rc;
+        if ((rc = launch_range_step<kCount>(b, A, bound))) return rc;
         // group the kept (cluster, query) pairs of level l by cluster
         const uint64_t mesh = b->count[l];
         const uint64_t tiles = (mesh + kScanThreads - 1) / kScanThreads;
@@ -1028,7 +1037,7 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
         }
         A.survStat = nullptr;
         A.fineStat = nullptr;
-        if ((rc = launch_range_step<kWrite>(ctx, A, bound))) return rc;
+        if ((rc = launch_range_step<kWrite>(b, A, bound))) return rc;
         cur = nxt;
     }
     // final step: exact filter of the finest candidates
@@ -1056,9 +1065,10 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     A.outCount = ctl + 2;
     A.idBits = idBits;
     A.work = ctl + 8;
-    if ((rc = launch_range_step<kFinal>(ctx, A, b->itemCap))) return rc;
+    if ((rc = launch_range_step<kFinal>(b, A, b->itemCap))) return rc;
     uint64_t* sorted = nullptr;
-    if ((rc = radix_sort(ctx, b->hits, b->hitsTmp, ctl + 2, A.cap, idBits + qBits, b->hist, &sorted, false, 0)))
+    if ((rc = radix_sort(ctx, b->hits, b->hitsTmp, ctl + 2, A.cap, idBits + qBits, b->hist, &sorted, false, 0,
+                         b->radixBlocksPerCu)))
         return rc;
     hipLaunchKernelGGL(k_query_offsets, dim3(static_cast<unsigned>((nq + 1 + 255) / 256)), dim3(256), 0, ctx->stream,
                        sorted, ctl + 2, A.cap, nq, idBits, d_out_off);
@@ -1078,7 +1088,8 @@ extern "C" int tblaeq_batch_range_graph(tblaeq_batch* b, const double* d_boxes, 
                                         uint64_t* d_out_ids, uint64_t out_cap, uint64_t* d_status) {
     if (!b) return TBLAEQ_ERR_INVALID;
     tblaeq_ctx* ctx = b->ctx;
-    const bool same = b->rangeExec && b->gBoxes == d_boxes && b->gNq == nq && b->gOff == d_out_off &&
+    const bool same = b->rangeExec && b->gParamVersion == b->paramVersion && b->gBoxes == d_boxes && b->gNq == nq &&
+                      b->gOff == d_out_off &&
                       b->gIds == d_out_ids && b->gCap == out_cap && b->gStatus == d_status;
     if (!same) {
         if (b->rangeExec) TB_HIP(hipGraphExecDestroy(b->rangeExec));
@@ -1103,6 +1114,7 @@ extern "C" int tblaeq_batch_range_graph(tblaeq_batch* b, const double* d_boxes, 
         b->gScanLaunches = ctx->scanLaunches - before;
         ctx->scanLaunches = before;
         b->gBoxes = d_boxes;
+        b->gParamVersion = b->paramVersion;
         b->gNq = nq;
         b->gOff = d_out_off;
         b->gIds = d_out_ids;

@@ -1126,30 +1126,46 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_merge(KnnMergeArgs A) {
 
 using namespace tblaeq;
 
+namespace {
+struct ParamSpec {
+    int id;
+    uint64_t tblaeq_batch::*field;
+    uint64_t lo, hi;
+};
+const ParamSpec kParams[] = {
+    {TBLAEQ_PARAM_KNN_SEED, &tblaeq_batch::knnSeed, 1, static_cast<uint64_t>(kKnnMaxM)},
+    {TBLAEQ_PARAM_KNN_WG_PER_CU, &tblaeq_batch::knnWgPerCu, 1, 16},
+    {TBLAEQ_PARAM_KNN_MODE, &tblaeq_batch::knnMode, 0, 1},
+    {TBLAEQ_PARAM_RANGE_BLOCKS_PER_CU, &tblaeq_batch::rangeBlocksPerCu, 1, 64},
+    {TBLAEQ_PARAM_STEP_BLOCKS_PER_CU, &tblaeq_batch::stepBlocksPerCu, 1, 64},
+    {TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU, &tblaeq_batch::radixBlocksPerCu, 1, 32},
+    {TBLAEQ_PARAM_HIT_STAGE, &tblaeq_batch::hitStage, 256, 1024},
+};
+const ParamSpec* find_param(int id) {
+    for (const auto& p : kParams)
+        if (p.id == id) return &p;
+    return nullptr;
+}
+}  // namespace
+
 extern "C" int tblaeq_batch_set_param(tblaeq_batch* b, int param, uint64_t value) {
     if (!b) {
         set_error("tblaeq_batch_set_param: null engine");
         return TBLAEQ_ERR_INVALID;
     }
-    switch (param) {
-    case TBLAEQ_PARAM_KNN_SEED:
-        if (value < 1 || value > static_cast<uint64_t>(kKnnMaxM)) break;
-        b->knnSeed = value;
-        return TBLAEQ_OK;
-    case TBLAEQ_PARAM_KNN_WG_PER_CU:
-        if (value < 1 || value > 16) break;
-        b->knnWgPerCu = value;
-        return TBLAEQ_OK;
-    case TBLAEQ_PARAM_KNN_MODE:
-        if (value > 1) break;
-        b->knnMode = value;
-        return TBLAEQ_OK;
-    default:
+    const ParamSpec* p = find_param(param);
+    if (!p) {
         set_error("tblaeq_batch_set_param: unknown parameter");
         return TBLAEQ_ERR_INVALID;
     }
-    set_error("tblaeq_batch_set_param: value out of range");
-    return TBLAEQ_ERR_INVALID;
+    const bool stageOk = param != TBLAEQ_PARAM_HIT_STAGE || value == 256 || value == 512 || value == 1024;
+    if (value < p->lo || value > p->hi || !stageOk) {
+        set_error("tblaeq_batch_set_param: value out of range");
+        return TBLAEQ_ERR_INVALID;
+    }
+    if (b->*(p->field) != value) ++b->paramVersion;
+    b->*(p->field) = value;
+    return TBLAEQ_OK;
 }
 
 extern "C" int tblaeq_batch_get_param(const tblaeq_batch* b, int param, uint64_t* value) {
@@ -1157,14 +1173,13 @@ extern "C" int tblaeq_batch_get_param(const tblaeq_batch* b, int param, uint64_t
         set_error("tblaeq_batch_get_param: invalid argument");
         return TBLAEQ_ERR_INVALID;
     }
-    switch (param) {
-    case TBLAEQ_PARAM_KNN_SEED: *value = b->knnSeed; return TBLAEQ_OK;
-    case TBLAEQ_PARAM_KNN_WG_PER_CU: *value = b->knnWgPerCu; return TBLAEQ_OK;
-    case TBLAEQ_PARAM_KNN_MODE: *value = b->knnMode; return TBLAEQ_OK;
-    default:
+    const ParamSpec* p = find_param(param);
+    if (!p) {
         set_error("tblaeq_batch_get_param: unknown parameter");
         return TBLAEQ_ERR_INVALID;
     }
+    *value = b->*(p->field);
+    return TBLAEQ_OK;
 }
 
 extern "C" int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const uint64_t* d_K, uint64_t nq,
@@ -1274,7 +1289,7 @@ extern "C" int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const u
         TB_HIP(hipGetLastError());
     }
     const uint64_t stepBlocks = std::min<uint64_t>((b->itemCap + kStepWaves - 1) / kStepWaves,
-                                                   static_cast<uint64_t>(ctx->cuCount) * 16);
+                                                   static_cast<uint64_t>(ctx->cuCount) * b->stepBlocksPerCu);
     const dim3 sg(static_cast<unsigned>(stepBlocks ? stepBlocks : 1)), st(kStepWaves * kWave);
     // refine groups: set 1 after a split last level, set 0 otherwise
     const uint64_t* rItems = b->items;

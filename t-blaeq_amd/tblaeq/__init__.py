@@ -80,11 +80,22 @@ SIGNATURES = {
     "tblaeq_batch_get_param": (C.c_int, [vp, C.c_int, u64p]),
     "tblaeq_handler_batch_set_param": (C.c_int, [vp, C.c_int, C.c_uint64]),
     "tblaeq_handler_batch_work": (C.c_int, [vp, u64p]),
+    "tblaeq_batch_autotune": (C.c_int, [vp, vp, C.c_uint64, vp, vp, C.c_uint64, C.c_uint64, C.c_int, f64p]),
+    "tblaeq_batch_save_params": (C.c_int, [vp, C.c_char_p]),
+    "tblaeq_batch_load_params": (C.c_int, [vp, C.c_char_p]),
+    "tblaeq_handler_batch_autotune": (C.c_int, [vp, f64p, C.c_uint64, f64p, u64p, C.c_uint64, C.c_int, f64p]),
+    "tblaeq_handler_batch_save_params": (C.c_int, [vp, C.c_char_p]),
+    "tblaeq_handler_batch_get_param": (C.c_int, [vp, C.c_int, u64p]),
+    "tblaeq_handler_batch_load_params": (C.c_int, [vp, C.c_char_p]),
 }
 
 PARAM_KNN_SEED = 1
 PARAM_KNN_WG_PER_CU = 2
 PARAM_KNN_MODE = 3
+PARAM_RANGE_BLOCKS_PER_CU = 4
+PARAM_STEP_BLOCKS_PER_CU = 5
+PARAM_RADIX_BLOCKS_PER_CU = 6
+PARAM_HIT_STAGE = 7
 
 
 class TblaeqError(RuntimeError):
@@ -216,6 +227,29 @@ class Handler:
 
     def batch_use_graph(self, on=True):
         check(lib().tblaeq_handler_batch_use_graph(self.h, int(on)), "batch_use_graph")
+
+    def batch_autotune(self, boxes=None, points=None, Ks=None, reps=3):
+        """Tunes the batched engine's launch parameters on sample queries; returns
+        {'range_ms': (before, after), 'knn_ms': (before, after)}."""
+        D = self.D
+        b = np.ascontiguousarray(boxes if boxes is not None else np.zeros((0, 2 * D)), np.float64)
+        p = np.ascontiguousarray(points if points is not None else np.zeros((0, D)), np.float64)
+        k = np.ascontiguousarray(np.broadcast_to(np.asarray(Ks if Ks is not None else 0, np.uint64), (p.shape[0],)))
+        out = np.zeros(4)
+        check(lib().tblaeq_handler_batch_autotune(self.h, _p(b, f64p), b.shape[0], _p(p, f64p), _p(k, u64p),
+                                                  p.shape[0], int(reps), _p(out, f64p)), "batch_autotune")
+        return {"range_ms": (out[0], out[1]), "knn_ms": (out[2], out[3])}
+
+    def batch_save_params(self, path):
+        check(lib().tblaeq_handler_batch_save_params(self.h, str(path).encode()), "batch_save_params")
+
+    def batch_load_params(self, path):
+        check(lib().tblaeq_handler_batch_load_params(self.h, str(path).encode()), "batch_load_params")
+
+    def batch_get_param(self, param):
+        v = C.c_uint64(0)
+        check(lib().tblaeq_handler_batch_get_param(self.h, int(param), C.byref(v)), "batch_get_param")
+        return v.value
 
     def batch_work(self):
         """(tests, candidate rows) of the last batch_run's final stage."""

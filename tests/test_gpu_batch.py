@@ -195,3 +195,34 @@ def test_batch_knn_hierarchy_heights(ratios):
     pts = W.knn_points(data, 60, 17, h.mesh_sizes, CFG["vmin"], CFG["vmax"], noise=0.2)
     Ks = np.random.default_rng(3).choice([0, 1, 7, 50, 120], len(pts))
     _check_knn_batch(h, o, pts, Ks)
+
+
+def test_autotune_keeps_results_and_params_roundtrip(tmp_path):
+    """The tuner only changes launch parameters: answers after tuning (graph and plain launches) still
+    equal the oracle's, and a saved tuning table restores the chosen values."""
+    h, o = _make(dict(CFG, N=30000, seed=17), max_q=128)
+    data = h.data()
+    boxes, _ = W.range_boxes(data, 64, 3, 1e-4, 2e-2)
+    pts = W.knn_points(data, 64, 4, h.mesh_sizes, CFG["vmin"], CFG["vmax"])
+    Ks = np.random.default_rng(2).choice([1, 10, 60], len(pts)).astype(np.uint64)
+    ms = h.batch_autotune(boxes, pts, Ks, reps=1)
+    assert ms["range_ms"][1] <= ms["range_ms"][0] and ms["knn_ms"][1] <= ms["knn_ms"][0]
+    params = {p: h.batch_get_param(p) for p in range(1, 8)}
+    assert params[tblaeq.PARAM_HIT_STAGE] in (256, 512, 1024)
+    _check_range_batch(h, o, boxes)
+    h.batch_set("range", boxes)
+    h.batch_run()  # captured graph with the tuned parameters
+    graph_res = h.batch_results()
+    for i, row in enumerate(boxes):
+        assert np.array_equal(graph_res[i], o.range(row[:h.D], row[h.D:]).refine_range(row[:h.D], row[h.D:]))
+    _check_knn_batch(h, o, pts, Ks)
+    path = tmp_path / "tuning.txt"
+    h.batch_save_params(path)
+    h.batch_set_param(tblaeq.PARAM_HIT_STAGE, 256 if params[tblaeq.PARAM_HIT_STAGE] != 256 else 1024)
+    h.batch_set_param(tblaeq.PARAM_KNN_SEED, 3)
+    h.batch_load_params(path)
+    assert {p: h.batch_get_param(p) for p in range(1, 8)} == params
+    bad = tmp_path / "bad.txt"
+    bad.write_text("hit_stage 300\n")
+    with pytest.raises(tblaeq.TblaeqError):
+        h.batch_load_params(bad)
