@@ -99,6 +99,66 @@ def run_plain(h, boxes, knn_q, Ks, steps, warmup, timing):
     return nq, wall, np.asarray(us), totals
 
 
+def run_serving(h, data, cfg, rates, seconds, log):
+    """Open-loop Poisson request stream through the continuous-batching server:
+    70% range (selectivity log-uniform 1e-6..1e-3), 30% KNN (K log-uniform 1..256,
+    capped at the engine's limit), at each offered rate for `seconds`."""
+    import random
+
+    import tblaeq
+    from tblaeq import workload as W
+
+    boxes, _ = W.range_boxes(data, 2048, cfg["seed"] + 3, 1e-6, 1e-3)
+    pts = W.knn_points(data, 2048, cfg["seed"] + 4, h.mesh_sizes, cfg["vmin"], cfg["vmax"])
+    rng = np.random.default_rng(cfg["seed"] + 5)
+    srv = tblaeq.Server(h, max_batch=4096)
+    out = []
+    try:
+        kmax = 200  # K + empty clusters + 1 must stay within the 256-entry LDS selection
+        Ks = np.clip(np.round(np.exp(rng.uniform(0.0, np.log(256.0), 2048))), 1, kmax).astype(np.int64)
+        # warm-up
+        for i in range(64):
+            srv.wait(srv.submit("range", boxes[i]))
+            srv.wait(srv.submit("knn", pts[i], int(Ks[i])))
+        for rate in rates:
+            r = random.Random(int(rate))
+            srv.reset_stats()
+            tickets = []
+            start = time.perf_counter()
+            t_next = start
+            i = 0
+            while True:
+                now = time.perf_counter()
+                if now - start >= seconds:
+                    break
+                if now < t_next:
+                    if t_next - now > 2e-4:
+                        time.sleep(t_next - now - 1e-4)
+                    continue
+                if r.random() < 0.7:
+                    tickets.append(srv.submit("range", boxes[i % 2048]))
+                else:
+                    tickets.append(srv.submit("knn", pts[i % 2048], int(Ks[i % 2048])))
+                i += 1
+                t_next += r.expovariate(rate)
+            offered_s = time.perf_counter() - start
+            for t in tickets:
+                srv.wait(t, 120.0)
+            done_s = time.perf_counter() - start
+            for t in tickets:
+                srv.release(t)
+            st = srv.stats()
+            rec = {"offered_rps": round(len(tickets) / offered_s, 1), "completed_rps": round(len(tickets) / done_s, 1),
+                   "requests": len(tickets), "p50_us": round(st["p50_us"], 1), "p99_us": round(st["p99_us"], 1),
+                   "mean_batch": round(st["mean_batch"], 1), "batches": int(st["batches"]),
+                   "deferred_by_hbm": int(st["deferred"])}
+            log("serving", rec)
+            out.append(rec)
+    finally:
+        srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +173,9 @@ def main():
     ap.add_argument("--pair-cap", type=int, default=1 << 28)
     ap.add_argument("--hit-cap", type=int, default=1 << 28)
     ap.add_argument("--small", action="store_true", help="N=500k D=5 hierarchy (quick check)")
+    ap.add_argument("--serve-rates", default="20000,100000",
+                    help="offered request rates (req/s) of the serving run; empty to skip")
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once on the GPU (a kernel of known "
                          "traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -255,6 +318,14 @@ def main():
                                 "per 64-box slice, so HBM is not its bound",
                         "plain": plain_roofline}
 
+    serving = None
+    if args.serve_rates and not args.no_batch:
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        serving = {"workload": "open-loop Poisson stream on the same index: 70% range (selectivity log-uniform "
+                               "1e-6..1e-3), 30% KNN (K log-uniform 1..256, capped at 200); continuous batching, admission by "
+                               "free HBM; latency = submit to result on the host",
+                   "runs": run_serving(h, data, cfg, rates, args.serve_seconds, log)}
+
     cpu = None
     if not args.no_cpu:
         import oracle_py as O
@@ -289,6 +360,7 @@ def main():
         "plain": {"qps": round(plain_qps, 3), "p50_us": round(float(np.percentile(us, 50)), 1),
                   "p99_us": round(float(np.percentile(us, 99)), 1), "queries": nq},
         "batched": batched,
+        "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

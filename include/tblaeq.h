@@ -35,6 +35,7 @@ extern "C" {
 #define TBLAEQ_ERR_CAPACITY 4    /* an output or scratch buffer is too small */
 #define TBLAEQ_ERR_IO 5          /* index file missing or malformed */
 #define TBLAEQ_ERR_NOMEM 6       /* device allocation failed */
+#define TBLAEQ_ERR_TIMEOUT 7     /* a wait ended before the work was done */
 
 typedef struct tblaeq_ctx tblaeq_ctx;
 

@@ -86,6 +86,15 @@ SIGNATURES = {
     "tblaeq_handler_batch_autotune": (C.c_int, [vp, f64p, C.c_uint64, f64p, u64p, C.c_uint64, C.c_int, f64p]),
     "tblaeq_handler_batch_save_params": (C.c_int, [vp, C.c_char_p]),
     "tblaeq_handler_batch_get_param": (C.c_int, [vp, C.c_int, u64p]),
+    # tblaeq_serve.h
+    "tblaeq_server_config_init": (None, [vp]),
+    "tblaeq_server_create": (C.c_int, [vp, vp, C.POINTER(vp)]),
+    "tblaeq_server_destroy": (C.c_int, [vp]),
+    "tblaeq_server_submit": (C.c_int, [vp, C.c_int, f64p, C.c_uint64, u64p]),
+    "tblaeq_server_wait": (C.c_int, [vp, C.c_uint64, C.c_double]),
+    "tblaeq_server_result": (C.c_int, [vp, C.c_uint64, u64p, C.c_uint64, f64p, u64p, C.c_int]),
+    "tblaeq_server_stats": (C.c_int, [vp, f64p]),
+    "tblaeq_server_reset_stats": (C.c_int, [vp]),
     "tblaeq_handler_batch_load_params": (C.c_int, [vp, C.c_char_p]),
 }
 
@@ -304,6 +313,73 @@ class Handler:
         if getattr(self, "h", None) and _LIB is not None:
             _LIB.tblaeq_handler_destroy(self.h)
             self.h = None
+
+
+class ServerConfig(C.Structure):
+    """tblaeq_server_config (tblaeq_serve.h)."""
+    _fields_ = [("max_batch", C.c_uint64), ("max_k", C.c_uint64), ("pair_cap", C.c_uint64), ("hit_cap", C.c_uint64),
+                ("hbm_fraction", C.c_double), ("range_request_bytes", C.c_uint64),
+                ("knn_request_bytes", C.c_uint64), ("max_wait_us", C.c_double)]
+
+
+class Server:
+    """Continuous-batching serving loop over a handler's index (tblaeq_serve.h).
+    Close it before the handler."""
+
+    def __init__(self, handler, **cfg):
+        c = ServerConfig()
+        lib().tblaeq_server_config_init(C.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self.handler = handler
+        self.D = handler.D
+        self.s = vp()
+        check(lib().tblaeq_server_create(handler.h, C.byref(c), C.byref(self.s)), "server_create")
+
+    def submit(self, qtype, q, K=0):
+        q = np.ascontiguousarray(q, np.float64)
+        t = C.c_uint64(0)
+        check(lib().tblaeq_server_submit(self.s, 0 if qtype == "range" else 1, _p(q, f64p), int(K), C.byref(t)),
+              "server_submit")
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        check(lib().tblaeq_server_wait(self.s, ticket, float(timeout)), "server_wait")
+
+    def result(self, ticket, release=True):
+        n = C.c_uint64(0)
+        check(lib().tblaeq_server_result(self.s, ticket, None, 0, None, C.byref(n), 0), "server_result")
+        ids = np.zeros(n.value, np.uint64)
+        d2 = np.zeros(n.value)
+        check(lib().tblaeq_server_result(self.s, ticket, _p(ids, u64p), n.value, _p(d2, f64p), C.byref(n),
+                                         int(release)), "server_result")
+        return ids, d2
+
+    def release(self, ticket):
+        """Forgets a finished request without copying its result."""
+        n = C.c_uint64(0)
+        check(lib().tblaeq_server_result(self.s, ticket, None, 0, None, C.byref(n), 1), "server_result")
+        return n.value
+
+    def stats(self):
+        o = np.zeros(8)
+        check(lib().tblaeq_server_stats(self.s, _p(o, f64p)), "server_stats")
+        keys = ("completed", "p50_us", "p99_us", "batches", "mean_batch", "deferred", "peak_inflight_bytes",
+                "budget_bytes")
+        return dict(zip(keys, o.tolist()))
+
+    def reset_stats(self):
+        check(lib().tblaeq_server_reset_stats(self.s), "server_reset_stats")
+
+    def close(self):
+        if self.s:
+            lib().tblaeq_server_destroy(self.s)
+            self.s = None
+
+    def __del__(self):
+        if getattr(self, "s", None) and _LIB is not None:
+            _LIB.tblaeq_server_destroy(self.s)
+            self.s = None
 
 
 class Result:
