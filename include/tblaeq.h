@@ -153,6 +153,17 @@ int tblaeq_refine_range(tblaeq_ctx* ctx, uint64_t D, const uint64_t* ids, const 
 int tblaeq_refine_knn_topk(tblaeq_ctx* ctx, uint64_t D, const uint64_t* ids, const double* vals, uint64_t count,
                            const double* d_q, uint64_t K, uint64_t* out_ids, double* out_d2, uint64_t* n_out);
 
+/* Learned KNN radius filter (no reference counterpart): the KNN compaction of
+ * a grid with the bound *d_radius in place of the STEP bound -- keeps the
+ * points with sqrt_rd(|q - c|^2) - r_c <= *d_radius in grid order, with their
+ * prolongation offsets; *d_out_count = kept points, *d_out_nnz = their
+ * children.  Used on the last level's STEP survivors so that only clusters
+ * within a proposed radius are prolonged. */
+int tblaeq_knn_radius_compact(tblaeq_ctx* ctx, const tblaeq_level* lv, const uint64_t* ids, const double* vals,
+                              const uint64_t* d_count, uint64_t count_bound, const double* d_q, const double* d_radius,
+                              uint64_t* out_ids, double* out_vals, uint64_t* out_off, uint64_t* d_out_count,
+                              uint64_t* d_out_nnz);
+
 /* --- sort ------------------------------------------------------------------
  * Block merge sort of (key, value) pairs, lexicographic, ascending.  LDS
  * bitonic tiles of 2048 pairs, then merge-path passes.  Replaces
@@ -201,8 +212,9 @@ int tblaeq_batch_range_graph(tblaeq_batch* b, const double* d_boxes, uint64_t nq
  * level) as the per-query path.  d_status[0] = candidate records refined
  * cluster-major, d_status[1] = overflow flags (non-zero: a capacity of
  * tblaeq_batch_reserve was exceeded, results invalid; split the batch),
- * d_status[2] / [3] = distance evaluations / candidate rows of the
- * cluster-major refine (d_status holds 4 words).
+ * d_status[2] = distance evaluations of the cluster-major refine,
+ * d_status[3] = distance and bound evaluations of the query-major expansions
+ * (seed clusters, last-level bound) (d_status holds 4 words).
  * Requires Kmax + empty clusters + 1 <= 256 and at most 8 levels (else
  * TBLAEQ_ERR_UNSUPPORTED).  d_level_surv / d_fine_count as for
  * tblaeq_batch_range (written, not accumulated). */

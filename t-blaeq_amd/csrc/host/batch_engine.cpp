@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "../../../include/tblaeq_handler.h"
+#include "../../../include/tblaeq_learn.h"
 #include "../ctx.hpp"
 #include "handler_internal.hpp"
 
@@ -76,6 +77,8 @@ int tblaeq_handler_batch_set_queries(tblaeq_handler* h, int type, const double* 
         st->type = type;
         st->nq = nq;
         hipCheck(hipMemcpy(st->dQ.data(), queries, nq * (type == 0 ? 2 * D : D) * 8, hipMemcpyHostToDevice), "H2D");
+        st->hQ.assign(queries, queries + nq * (type == 0 ? 2 * D : D));
+        if (type == 1) st->hK.assign(Ks, Ks + nq);
         if (type == 1) {
             hipCheck(hipMemcpy(st->dK.data(), Ks, nq * 8, hipMemcpyHostToDevice), "H2D");
             st->Kmax = *std::max_element(Ks, Ks + nq);
@@ -306,6 +309,145 @@ int tblaeq_handler_batch_load_params(tblaeq_handler* h, const char* path) {
     BatchState* st = stateOf(h);
     if (!st) return TBLAEQ_ERR_INVALID;
     return tblaeq_batch_load_params(st->b, path);
+}
+
+int tblaeq_handler_knn_features(tblaeq_handler* h, const double* points, const uint64_t* Ks, uint64_t n, double vmin,
+                                double scale, float* X) {
+    BatchState* st = stateOf(h);
+    if (!st || !points || !Ks || !X || n == 0) {
+        tblaeq::set_error("tblaeq_handler_knn_features: invalid argument or batch_init not called");
+        return TBLAEQ_ERR_INVALID;
+    }
+    return guarded("tblaeq_handler_knn_features", [&]() -> int {
+        const size_t D = h->h->index().D;
+        DeviceBuffer<double> dP(points, n * D);
+        DeviceBuffer<uint64_t> dK(Ks, n);
+        DeviceBuffer<float> dX(n * 48);
+        tbCheck(tblaeq_knn_features(st->b, dP.data(), dK.data(), n, vmin, scale, dX.data()), "features");
+        hipStream_t s = streamOf(h->h->index());
+        hipCheck(hipMemcpyAsync(X, dX.data(), n * 48 * 4, hipMemcpyDeviceToHost, s), "D2H");
+        hipCheck(hipStreamSynchronize(s), "sync");
+        return TBLAEQ_OK;
+    });
+}
+
+// Learned KNN over the uploaded batch: features -> network -> radii -> certified
+// batch; queries that fail certification are rerun with the seed method.
+int tblaeq_handler_batch_knn_learned(tblaeq_handler* h, tblaeq_mlp* m, double factor, int stats,
+                                     uint64_t* fallbacks) {
+    BatchState* st = stateOf(h);
+    if (!st || !m || st->type != 1 || st->nq == 0 || !(factor > 0.0)) {
+        tblaeq::set_error("tblaeq_handler_batch_knn_learned: upload a KNN batch first (and factor > 0)");
+        return TBLAEQ_ERR_INVALID;
+    }
+    return guarded("tblaeq_handler_batch_knn_learned", [&]() -> int {
+        IndexData& idx = h->h->index();
+        const size_t D = idx.D, L = idx.intervals;
+        hipStream_t s = streamOf(idx);
+        const uint64_t nq = st->nq, Kmax = st->Kmax;
+        double vmin = 0, scale = 1;
+        tblaeq_mlp_get_norm(m, &vmin, &scale);
+        DeviceBuffer<float> dX(nq * 48);
+        DeviceBuffer<double> dR(nq);
+        DeviceBuffer<uint8_t> dF(nq);
+        tbCheck(tblaeq_knn_features(st->b, st->dQ.data(), st->dK.data(), nq, vmin, scale, dX.data()), "features");
+        hipCheck(hipStreamSynchronize(s), "sync");
+        tbCheck(tblaeq_mlp_propose_radius(m, dX.data(), nq, scale, factor, dR.data()), "propose radius");
+        hipCheck(hipDeviceSynchronize(), "sync");
+        st->work[0] = st->work[1] = 0;
+        if (stats) {
+            hipCheck(hipMemsetAsync(st->dSurv.data(), 0, nq * L * 8, s), "memset");
+            hipCheck(hipMemsetAsync(st->dFine.data(), 0, nq * 8, s), "memset");
+        }
+        // certified batch, split in halves while a capacity is exceeded
+        std::vector<std::pair<uint64_t, uint64_t>> todo{{0, nq}};
+        while (!todo.empty()) {
+            auto [a, n] = todo.back();
+            todo.pop_back();
+            tbCheck(tblaeq_batch_knn_radius(st->b, st->dQ.data() + a * D, st->dK.data() + a, n, Kmax, dR.data() + a,
+                                            st->dIds.data() + a * Kmax, st->dD2.data() + a * Kmax, st->dN.data() + a,
+                                            dF.data() + a, st->dStatus.data(),
+                                            stats ? st->dSurv.data() + a * L : nullptr,
+                                            stats ? st->dFine.data() + a : nullptr),
+                    "learned batch");
+            uint64_t status[4];
+            hipCheck(hipMemcpyAsync(status, st->dStatus.data(), 32, hipMemcpyDeviceToHost, s), "D2H");
+            hipCheck(hipStreamSynchronize(s), "sync");
+            if (status[1] != 0) {
+                if (n == 1) {
+                    tblaeq::set_error("learned KNN: one query exceeds the engine capacities");
+                    return TBLAEQ_ERR_CAPACITY;
+                }
+                todo.push_back({a + n / 2, n - n / 2});
+                todo.push_back({a, n / 2});
+                continue;
+            }
+            st->work[0] += status[2];
+            st->work[1] += status[3];
+        }
+        std::vector<uint8_t> fb(nq);
+        hipCheck(hipMemcpy(fb.data(), dF.data(), nq, hipMemcpyDeviceToHost), "D2H");
+        std::vector<uint64_t> redo;
+        for (uint64_t q = 0; q < nq; ++q)
+            if (fb[q]) redo.push_back(q);
+        if (fallbacks) *fallbacks = redo.size();
+        if (!redo.empty()) {
+            // rerun the failed queries with the seed method and scatter their rows back
+            const uint64_t n = redo.size();
+            std::vector<double> q(n * D);
+            std::vector<uint64_t> k(n);
+            for (uint64_t i = 0; i < n; ++i) {
+                std::memcpy(q.data() + i * D, st->hQ.data() + redo[i] * D, D * 8);
+                k[i] = st->hK[redo[i]];
+            }
+            DeviceBuffer<double> dq(q.data(), n * D), dd2(n * Kmax);
+            DeviceBuffer<uint64_t> dk(k.data(), n), dids(n * Kmax), dn(n), dsurv(stats ? n * L : 1),
+                dfine(stats ? n : 1);
+            if (stats) {
+                hipCheck(hipMemsetAsync(dsurv.data(), 0, n * L * 8, s), "memset");
+                hipCheck(hipMemsetAsync(dfine.data(), 0, n * 8, s), "memset");
+            }
+            std::vector<std::pair<uint64_t, uint64_t>> todo2{{0, n}};
+            while (!todo2.empty()) {
+                auto [a, c] = todo2.back();
+                todo2.pop_back();
+                tbCheck(tblaeq_batch_knn(st->b, dq.data() + a * D, dk.data() + a, c, Kmax, dids.data() + a * Kmax,
+                                         dd2.data() + a * Kmax, dn.data() + a, st->dStatus.data(),
+                                         stats ? dsurv.data() + a * L : nullptr, stats ? dfine.data() + a : nullptr),
+                        "fallback batch");
+                uint64_t status[4];
+                hipCheck(hipMemcpyAsync(status, st->dStatus.data(), 32, hipMemcpyDeviceToHost, s), "D2H");
+                hipCheck(hipStreamSynchronize(s), "sync");
+                if (status[1] != 0) {
+                    if (c == 1) {
+                        tblaeq::set_error("learned KNN: one query exceeds the engine capacities");
+                        return TBLAEQ_ERR_CAPACITY;
+                    }
+                    todo2.push_back({a + c / 2, c - c / 2});
+                    todo2.push_back({a, c / 2});
+                    continue;
+                }
+                st->work[0] += status[2];
+                st->work[1] += status[3];
+            }
+            for (uint64_t i = 0; i < n; ++i) {
+                const uint64_t r = redo[i];
+                hipCheck(hipMemcpyAsync(st->dIds.data() + r * Kmax, dids.data() + i * Kmax, Kmax * 8,
+                                        hipMemcpyDeviceToDevice, s), "D2D");
+                hipCheck(hipMemcpyAsync(st->dD2.data() + r * Kmax, dd2.data() + i * Kmax, Kmax * 8,
+                                        hipMemcpyDeviceToDevice, s), "D2D");
+                hipCheck(hipMemcpyAsync(st->dN.data() + r, dn.data() + i, 8, hipMemcpyDeviceToDevice, s), "D2D");
+                if (stats) {
+                    hipCheck(hipMemcpyAsync(st->dSurv.data() + r * L, dsurv.data() + i * L, L * 8,
+                                            hipMemcpyDeviceToDevice, s), "D2D");
+                    hipCheck(hipMemcpyAsync(st->dFine.data() + r, dfine.data() + i, 8, hipMemcpyDeviceToDevice, s),
+                             "D2D");
+                }
+            }
+            hipCheck(hipStreamSynchronize(s), "sync");
+        }
+        return TBLAEQ_OK;
+    });
 }
 
 int tblaeq_handler_batch_free(tblaeq_handler* h) {

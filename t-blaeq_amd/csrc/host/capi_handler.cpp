@@ -3,6 +3,7 @@
 #include <iostream>
 
 #include "../../../include/tblaeq_handler.h"
+#include "../../../include/tblaeq_learn.h"
 #include "../ctx.hpp"
 #include "handler_internal.hpp"
 
@@ -164,6 +165,29 @@ int tblaeq_handler_query(tblaeq_handler* h, int type, const double* queries, uin
             delete r;
             throw std::invalid_argument("query dimension mismatch");
         }
+        *out = r;
+    });
+}
+
+int tblaeq_handler_query_learned(tblaeq_handler* h, const double* points, uint64_t nq, uint64_t K,
+                                 const double* radii, uint64_t* fallbacks, tblaeq_result** out) {
+    if (!h || !out || !points || !radii || nq == 0 || K == 0) return TBLAEQ_ERR_INVALID;
+    return guarded("tblaeq_handler_query_learned", [&] {
+        const int D = static_cast<int>(h->h->getDim());
+        Query q(static_cast<int>(nq), D, QueryType::POINT);
+        std::memcpy(q.data.data(), points, q.data.size() * 8);
+        QueryEngine::RunConfig cfg;
+        cfg.K = K;
+        cfg.refine = true;
+        cfg.radii = radii;
+        uint64_t fb = 0;
+        cfg.fallbacks = &fb;
+        auto* r = new tblaeq_result{h->h->performQuery(q, cfg), {}};
+        if (r->r.errorCode) {
+            delete r;
+            throw std::invalid_argument("query dimension mismatch");
+        }
+        if (fallbacks) *fallbacks = fb;
         *out = r;
     });
 }

@@ -16,7 +16,9 @@ struct BatchState {
     DeviceBuffer<uint64_t> dK, dOff, dOffTmp, dIds, dStatus, dSurv, dFine, dN;
     DeviceBuffer<double> dD2;
     uint64_t lastTotal = 0;
-    uint64_t work[2] = {0, 0};  // last run: final-stage tests and candidate rows (summed over sub-batches)
+    uint64_t work[2] = {0, 0};  // last run: status words 2 and 3 (summed over sub-batches)
+    std::vector<double> hQ;     // host copy of the uploaded queries
+    std::vector<uint64_t> hK;
     bool useGraph = true;
     ~BatchState() {
         if (b) tblaeq_batch_destroy(b);

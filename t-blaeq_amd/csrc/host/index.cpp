@@ -113,12 +113,12 @@ void IndexData::uploadPermanentData() {
     }
     dCoarsestIds.reset(coarsestIds.data(), coarsestIds.size());
     dCoarsestVals.reset(coarsestVals.data(), coarsestVals.size());
-    dCounts.resize(2 * intervals + 2);
-    std::vector<uint64_t> c(2 * intervals + 2, 0);
+    dCounts.resize(2 * intervals + 4);
+    std::vector<uint64_t> c(2 * intervals + 4, 0);
     c[2 * intervals] = coarsestIds.size();
     hipCheck(hipMemcpy(dCounts.data(), c.data(), c.size() * 8, hipMemcpyHostToDevice), "counts");
-    dQuery.resize(2 * D);
-    hipCheck(hipHostMalloc(reinterpret_cast<void**>(&hQuery), 2 * D * sizeof(double), hipHostMallocDefault),
+    dQuery.resize(2 * D + 1);
+    hipCheck(hipHostMalloc(reinterpret_cast<void**>(&hQuery), (2 * D + 1) * sizeof(double), hipHostMallocDefault),
              "hipHostMalloc");
     permanentDataOnDevice = true;
 }
@@ -141,6 +141,11 @@ void IndexData::allocLevelWork(size_t l) {
     W.survVals.resize(std::max<size_t>(cols * D, 1));
     W.outIds.resize(std::max<size_t>(nnz, 1));
     W.outVals.resize(std::max<size_t>(nnz * D, 1));
+    if (l + 1 == intervals) {
+        W.radIds.resize(std::max<size_t>(cols, 1));
+        W.radOff.resize(std::max<size_t>(cols, 1));
+        W.radVals.resize(std::max<size_t>(cols * D, 1));
+    }
 }
 
 tblaeq_level IndexData::levelDesc(size_t l) const {

@@ -238,8 +238,26 @@ LevelResult L0Strategy::runLevel(size_t l, SparseGrid* cur, QueryType t, const d
                                                  W.survIds.data(), W.survVals.data(), W.survOff.data(), dSurv, dOut),
                 "knn_lower_bound_sort_step");
     }
-    tbCheck(tblaeq_prolong_refactor(idx.ctx, &lv, W.survIds.data(), W.survVals.data(), W.survOff.data(), dSurv,
-                                    std::min<uint64_t>(bound, lv.cols), W.outIds.data(), W.outVals.data()),
+    const uint64_t* pIds = W.survIds.data();
+    const double* pVals = W.survVals.data();
+    const uint64_t* pOff = W.survOff.data();
+    const uint64_t* pCount = dSurv;
+    if (t != QueryType::RANGE && idx.learnedRadius > 0.0 && l + 1 == idx.intervals && W.radIds.data()) {
+        // learned radius: prolong only the survivors within the proposed radius
+        // (device copy of the radius staged after the query in idx.dQuery)
+        uint64_t* dRad = idx.dCounts.data() + 2 * idx.intervals + 2;
+        tbCheck(tblaeq_knn_radius_compact(idx.ctx, &lv, W.survIds.data(), W.survVals.data(), dSurv,
+                                          std::min<uint64_t>(bound, lv.cols), idx.dQuery.data(),
+                                          idx.dQuery.data() + 2 * D, W.radIds.data(), W.radVals.data(),
+                                          W.radOff.data(), dRad, dOut),
+                "knn_radius_compact");
+        pIds = W.radIds.data();
+        pVals = W.radVals.data();
+        pOff = W.radOff.data();
+        pCount = dRad;
+    }
+    tbCheck(tblaeq_prolong_refactor(idx.ctx, &lv, pIds, pVals, pOff, pCount, std::min<uint64_t>(bound, lv.cols),
+                                    W.outIds.data(), W.outVals.data()),
             "prolong_refactor");
     if (tmpCnt.data()) hipCheck(hipStreamSynchronize(streamOf(idx)), "sync");  // temporaries die here
     auto* g = new SparseGrid(idx.pTensors[l]->rows, D, 0, W.outIds.data(), W.outVals.data());

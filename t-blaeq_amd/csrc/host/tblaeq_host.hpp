@@ -212,6 +212,9 @@ struct LevelDevice {
 struct LevelWork {
     DeviceBuffer<uint64_t> survIds, survOff, outIds;
     DeviceBuffer<double> survVals, outVals;
+    // learned KNN radius: survivors within the proposed radius (last level)
+    DeviceBuffer<uint64_t> radIds, radOff;
+    DeviceBuffer<double> radVals;
 };
 
 struct IndexData {
@@ -236,7 +239,8 @@ struct IndexData {
     DeviceBuffer<double> dCoarsestVals;
     DeviceBuffer<uint64_t> dCounts;  // per level: [survivors, outputs] + coarsest count
     DeviceBuffer<double> dQuery;     // lo|hi or query point (uploaded once per query)
-    double* hQuery = nullptr;        // pinned staging for dQuery
+    double* hQuery = nullptr;        // pinned staging for dQuery (2D values + the learned radius)
+    double learnedRadius = 0.0;      // > 0: the last KNN level keeps only survivors with lb <= this
     IndexPolicy activePolicy;
     bool permanentDataOnDevice = false;
     bool workReady = false;
@@ -384,6 +388,11 @@ public:
         bool refine = false;   // run the exact refine after the traversal
         bool copyRefined = true;  // copy refined ids/distances to the host (else counts only)
         bool quiet = true;     // no console output inside the query loop
+        // learned KNN radius per query (refine must be on): the last level
+        // prolongs only survivors within the radius; a query whose K-th result
+        // lies beyond it (or that finds fewer than K) is rerun in full
+        const double* radii = nullptr;
+        uint64_t* fallbacks = nullptr;
     };
     static QueryResult run(IndexData& idx, IQueryStrategy& strategy, const Query& q, const RunConfig& cfg);
     // One query through the level loop; fills levelSizes and returns the final grid.

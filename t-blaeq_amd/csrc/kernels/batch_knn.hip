@@ -24,6 +24,7 @@
 // point, whatever the cluster sizes.
 #include <algorithm>
 
+#include "../../../include/tblaeq_learn.h"
 #include "../batch_internal.hpp"
 
 namespace tblaeq {
@@ -70,6 +71,7 @@ struct KnnArgs {
     // the best M (lb, id) keys of the last level.
     int split;
     uint64_t* tau;          // nq x 2
+    uint64_t* work;         // distance / bound evaluations of the flat expansions
     uint64_t* outIds;       // nq x Kmax
     double* outD2;
     uint64_t* outN;         // nq
@@ -204,7 +206,7 @@ __device__ __forceinline__ void knn_queue(KnnShared& S, int i, uint64_t start, u
 // scanned in place.
 template <int DM>
 __device__ void knn_flush(KnnShared& S, int tid, int D, const double* coords, const uint64_t* ids,
-                          const double* radius) {
+                          const double* radius, uint64_t* work) {
     const int lane = tid & (kWave - 1), wid = tid / kWave;
     const uint32_t n = S.listN;
     if (n == 0) return;
@@ -232,6 +234,7 @@ __device__ void knn_flush(KnnShared& S, int tid, int D, const double* coords, co
     if (tid == kKnnThreads - 1) S.lpre[n] = total;
     __syncthreads();
     const uint64_t all = S.lpre[n];
+    if (work && tid == 0) atomicAdd(reinterpret_cast<unsigned long long*>(work), all);
     for (uint64_t f0 = 0; f0 < all; f0 += kKnnThreads) {
         if (knn_needs_compact(S)) knn_compact(S, tid);
         const uint64_t f = f0 + tid;
@@ -363,7 +366,7 @@ __device__ void knn_pick_seeds(KnnShared& S, int tid, uint64_t q0, uint64_t nq, 
 // compacted buffer's threshold is the certified radius.
 template <int DM>
 __device__ void knn_expand_seeds(KnnShared& S, int tid, uint64_t q0, uint64_t nq, int D, const uint64_t* colRes,
-                                 const double* coords, const uint64_t* ids) {
+                                 const double* coords, const uint64_t* ids, uint64_t* work) {
     for (int i = 0; i < kKnnQB; ++i)
         for (uint32_t e = tid; e < S.seedN[i]; e += kKnnThreads) {
             const uint64_t cid = S.bv[i][e];
@@ -377,7 +380,7 @@ __device__ void knn_expand_seeds(KnnShared& S, int tid, uint64_t q0, uint64_t nq
         S.thrV[tid] = ~0ull;
     }
     __syncthreads();
-    knn_flush<DM>(S, tid, D, coords, ids, nullptr);
+    knn_flush<DM>(S, tid, D, coords, ids, nullptr, work);
     knn_compact(S, tid);
 }
 
@@ -494,7 +497,7 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
                     S.thrV[tid] = ~0ull;
                 }
                 __syncthreads();
-                knn_flush<DM>(S, tid, D, A.coords[L - 1], A.ids[L - 1], A.radiusPos[L - 1]);
+                knn_flush<DM>(S, tid, D, A.coords[L - 1], A.ids[L - 1], A.radiusPos[L - 1], A.work);
                 knn_compact(S, tid);
                 if (tid < kKnnQB && q0 + tid < A.nq) {
                     A.tau[2 * (q0 + tid)] = S.thrK[tid];
@@ -553,7 +556,7 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
             }
             // ---- last level (whole level in this kernel): seeds, certified radius
             knn_pick_seeds(S, tid, q0, A.nq, A.seedMin);
-            knn_expand_seeds<DM>(S, tid, q0, A.nq, D, A.colRes[L - 1], A.coords[L], A.ids[L]);
+            knn_expand_seeds<DM>(S, tid, q0, A.nq, D, A.colRes[L - 1], A.coords[L], A.ids[L], A.work);
             // ---- pass 2 of the last level: other survivors within the certified radius
             for (uint64_t j0 = 0; j0 < n; j0 += kKnnThreads) {
                 const uint64_t j = j0 + tid;
@@ -584,11 +587,11 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
                 __syncthreads();
                 if (S.listN >= static_cast<uint32_t>(kKnnFlush)) {
                     if (A.pairs) knn_emit_pairs(A, S, tid, q0);
-                    else knn_flush<DM>(S, tid, D, A.coords[L], A.ids[L], nullptr);
+                    else knn_flush<DM>(S, tid, D, A.coords[L], A.ids[L], nullptr, A.work);
                 }
             }
             if (A.pairs) knn_emit_pairs(A, S, tid, q0);
-            else knn_flush<DM>(S, tid, D, A.coords[L], A.ids[L], nullptr);
+            else knn_flush<DM>(S, tid, D, A.coords[L], A.ids[L], nullptr, A.work);
             knn_compact(S, tid);  // cluster-major: the seeds' top-K, refined later
             knn_write_topk(S, tid, q0, A.nq, A.Kmax, A.outIds, A.outD2, A.outN, A.thr);
             for (int i = 0; i < kKnnQB; ++i) {
@@ -857,6 +860,8 @@ struct KnnSelectArgs {
     uint64_t* outN;
     uint64_t* survStat;
     uint64_t* fineStat;
+    const double* radius;        // proposed search radii (learned mode), or null (seeds)
+    uint64_t* work;              // distance evaluations of the seed expansion
 };
 
 template <int DM>
@@ -903,6 +908,22 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_select(KnnSelectArgs A) {
         }
         knn_compact(S, tid);
         knn_step(S, tid, A.weightById, A.radiusById);
+        if (A.radius) {
+            // learned mode: no seeds; every survivor within the proposed radius
+            // is refined and each point within it becomes a record
+            if (tid < kKnnQB && q0 + tid < A.nq) {
+                const uint64_t q = q0 + tid;
+                const double r = A.radius[q];
+                A.ub[q] = S.ub[tid];
+                A.bnd[2 * q] = 0;
+                A.bnd[2 * q + 1] = 0;
+                A.outN[q] = 0;
+                A.thr[2 * q] = S.K[tid] > 0 ? okey(r * r) : 0;
+                A.thr[2 * q + 1] = ~0ull;
+            }
+            __syncthreads();
+            continue;
+        }
         knn_pick_seeds(S, tid, q0, A.nq, A.seedMin);
         if (tid < kKnnQB && q0 + tid < A.nq) {
             const uint64_t q = q0 + tid;
@@ -910,7 +931,7 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_select(KnnSelectArgs A) {
             A.bnd[2 * q] = S.bndK[tid];
             A.bnd[2 * q + 1] = S.bndV[tid];
         }
-        knn_expand_seeds<DM>(S, tid, q0, A.nq, D, A.colRes, A.coordsFine, A.idsFine);
+        knn_expand_seeds<DM>(S, tid, q0, A.nq, D, A.colRes, A.coordsFine, A.idsFine, A.work);
         knn_write_topk(S, tid, q0, A.nq, A.Kmax, A.outIds, A.outD2, A.outN, A.thr);
         __syncthreads();
     }
@@ -942,7 +963,7 @@ struct KnnClusterArgs {
     uint64_t* recCount;
     uint64_t* cntQ;           // records per query
     uint64_t* overflow;
-    uint64_t* work;           // [0] distance evaluations (child rows x queries), [1] child rows
+    uint64_t* work;           // distance evaluations (child rows x queries)
 };
 
 template <int DM>
@@ -1022,10 +1043,8 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_cluster(KnnClusterAr
         wRows += je - jb;
         wTests += (je - jb) * static_cast<uint64_t>(__popcll(__ballot(valid)));
     }
-    if (lane == 0 && A.work) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(A.work), wTests);
-        atomicAdd(reinterpret_cast<unsigned long long*>(A.work + 1), wRows);
-    }
+    if (lane == 0 && A.work) atomicAdd(reinterpret_cast<unsigned long long*>(A.work), wTests);
+    (void)wRows;
 }
 
 // Counting-sort scatter of the records into per-query segments; leaves cntQ zeroed.
@@ -1122,6 +1141,15 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_merge(KnnMergeArgs A) {
     }
 }
 
+// Learned mode: a query is exact when at least K candidates lie within its
+// radius (its records); otherwise it falls back.
+__global__ void k_certify(uint64_t nq, const uint64_t* __restrict__ Ks, const uint64_t* __restrict__ offQ,
+                          uint8_t* __restrict__ fallback) {
+    for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; q < nq;
+         q += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        fallback[q] = (Ks[q] > 0 && offQ[q + 1] - offQ[q] < Ks[q]) ? 1 : 0;
+}
+
 }  // namespace tblaeq
 
 using namespace tblaeq;
@@ -1182,9 +1210,32 @@ extern "C" int tblaeq_batch_get_param(const tblaeq_batch* b, int param, uint64_t
     return TBLAEQ_OK;
 }
 
+static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K, uint64_t nq, uint64_t Kmax,
+                    const double* d_radius, uint64_t* d_out_ids, double* d_out_d2, uint64_t* d_out_n,
+                    uint8_t* d_fallback, uint64_t* d_status, uint64_t* d_level_surv, uint64_t* d_fine_count);
+
 extern "C" int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const uint64_t* d_K, uint64_t nq,
                                 uint64_t Kmax, uint64_t* d_out_ids, double* d_out_d2, uint64_t* d_out_n,
                                 uint64_t* d_status, uint64_t* d_level_surv, uint64_t* d_fine_count) {
+    return knn_impl(b, d_points, d_K, nq, Kmax, nullptr, d_out_ids, d_out_d2, d_out_n, nullptr, d_status,
+                    d_level_surv, d_fine_count);
+}
+
+extern "C" int tblaeq_batch_knn_radius(tblaeq_batch* b, const double* d_points, const uint64_t* d_K, uint64_t nq,
+                                       uint64_t Kmax, const double* d_radius, uint64_t* d_out_ids, double* d_out_d2,
+                                       uint64_t* d_out_n, uint8_t* d_fallback, uint64_t* d_status,
+                                       uint64_t* d_level_surv, uint64_t* d_fine_count) {
+    if (!d_radius || !d_fallback) {
+        set_error("tblaeq_batch_knn_radius: radius and fallback arrays are required");
+        return TBLAEQ_ERR_INVALID;
+    }
+    return knn_impl(b, d_points, d_K, nq, Kmax, d_radius, d_out_ids, d_out_d2, d_out_n, d_fallback, d_status,
+                    d_level_surv, d_fine_count);
+}
+
+static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K, uint64_t nq, uint64_t Kmax,
+                    const double* d_radius, uint64_t* d_out_ids, double* d_out_d2, uint64_t* d_out_n,
+                    uint8_t* d_fallback, uint64_t* d_status, uint64_t* d_level_surv, uint64_t* d_fine_count) {
     if (!b || !d_points || !d_K || nq == 0 || !d_out_ids || !d_out_d2 || !d_out_n || !d_status) {
         set_error("tblaeq_batch_knn: invalid argument");
         return TBLAEQ_ERR_INVALID;
@@ -1266,6 +1317,7 @@ extern "C" int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const u
     A.fineStat = d_fine_count;
     A.split = split ? 1 : 0;
     A.tau = tau;
+    A.work = ctl + 9;
     if (clusterMajor) {
         A.pairs = b->pairs;
         A.pairCap = b->pairCap;
@@ -1379,6 +1431,8 @@ extern "C" int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const u
         SA.outN = d_out_n;
         SA.survStat = d_level_surv;
         SA.fineStat = d_fine_count;
+        SA.radius = d_radius;
+        SA.work = ctl + 9;
         {
             TimedScope ts(ctx, kKBatch0 + 10);
             const dim3 g(static_cast<unsigned>(wgGrid));
@@ -1453,6 +1507,15 @@ extern "C" int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const u
             const uint64_t mg = std::min<uint64_t>(groups, static_cast<uint64_t>(ctx->cuCount) * 4);
             hipLaunchKernelGGL(k_knn_merge, dim3(static_cast<unsigned>(mg)), dim3(kKnnThreads), 0, ctx->stream, M);
             TB_HIP(hipGetLastError());
+        }
+    }
+    if (d_fallback) {
+        if (split && d_radius) {
+            const uint64_t g = std::min<uint64_t>((nq + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 4);
+            hipLaunchKernelGGL(k_certify, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream, nq, d_K,
+                               offQ, d_fallback);
+        } else {
+            TB_HIP(hipMemsetAsync(d_fallback, 0, nq, ctx->stream));
         }
     }
     launch_copy_status(b, d_status);

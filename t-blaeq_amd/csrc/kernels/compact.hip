@@ -246,6 +246,24 @@ int launch_compact_knn(tblaeq_ctx* ctx, const tblaeq_level* lv, const uint64_t* 
     return launch_compact(ctx, kModeKnn, A, count_bound);
 }
 
+}  // namespace tblaeq
+
+extern "C" int tblaeq_knn_radius_compact(tblaeq_ctx* ctx, const tblaeq_level* lv, const uint64_t* ids,
+                                         const double* vals, const uint64_t* d_count, uint64_t count_bound,
+                                         const double* d_q, const double* d_radius, uint64_t* out_ids,
+                                         double* out_vals, uint64_t* out_off, uint64_t* d_out_count,
+                                         uint64_t* d_out_nnz) {
+    if (!ctx || !lv || !ids || !vals || !d_count || !d_q || !d_radius || !out_ids || !out_vals || !out_off ||
+        !d_out_count || !d_out_nnz) {
+        tblaeq::set_error("tblaeq_knn_radius_compact: invalid argument");
+        return TBLAEQ_ERR_INVALID;
+    }
+    return tblaeq::launch_compact_knn(ctx, lv, ids, vals, d_count, count_bound, d_q, d_radius, out_ids, out_vals,
+                                      out_off, d_out_count, d_out_nnz);
+}
+
+namespace tblaeq {
+
 int launch_compact_box(tblaeq_ctx* ctx, uint64_t D, const uint64_t* ids, const double* vals, uint64_t count,
                        const double* d_lo, const double* d_hi, uint64_t* out_ids, uint64_t* d_out_count) {
     CompactArgs A{};

@@ -15,6 +15,7 @@ _LIB = None
 u64p = C.POINTER(C.c_uint64)
 f64p = C.POINTER(C.c_double)
 u8p = C.POINTER(C.c_uint8)
+f32p = C.POINTER(C.c_float)
 vp = C.c_void_p
 
 # Every function declared in include/tblaeq.h and include/tblaeq_handler.h.
@@ -86,6 +87,26 @@ SIGNATURES = {
     "tblaeq_handler_batch_autotune": (C.c_int, [vp, f64p, C.c_uint64, f64p, u64p, C.c_uint64, C.c_int, f64p]),
     "tblaeq_handler_batch_save_params": (C.c_int, [vp, C.c_char_p]),
     "tblaeq_handler_batch_get_param": (C.c_int, [vp, C.c_int, u64p]),
+    # tblaeq_learn.h
+    "tblaeq_mlp_param_count": (C.c_uint64, []),
+    "tblaeq_mlp_create": (C.c_int, [vp, C.c_uint64, C.POINTER(vp)]),
+    "tblaeq_mlp_destroy": (C.c_int, [vp]),
+    "tblaeq_mlp_set_params": (C.c_int, [vp, f32p]),
+    "tblaeq_mlp_get_params": (C.c_int, [vp, f32p]),
+    "tblaeq_mlp_set_norm": (C.c_int, [vp, C.c_double, C.c_double]),
+    "tblaeq_mlp_get_norm": (C.c_int, [vp, f64p, f64p]),
+    "tblaeq_mlp_set_loss_weights": (C.c_int, [vp, f32p]),
+    "tblaeq_mlp_forward": (C.c_int, [vp, f32p, C.c_uint64, f32p]),
+    "tblaeq_mlp_forward_device": (C.c_int, [vp, vp, C.c_uint64, vp]),
+    "tblaeq_mlp_gradients": (C.c_int, [vp, f32p, f32p, C.c_uint64, f32p, f32p]),
+    "tblaeq_mlp_train": (C.c_int, [vp, f32p, f32p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_float, f32p]),
+    "tblaeq_knn_features": (C.c_int, [vp, vp, vp, C.c_uint64, C.c_double, C.c_double, vp]),
+    "tblaeq_mlp_propose_radius": (C.c_int, [vp, vp, C.c_uint64, C.c_double, C.c_double, vp]),
+    "tblaeq_batch_knn_radius": (C.c_int, [vp, vp, vp, C.c_uint64, C.c_uint64, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "tblaeq_handler_batch_knn_learned": (C.c_int, [vp, vp, C.c_double, C.c_int, u64p]),
+    "tblaeq_handler_knn_features": (C.c_int, [vp, f64p, u64p, C.c_uint64, C.c_double, C.c_double, f32p]),
+    "tblaeq_handler_query_learned": (C.c_int, [vp, f64p, C.c_uint64, C.c_uint64, f64p, u64p, C.POINTER(vp)]),
+    "tblaeq_knn_radius_compact": (C.c_int, [vp] * 13),
     # tblaeq_serve.h
     "tblaeq_server_config_init": (None, [vp]),
     "tblaeq_server_create": (C.c_int, [vp, vp, C.POINTER(vp)]),
@@ -260,6 +281,36 @@ class Handler:
         check(lib().tblaeq_handler_batch_get_param(self.h, int(param), C.byref(v)), "batch_get_param")
         return v.value
 
+    def knn_features(self, points, Ks, vmin, scale):
+        """Learned-radius features [n x 48] of host KNN queries (batch_init first)."""
+        p = np.ascontiguousarray(points, np.float64)
+        k = np.ascontiguousarray(np.broadcast_to(np.asarray(Ks, np.uint64), (p.shape[0],)))
+        X = np.zeros((p.shape[0], 48), np.float32)
+        check(lib().tblaeq_handler_knn_features(self.h, _p(p, f64p), _p(k, u64p), p.shape[0], float(vmin),
+                                                float(scale), _p(X, f32p)), "knn_features")
+        return X
+
+    def query_learned(self, points, K, radii):
+        """Per-query KNN with a learned radius per query (certified, reruns on a miss);
+        returns (Result with refined top-K, number of reruns)."""
+        q = np.ascontiguousarray(points, np.float64)
+        if q.ndim == 1:
+            q = q[None, :]
+        r = np.ascontiguousarray(np.broadcast_to(np.asarray(radii, np.float64), (q.shape[0],)))
+        fb = C.c_uint64(0)
+        out = vp()
+        check(lib().tblaeq_handler_query_learned(self.h, _p(q, f64p), q.shape[0], int(K), _p(r, f64p),
+                                                 C.byref(fb), C.byref(out)), "query_learned")
+        return Result(out, self.D, self.intervals), fb.value
+
+    def batch_knn_learned(self, mlp, factor=1.0, stats=False):
+        """Runs the uploaded KNN batch with radii proposed by `mlp`; returns the number of
+        queries that failed certification and were rerun by the seed method."""
+        f = C.c_uint64(0)
+        check(lib().tblaeq_handler_batch_knn_learned(self.h, mlp.m, float(factor), int(stats), C.byref(f)),
+              "batch_knn_learned")
+        return f.value
+
     def batch_work(self):
         """(tests, candidate rows) of the last batch_run's final stage."""
         w = np.zeros(2, np.uint64)
@@ -313,6 +364,65 @@ class Handler:
         if getattr(self, "h", None) and _LIB is not None:
             _LIB.tblaeq_handler_destroy(self.h)
             self.h = None
+
+
+class Mlp:
+    """The learned-radius network (tblaeq_learn.h) on a handler's stream."""
+
+    def __init__(self, handler, seed=7):
+        self.m = vp()
+        check(lib().tblaeq_mlp_create(handler.ctx(), int(seed), C.byref(self.m)), "mlp_create")
+        self.n_params = int(lib().tblaeq_mlp_param_count())
+
+    def get_params(self):
+        w = np.zeros(self.n_params, np.float32)
+        check(lib().tblaeq_mlp_get_params(self.m, _p(w, f32p)), "mlp_get_params")
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == self.n_params
+        check(lib().tblaeq_mlp_set_params(self.m, _p(w, f32p)), "mlp_set_params")
+
+    def set_norm(self, vmin, scale):
+        check(lib().tblaeq_mlp_set_norm(self.m, float(vmin), float(scale)), "mlp_set_norm")
+
+    def set_loss_weights(self, w4):
+        w = np.ascontiguousarray(w4, np.float32)
+        check(lib().tblaeq_mlp_set_loss_weights(self.m, _p(w, f32p)), "mlp_set_loss_weights")
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], 4), np.float32)
+        check(lib().tblaeq_mlp_forward(self.m, _p(X, f32p), X.shape[0], _p(Y, f32p)), "mlp_forward")
+        return Y
+
+    def gradients(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = np.zeros(1, np.float32)
+        check(lib().tblaeq_mlp_gradients(self.m, _p(X, f32p), _p(T, f32p), X.shape[0], _p(g, f32p),
+                                         _p(loss, f32p)), "mlp_gradients")
+        return g, float(loss[0])
+
+    def train(self, X, T, epochs=10, batch=1024, lr=1e-3):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        losses = np.zeros(epochs, np.float32)
+        check(lib().tblaeq_mlp_train(self.m, _p(X, f32p), _p(T, f32p), X.shape[0], int(epochs), int(batch),
+                                     float(lr), _p(losses, f32p)), "mlp_train")
+        return losses
+
+    def close(self):
+        if self.m:
+            lib().tblaeq_mlp_destroy(self.m)
+            self.m = None
+
+    def __del__(self):
+        if getattr(self, "m", None) and _LIB is not None:
+            _LIB.tblaeq_mlp_destroy(self.m)
+            self.m = None
 
 
 class ServerConfig(C.Structure):
