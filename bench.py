@@ -159,6 +159,70 @@ def run_serving(h, data, cfg, rates, seconds, log):
     return out
 
 
+def run_learned(h, data, cfg, knn_q, Ks, n_traces, epochs, plain_sample, log):
+    """Learned KNN radius on this index: traces from the batched engine, BF16-MFMA
+    training on the GPU, then the per-query traversal with and without the
+    learned radius on a sample (exact answers either way) and the batched engine
+    with learned radii against the seed method."""
+    import tblaeq
+    from tblaeq import learn
+
+    t0 = time.perf_counter()
+    X, T = learn.traces(h, n_traces, cfg["seed"] + 7, cfg["vmin"], cfg["vmax"])
+    t1 = time.perf_counter()
+    mlp = tblaeq.Mlp(h, seed=7)
+    mlp.set_norm(cfg["vmin"], cfg["vmax"] - cfg["vmin"])
+    losses = learn.fit(mlp, X, T, epochs=epochs, batch=1024, lr=2e-3)
+    t2 = time.perf_counter()
+    Y = mlp.forward(X[:20000])
+    resid = T[:20000, 0] - Y[:, 0]
+    factor = float(np.exp(np.quantile(resid, 0.9)))  # first radius covers 90% of the traces
+    out = {"traces": n_traces, "trace_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2), "epochs": epochs,
+           "loss_first_last": [round(float(losses[0]), 5), round(float(losses[-1]), 5)],
+           "log_dk_residual_rms": round(float(np.sqrt(np.mean(resid ** 2))), 4), "radius_factor": round(factor, 4)}
+    # per-query path, K = 10, sample of the benchmark's KNN points
+    P = plain_sample
+    pts = knn_q[:P]
+    radii = learn.propose_radii(h, mlp, pts, np.full(P, 10), cfg["vmin"], cfg["vmax"], factor=factor)
+    h.query("knn", pts[:4], K=10, refine=2)
+    h.query_learned(pts[:4], 10, radii[:4])
+    t0 = time.perf_counter()
+    plain = h.query("knn", pts, K=10, refine=True)
+    t1 = time.perf_counter()
+    learned, fb = h.query_learned(pts, 10, radii)
+    t2 = time.perf_counter()
+    same = all(np.array_equal(plain.refined(i)[0], learned.refined(i)[0]) for i in range(P))
+    fp = sum(plain.info(i)["fine"] for i in range(P))
+    fl = sum(learned.info(i)["fine"] for i in range(P))
+    out["per_query"] = {"queries": P, "K": 10, "plain_qps": round(P / (t1 - t0), 1),
+                        "learned_qps": round(P / (t2 - t1), 1), "speedup": round((t1 - t0) / (t2 - t1), 2),
+                        "retries_per_query": round(fb / P, 4), "prolonged_points_plain": int(fp),
+                        "prolonged_points_learned": int(fl), "answers_identical": bool(same)}
+    # batched engine: learned radii vs the seed method on the benchmark's KNN batch
+    h.batch_set("knn", knn_q, Ks)
+    h.batch_run()
+    ref = h.batch_results()
+    tb = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        h.batch_run()
+        tb.append(time.perf_counter() - t0)
+    tl = []
+    fbb = 0
+    for _ in range(3):
+        t0 = time.perf_counter()
+        fbb = h.batch_knn_learned(mlp, factor=factor)
+        tl.append(time.perf_counter() - t0)
+    res = h.batch_results()
+    same_b = all(np.array_equal(a[0], b[0]) for a, b in zip(ref, res))
+    out["batched"] = {"queries": len(knn_q), "seed_ms": round(min(tb) * 1e3, 3),
+                      "learned_ms": round(min(tl) * 1e3, 3), "fallback_rate": round(fbb / len(knn_q), 4),
+                      "answers_identical": bool(same_b)}
+    mlp.close()
+    log("learned", out)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,6 +240,8 @@ def main():
     ap.add_argument("--serve-rates", default="20000,100000",
                     help="offered request rates (req/s) of the serving run; empty to skip")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--learn-traces", type=int, default=200000, help="0 skips the learned-radius run")
+    ap.add_argument("--learn-epochs", type=int, default=20)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once on the GPU (a kernel of known "
                          "traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -318,6 +384,11 @@ def main():
                                 "per 64-box slice, so HBM is not its bound",
                         "plain": plain_roofline}
 
+    learned = None
+    if args.learn_traces and not args.no_batch:
+        learned = run_learned(h, data, cfg, knn_q, Ks.astype(np.uint64), args.learn_traces, args.learn_epochs,
+                              min(args.plain_sample, 96), log)
+
     serving = None
     if args.serve_rates and not args.no_batch:
         rates = [float(x) for x in args.serve_rates.split(",") if x]
@@ -360,6 +431,7 @@ def main():
         "plain": {"qps": round(plain_qps, 3), "p50_us": round(float(np.percentile(us, 50)), 1),
                   "p99_us": round(float(np.percentile(us, 99)), 1), "queries": nq},
         "batched": batched,
+        "learned": learned,
         "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -87,8 +87,9 @@ int tblaeq_handler_batch_knn_learned(struct tblaeq_handler* h, tblaeq_mlp* m, do
 /* Per-query (plain) path with a learned radius per query (host radii [nq],
  * e.g. scale * exp(y0) * factor from tblaeq_mlp_forward): the last level
  * prolongs only the STEP survivors within the radius; the exact top-K is
- * certified when K results lie within the radius, else the query is rerun in
- * full.  Result as tblaeq_handler_query with refine on; *fallbacks = reruns. */
+ * certified when K results lie within the radius, else the radius grows by
+ * 1.25 (at most three tries) before a full traversal.  Result as
+ * tblaeq_handler_query with refine on; *fallbacks = retries (misses). */
 struct tblaeq_result;
 int tblaeq_handler_query_learned(struct tblaeq_handler* h, const double* points, uint64_t nq, uint64_t K,
                                  const double* radii, uint64_t* fallbacks, struct tblaeq_result** out);

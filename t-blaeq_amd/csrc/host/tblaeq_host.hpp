@@ -390,7 +390,8 @@ public:
         bool quiet = true;     // no console output inside the query loop
         // learned KNN radius per query (refine must be on): the last level
         // prolongs only survivors within the radius; a query whose K-th result
-        // lies beyond it (or that finds fewer than K) is rerun in full
+        // lies beyond it (or that finds fewer than K) retries with a larger
+        // radius, then runs in full
         const double* radii = nullptr;
         uint64_t* fallbacks = nullptr;
     };

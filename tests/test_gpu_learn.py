@@ -214,7 +214,7 @@ def test_plain_learned_knn_gains_over_plain_traversal(idx):
     assert fine_learned * 2 < fine_plain, (fine_learned, fine_plain)
     _check(h, o, pts[:30], np.full(30, K), [learned.refined(i) for i in range(30)])
     bad, fb_all = h.query_learned(pts[:20], K, radii[:20] * 1e-6)
-    assert fb_all == 20
+    assert fb_all == 3 * 20  # three growing radii miss, then the full traversal
     for i in range(20):
         assert np.array_equal(bad.refined(i)[0], plain.refined(i)[0])
     mlp.close()
