@@ -45,7 +45,8 @@ struct tblaeq_batch {
     uint64_t* recAux = nullptr;  // [hitCap] grouped KNN record ids
     uint64_t* hist = nullptr;    // radix histograms
     uint64_t histCap = 0;
-    uint64_t* ctl = nullptr;     // [0] pairs [1] items [2] hits [3] overflow [4..] spare
+    uint64_t* ctl = nullptr;     // [0] pairs [1] items [2] hits [3] overflow [4..] spare; kCtlWords
+    uint64_t* queue = nullptr;   // [kQueueSlots] work-queue counters of the grid-stride kernels (ctl + kCtlWords)
     // captured range batch (replayed while the arguments are unchanged)
     hipGraph_t rangeGraph = nullptr;
     hipGraphExec_t rangeExec = nullptr;
@@ -75,6 +76,8 @@ constexpr int kSliceBits = 20;
 constexpr int kQueryBits = 24;
 constexpr int kStepWaves = 4;  // waves per workgroup of the cluster-major kernels
 
+constexpr int kCtlWords = 16;
+constexpr int kQueueSlots = 64;
 // ctl slots: [0] pairs [1] items [2] hits / records [3] overflow flags
 // (1 pairs, 2 items, 4 hits / records) [4] items of the second group set
 // [6], [7] scratch of count-only scans.
@@ -92,4 +95,8 @@ int launch_count_scan(tblaeq_batch* b, const uint64_t* cnt, uint64_t n, uint64_t
 void launch_copy_status(tblaeq_batch* b, uint64_t* status);
 // Fills n words with v (a kernel, so that captured batches stay replayable).
 void launch_fill(tblaeq_batch* b, uint64_t* p, uint64_t v, uint64_t n);
+// Blocks of `kernel` (with `threads` threads) that fit on one CU at once,
+// queried once per kernel.  Work-queue kernels launch at most this many per CU:
+// every wave is resident from the start and takes items until the queue is dry.
+int resident_blocks(const void* kernel, int threads);
 }  // namespace tblaeq

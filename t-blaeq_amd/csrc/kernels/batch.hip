@@ -13,6 +13,8 @@
 // path (RangePruning.cu:5-25: fma chain of the clamped differences against
 // r*r); the last step is the exact point-in-box filter of the candidates.
 #include <algorithm>
+#include <mutex>
+#include <unordered_map>
 
 #include "../batch_internal.hpp"
 #include "../scan_common.hpp"
@@ -54,10 +56,12 @@ __global__ void k_set_u64(uint64_t* p, uint64_t v, int n) {
 //
 // Work item = (cluster c of the previous level, slice of <= 64 queries that kept
 // c), or at level 0 a slice of queries over all coarsest points.  Lane = query,
-// box in registers.  The cluster's children are staged per wave in LDS, 64 at a
-// time, with coalesced vector loads (row stride DM; padding dimensions hold 0
-// in the children and in the boxes, which passes the box test and adds
-// fma(0,0,d) = d to the distance), then tested branch-free against every lane.
+// box in registers.  The cluster's children are read one row at a time with
+// wave-uniform scalar loads (coordinates, radius, bounding box into SGPRs; the
+// next row's load is issued before the current row's arithmetic) and tested
+// branch-free against every lane.  Padding dimensions (D < DM) hold 0 in the
+// child and in the box, which passes the box test and adds fma(0,0,d) = d to
+// the distance.
 //
 // Survivors of a prune are grouped by cluster without materialising pairs: the
 // count pass adds each child's survivor mask popcount to cnt[id]; after the
@@ -104,7 +108,34 @@ struct RangeStepArgs {
     uint64_t* fineStat;  // [nq]
     int level, L;
     uint64_t* work;      // final: [0] box tests, [1] child rows scanned
+    uint64_t* queue;     // final: work-queue counter (zero at launch)
 };
+
+// Uniform copy of a value every lane holds (keeps it in SGPRs).
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Next chunk [*begin, *end) of the nItems work items for this wave, taken from
+// the queue counter in chunks of `chunk`; false when the queue is dry.
+__device__ __forceinline__ bool next_items(uint64_t* queue, uint64_t chunk, uint64_t nItems, int lane,
+                                           uint64_t* begin, uint64_t* end) {
+    uint64_t base = 0;
+    if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(queue), chunk);
+    base = uniform_u64(__shfl(base, 0, kWave));
+    if (base >= nItems) return false;
+    *begin = base;
+    *end = base + chunk < nItems ? base + chunk : nItems;
+    return true;
+}
+
+// Items per queue grab: about 16 grabs per wave.
+__device__ __forceinline__ uint64_t queue_chunk(uint64_t nItems, uint64_t waves) {
+    const uint64_t c = nItems / (waves * 16);
+    return c ? c : 1;
+}
 
 // Appends a wave's staged hits to the global hit list (entries past the
 // capacity are dropped and flagged).
@@ -120,32 +151,30 @@ __device__ __forceinline__ void flush_hits(const RangeStepArgs& A, const uint64_
     wave_lds_sync();
 }
 
-template <int DM, int MODE, int STAGE>
+// Read-only device arrays addressed through the constant address space: a
+// wave-uniform load from such a pointer is a scalar load into SGPRs, so a child
+// row is fetched once per wave (s_load of the whole row) and enters the vector
+// compares as a scalar operand.  Only valid for arrays no kernel writes while
+// the reading kernel runs.
+typedef const __attribute__((address_space(4))) double* ConstF64;
+typedef const __attribute__((address_space(4))) uint64_t* ConstU64;
+typedef const __attribute__((address_space(4))) float* ConstF32;
+
+template <int DM, int MODE, bool EXACT>
 __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs A) {
-    constexpr bool FINAL = MODE == kFinal;
-    constexpr int kHitStage = STAGE;
-    __shared__ uint64_t s_hit[kStepWaves][FINAL ? kHitStage : 1];
-    uint64_t staged = 0, wTests = 0, wRows = 0;
-    __shared__ double s_x[kStepWaves][kWave * DM];
-    __shared__ double s_r[kStepWaves][FINAL ? 1 : kWave];
-    __shared__ float s_ab[kStepWaves][FINAL ? 1 : kWave * 2 * DM];
+    static_assert(MODE == kCount || MODE == kWrite, "prune modes only");
     const int lane = threadIdx.x & (kWave - 1);
     const uint64_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint64_t gw = blockIdx.x * static_cast<uint64_t>(kStepWaves) + wave;
     const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kStepWaves;
-    const int D = A.D;
-    uint64_t nItems = A.level0 ? (A.nq + kWave - 1) / kWave : *A.nItems;
+    const int D = EXACT ? DM : A.D;
+    const ConstF64 cx = (ConstF64)A.coords;
+    const ConstF64 crad = (ConstF64)A.radiusPos;
+    const ConstF32 cab = (ConstF32)A.aabb;
+    uint64_t nItems = A.level0 ? (A.nq + kWave - 1) / kWave : ((ConstU64)A.nItems)[0];
     if (!A.level0 && nItems > A.itemCap) nItems = A.itemCap;
     const uint64_t ltmask = (1ull << lane) - 1ull;
-    double* sx = s_x[wave];
-    double* sr = s_r[wave];
-    float* sab = s_ab[wave];
-    const bool useBox = !FINAL && A.aabb != nullptr;
-    if (D != DM) {
-        for (int t = lane; t < kWave * DM; t += kWave) sx[t] = 0.0;
-        if (!FINAL)
-            for (int t = lane; t < kWave * 2 * DM; t += kWave) sab[t] = 0.0f;
-    }
+    const bool useBox = A.aabb != nullptr;
     for (uint64_t item = gw; item < nItems; item += nw) {
         uint64_t jb, je, q = 0;
         bool valid;
@@ -156,92 +185,69 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
             jb = 0;
             je = A.count0;
         } else {
-            const uint64_t it = A.items[item];
+            const uint64_t it = ((ConstU64)A.items)[item];
             const uint64_t c = it >> kSliceBits;
             const uint64_t start = (it & ((1ull << kSliceBits) - 1)) * kWave;
-            const uint64_t g0 = A.off[c];
-            const uint64_t gn = A.off[c + 1] - g0;
+            const uint64_t g0 = ((ConstU64)A.off)[c];
+            const uint64_t gn = ((ConstU64)A.off)[c + 1] - g0;
             valid = start + lane < gn && g0 + start + lane < A.qcap;
             q = valid ? A.qlist[g0 + start + lane] : 0;
-            jb = A.colResPrev[c];
-            je = A.colResPrev[c + 1];
-        }
-        if (FINAL) {
-            wRows += je - jb;
-            wTests += (je - jb) * static_cast<uint64_t>(__popcll(__ballot(valid)));
+            jb = ((ConstU64)A.colResPrev)[c];
+            je = ((ConstU64)A.colResPrev)[c + 1];
         }
         double lo[DM], hi[DM];
         float flo[DM], fhi[DM];
 #pragma unroll
         for (int d = 0; d < DM; ++d) {
-            lo[d] = 0.0;
-            hi[d] = 0.0;
-            if (d < D) {
-                lo[d] = A.boxes[q * 2 * D + d];
-                hi[d] = A.boxes[q * 2 * D + D + d];
-            }
+            lo[d] = d < D ? A.boxes[q * 2 * D + d] : 0.0;
+            hi[d] = d < D ? A.boxes[q * 2 * D + D + d] : 0.0;
             flo[d] = __double2float_rd(lo[d]);
             fhi[d] = __double2float_ru(hi[d]);
         }
         uint64_t mine = 0, fine = 0;
         for (uint64_t j0 = jb; j0 < je; j0 += kWave) {
             const int nc = static_cast<int>(je - j0 < static_cast<uint64_t>(kWave) ? je - j0 : kWave);
-            {
-                const double* src = A.coords + j0 * D;
-                const int tot = nc * D;
-                if (D == DM) {
-                    for (int t = lane; t < tot; t += kWave) sx[t] = src[t];
-                } else {
-                    for (int t = lane; t < tot; t += kWave) sx[(t / D) * DM + t % D] = src[t];
-                }
-                if (!FINAL && lane < nc) sr[lane] = A.radiusPos[j0 + lane];
-                if (useBox) {
-                    const float* bsrc = A.aabb + j0 * 2 * D;
-                    const int btot = nc * 2 * D;
-                    for (int t = lane; t < btot; t += kWave) {
-                        const int row = t / (2 * D), rem = t % (2 * D);
-                        sab[row * 2 * DM + (rem < D ? rem : DM + rem - D)] = bsrc[t];
-                    }
-                }
-            }
             const uint64_t myId = lane < nc ? A.ids[j0 + lane] : 0;
-            wave_lds_sync();
             uint64_t myMask = 0, myBall = 0;
+            double xn[DM], rn;
+#pragma unroll
+            for (int d = 0; d < DM; ++d) xn[d] = d < D ? cx[j0 * D + d] : 0.0;
+            rn = crad[j0];
             for (int k = 0; k < nc; ++k) {
-                const double* x = sx + k * DM;
+                const uint64_t j = j0 + k;
                 double xv[DM];
 #pragma unroll
-                for (int d = 0; d < DM; ++d) xv[d] = x[d];
-                if (FINAL) {
-                    uint64_t m = __ballot(valid);
+                for (int d = 0; d < DM; ++d) xv[d] = xn[d];
+                const double r = rn;
+                if (k + 1 < nc) {
 #pragma unroll
-                    for (int d = 0; d < DM; ++d) m &= __ballot(xv[d] >= lo[d]) & __ballot(xv[d] <= hi[d]);
-                    if (lane == k) myMask = m;
-                } else {
-                    double dist = 0.0;
+                    for (int d = 0; d < DM; ++d) xn[d] = d < D ? cx[(j + 1) * D + d] : 0.0;
+                    rn = crad[j + 1];
+                }
+                double dist = 0.0;
+#pragma unroll
+                for (int d = 0; d < DM; ++d) {
+                    const double below = lo[d] - xv[d], above = xv[d] - hi[d];
+                    const double diff = xv[d] < lo[d] ? below : (xv[d] > hi[d] ? above : 0.0);
+                    dist = __builtin_fma(diff, diff, dist);
+                }
+                const bool ball = valid && dist <= r * r;
+                bool keep = ball;
+                if (useBox) {
 #pragma unroll
                     for (int d = 0; d < DM; ++d) {
-                        const double below = lo[d] - xv[d], above = xv[d] - hi[d];
-                        const double diff = xv[d] < lo[d] ? below : (xv[d] > hi[d] ? above : 0.0);
-                        dist = __builtin_fma(diff, diff, dist);
-                    }
-                    const double r = sr[k];
-                    const bool ball = valid && dist <= r * r;
-                    bool keep = ball;
-                    if (useBox) {
-                        const float* ab = sab + k * 2 * DM;
-#pragma unroll
-                        for (int d = 0; d < DM; ++d) keep = keep & (ab[DM + d] >= flo[d]) & (ab[d] <= fhi[d]);
-                    }
-                    const uint64_t m = __ballot(keep);
-                    if (lane == k) myMask = m;
-                    if (MODE == kCount) {
-                        const uint64_t mb = useBox ? __ballot(ball) : m;
-                        if (lane == k) myBall = mb;
+                        const float blo = d < D ? cab[j * 2 * D + d] : 0.0f;
+                        const float bhi = d < D ? cab[j * 2 * D + D + d] : 0.0f;
+                        keep = keep & (bhi >= flo[d]) & (blo <= fhi[d]);
                     }
                 }
+                const uint64_t m = __ballot(keep);
+                if (lane == k) myMask = m;
+                if (MODE == kCount) {
+                    const uint64_t mb = useBox ? __ballot(ball) : m;
+                    if (lane == k) myBall = mb;
+                }
             }
-            wave_lds_sync();
             if (MODE == kCount) {
                 // ball survivors per lane for the statistics; kept pairs per cluster
                 if (A.survStat || A.fineStat) {
@@ -260,61 +266,22 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
                               static_cast<unsigned long long>(__popcll(myMask)));
                 continue;
             }
-            if (MODE == kWrite) {
-                uint64_t myBase = 0, myOff = 0;
-                if (lane < nc && myMask) {
-                    const uint64_t pc = __popcll(myMask);
-                    myBase = atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[myId]), 0ull - pc) - pc;
-                    myOff = A.offNext[myId];
-                }
-                for (int k = 0; k < nc; ++k) {
-                    const uint64_t m = __shfl(myMask, k, kWave);
-                    const uint64_t base = __shfl(myOff, k, kWave) + __shfl(myBase, k, kWave);
-                    if (m == 0) continue;
-                    if ((m >> lane) & 1ull) {
-                        const uint64_t o = base + __popcll(m & ltmask);
-                        if (o < A.cap) A.qlistNext[o] = static_cast<uint32_t>(q);
-                        else atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 1ull);
-                    }
-                }
-                continue;
+            uint64_t myBase = 0, myOff = 0;
+            if (lane < nc && myMask) {
+                const uint64_t pc = __popcll(myMask);
+                myBase = atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[myId]), 0ull - pc) - pc;
+                myOff = A.offNext[myId];
             }
-            // final: hits are staged per wave in LDS and appended to the global
-            // list in runs of up to kHitStage (one atomic per run)
-            uint64_t total = lane < nc ? __popcll(myMask) : 0;
-            for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o, kWave);
-            if (total == 0) continue;
-            if (staged + total > static_cast<uint64_t>(kHitStage)) {
-                flush_hits(A, s_hit[wave], staged, lane);
-                staged = 0;
-            }
-            if (total > static_cast<uint64_t>(kHitStage)) {
-                uint64_t base = 0;
-                if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(A.outCount), total);
-                base = __shfl(base, 0, kWave);
-                uint64_t pos = base;
-                for (int k = 0; k < nc; ++k) {
-                    const uint64_t m = __shfl(myMask, k, kWave);
-                    const uint64_t id = __shfl(myId, k, kWave);
-                    if (m == 0) continue;
-                    if ((m >> lane) & 1ull) {
-                        const uint64_t o = pos + __popcll(m & ltmask);
-                        if (o < A.cap) A.out[o] = (q << A.idBits) | id;
-                    }
-                    pos += __popcll(m);
-                }
-                if (lane == 0 && base + total > A.cap) atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 4ull);
-                continue;
-            }
-            uint64_t pos = staged;
             for (int k = 0; k < nc; ++k) {
                 const uint64_t m = __shfl(myMask, k, kWave);
-                const uint64_t id = __shfl(myId, k, kWave);
+                const uint64_t base = __shfl(myOff, k, kWave) + __shfl(myBase, k, kWave);
                 if (m == 0) continue;
-                if ((m >> lane) & 1ull) s_hit[wave][pos + __popcll(m & ltmask)] = (q << A.idBits) | id;
-                pos += __popcll(m);
+                if ((m >> lane) & 1ull) {
+                    const uint64_t o = base + __popcll(m & ltmask);
+                    if (o < A.cap) A.qlistNext[o] = static_cast<uint32_t>(q);
+                    else atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 1ull);
+                }
             }
-            staged += total;
         }
         if (MODE == kCount && valid) {
             if (A.survStat && mine)
@@ -323,13 +290,203 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
                 atomicAdd(reinterpret_cast<unsigned long long*>(&A.fineStat[q]), fine);
         }
     }
-    if (FINAL) {
-        flush_hits(A, s_hit[wave], staged, lane);
-        // work done by this wave: box tests (child rows x queries) and child rows
-        if (lane == 0 && A.work) {
-            atomicAdd(reinterpret_cast<unsigned long long*>(A.work), wTests);
-            atomicAdd(reinterpret_cast<unsigned long long*>(A.work + 1), wRows);
+}
+
+// Lanes of m whose box [lo, hi] contains the wave-uniform point x in four
+// dimensions: a chain of v_cmpx compares narrows EXEC (starting from EXEC & m),
+// the survivors are read back and EXEC is restored inside the one asm block, so
+// no compiler-visible code runs with the narrowed EXEC.  Same predicate as
+// x >= lo && x <= hi per dimension (a NaN on either side compares false); the
+// mask stays in SGPRs with no per-compare scalar AND.
+__device__ __forceinline__ uint64_t box_mask4(uint64_t m, const double* x, const double* lo, const double* hi) {
+    uint64_t out, save;
+    asm volatile(
+        "s_mov_b64 %[sv], exec\n\t"
+        "s_and_b64 exec, exec, %[m]\n\t"
+        "v_cmpx_ge_f64 vcc, %[x0], %[l0]\n\t"
+        "v_cmpx_le_f64 vcc, %[x0], %[h0]\n\t"
+        "v_cmpx_ge_f64 vcc, %[x1], %[l1]\n\t"
+        "v_cmpx_le_f64 vcc, %[x1], %[h1]\n\t"
+        "v_cmpx_ge_f64 vcc, %[x2], %[l2]\n\t"
+        "v_cmpx_le_f64 vcc, %[x2], %[h2]\n\t"
+        "v_cmpx_ge_f64 vcc, %[x3], %[l3]\n\t"
+        "v_cmpx_le_f64 vcc, %[x3], %[h3]\n\t"
+        "s_mov_b64 %[out], exec\n\t"
+        "s_mov_b64 exec, %[sv]"
+        : [out] "=&s"(out), [sv] "=&s"(save)
+        : [m] "s"(m), [x0] "s"(x[0]), [x1] "s"(x[1]), [x2] "s"(x[2]), [x3] "s"(x[3]), [l0] "v"(lo[0]),
+          [l1] "v"(lo[1]), [l2] "v"(lo[2]), [l3] "v"(lo[3]), [h0] "v"(hi[0]), [h1] "v"(hi[1]), [h2] "v"(hi[2]),
+          [h3] "v"(hi[3])
+        : "vcc");
+    return out;
+}
+
+__device__ __forceinline__ uint64_t box_mask8(uint64_t m, const double* x, const double* lo, const double* hi) {
+    uint64_t out, save;
+    asm volatile(
+        "s_mov_b64 %[sv], exec\n\t"
+        "s_and_b64 exec, exec, %[m]\n\t"
+        "v_cmpx_ge_f64 vcc, %[x0], %[l0]\n\t"
+        "v_cmpx_le_f64 vcc, %[x0], %[h0]\n\t"
+        "v_cmpx_ge_f64 vcc, %[x1], %[l1]\n\t"
+        "v_cmpx_le_f64 vcc, %[x1], %[h1]\n\t"
+        "v_cmpx_ge_f64 vcc, %[x2], %[l2]\n\t"
+        "v_cmpx_le_f64 vcc, %[x2], %[h2]\n\t"
+        "v_cmpx_ge_f64 vcc, %[x3], %[l3]\n\t"
+        "v_cmpx_le_f64 vcc, %[x3], %[h3]\n\t"
+        "v_cmpx_ge_f64 vcc, %[x4], %[l4]\n\t"
+        "v_cmpx_le_f64 vcc, %[x4], %[h4]\n\t"
+        "v_cmpx_ge_f64 vcc, %[x5], %[l5]\n\t"
+        "v_cmpx_le_f64 vcc, %[x5], %[h5]\n\t"
+        "v_cmpx_ge_f64 vcc, %[x6], %[l6]\n\t"
+        "v_cmpx_le_f64 vcc, %[x6], %[h6]\n\t"
+        "v_cmpx_ge_f64 vcc, %[x7], %[l7]\n\t"
+        "v_cmpx_le_f64 vcc, %[x7], %[h7]\n\t"
+        "s_mov_b64 %[out], exec\n\t"
+        "s_mov_b64 exec, %[sv]"
+        : [out] "=&s"(out), [sv] "=&s"(save)
+        : [m] "s"(m), [x0] "s"(x[0]), [x1] "s"(x[1]), [x2] "s"(x[2]), [x3] "s"(x[3]), [x4] "s"(x[4]),
+          [x5] "s"(x[5]), [x6] "s"(x[6]), [x7] "s"(x[7]), [l0] "v"(lo[0]), [l1] "v"(lo[1]), [l2] "v"(lo[2]),
+          [l3] "v"(lo[3]), [l4] "v"(lo[4]), [l5] "v"(lo[5]), [l6] "v"(lo[6]), [l7] "v"(lo[7]), [h0] "v"(hi[0]),
+          [h1] "v"(hi[1]), [h2] "v"(hi[2]), [h3] "v"(hi[3]), [h4] "v"(hi[4]), [h5] "v"(hi[5]), [h6] "v"(hi[6]),
+          [h7] "v"(hi[7])
+        : "vcc");
+    return out;
+}
+
+template <int DM>
+__device__ __forceinline__ uint64_t box_mask(uint64_t m, const double* x, const double* lo, const double* hi) {
+    if constexpr (DM == 4) {
+        return box_mask4(m, x, lo, hi);
+    } else {
+#pragma unroll
+        for (int d = 0; d < DM; d += 8) m = box_mask8(m, x + d, lo + d, hi + d);
+        return m;
+    }
+}
+
+// Wave-uniform child row r of the cluster at `row` (D == DM: one wide scalar
+// load; padding dimensions read as 0).
+template <int DM, bool EXACT>
+__device__ __forceinline__ void load_row(double* x, ConstF64 row, uint32_t r, int D) {
+#pragma unroll
+    for (int d = 0; d < DM; ++d) x[d] = (EXACT || d < D) ? row[r * (EXACT ? DM : D) + d] : 0.0;
+}
+
+// Queries of work item (cluster, 64-query slice) `it` for this lane: *q and
+// whether the lane holds one.  Level 0 (no clusters) takes query slices.
+__device__ __forceinline__ bool slice_query(const RangeStepArgs& A, uint64_t c, uint64_t slice, int lane,
+                                            uint64_t* q) {
+    const uint64_t g0 = ((ConstU64)A.off)[c];
+    const uint64_t gn = ((ConstU64)A.off)[c + 1] - g0;
+    const uint64_t i = slice * kWave + lane;
+    const bool valid = i < gn && g0 + i < A.qcap;
+    *q = valid ? A.qlist[g0 + i] : 0;
+    return valid;
+}
+
+// The exact point-in-box filter of the last level (RangePruning.cu:5-25 with
+// r = 0 reduces to x in [lo, hi] per dimension).  A wave takes the work items
+// of an even slice 2s of a cluster together with its odd successor 2s + 1 (odd
+// items are skipped): every lane holds up to two query boxes in registers, so
+// each child row -- a scalar load into SGPRs, the next row in flight while the
+// current one is compared -- is tested against up to 128 boxes.  The hit mask of
+// a row and box set is box_mask.  Hits go straight into the wave's LDS stage and
+// are appended to the global list in runs of up to STAGE (one atomic per run).
+// Items come from a work queue (chunks of consecutive items per grab).
+template <int DM, bool EXACT, int STAGE>
+__global__ __launch_bounds__(kStepWaves * kWave) void k_range_final(RangeStepArgs A) {
+    __shared__ uint64_t s_hit[kStepWaves][STAGE];
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kStepWaves;
+    const int D = EXACT ? DM : A.D;
+    uint64_t nItems = A.level0 ? (A.nq + kWave - 1) / kWave : ((ConstU64)A.nItems)[0];
+    if (!A.level0 && nItems > A.itemCap) nItems = A.itemCap;
+    uint64_t* stage = s_hit[wave];
+    uint32_t staged = 0;
+    uint64_t wTests = 0, wRows = 0;
+    const uint64_t chunk = queue_chunk(nItems, nw);
+    // appends the hits of one row (lanes of m) to the stage
+    auto emit = [&](uint64_t m, uint64_t key) {
+        const uint32_t pc = static_cast<uint32_t>(__popcll(m));
+        if (staged + pc > static_cast<uint32_t>(STAGE)) {
+            flush_hits(A, stage, staged, lane);
+            staged = 0;
         }
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+        if ((m >> lane) & 1ull) stage[staged + rank] = key;
+        staged += pc;
+    };
+    uint64_t ib, ie;
+    while (next_items(A.queue, chunk, nItems, lane, &ib, &ie))
+    for (uint64_t item = ib; item < ie; ++item) {
+        uint64_t jb, je, q0 = 0, q1 = 0;
+        bool v0, v1 = false;
+        if (A.level0) {
+            const uint64_t qq = item * kWave + lane;
+            v0 = qq < A.nq;
+            q0 = v0 ? qq : 0;
+            jb = 0;
+            je = A.count0;
+        } else {
+            const uint64_t it = ((ConstU64)A.items)[item];
+            const uint64_t c = it >> kSliceBits;
+            const uint64_t slice = it & ((1ull << kSliceBits) - 1);
+            if (slice & 1) continue;  // done with its even partner
+            v0 = slice_query(A, c, slice, lane, &q0);
+            v1 = slice_query(A, c, slice + 1, lane, &q1);
+            jb = ((ConstU64)A.colResPrev)[c];
+            je = ((ConstU64)A.colResPrev)[c + 1];
+        }
+        const uint64_t vm0 = __ballot(v0), vm1 = __ballot(v1);
+        wRows += je - jb;
+        wTests += (je - jb) * static_cast<uint64_t>(__popcll(vm0) + __popcll(vm1));
+        if ((vm0 | vm1) == 0 || jb >= je) continue;
+        double lo0[DM], hi0[DM], lo1[DM], hi1[DM];
+#pragma unroll
+        for (int d = 0; d < DM; ++d) {
+            lo0[d] = d < D ? A.boxes[q0 * 2 * D + d] : 0.0;
+            hi0[d] = d < D ? A.boxes[q0 * 2 * D + D + d] : 0.0;
+            lo1[d] = d < D ? A.boxes[q1 * 2 * D + d] : 0.0;
+            hi1[d] = d < D ? A.boxes[q1 * 2 * D + D + d] : 0.0;
+        }
+        const uint64_t k0 = q0 << A.idBits, k1 = q1 << A.idBits;
+        const ConstF64 rows = (ConstF64)A.coords + jb * D;
+        const ConstU64 rid = (ConstU64)A.ids + jb;
+        const uint32_t n = static_cast<uint32_t>(je - jb);
+        double xa[DM], xb[DM];
+        uint64_t ida = rid[0], idb = 0;
+        load_row<DM, EXACT>(xa, rows, 0, D);
+        // one row: compare against both box sets, emit
+        auto test = [&](const double* x, uint64_t id) {
+            const uint64_t m0 = box_mask<DM>(vm0, x, lo0, hi0);
+            if (m0) emit(m0, k0 | id);
+            if (vm1) {
+                const uint64_t m1 = box_mask<DM>(vm1, x, lo1, hi1);
+                if (m1) emit(m1, k1 | id);
+            }
+        };
+        for (uint32_t r = 0; r < n; r += 2) {
+            if (r + 1 < n) {
+                load_row<DM, EXACT>(xb, rows, r + 1, D);
+                idb = rid[r + 1];
+            }
+            test(xa, ida);
+            if (r + 1 >= n) break;
+            if (r + 2 < n) {
+                load_row<DM, EXACT>(xa, rows, r + 2, D);
+                ida = rid[r + 2];
+            }
+            test(xb, idb);
+        }
+    }
+    flush_hits(A, stage, staged, lane);
+    // work done by this wave: box tests (child rows x queries) and child rows
+    if (lane == 0 && A.work) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.work), wTests);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.work + 1), wRows);
     }
 }
 
@@ -677,6 +834,18 @@ int radix_sort(tblaeq_ctx* ctx, uint64_t* a, uint64_t* b, const uint64_t* dn, ui
     return TBLAEQ_OK;
 }
 
+int resident_blocks(const void* kernel, int threads) {
+    static std::mutex mu;
+    static std::unordered_map<const void*, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(kernel);
+    if (it != cache.end()) return it->second;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess || n < 1) n = 1;
+    cache.emplace(kernel, n);
+    return n;
+}
+
 int bits_for(uint64_t v) {
     int b = 1;
     while (b < 64 && (v >> b) != 0) ++b;
@@ -684,11 +853,44 @@ int bits_for(uint64_t v) {
 }
 
 template <int MODE, int STAGE>
+auto range_kernel(int D) -> void (*)(RangeStepArgs) {
+    if constexpr (MODE == kFinal) {
+        if (D == 4) return k_range_final<4, true, STAGE>;
+        if (D == 8) return k_range_final<8, true, STAGE>;
+        if (D == 16) return k_range_final<16, true, STAGE>;
+        if (D < 4) return k_range_final<4, false, STAGE>;
+        if (D < 8) return k_range_final<8, false, STAGE>;
+        if (D < 16) return k_range_final<16, false, STAGE>;
+        return k_range_final<32, false, STAGE>;
+    } else {
+        if (D == 4) return k_range_step<4, MODE, true>;
+        if (D == 8) return k_range_step<8, MODE, true>;
+        if (D == 16) return k_range_step<16, MODE, true>;
+        if (D < 4) return k_range_step<4, MODE, false>;
+        if (D < 8) return k_range_step<8, MODE, false>;
+        if (D < 16) return k_range_step<16, MODE, false>;
+        return k_range_step<32, MODE, false>;
+    }
+}
+
+template <int MODE, int STAGE>
 void launch_range_dm(tblaeq_ctx* ctx, const RangeStepArgs& A, dim3 g, dim3 b) {
-    if (A.D <= 4) hipLaunchKernelGGL((k_range_step<4, MODE, STAGE>), g, b, 0, ctx->stream, A);
-    else if (A.D <= 8) hipLaunchKernelGGL((k_range_step<8, MODE, STAGE>), g, b, 0, ctx->stream, A);
-    else if (A.D <= 16) hipLaunchKernelGGL((k_range_step<16, MODE, STAGE>), g, b, 0, ctx->stream, A);
-    else hipLaunchKernelGGL((k_range_step<32, MODE, STAGE>), g, b, 0, ctx->stream, A);
+    void (*k)(RangeStepArgs) = range_kernel<MODE, STAGE>(A.D);
+    if (MODE == kFinal) {  // work-queue kernel: resident blocks only
+        const uint64_t cap =
+            static_cast<uint64_t>(ctx->cuCount) *
+            static_cast<uint64_t>(resident_blocks(reinterpret_cast<const void*>(k), static_cast<int>(b.x)));
+        if (g.x > cap) g.x = static_cast<unsigned>(cap);
+    }
+    hipLaunchKernelGGL(k, g, b, 0, ctx->stream, A);
+}
+
+// Queries the occupancy of this engine's final-filter kernels (outside any capture).
+void warm_range_kernels(tblaeq_batch* b) {
+    const int t = kStepWaves * kWave;
+    resident_blocks(reinterpret_cast<const void*>(range_kernel<kFinal, 256>(b->D)), t);
+    resident_blocks(reinterpret_cast<const void*>(range_kernel<kFinal, 512>(b->D)), t);
+    resident_blocks(reinterpret_cast<const void*>(range_kernel<kFinal, 1024>(b->D)), t);
 }
 
 template <int MODE>
@@ -905,9 +1107,10 @@ extern "C" int tblaeq_batch_create(tblaeq_ctx* ctx, const tblaeq_level* levels, 
         hipLaunchKernelGGL(k_aabb, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream, b->ids[L - 1], n,
                            levels[L - 1].col_res, b->coords[L], static_cast<int>(D), b->aabb);
     }
-    b->ctl = balloc<uint64_t>(b, 16, &rc);
+    b->ctl = balloc<uint64_t>(b, kCtlWords + kQueueSlots, &rc);
     if (rc) return fail(rc);
-    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, ctx->stream, b->ctl, 0ull, 16);
+    b->queue = b->ctl + kCtlWords;
+    launch_fill(b, b->ctl, 0, kCtlWords + kQueueSlots);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(hip_fail(hipGetLastError(), "batch create"));
     *out = b;
     return TBLAEQ_OK;
@@ -986,7 +1189,7 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     tblaeq_ctx* ctx = b->ctx;
     const int L = b->L;
     uint64_t* ctl = b->ctl;
-    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, ctx->stream, ctl, 0ull, 16);
+    launch_fill(b, ctl, 0, kCtlWords + kQueueSlots);
     RangeStepArgs A{};
     A.D = b->D;
     A.nq = nq;
@@ -1065,6 +1268,7 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     A.outCount = ctl + 2;
     A.idBits = idBits;
     A.work = ctl + 8;
+    A.queue = b->queue;
     if ((rc = launch_range_step<kFinal>(b, A, b->itemCap))) return rc;
     uint64_t* sorted = nullptr;
     if ((rc = radix_sort(ctx, b->hits, b->hitsTmp, ctl + 2, A.cap, idBits + qBits, b->hist, &sorted, false, 0,
@@ -1099,6 +1303,7 @@ extern "C" int tblaeq_batch_range_graph(tblaeq_batch* b, const double* d_boxes, 
         const bool timing = ctx->timing;
         ctx->timing = false;  // no event records inside the graph
         const uint64_t before = ctx->scanLaunches;
+        warm_range_kernels(b);
         TB_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         int rc = tblaeq_batch_range(b, d_boxes, nq, d_out_off, d_out_ids, out_cap, d_status, nullptr, nullptr);
         hipGraph_t g = nullptr;
