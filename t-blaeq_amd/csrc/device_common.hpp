@@ -109,6 +109,56 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Read-only device arrays addressed through the constant address space: a
+// wave-uniform load from such a pointer is a scalar load into SGPRs, so a child
+// row is fetched once per wave (s_load of the whole row) and enters the vector
+// compares as a scalar operand.  Only valid for arrays no kernel writes while
+// the reading kernel runs.
+typedef const __attribute__((address_space(4))) double* ConstF64;
+typedef const __attribute__((address_space(4))) uint64_t* ConstU64;
+typedef const __attribute__((address_space(4))) float* ConstF32;
+
+// Uniform copy of a value every lane holds (keeps it in SGPRs).
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Lane k's value (k wave-uniform) as a scalar.
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int k) {
+    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(v), k);
+    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(v >> 32), k);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Next chunk [*begin, *end) of the nItems work items for this wave, taken from
+// the queue counter in chunks of `chunk`; false when the queue is dry.
+__device__ __forceinline__ bool next_items(uint64_t* queue, uint64_t chunk, uint64_t nItems, int lane,
+                                           uint64_t* begin, uint64_t* end) {
+    uint64_t base = 0;
+    if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(queue), chunk);
+    base = uniform_u64(__shfl(base, 0, kWave));
+    if (base >= nItems) return false;
+    *begin = base;
+    *end = base + chunk < nItems ? base + chunk : nItems;
+    return true;
+}
+
+// Wave-uniform child row r of the cluster at `row` (D == DM: one wide scalar
+// load; padding dimensions read as 0).
+template <int DM, bool EXACT>
+__device__ __forceinline__ void scalar_row(double* x, ConstF64 row, uint32_t r, int D) {
+#pragma unroll
+    for (int d = 0; d < DM; ++d) x[d] = (EXACT || d < D) ? row[r * (EXACT ? DM : D) + d] : 0.0;
+}
+
+// Items per queue grab: about 16 grabs per wave.
+__device__ __forceinline__ uint64_t queue_chunk(uint64_t nItems, uint64_t waves) {
+    const uint64_t c = nItems / (waves * 16);
+    return c ? c : 1;
+}
+
 // Per-launch control block for the single-pass scans (decoupled look-back).
 // Status words carry their own epoch so the tile-status arrays never need to be
 // cleared between launches: word = epoch(27b) | valid(1b) | value(36b).

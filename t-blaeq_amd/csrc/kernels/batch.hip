@@ -111,32 +111,6 @@ struct RangeStepArgs {
     uint64_t* queue;     // final: work-queue counter (zero at launch)
 };
 
-// Uniform copy of a value every lane holds (keeps it in SGPRs).
-__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
-    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
-    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
-    return (static_cast<uint64_t>(hi) << 32) | lo;
-}
-
-// Next chunk [*begin, *end) of the nItems work items for this wave, taken from
-// the queue counter in chunks of `chunk`; false when the queue is dry.
-__device__ __forceinline__ bool next_items(uint64_t* queue, uint64_t chunk, uint64_t nItems, int lane,
-                                           uint64_t* begin, uint64_t* end) {
-    uint64_t base = 0;
-    if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(queue), chunk);
-    base = uniform_u64(__shfl(base, 0, kWave));
-    if (base >= nItems) return false;
-    *begin = base;
-    *end = base + chunk < nItems ? base + chunk : nItems;
-    return true;
-}
-
-// Items per queue grab: about 16 grabs per wave.
-__device__ __forceinline__ uint64_t queue_chunk(uint64_t nItems, uint64_t waves) {
-    const uint64_t c = nItems / (waves * 16);
-    return c ? c : 1;
-}
-
 // Appends a wave's staged hits to the global hit list (entries past the
 // capacity are dropped and flagged).
 __device__ __forceinline__ void flush_hits(const RangeStepArgs& A, const uint64_t* stage, uint64_t n, int lane) {
@@ -151,14 +125,6 @@ __device__ __forceinline__ void flush_hits(const RangeStepArgs& A, const uint64_
     wave_lds_sync();
 }
 
-// Read-only device arrays addressed through the constant address space: a
-// wave-uniform load from such a pointer is a scalar load into SGPRs, so a child
-// row is fetched once per wave (s_load of the whole row) and enters the vector
-// compares as a scalar operand.  Only valid for arrays no kernel writes while
-// the reading kernel runs.
-typedef const __attribute__((address_space(4))) double* ConstF64;
-typedef const __attribute__((address_space(4))) uint64_t* ConstU64;
-typedef const __attribute__((address_space(4))) float* ConstF32;
 
 template <int DM, int MODE, bool EXACT>
 __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs A) {
@@ -365,14 +331,6 @@ __device__ __forceinline__ uint64_t box_mask(uint64_t m, const double* x, const 
     }
 }
 
-// Wave-uniform child row r of the cluster at `row` (D == DM: one wide scalar
-// load; padding dimensions read as 0).
-template <int DM, bool EXACT>
-__device__ __forceinline__ void load_row(double* x, ConstF64 row, uint32_t r, int D) {
-#pragma unroll
-    for (int d = 0; d < DM; ++d) x[d] = (EXACT || d < D) ? row[r * (EXACT ? DM : D) + d] : 0.0;
-}
-
 // Queries of work item (cluster, 64-query slice) `it` for this lane: *q and
 // whether the lane holds one.  Level 0 (no clusters) takes query slices.
 __device__ __forceinline__ bool slice_query(const RangeStepArgs& A, uint64_t c, uint64_t slice, int lane,
@@ -458,7 +416,7 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_final(RangeStepArg
         const uint32_t n = static_cast<uint32_t>(je - jb);
         double xa[DM], xb[DM];
         uint64_t ida = rid[0], idb = 0;
-        load_row<DM, EXACT>(xa, rows, 0, D);
+        scalar_row<DM, EXACT>(xa, rows, 0, D);
         // one row: compare against both box sets, emit
         auto test = [&](const double* x, uint64_t id) {
             const uint64_t m0 = box_mask<DM>(vm0, x, lo0, hi0);
@@ -470,13 +428,13 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_final(RangeStepArg
         };
         for (uint32_t r = 0; r < n; r += 2) {
             if (r + 1 < n) {
-                load_row<DM, EXACT>(xb, rows, r + 1, D);
+                scalar_row<DM, EXACT>(xb, rows, r + 1, D);
                 idb = rid[r + 1];
             }
             test(xa, ida);
             if (r + 1 >= n) break;
             if (r + 2 < n) {
-                load_row<DM, EXACT>(xa, rows, r + 2, D);
+                scalar_row<DM, EXACT>(xa, rows, r + 2, D);
                 ida = rid[r + 2];
             }
             test(xb, idb);

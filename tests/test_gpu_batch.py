@@ -85,6 +85,16 @@ def test_batch_range_overflow_splits():
     _check_range_batch(h, o, boxes)
 
 
+@pytest.mark.parametrize("ratios", [(40,), (30, 10), (20, 8, 4, 3)])
+def test_batch_range_hierarchy_heights(ratios):
+    """Range batches over one level (the final filter reads the coarsest clusters' children), two and
+    four levels."""
+    h, o = _make(dict(CFG, N=24000, ratios=ratios, seed=14), max_q=96)
+    data = h.data()
+    boxes, _ = W.range_boxes(data, 80, 15, 1e-4, 5e-2)
+    _check_range_batch(h, o, boxes)
+
+
 def _check_knn_batch(h, o, pts, Ks):
     h.batch_set("knn", pts, Ks)
     h.batch_run(stats=True)
