@@ -37,6 +37,7 @@ struct tblaeq_batch {
     uint64_t* off2 = nullptr;
     uint64_t* items2 = nullptr;
     float* aabb = nullptr;       // [count[L-1] * 2D] bounding box of each last-level position's points
+    uint64_t* maskOff = nullptr; // [itemCap + 1] first keep-mask row of each work item of a level
     uint64_t* cnt = nullptr;     // [maxMesh]  per-cluster pair counts (kept zeroed between steps)
     uint64_t* off = nullptr;     // [maxMesh+1]
     uint64_t* items = nullptr;   // (id << 20 | slice)
@@ -78,6 +79,7 @@ constexpr int kStepWaves = 4;  // waves per workgroup of the cluster-major kerne
 
 constexpr int kCtlWords = 16;
 constexpr int kQueueSlots = 64;
+constexpr int kMaskQueue0 = 8;  // queue slots kMaskQueue0 + l: keep-mask rows reserved at level l
 // ctl slots: [0] pairs [1] items [2] hits / records [3] overflow flags
 // (1 pairs, 2 items, 4 hits / records) [4] items of the second group set
 // [6], [7] scratch of count-only scans.

@@ -109,6 +109,14 @@ struct RangeStepArgs {
     int level, L;
     uint64_t* work;      // final: [0] box tests, [1] child rows scanned
     uint64_t* queue;     // final: work-queue counter (zero at launch)
+    // prune: the count pass stores each item's per-row keep masks here when
+    // they fit in maskCap (item i's rows from maskOff[i], each wave reserving
+    // the rows of all its items on the maskNext counter; level 0: from
+    // i * count0), and the write pass reads them instead of recomputing
+    uint64_t* masks;
+    uint64_t* maskOff;
+    uint64_t* maskNext;
+    uint64_t maskCap;
 };
 
 // Appends a wave's staged hits to the global hit list (entries past the
@@ -126,6 +134,29 @@ __device__ __forceinline__ void flush_hits(const RangeStepArgs& A, const uint64_
 }
 
 
+// Write pass of a chunk of nc child rows: lane k holds row k's keep mask
+// (queries of the slice that keep child myId); each kept (child, query) pair
+// takes the next slot of the child's query list (cnt counted back down).
+__device__ __forceinline__ void write_kept(const RangeStepArgs& A, uint64_t myId, uint64_t myMask, int nc,
+                                           uint64_t q, int lane, uint64_t ltmask) {
+    uint64_t myBase = 0, myOff = 0;
+    if (lane < nc && myMask) {
+        const uint64_t pc = __popcll(myMask);
+        myBase = atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[myId]), 0ull - pc) - pc;
+        myOff = A.offNext[myId];
+    }
+    for (int k = 0; k < nc; ++k) {
+        const uint64_t m = readlane_u64(myMask, k);
+        if (m == 0) continue;
+        const uint64_t base = readlane_u64(myOff, k) + readlane_u64(myBase, k);
+        if ((m >> lane) & 1ull) {
+            const uint64_t o = base + __popcll(m & ltmask);
+            if (o < A.cap) A.qlistNext[o] = static_cast<uint32_t>(q);
+            else atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 1ull);
+        }
+    }
+}
+
 template <int DM, int MODE, bool EXACT>
 __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs A) {
     static_assert(MODE == kCount || MODE == kWrite, "prune modes only");
@@ -141,6 +172,17 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
     if (!A.level0 && nItems > A.itemCap) nItems = A.itemCap;
     const uint64_t ltmask = (1ull << lane) - 1ull;
     const bool useBox = A.aabb != nullptr;
+    // count pass: one reservation of keep-mask rows for all of this wave's items
+    uint64_t maskBase = 0;
+    if (MODE == kCount && A.masks && !A.level0) {
+        uint64_t rows = 0;
+        for (uint64_t item = gw; item < nItems; item += nw) {
+            const uint64_t c = ((ConstU64)A.items)[item] >> kSliceBits;
+            rows += ((ConstU64)A.colResPrev)[c + 1] - ((ConstU64)A.colResPrev)[c];
+        }
+        if (lane == 0 && rows) maskBase = atomicAdd(reinterpret_cast<unsigned long long*>(A.maskNext), rows);
+        maskBase = uniform_u64(__shfl(maskBase, 0, kWave));
+    }
     for (uint64_t item = gw; item < nItems; item += nw) {
         uint64_t jb, je, q = 0;
         bool valid;
@@ -160,6 +202,30 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
             q = valid ? A.qlist[g0 + start + lane] : 0;
             jb = ((ConstU64)A.colResPrev)[c];
             je = ((ConstU64)A.colResPrev)[c + 1];
+        }
+        uint64_t mo = 0;
+        bool stored = false;
+        if (A.masks) {
+            if (A.level0) {
+                mo = item * A.count0;
+            } else if (MODE == kCount) {
+                mo = maskBase;
+                maskBase += je - jb;
+                if (lane == 0) A.maskOff[item] = mo;
+            } else {
+                mo = A.maskOff[item];
+            }
+            stored = mo + (je - jb) <= A.maskCap;
+        }
+        if (MODE == kWrite && stored) {
+            // masks of the count pass: only the writes remain
+            for (uint64_t j0 = jb; j0 < je; j0 += kWave) {
+                const int nc = static_cast<int>(je - j0 < static_cast<uint64_t>(kWave) ? je - j0 : kWave);
+                const uint64_t myId = lane < nc ? A.ids[j0 + lane] : 0;
+                const uint64_t myMask = lane < nc ? A.masks[mo + (j0 - jb) + lane] : 0;
+                write_kept(A, myId, myMask, nc, q, lane, ltmask);
+            }
+            continue;
         }
         double lo[DM], hi[DM];
         float flo[DM], fhi[DM];
@@ -230,24 +296,10 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
                 if (lane < nc && myMask)
                     atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[myId]),
                               static_cast<unsigned long long>(__popcll(myMask)));
+                if (stored && lane < nc) A.masks[mo + (j0 - jb) + lane] = myMask;
                 continue;
             }
-            uint64_t myBase = 0, myOff = 0;
-            if (lane < nc && myMask) {
-                const uint64_t pc = __popcll(myMask);
-                myBase = atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[myId]), 0ull - pc) - pc;
-                myOff = A.offNext[myId];
-            }
-            for (int k = 0; k < nc; ++k) {
-                const uint64_t m = __shfl(myMask, k, kWave);
-                const uint64_t base = __shfl(myOff, k, kWave) + __shfl(myBase, k, kWave);
-                if (m == 0) continue;
-                if ((m >> lane) & 1ull) {
-                    const uint64_t o = base + __popcll(m & ltmask);
-                    if (o < A.cap) A.qlistNext[o] = static_cast<uint32_t>(q);
-                    else atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 1ull);
-                }
-            }
+            write_kept(A, myId, myMask, nc, q, lane, ltmask);
         }
         if (MODE == kCount && valid) {
             if (A.survStat && mine)
@@ -1102,6 +1154,8 @@ extern "C" int tblaeq_batch_reserve(tblaeq_batch* b, uint64_t max_queries, uint6
         if (rc) return rc;
         b->items2 = balloc<uint64_t>(b, b->itemCap, &rc);
         if (rc) return rc;
+        b->maskOff = balloc<uint64_t>(b, b->itemCap + 1, &rc);
+        if (rc) return rc;
         b->pairCap = pair_cap;
     }
     if (!b->cnt) {
@@ -1146,6 +1200,10 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     }
     tblaeq_ctx* ctx = b->ctx;
     const int L = b->L;
+    if (kMaskQueue0 + L > kQueueSlots) {
+        set_error("tblaeq_batch_range: too many levels");
+        return TBLAEQ_ERR_UNSUPPORTED;
+    }
     uint64_t* ctl = b->ctl;
     launch_fill(b, ctl, 0, kCtlWords + kQueueSlots);
     RangeStepArgs A{};
@@ -1185,6 +1243,11 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
         A.survStat = d_level_surv;
         A.fineStat = d_fine_count;
         const uint64_t bound = l == 0 ? (nq + kWave - 1) / kWave : b->itemCap;
+        // keep masks of the count pass, reused by the write pass
+        A.masks = b->hits;
+        A.maskCap = b->hitCap;
+        A.maskOff = b->maskOff;
+        A.maskNext = b->queue + kMaskQueue0 + l;
         if ((rc = launch_range_step<kCount>(b, A, bound))) return rc;
         // group the kept (cluster, query) pairs of level l by cluster
         const uint64_t mesh = b->count[l];
@@ -1227,6 +1290,7 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     A.idBits = idBits;
     A.work = ctl + 8;
     A.queue = b->queue;
+    A.masks = nullptr;
     if ((rc = launch_range_step<kFinal>(b, A, b->itemCap))) return rc;
     uint64_t* sorted = nullptr;
     if ((rc = radix_sort(ctx, b->hits, b->hitsTmp, ctl + 2, A.cap, idBits + qBits, b->hist, &sorted, false, 0,
