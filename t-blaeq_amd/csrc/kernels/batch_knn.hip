@@ -655,6 +655,13 @@ struct KnnLastArgs {
     uint64_t* overflow;
     uint64_t* survStat;
     uint64_t* fineStat;
+    // kLCount stores each item's per-row pair masks (item i's rows from
+    // maskOff[i], each wave reserving all its items' rows on maskNext) when
+    // they fit in maskCap; kLWrite reads them instead of recomputing
+    uint64_t* masks;
+    uint64_t* maskOff;
+    uint64_t* maskNext;
+    uint64_t maskCap;
 };
 
 constexpr int kRecIdBits = 40;
@@ -662,6 +669,28 @@ constexpr int kRecStage = 256;
 
 __device__ __forceinline__ double guard_bound(double t, double r) {
     return t + r + 1e-9 * (__builtin_fabs(t) + r);
+}
+
+// kLWrite for a chunk of nc child rows: lane k holds row k's pair mask
+// (queries of the slice that refine child cid); each pair takes the next slot
+// of the child's query list (cnt counted back down).
+__device__ __forceinline__ void write_pairs(const KnnLastArgs& A, uint64_t cid, uint64_t myMask, int nc, uint64_t q,
+                                            int lane, uint64_t ltmask) {
+    uint64_t myBase = 0;
+    if (lane < nc && myMask) {
+        const uint64_t pc = __popcll(myMask);
+        myBase = A.offNext[cid] + (atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[cid]), 0ull - pc) - pc);
+    }
+    for (int k = 0; k < nc; ++k) {
+        const uint64_t m = readlane_u64(myMask, k);
+        if (m == 0) continue;
+        const uint64_t base = readlane_u64(myBase, k);
+        if ((m >> lane) & 1ull) {
+            const uint64_t o = base + __popcll(m & ltmask);
+            if (o < A.cap) A.qlistNext[o] = static_cast<uint32_t>(q);
+            else atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 1ull);
+        }
+    }
 }
 
 template <int DM, int PASS>
@@ -684,6 +713,17 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_last(KnnLastArgs A) 
     if (D != DM)
         for (int t = lane; t < kWave * DM; t += kWave) sx[t] = 0.0;
     uint64_t staged = 0;
+    // kLCount: one reservation of pair-mask rows for all of this wave's items
+    uint64_t maskBase = 0;
+    if (PASS == kLCount && A.masks) {
+        uint64_t rows = 0;
+        for (uint64_t item = gw; item < nItems; item += nw) {
+            const uint64_t p = A.items[item] >> kSliceBits;
+            rows += A.colResPrev[p + 1] - A.colResPrev[p];
+        }
+        if (lane == 0 && rows) maskBase = atomicAdd(reinterpret_cast<unsigned long long*>(A.maskNext), rows);
+        maskBase = uniform_u64(__shfl(maskBase, 0, kWave));
+    }
     auto flushRec = [&]() {
         if (staged == 0) return;
         wave_lds_sync();
@@ -731,6 +771,28 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_last(KnnLastArgs A) 
         }
         if (PASS == kCand && tK != ~0ull) t1 = okey_inv(tK);
         const uint64_t jb = A.colResPrev[p], je = A.colResPrev[p + 1];
+        uint64_t mo = 0;
+        bool stored = false;
+        if (PASS != kCand && A.masks) {
+            if (PASS == kLCount) {
+                mo = maskBase;
+                maskBase += je - jb;
+                if (lane == 0) A.maskOff[item] = mo;
+            } else {
+                mo = A.maskOff[item];
+            }
+            stored = mo + (je - jb) <= A.maskCap;
+        }
+        if (PASS == kLWrite && stored) {
+            // pair masks of the count pass: only the writes remain
+            for (uint64_t j0 = jb; j0 < je; j0 += kWave) {
+                const int nc = static_cast<int>(je - j0 < static_cast<uint64_t>(kWave) ? je - j0 : kWave);
+                const uint64_t cid = lane < nc ? A.ids[j0 + lane] : 0;
+                const uint64_t myMask = lane < nc ? A.masks[mo + (j0 - jb) + lane] : 0;
+                write_pairs(A, cid, myMask, nc, q, lane, ltmask);
+            }
+            continue;
+        }
         uint64_t mine = 0, fine = 0;
         for (uint64_t j0 = jb; j0 < je; j0 += kWave) {
             const int nc = static_cast<int>(je - j0 < static_cast<uint64_t>(kWave) ? je - j0 : kWave);
@@ -804,24 +866,9 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_last(KnnLastArgs A) 
                 if (lane < nc && myMask)
                     atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[s_id[wave][lane]]),
                               static_cast<unsigned long long>(__popcll(myMask)));
+                if (stored && lane < nc) A.masks[mo + (j0 - jb) + lane] = myMask;
             } else if (PASS == kLWrite) {
-                uint64_t myBase = 0;
-                if (lane < nc && myMask) {
-                    const uint64_t cid = s_id[wave][lane];
-                    const uint64_t pc = __popcll(myMask);
-                    myBase = A.offNext[cid] +
-                             (atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[cid]), 0ull - pc) - pc);
-                }
-                for (int k = 0; k < nc; ++k) {
-                    const uint64_t m = __shfl(myMask, k, kWave);
-                    const uint64_t base = __shfl(myBase, k, kWave);
-                    if (m == 0) continue;
-                    if ((m >> lane) & 1ull) {
-                        const uint64_t o = base + __popcll(m & ltmask);
-                        if (o < A.cap) A.qlistNext[o] = static_cast<uint32_t>(q);
-                        else atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 1ull);
-                    }
-                }
+                write_pairs(A, lane < nc ? s_id[wave][lane] : 0, myMask, nc, q, lane, ltmask);
             }
             wave_lds_sync();
         }
@@ -1286,7 +1333,7 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
     uint64_t* offQ = cntQ + nq;
     uint8_t* flags = reinterpret_cast<uint8_t*>(offQ + nq + 1);
     uint64_t* ctl = b->ctl;
-    launch_fill(b, ctl, 0, 16);
+    launch_fill(b, ctl, 0, kCtlWords + kQueueSlots);
     A.D = b->D;
     A.L = L;
     A.nq = nq;
@@ -1381,6 +1428,10 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
         LA.qlistNext = b->qlist2;
         LA.cap = b->pairCap;
         LA.overflow = ctl + 3;
+        LA.masks = nullptr;  // kCand writes its records into b->hits
+        LA.maskOff = b->maskOff;
+        LA.maskNext = b->queue + kMaskQueue0;
+        LA.maskCap = b->hitCap;
         auto launchLast = [&](int pass) {
             TimedScope ts(ctx, kKBatch0 + 9);
 #define TB_LAST(DMV)                                                                             \
@@ -1445,6 +1496,7 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
         // survivors of the last level: statistics and the pairs to refine
         LA.survStat = d_level_surv;
         LA.fineStat = d_fine_count;
+        LA.masks = b->hits;  // free again: the records went to b->pairs / b->recAux
         launchLast(kLCount);
         TB_HIP(hipGetLastError());
         if ((rc = launch_group_scan(b, b->count[L - 1], b->off2, b->items2, ctl + 4))) return rc;
