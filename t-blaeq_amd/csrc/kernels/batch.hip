@@ -675,41 +675,47 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_hist(uint64_t* __restrict
     if (tid == 0) scan_exit(ctrl);
 }
 
-// Stable scatter.  Each key's rank among the tile's keys of its digit is found
-// in order (round, then wave, then lane); the tile is then laid out sorted by
-// digit in LDS and written from there, so consecutive threads write
-// consecutive addresses within each digit's run.  Tiles keep their order
-// through the digit-major scanned histogram.
+// Stable scatter.  Within a tile, wave w owns keys [1024 w, 1024 (w + 1)),
+// 64 per round.  Each wave ranks its keys against wave-private digit counters
+// in LDS (peers of equal digit found with 8 ballots; rank = counter + peers
+// below the lane), with no block barrier; the waves' counts then give every
+// digit's start in the tile and every wave's offset within the digit.  The
+// tile is laid out sorted by digit in LDS and written from there, so
+// consecutive threads write consecutive addresses within each digit's run.
+// Tiles keep their order through the digit-major scanned histogram.
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
                                                            const uint64_t* dn, uint64_t cap, int shift,
                                                            const uint64_t* __restrict__ hist, uint64_t idMask,
                                                            int extract) {
-    __shared__ uint32_t running[256];
+    constexpr int kWaves = kRsThreads / kWave;
+    constexpr uint32_t kPerWave = static_cast<uint32_t>(kRsPer) * kWave;
     __shared__ uint32_t lstart[256];
-    __shared__ uint32_t wc[kRsThreads / kWave][256];
+    __shared__ uint32_t wc[kWaves][256];
+    __shared__ uint32_t wsum[kWaves];
     __shared__ uint64_t sk[kRsTile];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     uint64_t n = *dn;
     if (n > cap) n = cap;
     const uint64_t ntiles = (n + kRsTile - 1) / kRsTile;
     const uint64_t ltmask = (1ull << lane) - 1ull;
+    uint32_t* mycnt = wc[wid];
     for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint64_t t0 = t * kRsTile;
         const uint32_t cnt = static_cast<uint32_t>(n - t0 < static_cast<uint64_t>(kRsTile) ? n - t0 : kRsTile);
+        const uint32_t e0 = static_cast<uint32_t>(wid) * kPerWave + static_cast<uint32_t>(lane);
         uint64_t key[kRsPer];
         uint32_t rank[kRsPer];
 #pragma unroll
         for (int r = 0; r < kRsPer; ++r) {
-            const uint32_t e = static_cast<uint32_t>(r) * kRsThreads + tid;
+            const uint32_t e = e0 + static_cast<uint32_t>(r) * kWave;
             key[r] = e < cnt ? in[t0 + e] : 0;
         }
-        running[tid] = 0;
-        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 256 / kWave; ++i) mycnt[lane + i * kWave] = 0;
+        wave_lds_sync();
 #pragma unroll
         for (int r = 0; r < kRsPer; ++r) {
-            for (int w = 0; w < kRsThreads / kWave; ++w) wc[w][tid] = 0;
-            __syncthreads();
-            const bool ok = static_cast<uint32_t>(r) * kRsThreads + tid < cnt;
+            const bool ok = e0 + static_cast<uint32_t>(r) * kWave < cnt;
             const uint32_t dig = ok ? static_cast<uint32_t>((key[r] >> shift) & 0xFF) : 0;
             uint64_t peers = __ballot(ok);
 #pragma unroll
@@ -717,40 +723,45 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* __res
                 const uint64_t bal = __ballot(((dig >> b) & 1u) != 0);
                 peers &= ((dig >> b) & 1u) ? bal : ~bal;
             }
-            const uint32_t rankW = static_cast<uint32_t>(__popcll(peers & ltmask));
-            if (ok && rankW == 0) wc[wid][dig] = static_cast<uint32_t>(__popcll(peers));
-            __syncthreads();
-            uint32_t before = 0;
-            if (ok) {
-                before = running[dig];
-                for (int w = 0; w < wid; ++w) before += wc[w][dig];
-            }
-            rank[r] = before + rankW;
-            __syncthreads();
-            uint32_t add = 0;
-            for (int w = 0; w < kRsThreads / kWave; ++w) add += wc[w][tid];
-            running[tid] += add;
-            __syncthreads();
+            const uint32_t below = static_cast<uint32_t>(__popcll(peers & ltmask));
+            const uint32_t pre = ok ? mycnt[dig] : 0;
+            rank[r] = pre + below;
+            wave_lds_sync();
+            if (ok && below == 0) mycnt[dig] = pre + static_cast<uint32_t>(__popcll(peers));
+            wave_lds_sync();
         }
-        // exclusive scan of the tile's digit counts (256 entries, 4 waves)
+        __syncthreads();
+        // digit starts in the tile (exclusive scan of the digit totals), then
+        // each wave's start within every digit
         {
-            const uint32_t v = running[tid];
+            uint32_t c[kWaves];
+            uint32_t v = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                c[w] = wc[w][tid];
+                v += c[w];
+            }
             uint32_t x = v;
             for (int o = 1; o < kWave; o <<= 1) {
                 const uint32_t y = __shfl_up(x, o, kWave);
                 if (lane >= o) x += y;
             }
-            if (lane == kWave - 1) wc[0][wid] = x;
+            if (lane == kWave - 1) wsum[wid] = x;
             __syncthreads();
             uint32_t p = x - v;
-            for (int w = 0; w < wid; ++w) p += wc[0][w];
+            for (int w = 0; w < wid; ++w) p += wsum[w];
             lstart[tid] = p;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                wc[w][tid] = p;
+                p += c[w];
+            }
             __syncthreads();
         }
 #pragma unroll
         for (int r = 0; r < kRsPer; ++r) {
-            const uint32_t e = static_cast<uint32_t>(r) * kRsThreads + tid;
-            if (e < cnt) sk[lstart[(key[r] >> shift) & 0xFF] + rank[r]] = key[r];
+            const uint32_t e = e0 + static_cast<uint32_t>(r) * kWave;
+            if (e < cnt) sk[mycnt[(key[r] >> shift) & 0xFF] + rank[r]] = key[r];
         }
         __syncthreads();
         for (uint32_t i = tid; i < cnt; i += kRsThreads) {
