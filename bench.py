@@ -242,6 +242,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--learn-traces", type=int, default=200000, help="0 skips the learned-radius run")
     ap.add_argument("--learn-epochs", type=int, default=20)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch parameters (profiling runs)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once on the GPU (a kernel of known "
                          "traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -307,7 +308,7 @@ def main():
         h.batch_init(args.nq, args.pair_cap, args.hit_cap)
         bK = Ks.astype(np.uint64)
         # launch parameters from the autotuner on this batch (untimed set-up)
-        tuned = h.batch_autotune(boxes, knn_q, bK, reps=3)
+        tuned = None if args.no_tune else h.batch_autotune(boxes, knn_q, bK, reps=3)
         tuned_params = {name: h.batch_get_param(pid) for name, pid in (
             ("knn_seed", 1), ("knn_wg_per_cu", 2), ("knn_mode", 3), ("range_blocks_per_cu", 4),
             ("step_blocks_per_cu", 5), ("radix_blocks_per_cu", 6), ("hit_stage", 7))}
@@ -362,14 +363,16 @@ def main():
                    "final_filter_tests": work_r[0], "final_filter_rows": work_r[1],
                    "knn_refine_evals": work_k[0], "knn_refine_rows": work_k[1],
                    "speedup_vs_plain": round(2 * args.nq / step_s / plain_qps, 2),
-                   "autotune": {"range_ms_before_after": [round(x, 3) for x in tuned["range_ms"]],
-                                "knn_ms_before_after": [round(x, 3) for x in tuned["knn_ms"]],
-                                "params": tuned_params}}
+                   "autotune": None if tuned is None else {
+                       "range_ms_before_after": [round(float(x), 3) for x in tuned["range_ms"]],
+                       "knn_ms_before_after": [round(float(x), 3) for x in tuned["knn_ms"]],
+                       "params": tuned_params}}
         log("batched", batched)
         # roofline of the dominant batched kernel: the exact box filter (one
         # launch per range batch).  Its work is 2D FP64 compares per candidate
-        # row x box (work_r[0] tests); it is bounded by FP64 vector issue, not
-        # by HBM (each child row is read once per 64-box slice).
+        # row x box (work_r[0] tests); it is bounded by FP64 compare issue, not
+        # by HBM (each child row is read once per pair of 64-box slices, as a
+        # scalar load).
         n_f, ms_f = kacc["range_final"]
         if n_f:
             avg_s = ms_f / 1e3 / n_f
@@ -377,11 +380,11 @@ def main():
             ach = ops / avg_s / 1e12
             roofline = {"bound": "mfma", "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": final_traffic(cfg),
-                        "kernel": "k_range_step<8,kFinal> (exact box filter of the batched range path)",
+                        "kernel": "k_range_final<8> (exact box filter of the batched range path)",
                         "launches": n_f, "avg_launch_us": round(avg_s * 1e6, 3), "ops_per_launch": ops,
-                        "note": "FP64 vector compares (2D per candidate row x box), counted against MI355X's FP64 "
-                                "peak (vector and matrix both 78.6 TFLOP/s); the kernel reads each child row once "
-                                "per 64-box slice, so HBM is not its bound",
+                        "note": "FP64 vector compares (2D per candidate row x box, a v_cmpx chain), counted "
+                                "against MI355X's FP64 peak (vector and matrix both 78.6 TFLOP/s); each child row is "
+                                "one scalar load per pair of 64-box slices, so HBM is not its bound",
                         "plain": plain_roofline}
 
     learned = None
