@@ -1,0 +1,79 @@
+/*
+ * GPU index build: k-means levels and the CSC P-tensor of each level.
+ *
+ * Replaces the reference's GPU build path, cuVS k-means (CUDAKmeans.cu:7-195,
+ * called from IndexBuilder.cu:44-151 when forceUseCPU is false), and the host
+ * P-tensor / map / radius construction (Setup.cuh:87-171, func.hpp:124-137).
+ *
+ * tblaeq_kmeans follows the CPU build's k-means (CPUKmeans.cuh:67-205):
+ *   - k-means++ seeding from mt19937_64(seed): the first center is
+ *     uniform_int(0, n-1), each further one is the first point whose
+ *     cumulative min squared distance reaches uniform_real(0, total); the
+ *     random draws are the same stream, the cumulative sums are a parallel
+ *     scan (a pick can differ from the sequential sum only when the threshold
+ *     falls within rounding of a boundary);
+ *   - Lloyd iterations until no label changes or max_iter: assignment by the
+ *     smallest squared distance, strict < (lowest index on ties).  The
+ *     distance ranking is computed on the FP32 matrix cores as
+ *     ||c||^2 - 2 x.c (x, c centred on the data's mid-range), and every point
+ *     whose best and second-best FP32 scores are within twice a proven error
+ *     bound is re-scored exactly in FP64 with the CPU build's arithmetic, so
+ *     the labels equal the CPU build's;
+ *   - centroid update: per-cluster FP64 sums in point order divided by the
+ *     count (bit-identical to the CPU build's sums); an empty cluster keeps
+ *     its centroid.
+ *
+ * Device pointers, stream-ordered on ctx; the call synchronises once per Lloyd
+ * iteration (the stopping test).  D <= 64.
+ */
+#ifndef TBLAEQ_BUILD_H
+#define TBLAEQ_BUILD_H
+
+#include <stdint.h>
+
+#include "tblaeq.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* d_x [n x D] FP64 (AoS) -> d_centers [k x D], d_labels [n].  stats (host,
+ * optional, 4 words): [0] Lloyd iterations run, [1] points re-scored in FP64
+ * (all iterations), [2] FP32 matrix-core score evaluations (points x centers,
+ * all iterations), [3] milliseconds in the assignment kernels. */
+int tblaeq_kmeans(tblaeq_ctx* ctx, const double* d_x, uint64_t n, uint64_t D, uint64_t k, uint64_t max_iter,
+                  uint64_t seed, double* d_centers, uint64_t* d_labels, uint64_t* stats);
+
+/* tblaeq_kmeans on host arrays (x [n x D] in; centers [k x D], labels [n] out),
+ * through device buffers of the call. */
+int tblaeq_kmeans_host(tblaeq_ctx* ctx, const double* x, uint64_t n, uint64_t D, uint64_t k, uint64_t max_iter,
+                       uint64_t seed, double* centers, uint64_t* labels, uint64_t* stats);
+
+/* CSC P-tensor of one level from its fine points, centroids and labels
+ * (Setup.cuh:87-146): columns = clusters, rows grouped by cluster in point
+ * order (a stable sort of (label, point) keys; the CPU build's std::sort may
+ * order the rows of a column differently), d_map[pos] = point index of row
+ * pos, d_vals[pos*D + d] = fine[map[pos]][d] / centers[col][d], and
+ * d_radius[c] = max over the cluster's points of sqrt(sum_d (c_d - f_d)^2)
+ * (Setup.cuh:148-171).  d_colRes [k+1], d_nnzPerCol [k], d_map [n],
+ * d_vals [n x D], d_radius [k]. */
+int tblaeq_build_level(tblaeq_ctx* ctx, const double* d_fine, uint64_t n, uint64_t D, const double* d_centers,
+                       uint64_t k, const uint64_t* d_labels, uint64_t* d_colRes, uint64_t* d_nnzPerCol,
+                       uint64_t* d_map, double* d_vals, double* d_radius);
+
+/* QueryHandler(forceCPU, PointCloud, name, height, ratios) (QueryHandler.cu:79-99,
+ * IndexBuilder.cu:44-151): k-means build of an in-memory dataset (data: N*D
+ * doubles, AoS).  forceCPU = 0 builds on the GPU (tblaeq_kmeans and
+ * tblaeq_build_level per level, k-means++ from seed 42, at most 7 Lloyd
+ * iterations, as the CPU build); forceCPU = 1 is the host build. */
+struct tblaeq_handler;
+int tblaeq_handler_build(const double* data, uint64_t N, uint64_t D, uint64_t height, const uint64_t* ratios,
+                         const char* name, int forceCPU, struct tblaeq_handler** out);
+/* Per level of a GPU build (finest first), the four tblaeq_kmeans stats words;
+ * *n = words available (0 for other indexes); copies min(*n, cap). */
+int tblaeq_handler_build_stats(struct tblaeq_handler* h, uint64_t* out, uint64_t cap, uint64_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TBLAEQ_BUILD_H */

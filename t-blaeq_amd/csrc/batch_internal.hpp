@@ -97,6 +97,17 @@ int launch_count_scan(tblaeq_batch* b, const uint64_t* cnt, uint64_t n, uint64_t
 void launch_copy_status(tblaeq_batch* b, uint64_t* status);
 // Fills n words with v (a kernel, so that captured batches stay replayable).
 void launch_fill(tblaeq_batch* b, uint64_t* p, uint64_t v, uint64_t n);
+// Exclusive scan of cnt[0..n) into off[0..n] (off[n] = total) on ctx's stream;
+// scratch2 = two device words.
+int scan_counts(tblaeq_ctx* ctx, const uint64_t* cnt, uint64_t n, uint64_t* off, uint64_t* scratch2);
+// Stable LSD radix sort of *dn (<= cap) u64 keys on bits [firstBit, firstBit +
+// nbits) between a and b (8-bit digits); *result = the buffer holding the
+// sorted keys.  hist needs radix_hist_words(cap) words.
+int radix_sort(tblaeq_ctx* ctx, uint64_t* a, uint64_t* b, const uint64_t* dn, uint64_t cap, int firstBit, int nbits,
+               uint64_t* hist, uint64_t** result, bool extractLast, uint64_t idMask, uint64_t blocksPerCu);
+uint64_t radix_hist_words(uint64_t cap);
+// Bits needed to represent v (at least 1).
+int bits_for(uint64_t v);
 // Blocks of `kernel` (with `threads` threads) that fit on one CU at once,
 // queried once per kernel.  Work-queue kernels launch at most this many per CU:
 // every wave is resident from the start and takes items until the queue is dry.

@@ -5,6 +5,7 @@
 #include "../../../include/tblaeq_handler.h"
 #include "../../../include/tblaeq_learn.h"
 #include "../ctx.hpp"
+#include "../../../include/tblaeq_build.h"
 #include "handler_internal.hpp"
 
 using namespace tblaeq_host;
@@ -58,6 +59,30 @@ int tblaeq_handler_create_from_data(const double* data, uint64_t N, uint64_t D, 
         auto* h = new tblaeq_handler();
         h->h = std::make_unique<QueryHandler>(true, pc, name ? name : "in-memory", height, toVec(ratios, height - 1));
         *out = h;
+    });
+}
+
+int tblaeq_handler_build(const double* data, uint64_t N, uint64_t D, uint64_t height, const uint64_t* ratios,
+                         const char* name, int forceCPU, tblaeq_handler** out) {
+    if (!out || !data || !ratios || N == 0 || D == 0 || height < 2) return TBLAEQ_ERR_INVALID;
+    return guarded("tblaeq_handler_build", [&] {
+        PointCloud pc;
+        pc.size = static_cast<int64_t>(N);
+        pc.dim = static_cast<int64_t>(D);
+        pc.data.assign(data, data + N * D);
+        auto* h = new tblaeq_handler();
+        h->h = std::make_unique<QueryHandler>(forceCPU != 0, pc, name ? name : "in-memory", height,
+                                              toVec(ratios, height - 1));
+        *out = h;
+    });
+}
+
+int tblaeq_handler_build_stats(tblaeq_handler* h, uint64_t* out, uint64_t cap, uint64_t* n) {
+    if (!h || !n) return TBLAEQ_ERR_INVALID;
+    return guarded("tblaeq_handler_build_stats", [&] {
+        const auto& s = h->h->index().buildStats;
+        *n = s.size();
+        if (out) std::copy(s.begin(), s.begin() + std::min<size_t>(cap, s.size()), out);
     });
 }
 

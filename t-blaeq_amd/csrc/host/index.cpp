@@ -10,6 +10,7 @@
 #include <random>
 #include <sstream>
 
+#include "../../../include/tblaeq_build.h"
 #include "tblaeq_host.hpp"
 
 namespace tblaeq_host {
@@ -445,10 +446,88 @@ IndexData* IndexBuilder::buildRandom(size_t N, size_t D, double valMin, double v
     return idx;
 }
 
+// IndexBuilder.cu:44-151 on the GPU (the reference's cuVS path when
+// forceUseCPU is false, CUDAKmeans.cu:7-195): per level tblaeq_kmeans
+// (k-means++ from mt19937_64(42), Lloyd for at most 7 iterations, the CPU
+// build's labels and centroids) and tblaeq_build_level (CSC P-tensor, map,
+// radius; rows of a column in point order).
+IndexData* IndexBuilder::buildGpu(const double* data, size_t N, size_t D, const std::string& name, size_t height,
+                                  const std::vector<size_t>& ratios) {
+    if (height < 2 || ratios.size() != height - 1) throw std::invalid_argument("build: bad height/ratios");
+    constexpr size_t kMaxIter = 7;
+    const size_t L = ratios.size();
+    auto idx = std::make_unique<IndexData>();
+    idx->D = D;
+    idx->N = N;
+    idx->height = height;
+    idx->intervals = L;
+    idx->ratios = ratios;
+    idx->datasetName = name;
+    idx->pTensors.resize(L);
+    idx->meshMaxRadius.resize(L);
+    idx->maps.resize(height);
+    idx->ensureCtx();
+    tblaeq_ctx* ctx = idx->ctx;
+    DeviceBuffer<double> dx;
+    dx.reset(data, N * D);
+    size_t n = N;
+    for (size_t i = 0; i < L; ++i) {
+        const size_t fineIdx = L - i, pIdx = fineIdx - 1;
+        const size_t k = computeCentroidCount(n, ratios[i]);
+        DeviceBuffer<double> dc;
+        DeviceBuffer<uint64_t> dlab, dColRes, dNnz, dMap;
+        DeviceBuffer<double> dVals, dRad;
+        dc.resize(k * D);
+        dlab.resize(n);
+        dColRes.resize(k + 1);
+        dNnz.resize(k);
+        dMap.resize(n);
+        dVals.resize(n * D);
+        dRad.resize(k);
+        uint64_t st[4] = {0, 0, 0, 0};
+        tbCheck(tblaeq_kmeans(ctx, dx.data(), n, D, k, kMaxIter, 42, dc.data(), dlab.data(), st), "tblaeq_kmeans");
+        tbCheck(tblaeq_build_level(ctx, dx.data(), n, D, dc.data(), k, dlab.data(), dColRes.data(), dNnz.data(),
+                                   dMap.data(), dVals.data(), dRad.data()),
+                "tblaeq_build_level");
+        idx->buildStats.insert(idx->buildStats.end(), st, st + 4);
+        idx->meshSizes.push_back(n);
+        auto P = std::make_unique<SparseTensorCscFormat>();
+        P->D = D;
+        P->rows = n;
+        P->cols = k;
+        P->nnz = n;
+        P->rowIds.resize(n);
+        std::iota(P->rowIds.begin(), P->rowIds.end(), uint64_t{0});
+        P->colRes.resize(k + 1);
+        P->nnzPerCol.resize(k);
+        P->vals.resize(n * D);
+        idx->maps[fineIdx].resize(n);
+        idx->meshMaxRadius[pIdx].resize(k);
+        dColRes.download(P->colRes.data(), k + 1);
+        dNnz.download(P->nnzPerCol.data(), k);
+        dVals.download(P->vals.data(), n * D);
+        dMap.download(idx->maps[fineIdx].data(), n);
+        dRad.download(idx->meshMaxRadius[pIdx].data(), k);
+        idx->pTensors[pIdx] = std::move(P);
+        if (i + 1 == L) {
+            idx->meshSizes.push_back(k);
+            idx->coarsestRows = k;
+            idx->coarsestIds.resize(k);
+            std::iota(idx->coarsestIds.begin(), idx->coarsestIds.end(), uint64_t{0});
+            idx->coarsestVals.resize(k * D);
+            dc.download(idx->coarsestVals.data(), k * D);
+        }
+        dx = std::move(dc);
+        n = k;
+    }
+    return idx.release();
+}
+
 // IndexBuilder.cu:44-151 with Lloyd k-means + k-means++ (CPUKmeans.cuh:67-205)
-// on the host.  The MI355X k-means build is a separate entry point.
-IndexData* IndexBuilder::build(const double* data, size_t N, size_t D, const std::string& name, bool /*forceUseCPU*/,
+// on the host, or on the GPU (buildGpu) when forceUseCPU is false.
+IndexData* IndexBuilder::build(const double* data, size_t N, size_t D, const std::string& name, bool forceUseCPU,
                                size_t height, const std::vector<size_t>& ratios) {
+    if (!forceUseCPU) return buildGpu(data, N, D, name, height, ratios);
     if (height < 2 || ratios.size() != height - 1) throw std::invalid_argument("build: bad height/ratios");
     constexpr size_t kMaxIter = 7;
     const size_t L = ratios.size();

@@ -71,6 +71,11 @@ public:
         p_ = nullptr;
         n_ = 0;
     }
+    // Copies n elements to the host (the caller has synchronised the stream
+    // that wrote them).
+    void download(T* host, size_t n) const {
+        if (n) hipCheck(hipMemcpy(host, p_, n * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    }
     T* data() const { return p_; }
     size_t size() const { return n_; }
     size_t bytes() const { return n_ * sizeof(T); }
@@ -230,6 +235,7 @@ struct IndexData {
     std::vector<uint64_t> coarsestIds;
     std::vector<double> coarsestVals;
     std::vector<double> finestData;  // generator output (finest layer), when generated
+    std::vector<uint64_t> buildStats;  // GPU build: per level (finest first) tblaeq_kmeans stats[4]
 
     // device side
     tblaeq_ctx* ctx = nullptr;
@@ -273,6 +279,8 @@ public:
     static const std::vector<size_t> kDefaultRatios;
     static constexpr size_t kDefaultHeight = 4;
     static size_t computeCentroidCount(size_t n, size_t ratio);
+    static IndexData* buildGpu(const double* data, size_t N, size_t D, const std::string& name, size_t height,
+                               const std::vector<size_t>& ratios);
     static IndexData* build(const double* data, size_t N, size_t D, const std::string& name, bool forceUseCPU,
                             size_t height, const std::vector<size_t>& ratios);
     static IndexData* buildRandom(size_t N, size_t D, double valMin, double valMax, bool isInt,

@@ -818,10 +818,14 @@ int scan_launch_grid(tblaeq_ctx* ctx, uint64_t tiles) {
     return static_cast<int>(tiles < cap ? (tiles ? tiles : 1) : cap);
 }
 
-// LSD radix sort of `dn` keys (device count, at most cap) on bits [0, nbits).
-// Ping-pongs between a and b; returns the buffer holding the result.  The last
-// pass can write key & idMask instead of the key.
-int radix_sort(tblaeq_ctx* ctx, uint64_t* a, uint64_t* b, const uint64_t* dn, uint64_t cap, int nbits,
+// Histogram words radix_sort needs for up to cap keys.
+uint64_t radix_hist_words(uint64_t cap) { return 256 * ((cap + kRsTile - 1) / kRsTile + 1); }
+
+// LSD radix sort of `dn` keys (device count, at most cap) on bits
+// [firstBit, firstBit + nbits), stable.  Ping-pongs between a and b; returns
+// the buffer holding the result.  The last pass can write key & idMask
+// instead of the key.
+int radix_sort(tblaeq_ctx* ctx, uint64_t* a, uint64_t* b, const uint64_t* dn, uint64_t cap, int firstBit, int nbits,
                uint64_t* hist, uint64_t** result, bool extractLast, uint64_t idMask, uint64_t blocksPerCu) {
     const uint64_t capTiles = (cap + kRsTile - 1) / kRsTile;
     const uint64_t gcap = static_cast<uint64_t>(ctx->cuCount) * blocksPerCu;
@@ -831,7 +835,7 @@ int radix_sort(tblaeq_ctx* ctx, uint64_t* a, uint64_t* b, const uint64_t* dn, ui
     uint64_t* src = a;
     uint64_t* dst = b;
     for (int p = 0; p < passes; ++p) {
-        const int shift = 8 * p;
+        const int shift = firstBit + 8 * p;
         {
             TimedScope ts(ctx, kKBatch0 + 4);
             hipLaunchKernelGGL(k_rs_hist, dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dn, cap, shift, hist);
@@ -991,6 +995,16 @@ int launch_group_scan(tblaeq_batch* b, uint64_t mesh, uint64_t* off, uint64_t* i
     TimedScope ts(ctx, kKBatch0 + 2);
     hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream, b->cnt,
                        mesh, off, items, b->itemCap, nItems, b->ctl + 3, ctx->status, ctx->ctrl);
+    TB_HIP(hipGetLastError());
+    return TBLAEQ_OK;
+}
+
+int scan_counts(tblaeq_ctx* ctx, const uint64_t* cnt, uint64_t n, uint64_t* off, uint64_t* scratch2) {
+    const uint64_t tiles = (n + kScanThreads - 1) / kScanThreads;
+    int rc = scan_prepare(ctx, tiles);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream, cnt, n,
+                       off, nullptr, 0ull, scratch2, scratch2 + 1, ctx->status, ctx->ctrl);
     TB_HIP(hipGetLastError());
     return TBLAEQ_OK;
 }
@@ -1304,7 +1318,7 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     A.masks = nullptr;
     if ((rc = launch_range_step<kFinal>(b, A, b->itemCap))) return rc;
     uint64_t* sorted = nullptr;
-    if ((rc = radix_sort(ctx, b->hits, b->hitsTmp, ctl + 2, A.cap, idBits + qBits, b->hist, &sorted, false, 0,
+    if ((rc = radix_sort(ctx, b->hits, b->hitsTmp, ctl + 2, A.cap, 0, idBits + qBits, b->hist, &sorted, false, 0,
                          b->radixBlocksPerCu)))
         return rc;
     hipLaunchKernelGGL(k_query_offsets, dim3(static_cast<unsigned>((nq + 1 + 255) / 256)), dim3(256), 0, ctx->stream,

@@ -47,6 +47,14 @@ SIGNATURES = {
     "tblaeq_handler_create_from_data": (C.c_int, [f64p, C.c_uint64, C.c_uint64, C.c_uint64, u64p, C.c_char_p,
                                                   C.POINTER(vp)]),
     "tblaeq_handler_load": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
+    # tblaeq_build.h
+    "tblaeq_kmeans": (C.c_int, [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, vp, vp, u64p]),
+    "tblaeq_kmeans_host": (C.c_int, [vp, f64p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, f64p,
+                                     u64p, u64p]),
+    "tblaeq_build_level": (C.c_int, [vp, vp, C.c_uint64, C.c_uint64, vp, C.c_uint64, vp, vp, vp, vp, vp, vp]),
+    "tblaeq_handler_build": (C.c_int, [f64p, C.c_uint64, C.c_uint64, C.c_uint64, u64p, C.c_char_p, C.c_int,
+                                       C.POINTER(vp)]),
+    "tblaeq_handler_build_stats": (C.c_int, [vp, u64p, C.c_uint64, u64p]),
     "tblaeq_handler_save": (C.c_int, [vp, C.c_char_p]),
     "tblaeq_handler_destroy": (C.c_int, [vp]),
     "tblaeq_handler_meta": (C.c_int, [vp, u64p, u64p, C.c_uint64]),
@@ -187,6 +195,37 @@ class Handler:
         check(lib().tblaeq_handler_create_from_data(_p(data, f64p), data.shape[0], data.shape[1], len(r) + 1,
                                                     _p(r, u64p), name.encode(), C.byref(h)), "create_from_data")
         return cls(h)
+
+    @classmethod
+    def build(cls, data, ratios=(100, 50, 20), name="in-memory", force_cpu=False):
+        """QueryHandler(forceCPU, PointCloud, name, height, ratios): k-means build, on the GPU unless
+        force_cpu (IndexBuilder.cu:44-151)."""
+        data = np.ascontiguousarray(data, np.float64)
+        r = np.asarray(ratios, np.uint64)
+        h = vp()
+        check(lib().tblaeq_handler_build(_p(data, f64p), data.shape[0], data.shape[1], len(r) + 1, _p(r, u64p),
+                                         name.encode(), int(bool(force_cpu)), C.byref(h)), "handler_build")
+        return cls(h)
+
+    def build_stats(self):
+        """Per level of a GPU build (finest first): Lloyd iterations, points re-scored in FP64, FP32
+        matrix-core score evaluations, milliseconds in the assignment kernels."""
+        n = C.c_uint64(0)
+        check(lib().tblaeq_handler_build_stats(self.h, None, 0, C.byref(n)), "build_stats")
+        out = np.zeros(n.value, np.uint64)
+        check(lib().tblaeq_handler_build_stats(self.h, _p(out, u64p), n.value, C.byref(n)), "build_stats")
+        return out.reshape(-1, 4)
+
+    def kmeans(self, x, k, max_iter=7, seed=42):
+        """tblaeq_kmeans on host arrays through this handler's context: (labels, centers, stats)."""
+        x = np.ascontiguousarray(x, np.float64)
+        n, D = x.shape
+        cen = np.zeros((k, D))
+        lab = np.zeros(n, np.uint64)
+        st = np.zeros(4, np.uint64)
+        check(lib().tblaeq_kmeans_host(self.ctx(), _p(x, f64p), n, D, k, max_iter, seed, _p(cen, f64p),
+                                       _p(lab, u64p), _p(st, u64p)), "kmeans")
+        return lab, cen, st
 
     @classmethod
     def load(cls, path):
