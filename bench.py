@@ -159,6 +159,61 @@ def run_serving(h, data, cfg, rates, seconds, log):
     return out
 
 
+FP32_MFMA_PEAK_TFLOPS = 157.3
+
+
+def run_build(n, D, ratios, cpu_n, log):
+    """GPU index build (k-means per level on the matrix cores + device CSC) of synthetic clustered
+    data, and the host build of a subsample beside the GPU build of the same subsample."""
+    import tblaeq
+
+    def data_of(m, seed):
+        rng = np.random.default_rng(seed)
+        k0 = max(1, m // ratios[0])
+        cent = rng.uniform(1.0, 100.0, (k0, D))
+        return np.ascontiguousarray(cent[rng.integers(0, k0, m)] + rng.normal(0.0, 1.5, (m, D)))
+
+    x = data_of(n, 11)
+    t0 = time.perf_counter()
+    hb = tblaeq.Handler.build(x, ratios, "bench", force_cpu=False)
+    gpu_s = time.perf_counter() - t0
+    st = hb.build_stats()
+    hb.close()
+    dp = 4
+    while dp < D:
+        dp *= 2
+    ms = float(st[0, 3])
+    flops = float(st[0, 2]) * 2 * dp
+    lvl = [{"iterations": int(r[0]), "rescored_points": int(r[1]), "score_evals": int(r[2]),
+            "assign_ms": int(r[3])} for r in st]
+    xs = data_of(cpu_n, 12)
+    t0 = time.perf_counter()
+    hg = tblaeq.Handler.build(xs, ratios, "bench-gpu", force_cpu=False)
+    gpu_small_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hc = tblaeq.Handler.build(xs, ratios, "bench-cpu", force_cpu=True)
+    cpu_small_s = time.perf_counter() - t0
+    # same hierarchy from both builds (the tests check labels and centroids bit for bit)
+    same = hg.mesh_sizes == hc.mesh_sizes
+    hg.close()
+    hc.close()
+    out = {"workload": "k-means index build of N=%d D=%d clustered points (%d centres, sigma 1.5), ratios %s, "
+                       "k-means++ seed 42, at most 7 Lloyd iterations per level" % (n, D, max(1, n // ratios[0]),
+                                                                                 ",".join(map(str, ratios))),
+           "gpu_s": round(gpu_s, 3), "levels": lvl,
+           "finest_level_ranking": {"bound": "mfma", "achieved": round(flops / (ms / 1e3) / 1e12, 2) if ms else None,
+                                    "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                    "frac": round(flops / (ms / 1e3) / 1e12 / FP32_MFMA_PEAK_TFLOPS, 4) if ms else None,
+                                    "note": "FP32 matrix-core score ||c||^2 - 2x.c (2 x padded D flops per point x "
+                                            "centre), assignment + FP64 re-score time"},
+           "rescored_fraction": round(float(st[0, 1]) / max(1.0, float(n) * float(st[0, 0])), 6),
+           "cpu_sample": {"N": cpu_n, "cpu_s": round(cpu_small_s, 3), "gpu_s": round(gpu_small_s, 3),
+                          "speedup": round(cpu_small_s / gpu_small_s, 1),
+                          "same_mesh_sizes": same}}
+    log("build", out)
+    return out
+
+
 def run_learned(h, data, cfg, knn_q, Ks, n_traces, epochs, plain_sample, log):
     """Learned KNN radius on this index: traces from the batched engine, BF16-MFMA
     training on the GPU, then the per-query traversal with and without the
@@ -243,6 +298,9 @@ def main():
     ap.add_argument("--learn-traces", type=int, default=200000, help="0 skips the learned-radius run")
     ap.add_argument("--learn-epochs", type=int, default=20)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch parameters (profiling runs)")
+    ap.add_argument("--build-n", type=int, default=1_000_000, help="points of the GPU index build run (0 skips it)")
+    ap.add_argument("--build-d", type=int, default=32)
+    ap.add_argument("--build-cpu-n", type=int, default=50_000, help="subsample built on the CPU beside the GPU")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once on the GPU (a kernel of known "
                          "traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -392,6 +450,10 @@ def main():
         learned = run_learned(h, data, cfg, knn_q, Ks.astype(np.uint64), args.learn_traces, args.learn_epochs,
                               min(args.plain_sample, 96), log)
 
+    build = None
+    if args.build_n:
+        build = run_build(args.build_n, args.build_d, (100, 50, 20), args.build_cpu_n, log)
+
     serving = None
     if args.serve_rates and not args.no_batch:
         rates = [float(x) for x in args.serve_rates.split(",") if x]
@@ -436,6 +498,7 @@ def main():
         "batched": batched,
         "learned": learned,
         "serving": serving,
+        "build": build,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
