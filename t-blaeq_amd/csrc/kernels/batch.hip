@@ -591,15 +591,18 @@ constexpr int kRsThreads = 256;
 constexpr int kRsPer = 16;
 constexpr int kRsTile = kRsThreads * kRsPer;
 
+template <int RB>
 __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint64_t* __restrict__ keys, const uint64_t* dn,
                                                         uint64_t cap, int shift, uint64_t* __restrict__ hist) {
-    __shared__ uint32_t lc[kRsThreads / kWave][256];
+    constexpr int kBins = 1 << RB;
+    __shared__ uint32_t lc[kRsThreads / kWave][kBins];
     const int tid = threadIdx.x, wid = tid / kWave;
     uint64_t n = *dn;
     if (n > cap) n = cap;
     const uint64_t ntiles = (n + kRsTile - 1) / kRsTile;
     for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        for (int w = 0; w < kRsThreads / kWave; ++w) lc[w][tid] = 0;
+        for (int w = 0; w < kRsThreads / kWave; ++w)
+            for (int d = tid; d < kBins; d += kRsThreads) lc[w][d] = 0;
         __syncthreads();
         uint64_t k[kRsPer];
 #pragma unroll
@@ -610,21 +613,23 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint64_t* __restri
 #pragma unroll
         for (int r = 0; r < kRsPer; ++r) {
             const uint64_t e = t * kRsTile + static_cast<uint64_t>(r) * kRsThreads + tid;
-            if (e < n) atomicAdd(&lc[wid][(k[r] >> shift) & 0xFF], 1u);
+            if (e < n) atomicAdd(&lc[wid][(k[r] >> shift) & (kBins - 1)], 1u);
         }
         __syncthreads();
-        uint32_t c = 0;
-        for (int w = 0; w < kRsThreads / kWave; ++w) c += lc[w][tid];
-        hist[static_cast<uint64_t>(tid) * ntiles + t] = c;
+        for (int d = tid; d < kBins; d += kRsThreads) {
+            uint32_t c = 0;
+            for (int w = 0; w < kRsThreads / kWave; ++w) c += lc[w][d];
+            hist[static_cast<uint64_t>(d) * ntiles + t] = c;
+        }
         __syncthreads();
     }
 }
 
-// In-place exclusive scan of a u64 array whose length is 256 * ntiles(dn);
-// each thread scans kRsPer consecutive entries.
+// In-place exclusive scan of a u64 array whose length is bins * ntiles(dn);
+// each thread scans kHsPer consecutive entries.
 constexpr int kHsPer = 16;
 __global__ __launch_bounds__(kScanThreads) void k_scan_hist(uint64_t* __restrict__ a, const uint64_t* dn,
-                                                            uint64_t cap, uint64_t* status, ScanCtrl* ctrl) {
+                                                            uint64_t cap, int bins, uint64_t* status, ScanCtrl* ctrl) {
     const uint32_t epoch = launch_epoch(ctrl);
     __shared__ uint64_t s_tile;
     __shared__ uint64_t s_w[kScanThreads / kWave];
@@ -632,7 +637,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_hist(uint64_t* __restrict
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     uint64_t nk = *dn;
     if (nk > cap) nk = cap;
-    const uint64_t n = 256 * ((nk + kRsTile - 1) / kRsTile);
+    const uint64_t n = static_cast<uint64_t>(bins) * ((nk + kRsTile - 1) / kRsTile);
     constexpr uint64_t kTile = static_cast<uint64_t>(kScanThreads) * kHsPer;
     const uint64_t ntiles = n == 0 ? 1 : (n + kTile - 1) / kTile;
     for (;;) {
@@ -675,22 +680,25 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_hist(uint64_t* __restrict
     if (tid == 0) scan_exit(ctrl);
 }
 
-// Stable scatter.  Within a tile, wave w owns keys [1024 w, 1024 (w + 1)),
-// 64 per round.  Each wave ranks its keys against wave-private digit counters
-// in LDS (peers of equal digit found with 8 ballots; rank = counter + peers
-// below the lane), with no block barrier; the waves' counts then give every
-// digit's start in the tile and every wave's offset within the digit.  The
-// tile is laid out sorted by digit in LDS and written from there, so
-// consecutive threads write consecutive addresses within each digit's run.
-// Tiles keep their order through the digit-major scanned histogram.
+// Stable scatter of RB-bit digits.  Within a tile, wave w owns keys
+// [1024 w, 1024 (w + 1)), 64 per round.  Each wave ranks its keys against
+// wave-private digit counters in LDS (peers of equal digit found with RB
+// ballots; rank = counter + peers below the lane), with no block barrier; the
+// waves' counts then give every digit's start in the tile and every wave's
+// offset within the digit.  The tile is laid out sorted by digit in LDS and
+// written from there, so consecutive threads write consecutive addresses
+// within each digit's run.  Tiles keep their order through the digit-major
+// scanned histogram.
+template <int RB>
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
                                                            const uint64_t* dn, uint64_t cap, int shift,
                                                            const uint64_t* __restrict__ hist, uint64_t idMask,
                                                            int extract) {
+    constexpr int kBins = 1 << RB;
     constexpr int kWaves = kRsThreads / kWave;
     constexpr uint32_t kPerWave = static_cast<uint32_t>(kRsPer) * kWave;
-    __shared__ uint32_t lstart[256];
-    __shared__ uint32_t wc[kWaves][256];
+    __shared__ uint32_t lstart[kBins];
+    __shared__ uint32_t wc[kWaves][kBins];
     __shared__ uint32_t wsum[kWaves];
     __shared__ uint64_t sk[kRsTile];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
@@ -711,15 +719,15 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* __res
             key[r] = e < cnt ? in[t0 + e] : 0;
         }
 #pragma unroll
-        for (int i = 0; i < 256 / kWave; ++i) mycnt[lane + i * kWave] = 0;
+        for (int i = 0; i < kBins / kWave; ++i) mycnt[lane + i * kWave] = 0;
         wave_lds_sync();
 #pragma unroll
         for (int r = 0; r < kRsPer; ++r) {
             const bool ok = e0 + static_cast<uint32_t>(r) * kWave < cnt;
-            const uint32_t dig = ok ? static_cast<uint32_t>((key[r] >> shift) & 0xFF) : 0;
+            const uint32_t dig = ok ? static_cast<uint32_t>((key[r] >> shift) & (kBins - 1)) : 0;
             uint64_t peers = __ballot(ok);
 #pragma unroll
-            for (int b = 0; b < 8; ++b) {
+            for (int b = 0; b < RB; ++b) {
                 const uint64_t bal = __ballot(((dig >> b) & 1u) != 0);
                 peers &= ((dig >> b) & 1u) ? bal : ~bal;
             }
@@ -731,14 +739,17 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* __res
             wave_lds_sync();
         }
         __syncthreads();
-        // digit starts in the tile (exclusive scan of the digit totals), then
-        // each wave's start within every digit
-        {
+        // digit starts in the tile (exclusive scan of the digit totals, kRsThreads
+        // digits per step), then each wave's start within every digit
+        uint32_t carry = 0;
+#pragma unroll
+        for (int h = 0; h < kBins / kRsThreads; ++h) {
+            const int d = h * kRsThreads + tid;
             uint32_t c[kWaves];
             uint32_t v = 0;
 #pragma unroll
             for (int w = 0; w < kWaves; ++w) {
-                c[w] = wc[w][tid];
+                c[w] = wc[w][d];
                 v += c[w];
             }
             uint32_t x = v;
@@ -748,25 +759,30 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* __res
             }
             if (lane == kWave - 1) wsum[wid] = x;
             __syncthreads();
-            uint32_t p = x - v;
-            for (int w = 0; w < wid; ++w) p += wsum[w];
-            lstart[tid] = p;
+            uint32_t p = carry + x - v;
+            uint32_t all = 0;
+            for (int w = 0; w < kWaves; ++w) {
+                if (w < wid) p += wsum[w];
+                all += wsum[w];
+            }
+            lstart[d] = p;
 #pragma unroll
             for (int w = 0; w < kWaves; ++w) {
-                wc[w][tid] = p;
+                wc[w][d] = p;
                 p += c[w];
             }
+            carry += all;
             __syncthreads();
         }
 #pragma unroll
         for (int r = 0; r < kRsPer; ++r) {
             const uint32_t e = e0 + static_cast<uint32_t>(r) * kWave;
-            if (e < cnt) sk[mycnt[(key[r] >> shift) & 0xFF] + rank[r]] = key[r];
+            if (e < cnt) sk[mycnt[(key[r] >> shift) & (kBins - 1)] + rank[r]] = key[r];
         }
         __syncthreads();
         for (uint32_t i = tid; i < cnt; i += kRsThreads) {
             const uint64_t k = sk[i];
-            const uint32_t dig = static_cast<uint32_t>((k >> shift) & 0xFF);
+            const uint32_t dig = static_cast<uint32_t>((k >> shift) & (kBins - 1));
             const uint64_t dst = hist[static_cast<uint64_t>(dig) * ntiles + t] + (i - lstart[dig]);
             out[dst] = extract ? (k & idMask) : k;
         }
@@ -818,39 +834,51 @@ int scan_launch_grid(tblaeq_ctx* ctx, uint64_t tiles) {
     return static_cast<int>(tiles < cap ? (tiles ? tiles : 1) : cap);
 }
 
-// Histogram words radix_sort needs for up to cap keys.
-uint64_t radix_hist_words(uint64_t cap) { return 256 * ((cap + kRsTile - 1) / kRsTile + 1); }
+// Histogram words radix_sort needs for up to cap keys (digits of up to 9 bits).
+uint64_t radix_hist_words(uint64_t cap) { return 512 * ((cap + kRsTile - 1) / kRsTile + 1); }
 
 // LSD radix sort of `dn` keys (device count, at most cap) on bits
-// [firstBit, firstBit + nbits), stable.  Ping-pongs between a and b; returns
-// the buffer holding the result.  The last pass can write key & idMask
-// instead of the key.
+// [firstBit, firstBit + nbits), stable: 9-bit digits when they take fewer
+// passes than 8-bit ones, else 8-bit.  Ping-pongs between a and b; returns the
+// buffer holding the result.  The last pass can write key & idMask instead of
+// the key.
 int radix_sort(tblaeq_ctx* ctx, uint64_t* a, uint64_t* b, const uint64_t* dn, uint64_t cap, int firstBit, int nbits,
                uint64_t* hist, uint64_t** result, bool extractLast, uint64_t idMask, uint64_t blocksPerCu) {
     const uint64_t capTiles = (cap + kRsTile - 1) / kRsTile;
     const uint64_t gcap = static_cast<uint64_t>(ctx->cuCount) * blocksPerCu;
     const int grid = static_cast<int>(capTiles < gcap ? (capTiles ? capTiles : 1) : gcap);
-    const uint64_t histTiles = (256 * (capTiles ? capTiles : 1) + kScanThreads * kHsPer - 1) / (kScanThreads * kHsPer);
-    const int passes = (nbits + 7) / 8;
+    const int rb = (nbits + 8) / 9 < (nbits + 7) / 8 ? 9 : 8;
+    const int bins = 1 << rb;
+    const uint64_t histTiles =
+        (static_cast<uint64_t>(bins) * (capTiles ? capTiles : 1) + kScanThreads * kHsPer - 1) / (kScanThreads * kHsPer);
+    const int passes = (nbits + rb - 1) / rb;
     uint64_t* src = a;
     uint64_t* dst = b;
     for (int p = 0; p < passes; ++p) {
-        const int shift = firstBit + 8 * p;
+        const int shift = firstBit + rb * p;
         {
             TimedScope ts(ctx, kKBatch0 + 4);
-            hipLaunchKernelGGL(k_rs_hist, dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dn, cap, shift, hist);
+            if (rb == 9)
+                hipLaunchKernelGGL(k_rs_hist<9>, dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dn, cap, shift, hist);
+            else
+                hipLaunchKernelGGL(k_rs_hist<8>, dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dn, cap, shift, hist);
         }
         int rc = scan_prepare(ctx, histTiles);
         if (rc) return rc;
         {
             TimedScope ts(ctx, kKBatch0 + 4);
             hipLaunchKernelGGL(k_scan_hist, dim3(scan_launch_grid(ctx, histTiles)), dim3(kScanThreads), 0,
-                               ctx->stream, hist, dn, cap, ctx->status, ctx->ctrl);
+                               ctx->stream, hist, dn, cap, bins, ctx->status, ctx->ctrl);
         }
         {
             TimedScope ts(ctx, kKBatch0 + 4);
-            hipLaunchKernelGGL(k_rs_scatter, dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dst, dn, cap, shift,
-                               hist, idMask, (extractLast && p == passes - 1) ? 1 : 0);
+            const int ex = (extractLast && p == passes - 1) ? 1 : 0;
+            if (rb == 9)
+                hipLaunchKernelGGL(k_rs_scatter<9>, dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dst, dn, cap,
+                                   shift, hist, idMask, ex);
+            else
+                hipLaunchKernelGGL(k_rs_scatter<8>, dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dst, dn, cap,
+                                   shift, hist, idMask, ex);
         }
         TB_HIP(hipGetLastError());
         std::swap(src, dst);
@@ -1199,8 +1227,7 @@ extern "C" int tblaeq_batch_reserve(tblaeq_batch* b, uint64_t max_queries, uint6
         if (rc) return rc;
         b->recAux = balloc<uint64_t>(b, hit_cap, &rc);
         if (rc) return rc;
-        const uint64_t tiles = (hit_cap + kRsTile - 1) / kRsTile;
-        b->histCap = 256 * tiles;
+        b->histCap = radix_hist_words(hit_cap);
         b->hist = balloc<uint64_t>(b, b->histCap, &rc);
         if (rc) return rc;
         b->hitCap = hit_cap;
@@ -1291,8 +1318,8 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     }
     // final step: exact filter of the finest candidates
     const uint64_t maxId = b->count[L];
-    const int idBits = bits_for(maxId);
-    const int qBits = bits_for(nq);
+    const int idBits = bits_for(maxId - 1);  // ids < count[L]
+    const int qBits = bits_for(nq - 1);      // queries < nq
     if (idBits + qBits > 64) {
         set_error("tblaeq_batch_range: id and query bits exceed 64");
         return TBLAEQ_ERR_UNSUPPORTED;
