@@ -112,4 +112,8 @@ int bits_for(uint64_t v);
 // queried once per kernel.  Work-queue kernels launch at most this many per CU:
 // every wave is resident from the start and takes items until the queue is dry.
 int resident_blocks(const void* kernel, int threads);
+// Moves n rows of `width` 8-byte words on ctx's stream: scatter ? dst[idx[i]] =
+// src[i] : dst[i] = src[idx[i]] (idx device, distinct for a scatter).
+void move_rows(tblaeq_ctx* ctx, bool scatter, const void* src, const uint64_t* idx, uint64_t n, uint64_t width,
+               void* dst);
 }  // namespace tblaeq

@@ -5,6 +5,7 @@
 
 #include "../../../include/tblaeq_handler.h"
 #include "../../../include/tblaeq_learn.h"
+#include "../batch_internal.hpp"
 #include "../ctx.hpp"
 #include "handler_internal.hpp"
 
@@ -347,103 +348,103 @@ int tblaeq_handler_batch_knn_learned(tblaeq_handler* h, tblaeq_mlp* m, double fa
         const uint64_t nq = st->nq, Kmax = st->Kmax;
         double vmin = 0, scale = 1;
         tblaeq_mlp_get_norm(m, &vmin, &scale);
-        DeviceBuffer<float> dX(nq * 48);
-        DeviceBuffer<double> dR(nq);
-        DeviceBuffer<uint8_t> dF(nq);
-        tbCheck(tblaeq_knn_features(st->b, st->dQ.data(), st->dK.data(), nq, vmin, scale, dX.data()), "features");
+        const uint64_t cap = st->maxQ;
+        if (st->lX.size() < cap * 48) {
+            st->lX.resize(cap * 48);
+            st->lR.resize(cap);
+            st->lF.resize(cap);
+            st->lIdx.resize(cap);
+            st->lQ.resize(cap * D);
+            st->lK.resize(cap);
+            st->lIds.resize(cap * st->Kmax);
+            st->lD2.resize(cap * st->Kmax);
+            st->lN.resize(cap);
+            st->lSurv.resize(cap * L);
+            st->lFine.resize(cap);
+        }
+        if (st->lIds.size() < cap * Kmax) {
+            st->lIds.resize(cap * Kmax);
+            st->lD2.resize(cap * Kmax);
+        }
+        float* dX = st->lX.data();
+        double* dR = st->lR.data();
+        uint8_t* dF = st->lF.data();
+        tbCheck(tblaeq_knn_features(st->b, st->dQ.data(), st->dK.data(), nq, vmin, scale, dX), "features");
         hipCheck(hipStreamSynchronize(s), "sync");
-        tbCheck(tblaeq_mlp_propose_radius(m, dX.data(), nq, scale, factor, dR.data()), "propose radius");
+        tbCheck(tblaeq_mlp_propose_radius(m, dX, nq, scale, factor, dR), "propose radius");
         hipCheck(hipDeviceSynchronize(), "sync");
         st->work[0] = st->work[1] = 0;
         if (stats) {
             hipCheck(hipMemsetAsync(st->dSurv.data(), 0, nq * L * 8, s), "memset");
             hipCheck(hipMemsetAsync(st->dFine.data(), 0, nq * 8, s), "memset");
         }
-        // certified batch, split in halves while a capacity is exceeded
-        std::vector<std::pair<uint64_t, uint64_t>> todo{{0, nq}};
-        while (!todo.empty()) {
-            auto [a, n] = todo.back();
-            todo.pop_back();
-            tbCheck(tblaeq_batch_knn_radius(st->b, st->dQ.data() + a * D, st->dK.data() + a, n, Kmax, dR.data() + a,
-                                            st->dIds.data() + a * Kmax, st->dD2.data() + a * Kmax, st->dN.data() + a,
-                                            dF.data() + a, st->dStatus.data(),
-                                            stats ? st->dSurv.data() + a * L : nullptr,
-                                            stats ? st->dFine.data() + a : nullptr),
-                    "learned batch");
-            uint64_t status[4];
-            hipCheck(hipMemcpyAsync(status, st->dStatus.data(), 32, hipMemcpyDeviceToHost, s), "D2H");
-            hipCheck(hipStreamSynchronize(s), "sync");
-            if (status[1] != 0) {
-                if (n == 1) {
-                    tblaeq::set_error("learned KNN: one query exceeds the engine capacities");
-                    return TBLAEQ_ERR_CAPACITY;
-                }
-                todo.push_back({a + n / 2, n - n / 2});
-                todo.push_back({a, n / 2});
-                continue;
-            }
-            st->work[0] += status[2];
-            st->work[1] += status[3];
-        }
-        std::vector<uint8_t> fb(nq);
-        hipCheck(hipMemcpy(fb.data(), dF.data(), nq, hipMemcpyDeviceToHost), "D2H");
-        std::vector<uint64_t> redo;
-        for (uint64_t q = 0; q < nq; ++q)
-            if (fb[q]) redo.push_back(q);
-        if (fallbacks) *fallbacks = redo.size();
-        if (!redo.empty()) {
-            // rerun the failed queries with the seed method and scatter their rows back
-            const uint64_t n = redo.size();
-            std::vector<double> q(n * D);
-            std::vector<uint64_t> k(n);
-            for (uint64_t i = 0; i < n; ++i) {
-                std::memcpy(q.data() + i * D, st->hQ.data() + redo[i] * D, D * 8);
-                k[i] = st->hK[redo[i]];
-            }
-            DeviceBuffer<double> dq(q.data(), n * D), dd2(n * Kmax);
-            DeviceBuffer<uint64_t> dk(k.data(), n), dids(n * Kmax), dn(n), dsurv(stats ? n * L : 1),
-                dfine(stats ? n : 1);
-            if (stats) {
-                hipCheck(hipMemsetAsync(dsurv.data(), 0, n * L * 8, s), "memset");
-                hipCheck(hipMemsetAsync(dfine.data(), 0, n * 8, s), "memset");
-            }
-            std::vector<std::pair<uint64_t, uint64_t>> todo2{{0, n}};
-            while (!todo2.empty()) {
-                auto [a, c] = todo2.back();
-                todo2.pop_back();
-                tbCheck(tblaeq_batch_knn(st->b, dq.data() + a * D, dk.data() + a, c, Kmax, dids.data() + a * Kmax,
-                                         dd2.data() + a * Kmax, dn.data() + a, st->dStatus.data(),
-                                         stats ? dsurv.data() + a * L : nullptr, stats ? dfine.data() + a : nullptr),
-                        "fallback batch");
+        // one engine call over [a, a + n), split in halves while a capacity is exceeded
+        auto run = [&](bool seed, const double* Q, const uint64_t* K, uint64_t n0, uint64_t* ids, double* d2,
+                       uint64_t* cnt, uint64_t* surv, uint64_t* fine) -> int {
+            std::vector<std::pair<uint64_t, uint64_t>> todo{{0, n0}};
+            while (!todo.empty()) {
+                auto [a, n] = todo.back();
+                todo.pop_back();
+                uint64_t* sv = stats ? surv + a * L : nullptr;
+                uint64_t* fn = stats ? fine + a : nullptr;
+                if (seed)
+                    tbCheck(tblaeq_batch_knn(st->b, Q + a * D, K + a, n, Kmax, ids + a * Kmax, d2 + a * Kmax, cnt + a,
+                                             st->dStatus.data(), sv, fn),
+                            "fallback batch");
+                else
+                    tbCheck(tblaeq_batch_knn_radius(st->b, Q + a * D, K + a, n, Kmax, dR + a, ids + a * Kmax,
+                                                    d2 + a * Kmax, cnt + a, dF + a, st->dStatus.data(), sv, fn),
+                            "learned batch");
                 uint64_t status[4];
                 hipCheck(hipMemcpyAsync(status, st->dStatus.data(), 32, hipMemcpyDeviceToHost, s), "D2H");
                 hipCheck(hipStreamSynchronize(s), "sync");
                 if (status[1] != 0) {
-                    if (c == 1) {
+                    if (n == 1) {
                         tblaeq::set_error("learned KNN: one query exceeds the engine capacities");
                         return TBLAEQ_ERR_CAPACITY;
                     }
-                    todo2.push_back({a + c / 2, c - c / 2});
-                    todo2.push_back({a, c / 2});
+                    todo.push_back({a + n / 2, n - n / 2});
+                    todo.push_back({a, n / 2});
                     continue;
                 }
                 st->work[0] += status[2];
                 st->work[1] += status[3];
             }
-            for (uint64_t i = 0; i < n; ++i) {
-                const uint64_t r = redo[i];
-                hipCheck(hipMemcpyAsync(st->dIds.data() + r * Kmax, dids.data() + i * Kmax, Kmax * 8,
-                                        hipMemcpyDeviceToDevice, s), "D2D");
-                hipCheck(hipMemcpyAsync(st->dD2.data() + r * Kmax, dd2.data() + i * Kmax, Kmax * 8,
-                                        hipMemcpyDeviceToDevice, s), "D2D");
-                hipCheck(hipMemcpyAsync(st->dN.data() + r, dn.data() + i, 8, hipMemcpyDeviceToDevice, s), "D2D");
-                if (stats) {
-                    hipCheck(hipMemcpyAsync(st->dSurv.data() + r * L, dsurv.data() + i * L, L * 8,
-                                            hipMemcpyDeviceToDevice, s), "D2D");
-                    hipCheck(hipMemcpyAsync(st->dFine.data() + r, dfine.data() + i, 8, hipMemcpyDeviceToDevice, s),
-                             "D2D");
-                }
+            return TBLAEQ_OK;
+        };
+        int rc = run(false, st->dQ.data(), st->dK.data(), nq, st->dIds.data(), st->dD2.data(), st->dN.data(),
+                     st->dSurv.data(), st->dFine.data());
+        if (rc) return rc;
+        std::vector<uint8_t> fb(nq);
+        hipCheck(hipMemcpy(fb.data(), dF, nq, hipMemcpyDeviceToHost), "D2H");
+        std::vector<uint64_t> redo;
+        for (uint64_t q = 0; q < nq; ++q)
+            if (fb[q]) redo.push_back(q);
+        if (fallbacks) *fallbacks = redo.size();
+        if (!redo.empty()) {
+            // rerun the failed queries with the seed method on gathered rows and
+            // scatter their result rows back
+            const uint64_t n = redo.size();
+            tblaeq_ctx* c = st->b->ctx;
+            uint64_t* ridx = st->lIdx.data();
+            hipCheck(hipMemcpyAsync(ridx, redo.data(), n * 8, hipMemcpyHostToDevice, s), "H2D");
+            tblaeq::move_rows(c, false, st->dQ.data(), ridx, n, D, st->lQ.data());
+            tblaeq::move_rows(c, false, st->dK.data(), ridx, n, 1, st->lK.data());
+            if (stats) {
+                hipCheck(hipMemsetAsync(st->lSurv.data(), 0, n * L * 8, s), "memset");
+                hipCheck(hipMemsetAsync(st->lFine.data(), 0, n * 8, s), "memset");
             }
+            rc = run(true, st->lQ.data(), st->lK.data(), n, st->lIds.data(), st->lD2.data(), st->lN.data(),
+                     st->lSurv.data(), st->lFine.data());
+            if (rc) return rc;
+            tblaeq::move_rows(c, true, st->lIds.data(), ridx, n, Kmax, st->dIds.data());
+            tblaeq::move_rows(c, true, st->lD2.data(), ridx, n, Kmax, st->dD2.data());
+            tblaeq::move_rows(c, true, st->lN.data(), ridx, n, 1, st->dN.data());
+            if (stats) {
+                tblaeq::move_rows(c, true, st->lSurv.data(), ridx, n, L, st->dSurv.data());
+                tblaeq::move_rows(c, true, st->lFine.data(), ridx, n, 1, st->dFine.data());
+            }
+            hipCheck(hipGetLastError(), "move rows");
             hipCheck(hipStreamSynchronize(s), "sync");
         }
         return TBLAEQ_OK;

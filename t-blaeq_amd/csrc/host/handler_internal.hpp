@@ -20,6 +20,12 @@ struct BatchState {
     std::vector<double> hQ;     // host copy of the uploaded queries
     std::vector<uint64_t> hK;
     bool useGraph = true;
+    // learned KNN: features, radii, fallback flags, and the fallback rerun's
+    // gathered queries and rows (allocated for maxQ on first use)
+    DeviceBuffer<float> lX;
+    DeviceBuffer<double> lR, lQ, lD2;
+    DeviceBuffer<uint8_t> lF;
+    DeviceBuffer<uint64_t> lIdx, lK, lIds, lN, lSurv, lFine;
     ~BatchState() {
         if (b) tblaeq_batch_destroy(b);
     }

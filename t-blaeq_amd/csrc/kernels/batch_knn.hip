@@ -1197,6 +1197,35 @@ __global__ void k_certify(uint64_t nq, const uint64_t* __restrict__ Ks, const ui
         fallback[q] = (Ks[q] > 0 && offQ[q + 1] - offQ[q] < Ks[q]) ? 1 : 0;
 }
 
+// Row moves by an index list (rows of `width` words): gather dst[i] = src[idx[i]],
+// scatter dst[idx[i]] = src[i].
+template <bool SCATTER>
+__global__ void k_move_rows(const uint64_t* __restrict__ src, const uint64_t* __restrict__ idx, uint64_t n,
+                            uint64_t width, uint64_t* __restrict__ dst) {
+    const uint64_t total = n * width;
+    for (uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; e < total;
+         e += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t i = e / width, j = e - i * width;
+        if (SCATTER) dst[idx[i] * width + j] = src[e];
+        else dst[e] = src[idx[i] * width + j];
+    }
+}
+
+void move_rows(tblaeq_ctx* ctx, bool scatter, const void* src, const uint64_t* idx, uint64_t n, uint64_t width,
+               void* dst) {
+    const uint64_t total = n * width;
+    if (total == 0) return;
+    const uint64_t cap = static_cast<uint64_t>(ctx->cuCount) * 8;
+    const uint64_t want = (total + 255) / 256;
+    const int grid = static_cast<int>(want < cap ? want : cap);
+    if (scatter)
+        hipLaunchKernelGGL(k_move_rows<true>, dim3(grid), dim3(256), 0, ctx->stream,
+                           static_cast<const uint64_t*>(src), idx, n, width, static_cast<uint64_t*>(dst));
+    else
+        hipLaunchKernelGGL(k_move_rows<false>, dim3(grid), dim3(256), 0, ctx->stream,
+                           static_cast<const uint64_t*>(src), idx, n, width, static_cast<uint64_t*>(dst));
+}
+
 }  // namespace tblaeq
 
 using namespace tblaeq;
