@@ -100,12 +100,25 @@ void launch_fill(tblaeq_batch* b, uint64_t* p, uint64_t v, uint64_t n);
 // Exclusive scan of cnt[0..n) into off[0..n] (off[n] = total) on ctx's stream;
 // scratch2 = two device words.
 int scan_counts(tblaeq_ctx* ctx, const uint64_t* cnt, uint64_t n, uint64_t* off, uint64_t* scratch2);
-// Stable LSD radix sort of *dn (<= cap) u64 keys on bits [firstBit, firstBit +
-// nbits) between a and b (8-bit digits); *result = the buffer holding the
-// sorted keys.  hist needs radix_hist_words(cap) words.
+// Outputs of the last radix pass: ids (optional) receives key & idMask instead
+// of the ping-pong buffer; groupOff (optional, nGroups + 1 words) receives the
+// first sorted position of every group key >> groupShift (groups without keys:
+// the next group's start; groupOff[nGroups] = key count; every key's group is
+// below nGroups).
+struct RadixOut {
+    uint64_t* ids = nullptr;
+    uint64_t idMask = ~0ull;
+    uint64_t* groupOff = nullptr;
+    int groupShift = 0;
+    uint64_t nGroups = 0;
+};
+// Stable LSD radix sort of *dn (<= cap < 2^30) u64 keys on bits [firstBit,
+// firstBit + nbits) between a and b (8- or 9-bit digits); *result = the buffer
+// holding the sorted keys (ro.ids when given).  hist needs
+// radix_hist_words(cap, ro.nGroups) words.
 int radix_sort(tblaeq_ctx* ctx, uint64_t* a, uint64_t* b, const uint64_t* dn, uint64_t cap, int firstBit, int nbits,
-               uint64_t* hist, uint64_t** result, bool extractLast, uint64_t idMask, uint64_t blocksPerCu);
-uint64_t radix_hist_words(uint64_t cap);
+               uint64_t* hist, uint64_t** result, const RadixOut& ro, uint64_t blocksPerCu);
+uint64_t radix_hist_words(uint64_t cap, uint64_t groups);
 // Bits needed to represent v (at least 1).
 int bits_for(uint64_t v);
 // Blocks of `kernel` (with `threads` threads) that fit on one CU at once,

@@ -584,130 +584,162 @@ __global__ void k_group_scatter(const uint64_t* __restrict__ pairs, const uint64
 }
 
 // ------------------------------------------------------------- radix sort
-// LSD radix sort of u64 keys, 8-bit digits, tiles of 4096 keys.  The key count
-// lives on the device; a persistent grid walks the tiles.
+// Single-sweep LSD radix sort of u64 keys: digits of RB = 8 or 9 bits, tiles of
+// 8192 keys, the key count on the device.  A counting kernel reads the keys
+// once for the first digit's counts; each pass is then one kernel that ranks a
+// tile's keys by digit, publishes the tile's digit counts, takes every digit's
+// exclusive prefix over the earlier tiles by decoupled look-back, and writes
+// each key straight to its place, while counting the next pass's digit (the
+// last pass: the keys of every group key >> groupShift) from the keys it holds.
+// Tiles are handed out by a ticket counter, so a tile only waits on tiles that
+// resident workgroups already hold.
 
-constexpr int kRsThreads = 256;
+constexpr int kRsThreads = 512;
+constexpr int kRsWaves = kRsThreads / kWave;
 constexpr int kRsPer = 16;
 constexpr int kRsTile = kRsThreads * kRsPer;
+constexpr int kRsMaxPasses = 8;
+constexpr int kRsMaxBins = 512;
+constexpr int kRsGroupLds = 8192;  // groups counted in LDS (more: global atomics)
+// scratch (u32 words): [0, 64) tile tickets of each pass (and two u64 words of
+// the group scan at [32, 36)), then the digit counts of each pass, then two
+// look-back regions of tiles x bins words, used by alternate passes, then the
+// u64 group counts
+constexpr uint64_t kRsCounts0 = 64;
+constexpr uint64_t kRsHdrWords = kRsCounts0 + static_cast<uint64_t>(kRsMaxPasses) * kRsMaxBins;
+// look-back word: flag (aggregate / inclusive prefix) in the top two bits
+constexpr uint32_t kRsAgg = 1u << 30, kRsInc = 2u << 30, kRsVal = (1u << 30) - 1;
+constexpr int kRsProbe = 8;  // predecessors read per look-back round
+// what a pass counts besides its own digit
+constexpr int kRsCountNext = 0, kRsCountNone = 1, kRsCountGroups = 2;
 
+// Lanes of the wave whose RB-bit digit equals this lane's, among the lanes set
+// in `valid`: one ballot per digit bit, each folded into the running mask of
+// ruled-out lanes by one bitop3 per half (out | (ballot ^ smeared bit): table
+// 0xF0 | (0xCC ^ 0xAA) = 0xF6 over the operands (out, ballot, bit)).
 template <int RB>
-__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint64_t* __restrict__ keys, const uint64_t* dn,
-                                                        uint64_t cap, int shift, uint64_t* __restrict__ hist) {
+__device__ __forceinline__ void digit_peers(uint32_t dig, uint64_t valid, uint32_t& peersLo, uint32_t& peersHi) {
+    uint32_t lo = ~static_cast<uint32_t>(valid), hi = ~static_cast<uint32_t>(valid >> 32);
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+        const uint32_t sm = static_cast<uint32_t>(static_cast<int32_t>(dig << (31 - b)) >> 31);
+        const uint64_t bal = __ballot(sm != 0);
+        lo = __builtin_amdgcn_bitop3_b32(lo, static_cast<uint32_t>(bal), sm, 0xF6);
+        hi = __builtin_amdgcn_bitop3_b32(hi, static_cast<uint32_t>(bal >> 32), sm, 0xF6);
+    }
+    peersLo = ~lo;
+    peersHi = ~hi;
+}
+
+// Exclusive scan of the kBins digit values of a workgroup of kRsThreads threads
+// (thread tid holds digit tid; threads past kBins hold 0); with barriers.
+template <int kBins>
+__device__ __forceinline__ uint32_t bins_excl_scan(uint32_t v, uint32_t* wsum) {
+    static_assert(kBins <= kRsThreads, "one digit per thread");
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    uint32_t x = v;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, kWave);
+        if (lane >= o) x += y;
+    }
+    if (lane == kWave - 1) wsum[wid] = x;
+    __syncthreads();
+    uint32_t p = x - v;
+#pragma unroll
+    for (int w = 0; w < kRsWaves; ++w)
+        if (w < wid) p += wsum[w];
+    __syncthreads();
+    return p;
+}
+
+// Clears the tickets, digit counts and group counts.
+__global__ void k_rs_begin(uint32_t* __restrict__ hdr, uint64_t* __restrict__ groupCnt, uint64_t nGroups) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    const uint64_t i0 = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    for (uint64_t i = i0; i < kRsHdrWords; i += stride) hdr[i] = 0;
+    if (groupCnt)
+        for (uint64_t i = i0; i < nGroups; i += stride) groupCnt[i] = 0;
+}
+
+// Counts of the first digit (at firstBit) over all keys; also clears look-back
+// region 0 for the tiles of this sort.
+template <int RB>
+__global__ __launch_bounds__(kRsThreads) void k_rs_counts(const uint64_t* __restrict__ keys, const uint64_t* dn,
+                                                          uint64_t cap, int firstBit, uint32_t* __restrict__ counts,
+                                                          uint32_t* __restrict__ status0) {
     constexpr int kBins = 1 << RB;
-    __shared__ uint32_t lc[kRsThreads / kWave][kBins];
+    __shared__ uint32_t lc[kRsWaves][kBins];  // per-wave counters
     const int tid = threadIdx.x, wid = tid / kWave;
+    for (int i = tid; i < kRsWaves * kBins; i += kRsThreads) (&lc[0][0])[i] = 0;
+    __syncthreads();
     uint64_t n = *dn;
     if (n > cap) n = cap;
     const uint64_t ntiles = (n + kRsTile - 1) / kRsTile;
     for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        for (int w = 0; w < kRsThreads / kWave; ++w)
-            for (int d = tid; d < kBins; d += kRsThreads) lc[w][d] = 0;
-        __syncthreads();
+        for (int d = tid; d < kBins; d += kRsThreads) status0[t * kBins + d] = 0;
         uint64_t k[kRsPer];
 #pragma unroll
         for (int r = 0; r < kRsPer; ++r) {
             const uint64_t e = t * kRsTile + static_cast<uint64_t>(r) * kRsThreads + tid;
-            k[r] = e < n ? keys[e] : ~0ull;
+            k[r] = e < n ? keys[e] : 0;
         }
 #pragma unroll
         for (int r = 0; r < kRsPer; ++r) {
             const uint64_t e = t * kRsTile + static_cast<uint64_t>(r) * kRsThreads + tid;
-            if (e < n) atomicAdd(&lc[wid][(k[r] >> shift) & (kBins - 1)], 1u);
+            if (e < n) atomicAdd(&lc[wid][(k[r] >> firstBit) & (kBins - 1)], 1u);
         }
-        __syncthreads();
-        for (int d = tid; d < kBins; d += kRsThreads) {
-            uint32_t c = 0;
-            for (int w = 0; w < kRsThreads / kWave; ++w) c += lc[w][d];
-            hist[static_cast<uint64_t>(d) * ntiles + t] = c;
-        }
-        __syncthreads();
+    }
+    __syncthreads();
+    for (int d = tid; d < kBins; d += kRsThreads) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < kRsWaves; ++w) c += lc[w][d];
+        if (c) atomicAdd(&counts[d], c);
     }
 }
 
-// In-place exclusive scan of a u64 array whose length is bins * ntiles(dn);
-// each thread scans kHsPer consecutive entries.
-constexpr int kHsPer = 16;
-__global__ __launch_bounds__(kScanThreads) void k_scan_hist(uint64_t* __restrict__ a, const uint64_t* dn,
-                                                            uint64_t cap, int bins, uint64_t* status, ScanCtrl* ctrl) {
-    const uint32_t epoch = launch_epoch(ctrl);
-    __shared__ uint64_t s_tile;
-    __shared__ uint64_t s_w[kScanThreads / kWave];
-    __shared__ uint64_t s_ex;
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
-    uint64_t nk = *dn;
-    if (nk > cap) nk = cap;
-    const uint64_t n = static_cast<uint64_t>(bins) * ((nk + kRsTile - 1) / kRsTile);
-    constexpr uint64_t kTile = static_cast<uint64_t>(kScanThreads) * kHsPer;
-    const uint64_t ntiles = n == 0 ? 1 : (n + kTile - 1) / kTile;
-    for (;;) {
-        if (tid == 0) s_tile = atomicAdd(reinterpret_cast<unsigned long long*>(&ctrl->tileCounter), 1ull);
-        __syncthreads();
-        const uint64_t tile = s_tile;
-        if (tile >= ntiles) break;
-        const uint64_t i0 = tile * kTile + static_cast<uint64_t>(tid) * kHsPer;
-        uint64_t v[kHsPer];
-        uint64_t sum = 0;
-#pragma unroll
-        for (int k = 0; k < kHsPer; ++k) {
-            v[k] = i0 + k < n ? a[i0 + k] : 0;
-            sum += v[k];
-        }
-        uint64_t x = sum;
-        for (int o = 1; o < kWave; o <<= 1) {
-            const uint64_t y = __shfl_up(x, o, kWave);
-            if (lane >= o) x += y;
-        }
-        if (lane == kWave - 1) s_w[wid] = x;
-        __syncthreads();
-        if (wid == 0) {
-            uint64_t t = lane < kScanThreads / kWave ? s_w[lane] : 0;
-            t = wave_sum_u64(t);
-            uint64_t e0, e1;
-            tile_prefix2(status, epoch, tile, lane, t, 0, e0, e1);
-            if (lane == 0) s_ex = e0;
-        }
-        __syncthreads();
-        uint64_t p = s_ex + x - sum;
-        for (int w = 0; w < wid; ++w) p += s_w[w];
-#pragma unroll
-        for (int k = 0; k < kHsPer; ++k) {
-            if (i0 + k < n) a[i0 + k] = p;
-            p += v[k];
-        }
-        __syncthreads();
-    }
-    if (tid == 0) scan_exit(ctrl);
-}
-
-// Stable scatter of RB-bit digits.  Within a tile, wave w owns keys
-// [1024 w, 1024 (w + 1)), 64 per round.  Each wave ranks its keys against
-// wave-private digit counters in LDS (peers of equal digit found with RB
-// ballots; rank = counter + peers below the lane), with no block barrier; the
-// waves' counts then give every digit's start in the tile and every wave's
-// offset within the digit.  The tile is laid out sorted by digit in LDS and
-// written from there, so consecutive threads write consecutive addresses
-// within each digit's run.  Tiles keep their order through the digit-major
-// scanned histogram.
-template <int RB>
-__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
-                                                           const uint64_t* dn, uint64_t cap, int shift,
-                                                           const uint64_t* __restrict__ hist, uint64_t idMask,
-                                                           int extract) {
+// One stable pass on the digit at `shift`.  Within a tile, wave w owns keys
+// [1024 w, 1024 (w + 1)), 64 per round, and ranks them against wave-private
+// digit counters in LDS (peers of equal digit found with RB ballots; rank =
+// counter + peers below the lane), with no workgroup barrier.  The waves'
+// counts give the tile's digit counts (thread d owns digit d), which are
+// published (tile 0: as its inclusive prefix) before the look-back; then
+// wave w's first output position for digit d is the digit's global start +
+// its count in earlier tiles + the counts of waves < w in this tile, and a
+// key goes to that position + its rank.  out = key & outMask.  COUNT:
+// kRsCountNext adds the next digit's counts into nextCounts, kRsCountGroups
+// the group counts into groupCnt.
+template <int RB, int COUNT>
+__global__ __launch_bounds__(kRsThreads, 4) void k_rs_onesweep(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                                                            const uint64_t* dn, uint64_t cap, int shift,
+                                                            const uint32_t* __restrict__ counts,
+                                                            uint32_t* __restrict__ nextCounts, uint32_t* status,
+                                                            uint32_t* __restrict__ statusNext, uint32_t* ticket,
+                                                            uint64_t outMask, uint64_t* __restrict__ groupCnt,
+                                                            int groupShift, uint64_t nGroups) {
     constexpr int kBins = 1 << RB;
-    constexpr int kWaves = kRsThreads / kWave;
     constexpr uint32_t kPerWave = static_cast<uint32_t>(kRsPer) * kWave;
-    __shared__ uint32_t lstart[kBins];
-    __shared__ uint32_t wc[kWaves][kBins];
-    __shared__ uint32_t wsum[kWaves];
-    __shared__ uint64_t sk[kRsTile];
+    constexpr int kSide = COUNT == kRsCountNext ? kBins : (COUNT == kRsCountGroups ? kRsGroupLds : 1);
+    __shared__ uint32_t wc[kRsWaves][kBins];  // wave counters, then wave output bases
+    __shared__ uint32_t side[kSide];          // next-digit or group counts of this workgroup
+    __shared__ uint32_t wsum[kRsWaves];
+    __shared__ uint32_t s_tile;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const bool owner = tid < kBins;  // thread tid owns digit tid
+    const bool ldsGroups = COUNT == kRsCountGroups && nGroups <= kRsGroupLds;
     uint64_t n = *dn;
     if (n > cap) n = cap;
     const uint64_t ntiles = (n + kRsTile - 1) / kRsTile;
-    const uint64_t ltmask = (1ull << lane) - 1ull;
     uint32_t* mycnt = wc[wid];
-    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    if (COUNT != kRsCountNone)
+        for (int i = tid; i < kSide; i += kRsThreads) side[i] = 0;
+    // global digit starts (exclusive scan of the digit counts)
+    const uint32_t gst = bins_excl_scan<kBins>(owner ? counts[tid] : 0u, wsum);
+    for (;;) {
+        if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+        __syncthreads();
+        const uint64_t t = s_tile;
+        if (t >= ntiles) break;
         const uint64_t t0 = t * kRsTile;
         const uint32_t cnt = static_cast<uint32_t>(n - t0 < static_cast<uint64_t>(kRsTile) ? n - t0 : kRsTile);
         const uint32_t e0 = static_cast<uint32_t>(wid) * kPerWave + static_cast<uint32_t>(lane);
@@ -719,104 +751,133 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* __res
             key[r] = e < cnt ? in[t0 + e] : 0;
         }
 #pragma unroll
-        for (int i = 0; i < kBins / kWave; ++i) mycnt[lane + i * kWave] = 0;
+        for (int i = 0; i < (kBins + kWave - 1) / kWave; ++i)
+            if (lane + i * kWave < kBins) mycnt[lane + i * kWave] = 0;
         wave_lds_sync();
 #pragma unroll
         for (int r = 0; r < kRsPer; ++r) {
             const bool ok = e0 + static_cast<uint32_t>(r) * kWave < cnt;
             const uint32_t dig = ok ? static_cast<uint32_t>((key[r] >> shift) & (kBins - 1)) : 0;
-            uint64_t peers = __ballot(ok);
-#pragma unroll
-            for (int b = 0; b < RB; ++b) {
-                const uint64_t bal = __ballot(((dig >> b) & 1u) != 0);
-                peers &= ((dig >> b) & 1u) ? bal : ~bal;
-            }
-            const uint32_t below = static_cast<uint32_t>(__popcll(peers & ltmask));
+            uint32_t plo, phi;
+            digit_peers<RB>(dig, __ballot(ok), plo, phi);
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
             const uint32_t pre = ok ? mycnt[dig] : 0;
             rank[r] = pre + below;
             wave_lds_sync();
-            if (ok && below == 0) mycnt[dig] = pre + static_cast<uint32_t>(__popcll(peers));
+            if (ok && below == 0) mycnt[dig] = pre + static_cast<uint32_t>(__popc(plo) + __popc(phi));
             wave_lds_sync();
+            if (COUNT == kRsCountNext && ok)
+                atomicAdd(&side[(key[r] >> (shift + RB)) & (kBins - 1)], 1u);
+            if (COUNT == kRsCountGroups && ok) {
+                const uint64_t g = key[r] >> groupShift;
+                if (ldsGroups) atomicAdd(&side[g], 1u);
+                else atomicAdd(reinterpret_cast<unsigned long long*>(groupCnt + g), 1ull);
+            }
         }
         __syncthreads();
-        // digit starts in the tile (exclusive scan of the digit totals, kRsThreads
-        // digits per step), then each wave's start within every digit
-        uint32_t carry = 0;
+        uint32_t c[kRsWaves];
+        uint32_t v = 0;
+        if (owner) {
 #pragma unroll
-        for (int h = 0; h < kBins / kRsThreads; ++h) {
-            const int d = h * kRsThreads + tid;
-            uint32_t c[kWaves];
-            uint32_t v = 0;
-#pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                c[w] = wc[w][d];
+            for (int w = 0; w < kRsWaves; ++w) {
+                c[w] = wc[w][tid];
                 v += c[w];
             }
-            uint32_t x = v;
-            for (int o = 1; o < kWave; o <<= 1) {
-                const uint32_t y = __shfl_up(x, o, kWave);
-                if (lane >= o) x += y;
-            }
-            if (lane == kWave - 1) wsum[wid] = x;
-            __syncthreads();
-            uint32_t p = carry + x - v;
-            uint32_t all = 0;
-            for (int w = 0; w < kWaves; ++w) {
-                if (w < wid) p += wsum[w];
-                all += wsum[w];
-            }
-            lstart[d] = p;
+            st_agent_u32(status + t * kBins + tid, (t == 0 ? kRsInc : kRsAgg) | v);
+            if (statusNext) statusNext[t * kBins + tid] = 0;
+        }
+        // look-back: the digit's exclusive prefix over the earlier tiles,
+        // reading kRsProbe predecessors per round and summing aggregates up to
+        // the nearest inclusive prefix; a predecessor that has not published
+        // yet is read again in the next round
+        uint32_t pre = 0;
+        if (owner && t > 0) {
+            int64_t next = static_cast<int64_t>(t) - 1;
+            for (;;) {
+                uint32_t sw[kRsProbe];
 #pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                wc[w][d] = p;
+                for (int k = 0; k < kRsProbe; ++k) {
+                    const int64_t j = next - k;
+                    sw[k] = j >= 0 ? ld_agent_u32(status + static_cast<uint64_t>(j) * kBins + tid) : kRsInc;
+                }
+                bool stop = false, done = false;
+                int used = 0;
+#pragma unroll
+                for (int k = 0; k < kRsProbe; ++k) {
+                    if (!stop) {
+                        const uint32_t f = sw[k] & ~kRsVal;
+                        if (f == 0) {
+                            stop = true;
+                        } else {
+                            pre += sw[k] & kRsVal;
+                            ++used;
+                            if (f & kRsInc) {
+                                stop = true;
+                                done = true;
+                            }
+                        }
+                    }
+                }
+                if (done) break;
+                next -= used;
+                if (used < kRsProbe) __builtin_amdgcn_s_sleep(1);
+            }
+            st_agent_u32(status + t * kBins + tid, kRsInc | (pre + v));
+        }
+        if (owner) {
+            uint32_t p = gst + pre;
+#pragma unroll
+            for (int w = 0; w < kRsWaves; ++w) {
+                wc[w][tid] = p;
                 p += c[w];
             }
-            carry += all;
-            __syncthreads();
         }
+        __syncthreads();
 #pragma unroll
         for (int r = 0; r < kRsPer; ++r) {
             const uint32_t e = e0 + static_cast<uint32_t>(r) * kWave;
-            if (e < cnt) sk[mycnt[(key[r] >> shift) & (kBins - 1)] + rank[r]] = key[r];
-        }
-        __syncthreads();
-        for (uint32_t i = tid; i < cnt; i += kRsThreads) {
-            const uint64_t k = sk[i];
-            const uint32_t dig = static_cast<uint32_t>((k >> shift) & (kBins - 1));
-            const uint64_t dst = hist[static_cast<uint64_t>(dig) * ntiles + t] + (i - lstart[dig]);
-            out[dst] = extract ? (k & idMask) : k;
+            if (e < cnt) out[mycnt[(key[r] >> shift) & (kBins - 1)] + rank[r]] = key[r] & outMask;
         }
         __syncthreads();
     }
+    if (COUNT == kRsCountNext)
+        for (int d = tid; d < kBins; d += kRsThreads)
+            if (side[d]) atomicAdd(&nextCounts[d], side[d]);
+    if (COUNT == kRsCountGroups && ldsGroups)
+        for (int i = tid; i < static_cast<int>(nGroups); i += kRsThreads)
+            if (side[i]) atomicAdd(reinterpret_cast<unsigned long long*>(groupCnt + i),
+                                   static_cast<unsigned long long>(side[i]));
 }
 
-__global__ void k_query_offsets(const uint64_t* __restrict__ keys, const uint64_t* dn, uint64_t cap, uint64_t nq,
-                                int idBits, uint64_t* __restrict__ outOff) {
-    const uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (q > nq) return;
-    uint64_t n = *dn;
-    if (n > cap) n = cap;
-    if (q == nq) {
-        outOff[q] = n;
-        return;
+// off[0..n] = exclusive scan of cnt[0..n) (off[n] = total), one workgroup.
+constexpr int kGsThreads = 1024;
+__global__ __launch_bounds__(kGsThreads) void k_rs_group_scan(const uint64_t* __restrict__ cnt, uint64_t n,
+                                                              uint64_t* __restrict__ off) {
+    __shared__ uint64_t part[kGsThreads / kWave];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const uint64_t per = (n + kGsThreads - 1) / kGsThreads;
+    const uint64_t a = static_cast<uint64_t>(tid) * per;
+    const uint64_t b = a + per < n ? a + per : n;
+    uint64_t sum = 0;
+    for (uint64_t i = a; i < b; ++i) sum += cnt[i];
+    uint64_t x = sum;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint64_t y = __shfl_up(x, o, kWave);
+        if (lane >= o) x += y;
     }
-    const uint64_t target = q << idBits;
-    uint64_t lo = 0, hi = n;
-    while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        if (keys[mid] < target) lo = mid + 1;
-        else hi = mid;
+    if (lane == kWave - 1) part[wid] = x;
+    __syncthreads();
+    uint64_t p = x - sum, total = 0;
+    for (int w = 0; w < kGsThreads / kWave; ++w) {
+        if (w < wid) p += part[w];
+        total += part[w];
     }
-    outOff[q] = lo;
-}
-
-__global__ void k_extract_ids(const uint64_t* __restrict__ keys, const uint64_t* dn, uint64_t cap, uint64_t mask,
-                              uint64_t* __restrict__ out) {
-    uint64_t n = *dn;
-    if (n > cap) n = cap;
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
-         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
-        out[i] = keys[i] & mask;
+    for (uint64_t i = a; i < b; ++i) {
+        const uint64_t c = cnt[i];
+        off[i] = p;
+        p += c;
+    }
+    if (tid == 0) off[n] = total;
 }
 
 // status[0] = hits, status[1] = overflow flags (1 pairs, 2 items, 4 hits)
@@ -834,57 +895,95 @@ int scan_launch_grid(tblaeq_ctx* ctx, uint64_t tiles) {
     return static_cast<int>(tiles < cap ? (tiles ? tiles : 1) : cap);
 }
 
-// Histogram words radix_sort needs for up to cap keys (digits of up to 9 bits).
-uint64_t radix_hist_words(uint64_t cap) { return 512 * ((cap + kRsTile - 1) / kRsTile + 1); }
+// Scratch words (u64) radix_sort needs for up to cap keys and `groups` groups.
+uint64_t radix_hist_words(uint64_t cap, uint64_t groups) {
+    const uint64_t tiles = (cap + kRsTile - 1) / kRsTile;
+    return (kRsHdrWords + 2 * tiles * kRsMaxBins + 1) / 2 + groups;
+}
 
-// LSD radix sort of `dn` keys (device count, at most cap) on bits
-// [firstBit, firstBit + nbits), stable: 9-bit digits when they take fewer
-// passes than 8-bit ones, else 8-bit.  Ping-pongs between a and b; returns the
-// buffer holding the result.  The last pass can write key & idMask instead of
-// the key.
-int radix_sort(tblaeq_ctx* ctx, uint64_t* a, uint64_t* b, const uint64_t* dn, uint64_t cap, int firstBit, int nbits,
-               uint64_t* hist, uint64_t** result, bool extractLast, uint64_t idMask, uint64_t blocksPerCu) {
+template <int RB, int COUNT>
+static void launch_onesweep(tblaeq_ctx* ctx, const uint64_t* src, uint64_t* dst, const uint64_t* dn, uint64_t cap,
+                            int shift, const uint32_t* counts, uint32_t* nextCounts, uint32_t* status,
+                            uint32_t* statusNext, uint32_t* ticket, uint64_t outMask, uint64_t* groupCnt,
+                            int groupShift, uint64_t nGroups) {
     const uint64_t capTiles = (cap + kRsTile - 1) / kRsTile;
+    const uint64_t scap = static_cast<uint64_t>(ctx->cuCount) *
+                          resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<RB, COUNT>), kRsThreads);
+    const int grid = static_cast<int>(capTiles < scap ? (capTiles ? capTiles : 1) : scap);
+    hipLaunchKernelGGL((k_rs_onesweep<RB, COUNT>), dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dst, dn, cap,
+                       shift, counts, nextCounts, status, statusNext, ticket, outMask, groupCnt, groupShift, nGroups);
+}
+
+template <int RB>
+static int radix_passes(tblaeq_ctx* ctx, uint64_t* a, uint64_t* b, const uint64_t* dn, uint64_t cap, int firstBit,
+                        int passes, uint64_t* scratch, uint64_t** result, const RadixOut& ro, uint64_t blocksPerCu) {
+    constexpr int kBins = 1 << RB;
+    const uint64_t capTiles = (cap + kRsTile - 1) / kRsTile;
+    uint32_t* hdr = reinterpret_cast<uint32_t*>(scratch);
+    uint32_t* counts = hdr + kRsCounts0;
+    uint32_t* region[2] = {hdr + kRsHdrWords, hdr + kRsHdrWords + capTiles * kRsMaxBins};
+    uint64_t* groupCnt = ro.groupOff ? scratch + (kRsHdrWords + 2 * capTiles * kRsMaxBins + 1) / 2 : nullptr;
     const uint64_t gcap = static_cast<uint64_t>(ctx->cuCount) * blocksPerCu;
-    const int grid = static_cast<int>(capTiles < gcap ? (capTiles ? capTiles : 1) : gcap);
-    const int rb = (nbits + 8) / 9 < (nbits + 7) / 8 ? 9 : 8;
-    const int bins = 1 << rb;
-    const uint64_t histTiles =
-        (static_cast<uint64_t>(bins) * (capTiles ? capTiles : 1) + kScanThreads * kHsPer - 1) / (kScanThreads * kHsPer);
-    const int passes = (nbits + rb - 1) / rb;
+    const int gridC = static_cast<int>(capTiles < gcap ? (capTiles ? capTiles : 1) : gcap);
+    TimedScope ts(ctx, kKBatch0 + 4);
+    hipLaunchKernelGGL(k_rs_begin, dim3(16), dim3(256), 0, ctx->stream, hdr, groupCnt, ro.nGroups);
+    hipLaunchKernelGGL(k_rs_counts<RB>, dim3(gridC), dim3(kRsThreads), 0, ctx->stream, a, dn, cap, firstBit, counts,
+                       region[0]);
     uint64_t* src = a;
     uint64_t* dst = b;
     for (int p = 0; p < passes; ++p) {
-        const int shift = firstBit + rb * p;
-        {
-            TimedScope ts(ctx, kKBatch0 + 4);
-            if (rb == 9)
-                hipLaunchKernelGGL(k_rs_hist<9>, dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dn, cap, shift, hist);
-            else
-                hipLaunchKernelGGL(k_rs_hist<8>, dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dn, cap, shift, hist);
+        const bool last = p == passes - 1;
+        uint64_t* o = (last && ro.ids) ? ro.ids : dst;
+        uint32_t* cp = counts + static_cast<uint64_t>(p) * kBins;
+        uint32_t* cn = last ? nullptr : cp + kBins;
+        uint32_t* sn = last ? nullptr : region[(p + 1) & 1];
+        const int shift = firstBit + RB * p;
+        if (!last)
+            launch_onesweep<RB, kRsCountNext>(ctx, src, o, dn, cap, shift, cp, cn, region[p & 1], sn, hdr + p, ~0ull,
+                                              nullptr, 0, 0);
+        else if (ro.groupOff)
+            launch_onesweep<RB, kRsCountGroups>(ctx, src, o, dn, cap, shift, cp, nullptr, region[p & 1], nullptr,
+                                                hdr + p, ro.idMask, groupCnt, ro.groupShift, ro.nGroups);
+        else
+            launch_onesweep<RB, kRsCountNone>(ctx, src, o, dn, cap, shift, cp, nullptr, region[p & 1], nullptr,
+                                              hdr + p, ro.idMask, nullptr, 0, 0);
+        src = o;
+        dst = (src == a) ? b : a;
+    }
+    TB_HIP(hipGetLastError());
+    if (ro.groupOff) {
+        // group starts = exclusive scan of the group counts (total at [nGroups])
+        if (ro.nGroups <= (1u << 20)) {
+            hipLaunchKernelGGL(k_rs_group_scan, dim3(1), dim3(kGsThreads), 0, ctx->stream, groupCnt, ro.nGroups,
+                               ro.groupOff);
+            TB_HIP(hipGetLastError());
+        } else {
+            int rc = scan_counts(ctx, groupCnt, ro.nGroups, ro.groupOff, reinterpret_cast<uint64_t*>(hdr + 32));
+            if (rc) return rc;
         }
-        int rc = scan_prepare(ctx, histTiles);
-        if (rc) return rc;
-        {
-            TimedScope ts(ctx, kKBatch0 + 4);
-            hipLaunchKernelGGL(k_scan_hist, dim3(scan_launch_grid(ctx, histTiles)), dim3(kScanThreads), 0,
-                               ctx->stream, hist, dn, cap, bins, ctx->status, ctx->ctrl);
-        }
-        {
-            TimedScope ts(ctx, kKBatch0 + 4);
-            const int ex = (extractLast && p == passes - 1) ? 1 : 0;
-            if (rb == 9)
-                hipLaunchKernelGGL(k_rs_scatter<9>, dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dst, dn, cap,
-                                   shift, hist, idMask, ex);
-            else
-                hipLaunchKernelGGL(k_rs_scatter<8>, dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dst, dn, cap,
-                                   shift, hist, idMask, ex);
-        }
-        TB_HIP(hipGetLastError());
-        std::swap(src, dst);
     }
     *result = src;
     return TBLAEQ_OK;
+}
+
+// Stable LSD radix sort of `dn` keys (device count, at most cap < 2^30) on bits
+// [firstBit, firstBit + nbits): 9-bit digits when they take fewer passes than
+// 8-bit ones, else 8-bit.  Ping-pongs between a and b; *result = the buffer
+// holding the result (ro.ids when given).
+int radix_sort(tblaeq_ctx* ctx, uint64_t* a, uint64_t* b, const uint64_t* dn, uint64_t cap, int firstBit, int nbits,
+               uint64_t* hist, uint64_t** result, const RadixOut& ro, uint64_t blocksPerCu) {
+    if (cap > kRsVal) {
+        set_error("radix sort: more than 2^30 - 1 keys");
+        return TBLAEQ_ERR_CAPACITY;
+    }
+    if (nbits < 1 || firstBit < 0 || firstBit + nbits > 64) {
+        set_error("radix sort: key bits out of range");
+        return TBLAEQ_ERR_INVALID;
+    }
+    const int rb = (nbits + 8) / 9 < (nbits + 7) / 8 ? 9 : 8;
+    const int passes = (nbits + rb - 1) / rb;
+    if (rb == 9) return radix_passes<9>(ctx, a, b, dn, cap, firstBit, passes, hist, result, ro, blocksPerCu);
+    return radix_passes<8>(ctx, a, b, dn, cap, firstBit, passes, hist, result, ro, blocksPerCu);
 }
 
 int resident_blocks(const void* kernel, int threads) {
@@ -938,12 +1037,18 @@ void launch_range_dm(tblaeq_ctx* ctx, const RangeStepArgs& A, dim3 g, dim3 b) {
     hipLaunchKernelGGL(k, g, b, 0, ctx->stream, A);
 }
 
-// Queries the occupancy of this engine's final-filter kernels (outside any capture).
+// Queries the occupancy of this engine's final-filter and radix kernels (outside any capture).
 void warm_range_kernels(tblaeq_batch* b) {
     const int t = kStepWaves * kWave;
     resident_blocks(reinterpret_cast<const void*>(range_kernel<kFinal, 256>(b->D)), t);
     resident_blocks(reinterpret_cast<const void*>(range_kernel<kFinal, 512>(b->D)), t);
     resident_blocks(reinterpret_cast<const void*>(range_kernel<kFinal, 1024>(b->D)), t);
+    resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<8, kRsCountNext>), kRsThreads);
+    resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<8, kRsCountNone>), kRsThreads);
+    resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<8, kRsCountGroups>), kRsThreads);
+    resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<9, kRsCountNext>), kRsThreads);
+    resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<9, kRsCountNone>), kRsThreads);
+    resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<9, kRsCountGroups>), kRsThreads);
 }
 
 template <int MODE>
@@ -1227,12 +1332,14 @@ extern "C" int tblaeq_batch_reserve(tblaeq_batch* b, uint64_t max_queries, uint6
         if (rc) return rc;
         b->recAux = balloc<uint64_t>(b, hit_cap, &rc);
         if (rc) return rc;
-        b->histCap = radix_hist_words(hit_cap);
-        b->hist = balloc<uint64_t>(b, b->histCap, &rc);
-        if (rc) return rc;
         b->hitCap = hit_cap;
     }
     b->maxQ = std::max(b->maxQ, max_queries);
+    if (radix_hist_words(b->hitCap, b->maxQ) > b->histCap) {
+        b->histCap = radix_hist_words(b->hitCap, b->maxQ);
+        b->hist = balloc<uint64_t>(b, b->histCap, &rc);
+        if (rc) return rc;
+    }
     // size the look-back status array for every scan a batch runs, so that a
     // captured batch never allocates
     const uint64_t histTiles = (b->histCap + kScanThreads - 1) / kScanThreads;
@@ -1344,19 +1451,18 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     A.queue = b->queue;
     A.masks = nullptr;
     if ((rc = launch_range_step<kFinal>(b, A, b->itemCap))) return rc;
+    // sort the (query, id) keys; the last pass writes the ids to the caller's
+    // buffer and the first position of every query to d_out_off
     uint64_t* sorted = nullptr;
-    if ((rc = radix_sort(ctx, b->hits, b->hitsTmp, ctl + 2, A.cap, 0, idBits + qBits, b->hist, &sorted, false, 0,
+    RadixOut ro;
+    ro.ids = d_out_ids;
+    ro.idMask = (idBits >= 64) ? ~0ull : ((1ull << idBits) - 1);
+    ro.groupOff = d_out_off;
+    ro.groupShift = idBits;
+    ro.nGroups = nq;
+    if ((rc = radix_sort(ctx, b->hits, b->hitsTmp, ctl + 2, A.cap, 0, idBits + qBits, b->hist, &sorted, ro,
                          b->radixBlocksPerCu)))
         return rc;
-    hipLaunchKernelGGL(k_query_offsets, dim3(static_cast<unsigned>((nq + 1 + 255) / 256)), dim3(256), 0, ctx->stream,
-                       sorted, ctl + 2, A.cap, nq, idBits, d_out_off);
-    // ids = key & mask, written to the caller's buffer
-    {
-        const uint64_t mask = (idBits >= 64) ? ~0ull : ((1ull << idBits) - 1);
-        const uint64_t g = std::min<uint64_t>((A.cap + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 8);
-        hipLaunchKernelGGL(k_extract_ids, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream, sorted,
-                           ctl + 2, A.cap, mask, d_out_ids);
-    }
     hipLaunchKernelGGL(k_copy_status, dim3(1), dim3(64), 0, ctx->stream, ctl, d_status);
     TB_HIP(hipGetLastError());
     return TBLAEQ_OK;

@@ -510,7 +510,7 @@ extern "C" int tblaeq_kmeans(tblaeq_ctx* ctx, const double* d_x, uint64_t n, uin
     uint64_t* keysB = S.get<uint64_t>(n, &rc);
     uint64_t* cnt = S.get<uint64_t>(k, &rc);
     uint64_t* off = S.get<uint64_t>(k + 1, &rc);
-    uint64_t* hist = S.get<uint64_t>(radix_hist_words(n), &rc);
+    uint64_t* hist = S.get<uint64_t>(radix_hist_words(n, 0), &rc);
     uint64_t* dn = S.get<uint64_t>(1, &rc);
     if (rc) return rc;
     // data bounds for the centring (per dimension min / max on the host side of a small copy)
@@ -593,7 +593,7 @@ extern "C" int tblaeq_kmeans(tblaeq_ctx* ctx, const double* d_x, uint64_t n, uin
         hipLaunchKernelGGL(k_km_zero, dim3(grid_for(ctx, k, 256)), dim3(256), 0, st, cnt, k);
         hipLaunchKernelGGL(k_km_keys, dim3(grid_for(ctx, n, 256)), dim3(256), 0, st, d_labels, n, idxBits, keysA, cnt);
         uint64_t* sorted = nullptr;
-        if ((rc = radix_sort(ctx, keysA, keysB, dn, n, idxBits, labBits, hist, &sorted, false, 0, 8))) break;
+        if ((rc = radix_sort(ctx, keysA, keysB, dn, n, idxBits, labBits, hist, &sorted, RadixOut(), 8))) break;
         if ((rc = scan_counts(ctx, cnt, k, off, ctl + 3))) break;
         hipLaunchKernelGGL(k_km_update, dim3(grid_for(ctx, k * D, 256)), dim3(256), 0, st, d_x, static_cast<int>(D),
                            sorted, idxMask, off, k, d_centers);
@@ -638,7 +638,7 @@ extern "C" int tblaeq_build_level(tblaeq_ctx* ctx, const double* d_fine, uint64_
     Scratch S;
     uint64_t* keysA = S.get<uint64_t>(n, &rc);
     uint64_t* keysB = S.get<uint64_t>(n, &rc);
-    uint64_t* hist = S.get<uint64_t>(radix_hist_words(n), &rc);
+    uint64_t* hist = S.get<uint64_t>(radix_hist_words(n, 0), &rc);
     uint64_t* dn = S.get<uint64_t>(1, &rc);
     uint64_t* r2 = S.get<uint64_t>(k, &rc);
     uint64_t* scratch = S.get<uint64_t>(2, &rc);
@@ -650,7 +650,7 @@ extern "C" int tblaeq_build_level(tblaeq_ctx* ctx, const double* d_fine, uint64_
     const uint64_t hn = n;
     TB_HIP(hipMemcpyAsync(dn, &hn, sizeof(uint64_t), hipMemcpyHostToDevice, st));
     uint64_t* sorted = nullptr;
-    if ((rc = radix_sort(ctx, keysA, keysB, dn, n, idxBits, labBits, hist, &sorted, false, 0, 8))) return rc;
+    if ((rc = radix_sort(ctx, keysA, keysB, dn, n, idxBits, labBits, hist, &sorted, RadixOut(), 8))) return rc;
     if ((rc = scan_counts(ctx, d_nnzPerCol, k, d_colRes, scratch))) return rc;
     hipLaunchKernelGGL(k_csc_rows, dim3(grid_for(ctx, n, 256)), dim3(256), 0, st, d_fine, n, static_cast<int>(D),
                        d_centers, sorted, idxBits, d_map, d_vals, r2);
