@@ -77,6 +77,13 @@ constexpr int kSliceBits = 20;
 constexpr int kQueryBits = 24;
 constexpr int kStepWaves = 4;  // waves per workgroup of the cluster-major kernels
 
+// chunked counting scatters: kAggPer elements per thread, targets ranked in
+// LDS when there are at most kAggLds of them
+constexpr int kAggThreads = 256;
+constexpr int kAggPer = 16;
+constexpr int kAggChunk = kAggThreads * kAggPer;
+constexpr int kAggLds = 8192;
+
 constexpr int kCtlWords = 16;
 constexpr int kQueueSlots = 64;
 constexpr int kMaskQueue0 = 8;  // queue slots kMaskQueue0 + l: keep-mask rows reserved at level l

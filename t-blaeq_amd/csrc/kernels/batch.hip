@@ -566,20 +566,64 @@ __global__ __launch_bounds__(kScanThreads) void k_group_scan(const uint64_t* __r
     if (tid == 0) scan_exit(ctrl);
 }
 
-// Counting-sort scatter of (id, q) pairs into grouped query lists; leaves cnt zeroed.
-__global__ void k_group_scatter(const uint64_t* __restrict__ pairs, const uint64_t* dn, uint64_t cap, uint64_t mesh,
-                                uint64_t* __restrict__ cnt, const uint64_t* __restrict__ off,
-                                uint32_t* __restrict__ qlist) {
+// Counting-sort scatter of (id, q) pairs into grouped query lists; leaves cnt
+// zeroed.  A workgroup takes chunks of kAggChunk pairs; with mesh <= kAggLds it
+// ranks the chunk's pairs per id in LDS and reserves each id's run with one
+// global atomic (a run's slots count down from cnt[id]), else one atomic per
+// pair.  The order inside a list is not fixed.
+__global__ __launch_bounds__(kAggThreads) void k_group_scatter(const uint64_t* __restrict__ pairs, const uint64_t* dn,
+                                                               uint64_t cap, uint64_t mesh,
+                                                               uint64_t* __restrict__ cnt,
+                                                               const uint64_t* __restrict__ off,
+                                                               uint32_t* __restrict__ qlist) {
+    __shared__ uint32_t lc[kAggLds];
+    const int tid = threadIdx.x;
     uint64_t n = *dn;
     if (n > cap) n = cap;
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
-         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        const uint64_t p = pairs[i];
-        const uint64_t id = p >> kQueryBits;
-        if (id >= mesh) continue;  // sentinel of an overflowed allocation
-        const uint32_t q = static_cast<uint32_t>(p & ((1ull << kQueryBits) - 1));
-        const uint64_t pos = atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[id]), ~0ull) - 1;
-        qlist[off[id] + pos] = q;
+    if (mesh > kAggLds) {
+        for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + tid; i < n;
+             i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+            const uint64_t p = pairs[i];
+            const uint64_t id = p >> kQueryBits;
+            if (id >= mesh) continue;  // sentinel of an overflowed allocation
+            const uint32_t q = static_cast<uint32_t>(p & ((1ull << kQueryBits) - 1));
+            const uint64_t pos = atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[id]), ~0ull) - 1;
+            qlist[off[id] + pos] = q;
+        }
+        return;
+    }
+    for (uint64_t i = tid; i < mesh; i += kAggThreads) lc[i] = 0;
+    __syncthreads();
+    const uint64_t chunks = (n + kAggChunk - 1) / kAggChunk;
+    for (uint64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        uint64_t p[kAggPer];
+        uint32_t rank[kAggPer];
+#pragma unroll
+        for (int r = 0; r < kAggPer; ++r) {
+            const uint64_t i = c * kAggChunk + static_cast<uint64_t>(r) * kAggThreads + tid;
+            p[r] = i < n ? pairs[i] : ~0ull;
+            const uint64_t id = p[r] >> kQueryBits;
+            rank[r] = id < mesh ? atomicAdd(&lc[id], 1u) : 0;
+        }
+        __syncthreads();
+        for (uint64_t id = tid; id < mesh; id += kAggThreads) {
+            const uint32_t k = lc[id];
+            if (k) lc[id] = static_cast<uint32_t>(
+                       atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[id]), 0ull - k) - k);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kAggPer; ++r) {
+            const uint64_t id = p[r] >> kQueryBits;
+            if (id < mesh) qlist[off[id] + lc[id] + rank[r]] = static_cast<uint32_t>(p[r] & ((1ull << kQueryBits) - 1));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kAggPer; ++r) {
+            const uint64_t id = p[r] >> kQueryBits;
+            if (id < mesh) lc[id] = 0;
+        }
+        __syncthreads();
     }
 }
 
@@ -1111,8 +1155,9 @@ int launch_grouping(tblaeq_batch* b, uint64_t mesh) {
     }
     {
         TimedScope ts(ctx, kKBatch0 + 3);
-        const uint64_t g = std::min<uint64_t>((b->pairCap + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 8);
-        hipLaunchKernelGGL(k_group_scatter, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream,
+        const uint64_t g = std::min<uint64_t>((b->pairCap + kAggChunk - 1) / kAggChunk,
+                                              static_cast<uint64_t>(ctx->cuCount) * 4);
+        hipLaunchKernelGGL(k_group_scatter, dim3(static_cast<unsigned>(g ? g : 1)), dim3(kAggThreads), 0, ctx->stream,
                            b->pairs, ctl + 0, b->pairCap, mesh, b->cnt, b->off, b->qlist);
     }
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, ctx->stream, ctl, 0ull, 1);

@@ -1094,20 +1094,68 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_cluster(KnnClusterAr
     (void)wRows;
 }
 
-// Counting-sort scatter of the records into per-query segments; leaves cntQ zeroed.
-__global__ void k_rec_scatter(const uint64_t* __restrict__ recK, const uint64_t* __restrict__ recV, const uint64_t* dn,
-                              uint64_t cap, uint64_t* __restrict__ cntQ, const uint64_t* __restrict__ offQ,
-                              uint64_t* __restrict__ gK, uint64_t* __restrict__ gV) {
+// Counting-sort scatter of the records into per-query segments; leaves cntQ
+// zeroed.  Chunks of kAggChunk records; with nq <= kAggLds a chunk's records are
+// ranked per query in LDS and each query's run is reserved with one global
+// atomic (slots count down from cntQ[q]), else one atomic per record.
+__global__ __launch_bounds__(kAggThreads) void k_rec_scatter(const uint64_t* __restrict__ recK,
+                                                             const uint64_t* __restrict__ recV, const uint64_t* dn,
+                                                             uint64_t cap, uint64_t nq, uint64_t* __restrict__ cntQ,
+                                                             const uint64_t* __restrict__ offQ,
+                                                             uint64_t* __restrict__ gK, uint64_t* __restrict__ gV) {
+    __shared__ uint32_t lc[kAggLds];
+    const int tid = threadIdx.x;
     uint64_t n = *dn;
     if (n > cap) n = cap;
     const uint64_t idMask = (1ull << kRecIdBits) - 1;
-    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
-         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        const uint64_t v = recV[i];
-        const uint64_t q = v >> kRecIdBits;
-        const uint64_t pos = atomicAdd(reinterpret_cast<unsigned long long*>(&cntQ[q]), ~0ull) - 1;
-        gK[offQ[q] + pos] = recK[i];
-        gV[offQ[q] + pos] = v & idMask;
+    if (nq > kAggLds) {
+        for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + tid; i < n;
+             i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+            const uint64_t v = recV[i];
+            const uint64_t q = v >> kRecIdBits;
+            const uint64_t pos = atomicAdd(reinterpret_cast<unsigned long long*>(&cntQ[q]), ~0ull) - 1;
+            gK[offQ[q] + pos] = recK[i];
+            gV[offQ[q] + pos] = v & idMask;
+        }
+        return;
+    }
+    for (uint64_t i = tid; i < nq; i += kAggThreads) lc[i] = 0;
+    __syncthreads();
+    const uint64_t chunks = (n + kAggChunk - 1) / kAggChunk;
+    for (uint64_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        uint64_t v[kAggPer];
+        uint32_t rank[kAggPer];
+#pragma unroll
+        for (int r = 0; r < kAggPer; ++r) {
+            const uint64_t i = c * kAggChunk + static_cast<uint64_t>(r) * kAggThreads + tid;
+            v[r] = i < n ? recV[i] : ~0ull;
+            const uint64_t q = v[r] >> kRecIdBits;
+            rank[r] = q < nq ? atomicAdd(&lc[q], 1u) : 0;
+        }
+        __syncthreads();
+        for (uint64_t q = tid; q < nq; q += kAggThreads) {
+            const uint32_t k = lc[q];
+            if (k) lc[q] = static_cast<uint32_t>(
+                       atomicAdd(reinterpret_cast<unsigned long long*>(&cntQ[q]), 0ull - k) - k);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kAggPer; ++r) {
+            const uint64_t q = v[r] >> kRecIdBits;
+            if (q < nq) {
+                const uint64_t i = c * kAggChunk + static_cast<uint64_t>(r) * kAggThreads + tid;
+                const uint64_t pos = offQ[q] + lc[q] + rank[r];
+                gK[pos] = recK[i];
+                gV[pos] = v[r] & idMask;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kAggPer; ++r) {
+            const uint64_t q = v[r] >> kRecIdBits;
+            if (q < nq) lc[q] = 0;
+        }
+        __syncthreads();
     }
 }
 
@@ -1481,9 +1529,11 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
         TB_HIP(hipGetLastError());
         if ((rc = launch_count_scan(b, cntQ, nq, offQ))) return rc;
         {
-            const uint64_t g = std::min<uint64_t>((recCap + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 8);
-            hipLaunchKernelGGL(k_rec_scatter, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream,
-                               b->hits, b->hitsTmp, ctl + 2, recCap, cntQ, offQ, b->pairs, b->recAux);
+            const uint64_t g = std::min<uint64_t>((recCap + kAggChunk - 1) / kAggChunk,
+                                                  static_cast<uint64_t>(ctx->cuCount) * 4);
+            hipLaunchKernelGGL(k_rec_scatter, dim3(static_cast<unsigned>(g ? g : 1)), dim3(kAggThreads), 0,
+                               ctx->stream, b->hits, b->hitsTmp, ctl + 2, recCap, nq, cntQ, offQ, b->pairs,
+                               b->recAux);
         }
         launch_fill(b, ctl + 2, 0, 1);
         KnnSelectArgs SA{};
@@ -1572,9 +1622,11 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
         if ((rc = launch_count_scan(b, cntQ, nq, offQ))) return rc;
         {
             TimedScope ts(ctx, kKBatch0 + 8);
-            const uint64_t g = std::min<uint64_t>((recCap + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 8);
-            hipLaunchKernelGGL(k_rec_scatter, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream,
-                               b->hits, b->hitsTmp, ctl + 2, recCap, cntQ, offQ, b->pairs, b->recAux);
+            const uint64_t g = std::min<uint64_t>((recCap + kAggChunk - 1) / kAggChunk,
+                                                  static_cast<uint64_t>(ctx->cuCount) * 4);
+            hipLaunchKernelGGL(k_rec_scatter, dim3(static_cast<unsigned>(g ? g : 1)), dim3(kAggThreads), 0,
+                               ctx->stream, b->hits, b->hitsTmp, ctl + 2, recCap, nq, cntQ, offQ, b->pairs,
+                               b->recAux);
             KnnMergeArgs M{};
             M.nq = nq;
             M.Ks = d_K;
