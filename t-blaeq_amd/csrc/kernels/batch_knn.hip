@@ -1013,6 +1013,38 @@ struct KnnClusterArgs {
     uint64_t* work;           // distance evaluations (child rows x queries)
 };
 
+// One row test of the refine: acc against the lane's threshold (tD as a double;
+// keys only for ties and NaNs); emitting lanes append (okey(acc), q, *idp)
+// records with one reservation per row.
+__device__ __forceinline__ void refine_emit(const KnnClusterArgs& A, double acc, double tD, uint64_t tK, uint64_t tV,
+                                            const uint64_t* idp, uint64_t q, int lane, uint64_t ltmask,
+                                            uint64_t& mine) {
+    bool emit = acc < tD;
+    const bool tie = !(emit || acc > tD);
+    if (__ballot(tie)) {
+        if (tie) {
+            const uint64_t key = okey(acc);
+            emit = key < tK || (key == tK && *idp < tV);
+        }
+    }
+    const uint64_t m = __ballot(emit);
+    if (m) {
+        uint64_t base = 0;
+        if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(A.recCount), __popcll(m));
+        base = __shfl(base, 0, kWave);
+        if (emit) {
+            const uint64_t pos = base + __popcll(m & ltmask);
+            if (pos < A.recCap) {
+                A.recK[pos] = okey(acc);
+                A.recV[pos] = (q << kRecIdBits) | *idp;
+                ++mine;
+            }
+        }
+        if (lane == 0 && base + __popcll(m) > A.recCap)
+            atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 4ull);
+    }
+}
+
 template <int DM>
 __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_cluster(KnnClusterArgs A) {
     __shared__ double s_x[kStepWaves][kWave * DM];
@@ -1041,6 +1073,10 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_cluster(KnnClusterAr
 #pragma unroll
         for (int d = 0; d < DM; ++d) qp[d] = (d < D) ? A.points[q * D + d] : 0.0;
         const uint64_t tK = valid ? A.thr[2 * q] : 0, tV = valid ? A.thr[2 * q + 1] : 0;
+        // the threshold as a double: keys of non-negative distances order like
+        // the doubles, so only ties and NaNs (tK = ~0 maps to a NaN) need the keys;
+        // an idle lane compares above -inf and never emits
+        const double tD = valid ? okey_inv(tK) : -__builtin_inf();
         const uint64_t jb = A.colRes[c], je = A.colRes[c + 1];
         uint64_t mine = 0;
         for (uint64_t j0 = jb; j0 < je; j0 += kWave) {
@@ -1064,25 +1100,7 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_cluster(KnnClusterAr
                     const double diff = qp[d] - x[d];
                     acc = __builtin_fma(diff, diff, acc);
                 }
-                const uint64_t key = okey(acc);
-                const uint64_t id = s_id[wave][k];
-                const bool emit = valid && (key < tK || (key == tK && id < tV));
-                const uint64_t m = __ballot(emit);
-                if (m) {
-                    uint64_t base = 0;
-                    if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(A.recCount), __popcll(m));
-                    base = __shfl(base, 0, kWave);
-                    if (emit) {
-                        const uint64_t pos = base + __popcll(m & ltmask);
-                        if (pos < A.recCap) {
-                            A.recK[pos] = key;
-                            A.recV[pos] = (q << kRecIdBits) | id;
-                            ++mine;
-                        }
-                    }
-                    if (lane == 0 && base + __popcll(m) > A.recCap)
-                        atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 4ull);
-                }
+                refine_emit(A, acc, tD, tK, tV, &s_id[wave][k], q, lane, ltmask, mine);
             }
             wave_lds_sync();
         }
