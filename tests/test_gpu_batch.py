@@ -30,7 +30,10 @@ def idx():
 
 
 def _check_range_batch(h, o, boxes):
+    """Results without statistics (the timed path) and with them, against the oracle."""
     h.batch_set("range", boxes)
+    h.batch_run()
+    plain = h.batch_results()
     h.batch_run(stats=True)
     res = h.batch_results()
     surv, fine = h.batch_stats()
@@ -38,7 +41,9 @@ def _check_range_batch(h, o, boxes):
     for i, row in enumerate(boxes):
         lo, hi = row[:D], row[D:]
         r = o.range(lo, hi)
-        assert np.array_equal(res[i], r.refine_range(lo, hi)), "query %d hits differ" % i
+        exp = r.refine_range(lo, hi)
+        assert np.array_equal(plain[i], exp), "query %d hits differ (no statistics)" % i
+        assert np.array_equal(res[i], exp), "query %d hits differ" % i
         _, s, out = r.levels()
         assert np.array_equal(surv[i], s), "query %d survivors per level differ" % i
         assert fine[i] == out[-1]
@@ -96,7 +101,10 @@ def test_batch_range_hierarchy_heights(ratios):
 
 
 def _check_knn_batch(h, o, pts, Ks):
+    """Results without statistics (the timed path) and with them, against the oracle."""
     h.batch_set("knn", pts, Ks)
+    h.batch_run()
+    plain = h.batch_results()
     h.batch_run(stats=True)
     res = h.batch_results()
     surv, fine = h.batch_stats()
@@ -104,9 +112,9 @@ def _check_knn_batch(h, o, pts, Ks):
         K = int(Ks[i])
         r = o.knn(q, K)
         eid, ed2 = r.refine_knn(q, K)
-        gid, gd2 = res[i]
-        assert np.array_equal(gid, eid), "query %d ids differ" % i
-        assert np.array_equal(gd2.view(np.uint64), ed2.view(np.uint64)), "query %d distances differ" % i
+        for gid, gd2 in (plain[i], res[i]):
+            assert np.array_equal(gid, eid), "query %d ids differ" % i
+            assert np.array_equal(gd2.view(np.uint64), ed2.view(np.uint64)), "query %d distances differ" % i
         _, s, out = r.levels()
         assert np.array_equal(surv[i], s), "query %d survivors per level differ" % i
         assert fine[i] == out[-1]
