@@ -630,18 +630,20 @@ __global__ __launch_bounds__(kAggThreads) void k_group_scatter(const uint64_t* _
 // ------------------------------------------------------------- radix sort
 // Single-sweep LSD radix sort of u64 keys: digits of RB = 8 or 9 bits, tiles of
 // 8192 keys, the key count on the device.  A counting kernel reads the keys
-// once for the first digit's counts; each pass is then one kernel that ranks a
-// tile's keys by digit, publishes the tile's digit counts, takes every digit's
-// exclusive prefix over the earlier tiles by decoupled look-back, and writes
-// each key straight to its place, while counting the next pass's digit (the
-// last pass: the keys of every group key >> groupShift) from the keys it holds.
-// Tiles are handed out by a ticket counter, so a tile only waits on tiles that
-// resident workgroups already hold.
+// once for the first digit's counts (and, optionally, the keys of every group
+// key >> groupShift); each pass is then one kernel that ranks a tile's keys by
+// digit, publishes the tile's digit counts, takes every digit's exclusive
+// prefix over the earlier tiles by decoupled look-back, lays the tile out in
+// digit order in LDS (two halves of 4096 keys) and writes each digit's run
+// with consecutive threads on consecutive addresses, while counting the next
+// pass's digit from the keys it holds.  Tiles are handed out by a ticket
+// counter, so a tile only waits on tiles that resident workgroups already hold.
 
 constexpr int kRsThreads = 512;
 constexpr int kRsWaves = kRsThreads / kWave;
 constexpr int kRsPer = 16;
 constexpr int kRsTile = kRsThreads * kRsPer;
+constexpr int kRsHalf = kRsTile / 2;  // keys staged in LDS per round
 constexpr int kRsMaxPasses = 8;
 constexpr int kRsMaxBins = 512;
 constexpr int kRsGroupLds = 8192;  // groups counted in LDS (more: global atomics)
@@ -654,8 +656,6 @@ constexpr uint64_t kRsHdrWords = kRsCounts0 + static_cast<uint64_t>(kRsMaxPasses
 // look-back word: flag (aggregate / inclusive prefix) in the top two bits
 constexpr uint32_t kRsAgg = 1u << 30, kRsInc = 2u << 30, kRsVal = (1u << 30) - 1;
 constexpr int kRsProbe = 8;  // predecessors read per look-back round
-// what a pass counts besides its own digit
-constexpr int kRsCountNext = 0, kRsCountNone = 1, kRsCountGroups = 2;
 
 // Lanes of the wave whose RB-bit digit equals this lane's, among the lanes set
 // in `valid`: one ballot per digit bit, each folded into the running mask of
@@ -705,16 +705,23 @@ __global__ void k_rs_begin(uint32_t* __restrict__ hdr, uint64_t* __restrict__ gr
         for (uint64_t i = i0; i < nGroups; i += stride) groupCnt[i] = 0;
 }
 
-// Counts of the first digit (at firstBit) over all keys; also clears look-back
-// region 0 for the tiles of this sort.
+// Counts of the first digit (at firstBit) over all keys and, with groupCnt, of
+// every group key >> groupShift (a row whose 64 keys share a group adds once);
+// also clears look-back region 0 for the tiles of this sort.
 template <int RB>
 __global__ __launch_bounds__(kRsThreads) void k_rs_counts(const uint64_t* __restrict__ keys, const uint64_t* dn,
                                                           uint64_t cap, int firstBit, uint32_t* __restrict__ counts,
-                                                          uint32_t* __restrict__ status0) {
+                                                          uint32_t* __restrict__ status0,
+                                                          uint64_t* __restrict__ groupCnt, int groupShift,
+                                                          uint64_t nGroups) {
     constexpr int kBins = 1 << RB;
     __shared__ uint32_t lc[kRsWaves][kBins];  // per-wave counters
-    const int tid = threadIdx.x, wid = tid / kWave;
+    __shared__ uint32_t lg[kRsGroupLds];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const bool ldsGroups = groupCnt && nGroups <= kRsGroupLds;
     for (int i = tid; i < kRsWaves * kBins; i += kRsThreads) (&lc[0][0])[i] = 0;
+    if (ldsGroups)
+        for (int i = tid; i < static_cast<int>(nGroups); i += kRsThreads) lg[i] = 0;
     __syncthreads();
     uint64_t n = *dn;
     if (n > cap) n = cap;
@@ -729,8 +736,21 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_counts(const uint64_t* __rest
         }
 #pragma unroll
         for (int r = 0; r < kRsPer; ++r) {
-            const uint64_t e = t * kRsTile + static_cast<uint64_t>(r) * kRsThreads + tid;
-            if (e < n) atomicAdd(&lc[wid][(k[r] >> firstBit) & (kBins - 1)], 1u);
+            const bool ok = t * kRsTile + static_cast<uint64_t>(r) * kRsThreads + tid < n;
+            if (ok) atomicAdd(&lc[wid][(k[r] >> firstBit) & (kBins - 1)], 1u);
+            if (groupCnt) {
+                const uint64_t g = k[r] >> groupShift;
+                const uint64_t g0 = readlane_u64(g, 0);
+                if (__ballot(ok) == ~0ull && __ballot(g != g0) == 0) {
+                    if (lane == 0) {
+                        if (ldsGroups) atomicAdd(&lg[g0], static_cast<uint32_t>(kWave));
+                        else atomicAdd(reinterpret_cast<unsigned long long*>(groupCnt + g0), 64ull);
+                    }
+                } else if (ok) {
+                    if (ldsGroups) atomicAdd(&lg[g], 1u);
+                    else atomicAdd(reinterpret_cast<unsigned long long*>(groupCnt + g), 1ull);
+                }
+            }
         }
     }
     __syncthreads();
@@ -740,6 +760,10 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_counts(const uint64_t* __rest
         for (int w = 0; w < kRsWaves; ++w) c += lc[w][d];
         if (c) atomicAdd(&counts[d], c);
     }
+    if (ldsGroups)
+        for (int i = tid; i < static_cast<int>(nGroups); i += kRsThreads)
+            if (lg[i]) atomicAdd(reinterpret_cast<unsigned long long*>(groupCnt + i),
+                                 static_cast<unsigned long long>(lg[i]));
 }
 
 // One stable pass on the digit at `shift`.  Within a tile, wave w owns keys
@@ -747,36 +771,34 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_counts(const uint64_t* __rest
 // digit counters in LDS (peers of equal digit found with RB ballots; rank =
 // counter + peers below the lane), with no workgroup barrier.  The waves'
 // counts give the tile's digit counts (thread d owns digit d), which are
-// published (tile 0: as its inclusive prefix) before the look-back; then
-// wave w's first output position for digit d is the digit's global start +
-// its count in earlier tiles + the counts of waves < w in this tile, and a
-// key goes to that position + its rank.  out = key & outMask.  COUNT:
-// kRsCountNext adds the next digit's counts into nextCounts, kRsCountGroups
-// the group counts into groupCnt.
-template <int RB, int COUNT>
+// published (tile 0: as its inclusive prefix) before the look-back; a key's
+// place in the tile is its digit's start in the tile + the counts of waves < w
+// + its rank, and a digit's run goes to the digit's global start + its count in
+// earlier tiles.  out = key & outMask.  NEXT: the next digit's counts (at
+// shift + RB) are added into nextCounts.
+template <int RB, bool NEXT>
 __global__ __launch_bounds__(kRsThreads, 4) void k_rs_onesweep(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
-                                                            const uint64_t* dn, uint64_t cap, int shift,
-                                                            const uint32_t* __restrict__ counts,
-                                                            uint32_t* __restrict__ nextCounts, uint32_t* status,
-                                                            uint32_t* __restrict__ statusNext, uint32_t* ticket,
-                                                            uint64_t outMask, uint64_t* __restrict__ groupCnt,
-                                                            int groupShift, uint64_t nGroups) {
+                                                               const uint64_t* dn, uint64_t cap, int shift,
+                                                               const uint32_t* __restrict__ counts,
+                                                               uint32_t* __restrict__ nextCounts, uint32_t* status,
+                                                               uint32_t* __restrict__ statusNext, uint32_t* ticket,
+                                                               uint64_t outMask) {
     constexpr int kBins = 1 << RB;
     constexpr uint32_t kPerWave = static_cast<uint32_t>(kRsPer) * kWave;
-    constexpr int kSide = COUNT == kRsCountNext ? kBins : (COUNT == kRsCountGroups ? kRsGroupLds : 1);
-    __shared__ uint32_t wc[kRsWaves][kBins];  // wave counters, then wave output bases
-    __shared__ uint32_t side[kSide];          // next-digit or group counts of this workgroup
+    __shared__ uint32_t wc[kRsWaves][kBins];  // wave counters, then each wave's place in the tile per digit
+    __shared__ uint32_t gofs[kBins];          // output position of a run = gofs[d] + place in the tile
+    __shared__ uint32_t side[NEXT ? kBins : 1];
+    __shared__ uint64_t sk[kRsHalf];
     __shared__ uint32_t wsum[kRsWaves];
     __shared__ uint32_t s_tile;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const bool owner = tid < kBins;  // thread tid owns digit tid
-    const bool ldsGroups = COUNT == kRsCountGroups && nGroups <= kRsGroupLds;
     uint64_t n = *dn;
     if (n > cap) n = cap;
     const uint64_t ntiles = (n + kRsTile - 1) / kRsTile;
     uint32_t* mycnt = wc[wid];
-    if (COUNT != kRsCountNone)
-        for (int i = tid; i < kSide; i += kRsThreads) side[i] = 0;
+    if (NEXT)
+        for (int i = tid; i < kBins; i += kRsThreads) side[i] = 0;
     // global digit starts (exclusive scan of the digit counts)
     const uint32_t gst = bins_excl_scan<kBins>(owner ? counts[tid] : 0u, wsum);
     for (;;) {
@@ -810,26 +832,17 @@ __global__ __launch_bounds__(kRsThreads, 4) void k_rs_onesweep(const uint64_t* _
             wave_lds_sync();
             if (ok && below == 0) mycnt[dig] = pre + static_cast<uint32_t>(__popc(plo) + __popc(phi));
             wave_lds_sync();
-            if (COUNT == kRsCountNext && ok)
-                atomicAdd(&side[(key[r] >> (shift + RB)) & (kBins - 1)], 1u);
-            if (COUNT == kRsCountGroups && ok) {
-                const uint64_t g = key[r] >> groupShift;
-                if (ldsGroups) atomicAdd(&side[g], 1u);
-                else atomicAdd(reinterpret_cast<unsigned long long*>(groupCnt + g), 1ull);
-            }
+            if (NEXT && ok) atomicAdd(&side[(key[r] >> (shift + RB)) & (kBins - 1)], 1u);
         }
         __syncthreads();
-        uint32_t c[kRsWaves];
         uint32_t v = 0;
         if (owner) {
 #pragma unroll
-            for (int w = 0; w < kRsWaves; ++w) {
-                c[w] = wc[w][tid];
-                v += c[w];
-            }
+            for (int w = 0; w < kRsWaves; ++w) v += wc[w][tid];
             st_agent_u32(status + t * kBins + tid, (t == 0 ? kRsInc : kRsAgg) | v);
             if (statusNext) statusNext[t * kBins + tid] = 0;
         }
+        const uint32_t ex = bins_excl_scan<kBins>(v, wsum);  // the digit's start in the tile
         // look-back: the digit's exclusive prefix over the earlier tiles,
         // reading kRsProbe predecessors per round and summing aggregates up to
         // the nearest inclusive prefix; a predecessor that has not published
@@ -869,28 +882,37 @@ __global__ __launch_bounds__(kRsThreads, 4) void k_rs_onesweep(const uint64_t* _
             st_agent_u32(status + t * kBins + tid, kRsInc | (pre + v));
         }
         if (owner) {
-            uint32_t p = gst + pre;
+            uint32_t p = ex;
 #pragma unroll
             for (int w = 0; w < kRsWaves; ++w) {
+                const uint32_t c = wc[w][tid];
                 wc[w][tid] = p;
-                p += c[w];
+                p += c;
             }
+            gofs[tid] = gst + pre - ex;
         }
         __syncthreads();
+        // rank -> place in the tile
 #pragma unroll
-        for (int r = 0; r < kRsPer; ++r) {
-            const uint32_t e = e0 + static_cast<uint32_t>(r) * kWave;
-            if (e < cnt) out[mycnt[(key[r] >> shift) & (kBins - 1)] + rank[r]] = key[r] & outMask;
+        for (int r = 0; r < kRsPer; ++r) rank[r] += mycnt[(key[r] >> shift) & (kBins - 1)];
+        for (uint32_t h0 = 0; h0 < cnt; h0 += kRsHalf) {
+#pragma unroll
+            for (int r = 0; r < kRsPer; ++r) {
+                const uint32_t e = e0 + static_cast<uint32_t>(r) * kWave;
+                if (e < cnt && rank[r] - h0 < static_cast<uint32_t>(kRsHalf)) sk[rank[r] - h0] = key[r];
+            }
+            __syncthreads();
+            const uint32_t nh = cnt - h0 < static_cast<uint32_t>(kRsHalf) ? cnt - h0 : kRsHalf;
+            for (uint32_t i = tid; i < nh; i += kRsThreads) {
+                const uint64_t k = sk[i];
+                out[gofs[(k >> shift) & (kBins - 1)] + h0 + i] = k & outMask;
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
-    if (COUNT == kRsCountNext)
+    if (NEXT)
         for (int d = tid; d < kBins; d += kRsThreads)
             if (side[d]) atomicAdd(&nextCounts[d], side[d]);
-    if (COUNT == kRsCountGroups && ldsGroups)
-        for (int i = tid; i < static_cast<int>(nGroups); i += kRsThreads)
-            if (side[i]) atomicAdd(reinterpret_cast<unsigned long long*>(groupCnt + i),
-                                   static_cast<unsigned long long>(side[i]));
 }
 
 // off[0..n] = exclusive scan of cnt[0..n) (off[n] = total), one workgroup.
@@ -945,17 +967,16 @@ uint64_t radix_hist_words(uint64_t cap, uint64_t groups) {
     return (kRsHdrWords + 2 * tiles * kRsMaxBins + 1) / 2 + groups;
 }
 
-template <int RB, int COUNT>
+template <int RB, bool NEXT>
 static void launch_onesweep(tblaeq_ctx* ctx, const uint64_t* src, uint64_t* dst, const uint64_t* dn, uint64_t cap,
                             int shift, const uint32_t* counts, uint32_t* nextCounts, uint32_t* status,
-                            uint32_t* statusNext, uint32_t* ticket, uint64_t outMask, uint64_t* groupCnt,
-                            int groupShift, uint64_t nGroups) {
+                            uint32_t* statusNext, uint32_t* ticket, uint64_t outMask) {
     const uint64_t capTiles = (cap + kRsTile - 1) / kRsTile;
     const uint64_t scap = static_cast<uint64_t>(ctx->cuCount) *
-                          resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<RB, COUNT>), kRsThreads);
+                          resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<RB, NEXT>), kRsThreads);
     const int grid = static_cast<int>(capTiles < scap ? (capTiles ? capTiles : 1) : scap);
-    hipLaunchKernelGGL((k_rs_onesweep<RB, COUNT>), dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dst, dn, cap,
-                       shift, counts, nextCounts, status, statusNext, ticket, outMask, groupCnt, groupShift, nGroups);
+    hipLaunchKernelGGL((k_rs_onesweep<RB, NEXT>), dim3(grid), dim3(kRsThreads), 0, ctx->stream, src, dst, dn, cap,
+                       shift, counts, nextCounts, status, statusNext, ticket, outMask);
 }
 
 template <int RB>
@@ -972,25 +993,20 @@ static int radix_passes(tblaeq_ctx* ctx, uint64_t* a, uint64_t* b, const uint64_
     TimedScope ts(ctx, kKBatch0 + 4);
     hipLaunchKernelGGL(k_rs_begin, dim3(16), dim3(256), 0, ctx->stream, hdr, groupCnt, ro.nGroups);
     hipLaunchKernelGGL(k_rs_counts<RB>, dim3(gridC), dim3(kRsThreads), 0, ctx->stream, a, dn, cap, firstBit, counts,
-                       region[0]);
+                       region[0], groupCnt, ro.groupShift, ro.nGroups);
     uint64_t* src = a;
     uint64_t* dst = b;
     for (int p = 0; p < passes; ++p) {
         const bool last = p == passes - 1;
         uint64_t* o = (last && ro.ids) ? ro.ids : dst;
         uint32_t* cp = counts + static_cast<uint64_t>(p) * kBins;
-        uint32_t* cn = last ? nullptr : cp + kBins;
-        uint32_t* sn = last ? nullptr : region[(p + 1) & 1];
         const int shift = firstBit + RB * p;
         if (!last)
-            launch_onesweep<RB, kRsCountNext>(ctx, src, o, dn, cap, shift, cp, cn, region[p & 1], sn, hdr + p, ~0ull,
-                                              nullptr, 0, 0);
-        else if (ro.groupOff)
-            launch_onesweep<RB, kRsCountGroups>(ctx, src, o, dn, cap, shift, cp, nullptr, region[p & 1], nullptr,
-                                                hdr + p, ro.idMask, groupCnt, ro.groupShift, ro.nGroups);
+            launch_onesweep<RB, true>(ctx, src, o, dn, cap, shift, cp, cp + kBins, region[p & 1],
+                                      region[(p + 1) & 1], hdr + p, ~0ull);
         else
-            launch_onesweep<RB, kRsCountNone>(ctx, src, o, dn, cap, shift, cp, nullptr, region[p & 1], nullptr,
-                                              hdr + p, ro.idMask, nullptr, 0, 0);
+            launch_onesweep<RB, false>(ctx, src, o, dn, cap, shift, cp, nullptr, region[p & 1], nullptr, hdr + p,
+                                       ro.idMask);
         src = o;
         dst = (src == a) ? b : a;
     }
@@ -1087,12 +1103,10 @@ void warm_range_kernels(tblaeq_batch* b) {
     resident_blocks(reinterpret_cast<const void*>(range_kernel<kFinal, 256>(b->D)), t);
     resident_blocks(reinterpret_cast<const void*>(range_kernel<kFinal, 512>(b->D)), t);
     resident_blocks(reinterpret_cast<const void*>(range_kernel<kFinal, 1024>(b->D)), t);
-    resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<8, kRsCountNext>), kRsThreads);
-    resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<8, kRsCountNone>), kRsThreads);
-    resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<8, kRsCountGroups>), kRsThreads);
-    resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<9, kRsCountNext>), kRsThreads);
-    resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<9, kRsCountNone>), kRsThreads);
-    resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<9, kRsCountGroups>), kRsThreads);
+    resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<8, true>), kRsThreads);
+    resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<8, false>), kRsThreads);
+    resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<9, true>), kRsThreads);
+    resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<9, false>), kRsThreads);
 }
 
 template <int MODE>
