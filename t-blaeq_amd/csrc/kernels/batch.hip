@@ -236,6 +236,12 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
             flo[d] = __double2float_rd(lo[d]);
             fhi[d] = __double2float_ru(hi[d]);
         }
+        // with no inverted box in the wave, the distance term max(lo - x, x - hi, 0)
+        // equals the conditional form (NaN bounds and coordinates included)
+        bool inverted = false;
+#pragma unroll
+        for (int d = 0; d < DM; ++d) inverted |= lo[d] > hi[d];
+        const bool fastDist = __ballot(valid && inverted) == 0;
         uint64_t mine = 0, fine = 0;
         for (uint64_t j0 = jb; j0 < je; j0 += kWave) {
             const int nc = static_cast<int>(je - j0 < static_cast<uint64_t>(kWave) ? je - j0 : kWave);
@@ -257,11 +263,19 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
                     rn = crad[j + 1];
                 }
                 double dist = 0.0;
+                if (fastDist) {
 #pragma unroll
-                for (int d = 0; d < DM; ++d) {
-                    const double below = lo[d] - xv[d], above = xv[d] - hi[d];
-                    const double diff = xv[d] < lo[d] ? below : (xv[d] > hi[d] ? above : 0.0);
-                    dist = __builtin_fma(diff, diff, dist);
+                    for (int d = 0; d < DM; ++d) {
+                        const double diff = fmax(fmax(lo[d] - xv[d], xv[d] - hi[d]), 0.0);
+                        dist = __builtin_fma(diff, diff, dist);
+                    }
+                } else {
+#pragma unroll
+                    for (int d = 0; d < DM; ++d) {
+                        const double below = lo[d] - xv[d], above = xv[d] - hi[d];
+                        const double diff = xv[d] < lo[d] ? below : (xv[d] > hi[d] ? above : 0.0);
+                        dist = __builtin_fma(diff, diff, dist);
+                    }
                 }
                 const bool ball = valid && dist <= r * r;
                 bool keep = ball;
