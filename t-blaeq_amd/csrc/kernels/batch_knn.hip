@@ -99,6 +99,8 @@ struct KnnShared {
     uint64_t surv[kKnnQB];
     uint64_t fine[kKnnQB];
     uint64_t tot[kKnnQB];
+    uint32_t need[2][kKnnQB];       // candidates of the current round that beat the threshold (by round parity)
+    uint32_t fill[2][kKnnQB];       // slots taken by the current round's inserts
 };
 
 __device__ __forceinline__ bool lt_pair(uint64_t ak, uint64_t av, uint64_t bk, uint64_t bv) {
@@ -106,63 +108,134 @@ __device__ __forceinline__ bool lt_pair(uint64_t ak, uint64_t av, uint64_t bk, u
 }
 
 // Sorts every query's buffer (entries beyond cnt are treated as sentinels) and
-// keeps its first M entries; updates the insertion threshold.
+// keeps its first M entries; updates the insertion threshold.  Wave i sorts
+// query i's buffer in registers (lane l holds entries r * 64 + l, r < 8) with
+// a bitonic network: partners 64 or more apart are in the same lane, nearer
+// ones are exchanged with __shfl_xor; one workgroup barrier at the end.
+static_assert(kKnnQB == kKnnThreads / kWave, "one wave per query buffer");
 __device__ void knn_compact(KnnShared& S, int tid) {
-    for (int i = 0; i < kKnnQB; ++i)
-        for (int e = tid; e < kKnnCap; e += kKnnThreads)
-            if (e >= static_cast<int>(S.cnt[i])) {
-                S.bk[i][e] = ~0ull;
-                S.bv[i][e] = ~0ull;
-            }
-    __syncthreads();
+    constexpr int R = kKnnCap / kWave;
+    const int lane = tid & (kWave - 1), qi = tid / kWave;
+    uint64_t* bk = S.bk[qi];
+    uint64_t* bv = S.bv[qi];
+    const uint32_t cnt = S.cnt[qi];
+    uint64_t K[R], V[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t e = static_cast<uint32_t>(r * kWave + lane);
+        K[r] = e < cnt ? bk[e] : ~0ull;
+        V[r] = e < cnt ? bv[e] : ~0ull;
+    }
+#pragma unroll
     for (int k = 2; k <= kKnnCap; k <<= 1) {
+#pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = tid; t < kKnnQB * (kKnnCap / 2); t += kKnnThreads) {
-                const int qi = t / (kKnnCap / 2);
-                const int tt = t % (kKnnCap / 2);
-                const int a = 2 * tt - (tt & (j - 1));
-                const int b = a + j;
-                const bool up = (a & k) == 0;
-                const uint64_t ak = S.bk[qi][a], av = S.bv[qi][a], bk = S.bk[qi][b], bv = S.bv[qi][b];
-                const bool gt = lt_pair(bk, bv, ak, av);
-                if (gt == up) {
-                    S.bk[qi][a] = bk;
-                    S.bv[qi][a] = bv;
-                    S.bk[qi][b] = ak;
-                    S.bv[qi][b] = av;
+            if (j >= kWave) {
+                const int jr = j / kWave;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (r & jr) continue;
+                    const int r2 = r | jr;
+                    const bool up = ((r * kWave + lane) & k) == 0;
+                    const bool hiLess = lt_pair(K[r2], V[r2], K[r], V[r]);
+                    if (hiLess == up) {
+                        const uint64_t tk = K[r], tv = V[r];
+                        K[r] = K[r2];
+                        V[r] = V[r2];
+                        K[r2] = tk;
+                        V[r2] = tv;
+                    }
+                }
+            } else {
+                const bool lower = (lane & j) == 0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint64_t pk = __shfl_xor(K[r], j, kWave), pv = __shfl_xor(V[r], j, kWave);
+                    const bool up = ((r * kWave + lane) & k) == 0;
+                    const bool pLess = lt_pair(pk, pv, K[r], V[r]);
+                    // the lower index of a pair keeps the smaller entry when up
+                    if (pLess == (lower == up)) {
+                        K[r] = pk;
+                        V[r] = pv;
+                    }
                 }
             }
-            __syncthreads();
         }
     }
-    if (tid < kKnnQB) {
-        const uint32_t m = S.M[tid];
-        if (S.cnt[tid] > m) S.cnt[tid] = m;
-        if (m > 0 && S.cnt[tid] >= m) {
-            S.thrK[tid] = S.bk[tid][m - 1];
-            S.thrV[tid] = S.bv[tid][m - 1];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        bk[r * kWave + lane] = K[r];
+        bv[r * kWave + lane] = V[r];
+    }
+    wave_lds_sync();
+    if (lane == 0) {
+        const uint32_t m = S.M[qi];
+        if (S.cnt[qi] > m) S.cnt[qi] = m;
+        if (m > 0 && S.cnt[qi] >= m) {
+            S.thrK[qi] = bk[m - 1];
+            S.thrV[qi] = bv[m - 1];
         } else {
-            S.thrK[tid] = ~0ull;
-            S.thrV[tid] = ~0ull;
+            S.thrK[qi] = ~0ull;
+            S.thrV[qi] = ~0ull;
         }
     }
     __syncthreads();
 }
 
-__device__ __forceinline__ void knn_insert(KnnShared& S, int i, uint64_t key, uint64_t val) {
-    if (S.M[i] == 0) return;
-    if (!lt_pair(key, val, S.thrK[i], S.thrV[i])) return;
-    const uint32_t p = atomicAdd(&S.cnt[i], 1u);
-    if (p < kKnnCap) {
-        S.bk[i][p] = key;
-        S.bv[i][p] = val;
+// One round of insertions, up to one candidate per query slot per thread
+// (have[i], key[i], val[i]): the candidates that beat their query's threshold
+// are counted first, the buffers are compacted only when one of them would
+// overflow (the tightened thresholds then apply), and every remaining
+// candidate gets a slot.  S.cnt is not written between the first barrier and
+// the decision of any wave: slots are taken on a separate counter (fill) and
+// added to cnt by the first threads after the inserts, followed by a barrier.
+// Rounds alternate the parity `par` of their counters; the counters are zero
+// between rounds.  Contains workgroup barriers: every thread calls it.
+__device__ void knn_round(KnnShared& S, int tid, const bool (&have)[kKnnQB], const uint64_t (&key)[kKnnQB],
+                          const uint64_t (&val)[kKnnQB], int par) {
+    bool pass[kKnnQB];
+#pragma unroll
+    for (int i = 0; i < kKnnQB; ++i) {
+        pass[i] = have[i] && S.M[i] > 0 && lt_pair(key[i], val[i], S.thrK[i], S.thrV[i]);
+        if (pass[i]) atomicAdd(&S.need[par][i], 1u);
     }
+    __syncthreads();
+    bool over = false;
+#pragma unroll
+    for (int i = 0; i < kKnnQB; ++i) over = over || S.cnt[i] + S.need[par][i] > static_cast<uint32_t>(kKnnCap);
+    if (over) {
+        knn_compact(S, tid);
+#pragma unroll
+        for (int i = 0; i < kKnnQB; ++i) pass[i] = pass[i] && lt_pair(key[i], val[i], S.thrK[i], S.thrV[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < kKnnQB; ++i)
+        if (pass[i]) {
+            const uint32_t p = S.cnt[i] + atomicAdd(&S.fill[par][i], 1u);
+            S.bk[i][p] = key[i];
+            S.bv[i][p] = val[i];
+        }
+    __syncthreads();
+    if (tid < kKnnQB) {
+        S.cnt[tid] += S.fill[par][tid];
+        S.fill[par][tid] = 0;
+        S.need[par][tid] = 0;
+    }
+    __syncthreads();
 }
 
-__device__ __forceinline__ bool knn_needs_compact(const KnnShared& S) {
-    bool need = false;
-    for (int i = 0; i < kKnnQB; ++i) need = need || (S.cnt[i] > static_cast<uint32_t>(kKnnCap - kKnnThreads));
-    return need;
+// knn_round with one candidate per thread, for query slot qi.
+__device__ __forceinline__ void knn_round1(KnnShared& S, int tid, int qi, bool have, uint64_t key, uint64_t val,
+                                           int par) {
+    bool h[kKnnQB];
+    uint64_t k[kKnnQB], v[kKnnQB];
+#pragma unroll
+    for (int i = 0; i < kKnnQB; ++i) {
+        h[i] = have && i == qi;
+        k[i] = key;
+        v[i] = val;
+    }
+    knn_round(S, tid, h, k, v, par);
 }
 
 // Coordinates are loaded into registers once per point (all DM loads issued
@@ -235,9 +308,11 @@ __device__ void knn_flush(KnnShared& S, int tid, int D, const double* coords, co
     __syncthreads();
     const uint64_t all = S.lpre[n];
     if (work && tid == 0) atomicAdd(reinterpret_cast<unsigned long long*>(work), all);
-    for (uint64_t f0 = 0; f0 < all; f0 += kKnnThreads) {
-        if (knn_needs_compact(S)) knn_compact(S, tid);
+    int par = 0;
+    for (uint64_t f0 = 0; f0 < all; f0 += kKnnThreads, par ^= 1) {
         const uint64_t f = f0 + tid;
+        int i = 0;
+        uint64_t key = 0, val = 0;
         if (f < all) {
             uint32_t lo = 0, hi = n;  // lpre[lo] <= f < lpre[hi]
             while (hi - lo > 1) {
@@ -246,14 +321,14 @@ __device__ void knn_flush(KnnShared& S, int tid, int D, const double* coords, co
                 else hi = mid;
             }
             const uint64_t j = S.lstart[lo] + (f - S.lpre[lo]);
-            const int i = S.listQ[lo];
+            i = S.listQ[lo];
             double xv[DM];
             load_row<DM>(coords + j * D, D, xv);
             const double d2 = d2_reg<DM>(xv, S.qp[i]);
-            const uint64_t key = radius ? okey(sqrt_rd(d2) - radius[j]) : okey(d2);
-            knn_insert(S, i, key, ids[j]);
+            key = radius ? okey(sqrt_rd(d2) - radius[j]) : okey(d2);
+            val = ids[j];
         }
-        __syncthreads();
+        knn_round1(S, tid, i, f < all, key, val, par);
     }
     if (tid == 0) S.listN = 0;
     __syncthreads();
@@ -408,6 +483,10 @@ __device__ void knn_write_topk(const KnnShared& S, int tid, uint64_t q0, uint64_
 template <int DM>
 __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
     __shared__ KnnShared S;
+    if (threadIdx.x < 2 * kKnnQB) {  // zero between rounds from here on
+        (&S.need[0][0])[threadIdx.x] = 0;
+        (&S.fill[0][0])[threadIdx.x] = 0;
+    }
     const int tid = threadIdx.x;
     const int D = A.D, L = A.L;
     const int Lend = A.split ? L - 1 : L;
@@ -447,22 +526,33 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
                 S.thrV[tid] = ~0ull;
             }
             __syncthreads();
-            for (uint64_t j0 = 0; j0 < n; j0 += kKnnThreads) {
-                if (knn_needs_compact(S)) knn_compact(S, tid);
+            int par = 0;
+            for (uint64_t j0 = 0; j0 < n; j0 += kKnnThreads, par ^= 1) {
                 const uint64_t j = j0 + tid;
+                bool have[kKnnQB];
+                uint64_t key[kKnnQB], val[kKnnQB];
+#pragma unroll
+                for (int i = 0; i < kKnnQB; ++i) {
+                    have[i] = false;
+                    key[i] = 0;
+                    val[i] = 0;
+                }
                 if (j < n) {
                     double xv[DM];
                     load_row<DM>(X + j * D, D, xv);
                     const double r = R[j];
                     const uint64_t id = I[j];
+#pragma unroll
                     for (int i = 0; i < kKnnQB; ++i) {
                         const uint64_t q = q0 + i;
                         if (q >= A.nq) continue;
                         if (l > 0 && !A.flags[q * A.flagStride + A.flagOff[l - 1] + Pp[j]]) continue;
-                        knn_insert(S, i, okey(lb_reg<DM>(xv, S.qp[i], r)), id);
+                        have[i] = true;
+                        key[i] = okey(lb_reg<DM>(xv, S.qp[i], r));
+                        val[i] = id;
                     }
                 }
-                __syncthreads();
+                knn_round(S, tid, have, key, val, par);
             }
             knn_compact(S, tid);
             knn_step(S, tid, A.weightById[l], A.radiusById[l]);
@@ -914,6 +1004,10 @@ struct KnnSelectArgs {
 template <int DM>
 __global__ __launch_bounds__(kKnnThreads) void k_knn_select(KnnSelectArgs A) {
     __shared__ KnnShared S;
+    if (threadIdx.x < 2 * kKnnQB) {  // zero between rounds from here on
+        (&S.need[0][0])[threadIdx.x] = 0;
+        (&S.fill[0][0])[threadIdx.x] = 0;
+    }
     const int tid = threadIdx.x;
     const int D = A.D;
     const uint64_t ngroups = (A.nq + kKnnQB - 1) / kKnnQB;
@@ -942,15 +1036,15 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_select(KnnSelectArgs A) {
             S.qp[i][d] = q < A.nq && d < D ? A.points[q * D + d] : 0.0;
         }
         __syncthreads();
+        int par = 0;
         for (int i = 0; i < kKnnQB; ++i) {
             const uint64_t q = q0 + i;
             if (q >= A.nq || S.M[i] == 0) continue;
             const uint64_t r0 = A.offQ[q], rn = A.offQ[q + 1] - r0;
-            for (uint64_t b = 0; b < rn; b += kKnnThreads) {
-                if (knn_needs_compact(S)) knn_compact(S, tid);
+            for (uint64_t b = 0; b < rn; b += kKnnThreads, par ^= 1) {
                 const uint64_t r = b + tid;
-                if (r < rn) knn_insert(S, i, A.gK[r0 + r], A.gV[r0 + r]);
-                __syncthreads();
+                const bool have = r < rn;
+                knn_round1(S, tid, i, have, have ? A.gK[r0 + r] : 0, have ? A.gV[r0 + r] : 0, par);
             }
         }
         knn_compact(S, tid);
@@ -1193,6 +1287,10 @@ struct KnnMergeArgs {
 
 __global__ __launch_bounds__(kKnnThreads) void k_knn_merge(KnnMergeArgs A) {
     __shared__ KnnShared S;
+    if (threadIdx.x < 2 * kKnnQB) {  // zero between rounds from here on
+        (&S.need[0][0])[threadIdx.x] = 0;
+        (&S.fill[0][0])[threadIdx.x] = 0;
+    }
     __shared__ uint64_t s_r0[kKnnQB], s_rn[kKnnQB];
     __shared__ uint32_t s_any;
     const int tid = threadIdx.x;
@@ -1229,14 +1327,14 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_merge(KnnMergeArgs A) {
             if (tid == 0) S.cnt[i] = n0;
         }
         __syncthreads();
+        int par = 0;
         for (int i = 0; i < kKnnQB; ++i) {
             if (S.M[i] == 0) continue;
             const uint64_t r0 = s_r0[i], rn = s_rn[i];
-            for (uint64_t b = 0; b < rn; b += kKnnThreads) {
-                if (knn_needs_compact(S)) knn_compact(S, tid);
+            for (uint64_t b = 0; b < rn; b += kKnnThreads, par ^= 1) {
                 const uint64_t r = b + tid;
-                if (r < rn) knn_insert(S, i, A.gK[r0 + r], A.gV[r0 + r]);
-                __syncthreads();
+                const bool have = r < rn;
+                knn_round1(S, tid, i, have, have ? A.gK[r0 + r] : 0, have ? A.gV[r0 + r] : 0, par);
             }
         }
         knn_compact(S, tid);
