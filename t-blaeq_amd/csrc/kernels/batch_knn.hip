@@ -188,7 +188,9 @@ __device__ void knn_compact(KnnShared& S, int tid) {
 // overflow (the tightened thresholds then apply), and every remaining
 // candidate gets a slot.  S.cnt is not written between the first barrier and
 // the decision of any wave: slots are taken on a separate counter (fill) and
-// added to cnt by the first threads after the inserts, followed by a barrier.
+// added to cnt by the first threads after the second barrier, which the next
+// round's first barrier orders before any read; after the last round of a
+// sequence the caller places one more barrier before S.cnt is read again.
 // Rounds alternate the parity `par` of their counters; the counters are zero
 // between rounds.  Contains workgroup barriers: every thread calls it.
 __device__ void knn_round(KnnShared& S, int tid, const bool (&have)[kKnnQB], const uint64_t (&key)[kKnnQB],
@@ -221,7 +223,6 @@ __device__ void knn_round(KnnShared& S, int tid, const bool (&have)[kKnnQB], con
         S.fill[par][tid] = 0;
         S.need[par][tid] = 0;
     }
-    __syncthreads();
 }
 
 // knn_round with one candidate per thread, for query slot qi.
@@ -308,27 +309,53 @@ __device__ void knn_flush(KnnShared& S, int tid, int D, const double* coords, co
     __syncthreads();
     const uint64_t all = S.lpre[n];
     if (work && tid == 0) atomicAdd(reinterpret_cast<unsigned long long*>(work), all);
+    // flat index f -> (query slot, point); the next round's point is located
+    // and its loads issued before the current round's barriers
+    auto locate = [&](uint64_t f, int& i, uint64_t& j) {
+        uint32_t lo = 0, hi = n;  // lpre[lo] <= f < lpre[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (S.lpre[mid] <= f) lo = mid;
+            else hi = mid;
+        }
+        j = S.lstart[lo] + (f - S.lpre[lo]);
+        i = S.listQ[lo];
+    };
+    int ni = 0;
+    uint64_t nj = 0, nid = 0;
+    double nx[DM], nr = 0.0;
+    bool nhave = static_cast<uint64_t>(tid) < all;
+#pragma unroll
+    for (int d = 0; d < DM; ++d) nx[d] = 0.0;
+    if (nhave) {
+        locate(static_cast<uint64_t>(tid), ni, nj);
+        load_row<DM>(coords + nj * D, D, nx);
+        nid = ids[nj];
+        if (radius) nr = radius[nj];
+    }
     int par = 0;
     for (uint64_t f0 = 0; f0 < all; f0 += kKnnThreads, par ^= 1) {
-        const uint64_t f = f0 + tid;
-        int i = 0;
-        uint64_t key = 0, val = 0;
-        if (f < all) {
-            uint32_t lo = 0, hi = n;  // lpre[lo] <= f < lpre[hi]
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (S.lpre[mid] <= f) lo = mid;
-                else hi = mid;
-            }
-            const uint64_t j = S.lstart[lo] + (f - S.lpre[lo]);
-            i = S.listQ[lo];
-            double xv[DM];
-            load_row<DM>(coords + j * D, D, xv);
-            const double d2 = d2_reg<DM>(xv, S.qp[i]);
-            key = radius ? okey(sqrt_rd(d2) - radius[j]) : okey(d2);
-            val = ids[j];
+        const bool have = nhave;
+        const int i = ni;
+        const uint64_t val = nid;
+        const double r = nr;
+        double xv[DM];
+#pragma unroll
+        for (int d = 0; d < DM; ++d) xv[d] = nx[d];
+        const uint64_t fn = f0 + kKnnThreads + tid;
+        nhave = fn < all;
+        if (nhave) {
+            locate(fn, ni, nj);
+            load_row<DM>(coords + nj * D, D, nx);
+            nid = ids[nj];
+            if (radius) nr = radius[nj];
         }
-        knn_round1(S, tid, i, f < all, key, val, par);
+        uint64_t key = 0;
+        if (have) {
+            const double d2 = d2_reg<DM>(xv, S.qp[i]);
+            key = radius ? okey(sqrt_rd(d2) - r) : okey(d2);
+        }
+        knn_round1(S, tid, i, have, key, val, par);
     }
     if (tid == 0) S.listN = 0;
     __syncthreads();
@@ -554,6 +581,7 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
                 }
                 knn_round(S, tid, have, key, val, par);
             }
+            __syncthreads();
             knn_compact(S, tid);
             knn_step(S, tid, A.weightById[l], A.radiusById[l]);
             if (A.split && l == L - 2) {
@@ -1047,6 +1075,7 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_select(KnnSelectArgs A) {
                 knn_round1(S, tid, i, have, have ? A.gK[r0 + r] : 0, have ? A.gV[r0 + r] : 0, par);
             }
         }
+        __syncthreads();
         knn_compact(S, tid);
         knn_step(S, tid, A.weightById, A.radiusById);
         if (A.radius) {
@@ -1337,6 +1366,7 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_merge(KnnMergeArgs A) {
                 knn_round1(S, tid, i, have, have ? A.gK[r0 + r] : 0, have ? A.gV[r0 + r] : 0, par);
             }
         }
+        __syncthreads();
         knn_compact(S, tid);
         for (int i = 0; i < kKnnQB; ++i) {
             if (S.M[i] == 0) continue;
