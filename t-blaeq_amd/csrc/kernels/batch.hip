@@ -584,7 +584,9 @@ __global__ __launch_bounds__(kScanThreads) void k_group_scan(const uint64_t* __r
 // zeroed.  A workgroup takes chunks of kAggChunk pairs; with mesh <= kAggLds it
 // ranks the chunk's pairs per id in LDS and reserves each id's run with one
 // global atomic (a run's slots count down from cnt[id]), else one atomic per
-// pair.  The order inside a list is not fixed.
+// pair.  The order inside a list is not fixed.  Slots at or beyond cap (counts
+// of pairs that overflowed the pair buffer, a batch that is rerun) are not
+// written.
 __global__ __launch_bounds__(kAggThreads) void k_group_scatter(const uint64_t* __restrict__ pairs, const uint64_t* dn,
                                                                uint64_t cap, uint64_t mesh,
                                                                uint64_t* __restrict__ cnt,
@@ -602,7 +604,7 @@ __global__ __launch_bounds__(kAggThreads) void k_group_scatter(const uint64_t* _
             if (id >= mesh) continue;  // sentinel of an overflowed allocation
             const uint32_t q = static_cast<uint32_t>(p & ((1ull << kQueryBits) - 1));
             const uint64_t pos = atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[id]), ~0ull) - 1;
-            qlist[off[id] + pos] = q;
+            if (off[id] + pos < cap) qlist[off[id] + pos] = q;
         }
         return;
     }
@@ -629,7 +631,8 @@ __global__ __launch_bounds__(kAggThreads) void k_group_scatter(const uint64_t* _
 #pragma unroll
         for (int r = 0; r < kAggPer; ++r) {
             const uint64_t id = p[r] >> kQueryBits;
-            if (id < mesh) qlist[off[id] + lc[id] + rank[r]] = static_cast<uint32_t>(p[r] & ((1ull << kQueryBits) - 1));
+            if (id < mesh && off[id] + lc[id] + rank[r] < cap)
+                qlist[off[id] + lc[id] + rank[r]] = static_cast<uint32_t>(p[r] & ((1ull << kQueryBits) - 1));
         }
         __syncthreads();
 #pragma unroll
@@ -1188,6 +1191,9 @@ int launch_grouping(tblaeq_batch* b, uint64_t mesh) {
         hipLaunchKernelGGL(k_group_scatter, dim3(static_cast<unsigned>(g ? g : 1)), dim3(kAggThreads), 0, ctx->stream,
                            b->pairs, ctl + 0, b->pairCap, mesh, b->cnt, b->off, b->qlist);
     }
+    // the counts also cover pairs that overflowed the pair buffer (the batch is
+    // then rerun): clear what the scatter did not count down
+    launch_fill(b, b->cnt, 0, mesh);
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, ctx->stream, ctl, 0ull, 1);
     TB_HIP(hipGetLastError());
     return TBLAEQ_OK;

@@ -368,8 +368,9 @@ __device__ __forceinline__ void knn_pair_queue(KnnShared& S, int i, uint64_t cid
     S.listQ[p] = static_cast<uint8_t>(i);
 }
 
-// Appends the queued pairs to the global pair list (one atomic per flush) and
-// counts them per cluster for the grouping scan; the queue is empty afterwards.
+// Appends the queued pairs to the global pair list (one atomic per flush); the
+// queue is empty afterwards.  (Their per-cluster counts are added by the
+// caller, one atomic per cluster and workgroup.)
 __device__ void knn_emit_pairs(const KnnArgs& A, KnnShared& S, int tid, uint64_t q0) {
     const uint32_t n = S.listN;
     if (n == 0) return;
@@ -378,11 +379,7 @@ __device__ void knn_emit_pairs(const KnnArgs& A, KnnShared& S, int tid, uint64_t
     const uint64_t base = S.wsum[0];
     for (uint32_t e = tid; e < n; e += kKnnThreads) {
         const uint64_t pos = base + e;
-        if (pos < A.pairCap) {
-            const uint64_t cid = S.lstart[e];
-            A.pairs[pos] = (cid << kQueryBits) | (q0 + S.listQ[e]);
-            atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[cid]), 1ull);
-        }
+        if (pos < A.pairCap) A.pairs[pos] = (S.lstart[e] << kQueryBits) | (q0 + S.listQ[e]);
     }
     if (tid == 0 && base + n > A.pairCap) atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 1ull);
     __syncthreads();
@@ -629,6 +626,7 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
                         load_row<DM>(X + j * D, D, xv);
                         const double r = R[j];
                         const uint64_t id = I[j];
+                        uint64_t kept = 0;
                         for (int i = 0; i < kKnnQB; ++i) {
                             const uint64_t q = q0 + i;
                             if (q >= A.nq) continue;
@@ -636,7 +634,10 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
                             if (!(lb_reg<DM>(xv, S.qp[i], r) <= S.ub[i])) continue;
                             atomicAdd(reinterpret_cast<unsigned long long*>(&S.surv[i]), 1ull);
                             knn_pair_queue(S, i, id);
+                            ++kept;
                         }
+                        // pairs of this cluster for the grouping scan, one atomic for the workgroup's queries
+                        if (kept) atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[id]), kept);
                     }
                     __syncthreads();
                     if (S.listN >= static_cast<uint32_t>(kKnnFlush)) knn_emit_pairs(A, S, tid, q0);
@@ -683,6 +684,7 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
                     load_row<DM>(X + j * D, D, xv);
                     const double r = R[j];
                     const uint64_t id = I[j];
+                    uint64_t kept = 0;
                     for (int i = 0; i < kKnnQB; ++i) {
                         const uint64_t q = q0 + i;
                         if (q >= A.nq) continue;
@@ -698,9 +700,14 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
                             const double rho = __builtin_sqrt(okey_inv(tk));
                             if (lb > rho + 1e-9 * (1.0 + rho + r)) continue;
                         }
-                        if (A.pairs) knn_pair_queue(S, i, id);
-                        else knn_queue(S, i, A.colRes[L - 1][id], A.colRes[L - 1][id + 1] - A.colRes[L - 1][id]);
+                        if (A.pairs) {
+                            knn_pair_queue(S, i, id);
+                            ++kept;
+                        } else {
+                            knn_queue(S, i, A.colRes[L - 1][id], A.colRes[L - 1][id + 1] - A.colRes[L - 1][id]);
+                        }
                     }
+                    if (kept) atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[id]), kept);
                 }
                 __syncthreads();
                 if (S.listN >= static_cast<uint32_t>(kKnnFlush)) {
@@ -864,8 +871,12 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_last(KnnLastArgs A) 
         const uint64_t start = (it & ((1ull << kSliceBits) - 1)) * kWave;
         const uint64_t g0 = A.off[p];
         const uint64_t gn = A.off[p + 1] - g0;
-        const bool valid = start + lane < gn && g0 + start + lane < A.qcap;
-        const uint64_t q = valid ? A.qlist[g0 + start + lane] : 0;
+        // a list slot past the written pairs (a pair buffer overflow, rerun by the
+        // host) may hold a stale query number: only queries below nq are taken
+        const bool inList = start + lane < gn && g0 + start + lane < A.qcap;
+        const uint64_t ql = inList ? A.qlist[g0 + start + lane] : 0;
+        const bool valid = inList && ql < A.nq;
+        const uint64_t q = valid ? ql : 0;
         double qp[DM];
 #pragma unroll
         for (int d = 0; d < DM; ++d) qp[d] = (d < D) ? A.points[q * D + d] : 0.0;
@@ -1116,6 +1127,7 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_select(KnnSelectArgs A) {
 // few points that can still displace a seed result.
 struct KnnClusterArgs {
     int D;
+    uint64_t nq;
     const uint64_t* items;
     const uint64_t* nItems;
     uint64_t itemCap;
@@ -1190,8 +1202,12 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_cluster(KnnClusterAr
         const uint64_t start = (it & ((1ull << kSliceBits) - 1)) * kWave;
         const uint64_t g0 = A.off[c];
         const uint64_t gn = A.off[c + 1] - g0;
-        const bool valid = start + lane < gn && g0 + start + lane < A.qcap;
-        const uint64_t q = valid ? A.qlist[g0 + start + lane] : 0;
+        // a list slot past the written pairs (a pair buffer overflow, rerun by the
+        // host) may hold a stale query number: only queries below nq are taken
+        const bool inList = start + lane < gn && g0 + start + lane < A.qcap;
+        const uint64_t ql = inList ? A.qlist[g0 + start + lane] : 0;
+        const bool valid = inList && ql < A.nq;
+        const uint64_t q = valid ? ql : 0;
         double qp[DM];
 #pragma unroll
         for (int d = 0; d < DM; ++d) qp[d] = (d < D) ? A.points[q * D + d] : 0.0;
@@ -1739,6 +1755,7 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
     if (clusterMajor) {
         KnnClusterArgs C{};
         C.D = b->D;
+        C.nq = nq;
         C.items = rItems;
         C.nItems = rN;
         C.itemCap = b->itemCap;
