@@ -221,6 +221,13 @@ int tblaeq_batch_range_graph(tblaeq_batch* b, const double* d_boxes, uint64_t nq
 int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const uint64_t* d_K, uint64_t nq, uint64_t Kmax,
                      uint64_t* d_out_ids, double* d_out_d2, uint64_t* d_out_n, uint64_t* d_status,
                      uint64_t* d_level_surv, uint64_t* d_fine_count);
+/* tblaeq_batch_knn as one hipGraph launch: the first call with a given set of
+ * arguments captures the whole batch (levels, grouping, last-level passes,
+ * selection, refine, merge) on the context stream; later calls with the same
+ * arguments replay it (the buffers' contents may change between calls).  No
+ * statistics outputs. */
+int tblaeq_batch_knn_graph(tblaeq_batch* b, const double* d_points, const uint64_t* d_K, uint64_t nq, uint64_t Kmax,
+                           uint64_t* d_out_ids, double* d_out_d2, uint64_t* d_out_n, uint64_t* d_status);
 
 /* Launch parameters of the batched engine (no effect on results, only on
  * speed); tblaeq_batch_get_param reads the current value.

@@ -51,6 +51,21 @@ struct tblaeq_batch {
     // captured range batch (replayed while the arguments are unchanged)
     hipGraph_t rangeGraph = nullptr;
     hipGraphExec_t rangeExec = nullptr;
+    // captured KNN batch and the arguments it was captured with
+    hipGraph_t knnGraph = nullptr;
+    hipGraphExec_t knnExec = nullptr;
+    const void* kPoints = nullptr;
+    const void* kK = nullptr;
+    uint64_t kNq = 0, kKmax = 0, kScanLaunches = 0, kParamVersion = 0;
+    void* kIds = nullptr;
+    void* kD2 = nullptr;
+    void* kN = nullptr;
+    void* kStatus = nullptr;
+    // context buffers baked into the captured graphs (a reallocation recaptures)
+    const void* gCtxStatus = nullptr;
+    const void* gCtxWs = nullptr;
+    const void* kCtxStatus = nullptr;
+    const void* kCtxWs = nullptr;
     const void* gBoxes = nullptr;
     uint64_t gNq = 0, gCap = 0;
     void* gOff = nullptr;

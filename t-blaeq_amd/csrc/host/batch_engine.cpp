@@ -111,10 +111,15 @@ int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total) {
             while (!todo.empty()) {
                 auto [a, n] = todo.back();
                 todo.pop_back();
-                int rc = tblaeq_batch_knn(st->b, st->dQ.data() + a * D, st->dK.data() + a, n, st->Kmax,
-                                          st->dIds.data() + a * st->Kmax, st->dD2.data() + a * st->Kmax,
-                                          st->dN.data() + a, st->dStatus.data(), stats ? st->dSurv.data() + a * L : nullptr,
-                                          stats ? st->dFine.data() + a : nullptr);
+                int rc = (!stats && st->useGraph)
+                             ? tblaeq_batch_knn_graph(st->b, st->dQ.data() + a * D, st->dK.data() + a, n, st->Kmax,
+                                                      st->dIds.data() + a * st->Kmax, st->dD2.data() + a * st->Kmax,
+                                                      st->dN.data() + a, st->dStatus.data())
+                             : tblaeq_batch_knn(st->b, st->dQ.data() + a * D, st->dK.data() + a, n, st->Kmax,
+                                                st->dIds.data() + a * st->Kmax, st->dD2.data() + a * st->Kmax,
+                                                st->dN.data() + a, st->dStatus.data(),
+                                                stats ? st->dSurv.data() + a * L : nullptr,
+                                                stats ? st->dFine.data() + a : nullptr);
                 if (rc) return rc;
                 uint64_t status[4];
                 hipCheck(hipMemcpyAsync(status, st->dStatus.data(), 32, hipMemcpyDeviceToHost, s), "D2H");

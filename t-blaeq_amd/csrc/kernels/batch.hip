@@ -1368,6 +1368,8 @@ extern "C" int tblaeq_batch_destroy(tblaeq_batch* b) {
     if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
     if (b->rangeExec) (void)hipGraphExecDestroy(b->rangeExec);
     if (b->rangeGraph) (void)hipGraphDestroy(b->rangeGraph);
+    if (b->knnExec) (void)hipGraphExecDestroy(b->knnExec);
+    if (b->knnGraph) (void)hipGraphDestroy(b->knnGraph);
     for (void* p : b->owned) (void)hipFree(p);
     delete b;
     return TBLAEQ_OK;
@@ -1552,8 +1554,8 @@ extern "C" int tblaeq_batch_range_graph(tblaeq_batch* b, const double* d_boxes, 
     if (!b) return TBLAEQ_ERR_INVALID;
     tblaeq_ctx* ctx = b->ctx;
     const bool same = b->rangeExec && b->gParamVersion == b->paramVersion && b->gBoxes == d_boxes && b->gNq == nq &&
-                      b->gOff == d_out_off &&
-                      b->gIds == d_out_ids && b->gCap == out_cap && b->gStatus == d_status;
+                      b->gOff == d_out_off && b->gIds == d_out_ids && b->gCap == out_cap && b->gStatus == d_status &&
+                      b->gCtxStatus == ctx->status && b->gCtxWs == ctx->ws;
     if (!same) {
         if (b->rangeExec) TB_HIP(hipGraphExecDestroy(b->rangeExec));
         if (b->rangeGraph) TB_HIP(hipGraphDestroy(b->rangeGraph));
@@ -1584,6 +1586,8 @@ extern "C" int tblaeq_batch_range_graph(tblaeq_batch* b, const double* d_boxes, 
         b->gIds = d_out_ids;
         b->gCap = out_cap;
         b->gStatus = d_status;
+        b->gCtxStatus = ctx->status;
+        b->gCtxWs = ctx->ws;
     }
     // keep the host count of scan launches (epoch wrap protection) in step with replays
     for (uint64_t i = 0; i < b->gScanLaunches; ++i) {

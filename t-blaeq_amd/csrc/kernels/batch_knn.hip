@@ -1500,6 +1500,73 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
                     const double* d_radius, uint64_t* d_out_ids, double* d_out_d2, uint64_t* d_out_n,
                     uint8_t* d_fallback, uint64_t* d_status, uint64_t* d_level_surv, uint64_t* d_fine_count);
 
+// Context workspace of a KNN batch of nq queries:
+// thr | tau | bnd [2nq each] | ub [nq] | cntQ [nq] | offQ [nq+1] | flags [nq * (points of levels 0..L-2) bytes].
+static uint64_t knn_ws_bytes(const tblaeq_batch* b, uint64_t nq) {
+    uint64_t stride = 0;
+    for (int l = 0; l + 1 < b->L; ++l) stride += b->count[l];
+    if (stride == 0) stride = 1;
+    const uint64_t words = 6 * nq + nq + nq + nq + 1;
+    return 8 * words + nq * stride + 64;
+}
+
+extern "C" int tblaeq_batch_knn_graph(tblaeq_batch* b, const double* d_points, const uint64_t* d_K, uint64_t nq,
+                                      uint64_t Kmax, uint64_t* d_out_ids, double* d_out_d2, uint64_t* d_out_n,
+                                      uint64_t* d_status) {
+    if (!b) return TBLAEQ_ERR_INVALID;
+    tblaeq_ctx* ctx = b->ctx;
+    const bool same = b->knnExec && b->kParamVersion == b->paramVersion && b->kPoints == d_points && b->kK == d_K &&
+                      b->kNq == nq && b->kKmax == Kmax && b->kIds == d_out_ids && b->kD2 == d_out_d2 &&
+                      b->kN == d_out_n && b->kStatus == d_status && b->kCtxStatus == ctx->status &&
+                      b->kCtxWs == ctx->ws && ctx->wsBytes >= knn_ws_bytes(b, nq);
+    if (!same) {
+        if (b->knnExec) TB_HIP(hipGraphExecDestroy(b->knnExec));
+        if (b->knnGraph) TB_HIP(hipGraphDestroy(b->knnGraph));
+        b->knnExec = nullptr;
+        b->knnGraph = nullptr;
+        // the workspace must not be allocated inside the capture
+        int rc = ws_reserve(ctx, knn_ws_bytes(b, nq));
+        if (rc) return rc;
+        const bool timing = ctx->timing;
+        ctx->timing = false;  // no event records inside the graph
+        const uint64_t before = ctx->scanLaunches;
+        TB_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        rc = knn_impl(b, d_points, d_K, nq, Kmax, nullptr, d_out_ids, d_out_d2, d_out_n, nullptr, d_status, nullptr,
+                      nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        ctx->timing = timing;
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        TB_HIP(e);
+        b->knnGraph = g;
+        TB_HIP(hipGraphInstantiate(&b->knnExec, g, nullptr, nullptr, 0));
+        b->kScanLaunches = ctx->scanLaunches - before;
+        ctx->scanLaunches = before;
+        b->kParamVersion = b->paramVersion;
+        b->kPoints = d_points;
+        b->kK = d_K;
+        b->kNq = nq;
+        b->kKmax = Kmax;
+        b->kIds = d_out_ids;
+        b->kD2 = d_out_d2;
+        b->kN = d_out_n;
+        b->kStatus = d_status;
+        b->kCtxStatus = ctx->status;
+        b->kCtxWs = ctx->ws;
+    }
+    // keep the host count of scan launches (epoch wrap protection) in step with replays
+    for (uint64_t i = 0; i < b->kScanLaunches; ++i) {
+        int rc = scan_prepare(ctx, 1);
+        if (rc) return rc;
+    }
+    TimedScope ts(ctx, kKBatch0 + 6);
+    TB_HIP(hipGraphLaunch(b->knnExec, ctx->stream));
+    return TBLAEQ_OK;
+}
+
 extern "C" int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const uint64_t* d_K, uint64_t nq,
                                 uint64_t Kmax, uint64_t* d_out_ids, double* d_out_d2, uint64_t* d_out_n,
                                 uint64_t* d_status, uint64_t* d_level_surv, uint64_t* d_fine_count) {
@@ -1560,9 +1627,7 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
     if (stride == 0) stride = 1;
     // split: levels 0..L-2 query-major, the last level cluster-major
     const bool split = clusterMajor && L >= 2;
-    // workspace: thr | tau | bnd [2nq each] | ub [nq] | cntQ [nq] | offQ [nq+1] | flags [nq*stride bytes]
-    const uint64_t words = 6 * nq + nq + nq + nq + 1;
-    int rc = ws_reserve(ctx, 8 * words + nq * stride + 64);
+    int rc = ws_reserve(ctx, knn_ws_bytes(b, nq));
     if (rc) return rc;
     uint64_t* thr = static_cast<uint64_t*>(ctx->ws);
     uint64_t* tau = thr + 2 * nq;

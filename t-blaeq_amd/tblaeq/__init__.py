@@ -85,6 +85,7 @@ SIGNATURES = {
     "tblaeq_handler_batch_free": (C.c_int, [vp]),
     "tblaeq_handler_batch_use_graph": (C.c_int, [vp, C.c_int]),
     "tblaeq_batch_range_graph": (C.c_int, [vp, vp, C.c_uint64, vp, vp, C.c_uint64, vp]),
+    "tblaeq_batch_knn_graph": (C.c_int, [vp, vp, vp, C.c_uint64, C.c_uint64, vp, vp, vp, vp]),
     "tblaeq_batch_set_param": (C.c_int, [vp, C.c_int, C.c_uint64]),
     "tblaeq_batch_get_param": (C.c_int, [vp, C.c_int, u64p]),
     "tblaeq_handler_batch_set_param": (C.c_int, [vp, C.c_int, C.c_uint64]),
