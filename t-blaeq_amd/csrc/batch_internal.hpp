@@ -37,6 +37,9 @@ struct tblaeq_batch {
     uint64_t* off2 = nullptr;
     uint64_t* items2 = nullptr;
     float* aabb = nullptr;       // [count[L-1] * 2D] bounding box of each last-level position's points
+    double* enclC = nullptr;     // [count[L-2] * D] (L >= 2) centre of each level-(L-2) cluster, by id
+    double* enclR = nullptr;     // [count[L-2]] radius of a ball around it holding its children's balls
+    float* enclBox = nullptr;    // [count[L-2] * 2D] box lo[D] | hi[D] holding its children's balls, by id
     uint64_t* maskOff = nullptr; // [itemCap + 1] first keep-mask row of each work item of a level
     uint64_t* cnt = nullptr;     // [maxMesh]  per-cluster pair counts (kept zeroed between steps)
     uint64_t* off = nullptr;     // [maxMesh+1]

@@ -1173,6 +1173,56 @@ __global__ void k_aabb(const uint64_t* __restrict__ ids, uint64_t n, const uint6
     }
 }
 
+// Ball and box around each cluster (position j of a level, id ids[j], centre
+// coords[j]) that hold the balls of all its children (positions
+// colRes[id]..colRes[id+1] of the next level, centres childCoords, radii
+// childRadius): R >= |centre - child centre| + child radius for every child,
+// rounded up by a relative 1e-12; box lo[D] | hi[D] = the extent of the child
+// balls, rounded outward to float.  Written by id; a cluster without children
+// gets R = 0 and an empty box (+inf | -inf).
+__global__ void k_enclose(const uint64_t* __restrict__ ids, uint64_t n, const uint64_t* __restrict__ colRes,
+                          const double* __restrict__ coords, const double* __restrict__ childCoords,
+                          const double* __restrict__ childRadius, int D, double* __restrict__ cOut,
+                          double* __restrict__ rOut, float* __restrict__ boxOut) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t w0 = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / kWave;
+    const uint64_t nw = static_cast<uint64_t>(gridDim.x) * blockDim.x / kWave;
+    for (uint64_t j = w0; j < n; j += nw) {
+        const uint64_t c = ids[j];
+        const double* ctr = coords + j * D;
+        const uint64_t cb = colRes[c], ce = colRes[c + 1];
+        double r = 0.0;
+        for (uint64_t e = cb + lane; e < ce; e += kWave) {
+            const double* x = childCoords + e * D;
+            double d2 = 0.0;
+            for (int d = 0; d < D; ++d) {
+                const double diff = x[d] - ctr[d];
+                d2 = __builtin_fma(diff, diff, d2);
+            }
+            r = fmax(r, __builtin_sqrt(d2) + childRadius[e]);
+        }
+        r = wave_max_f64(r);
+        if (lane == 0) rOut[c] = r + r * 1e-12;
+        for (int d = lane; d < D; d += kWave) cOut[c * D + d] = ctr[d];
+        for (int d = 0; d < D; ++d) {
+            double mn = __builtin_inf(), mx = -__builtin_inf();
+            for (uint64_t e = cb + lane; e < ce; e += kWave) {
+                const double v = childCoords[e * D + d], cr = childRadius[e];
+                mn = fmin(mn, v - cr);
+                mx = fmax(mx, v + cr);
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                mn = fmin(mn, __shfl_xor(mn, o, kWave));
+                mx = fmax(mx, __shfl_xor(mx, o, kWave));
+            }
+            if (lane == 0) {
+                boxOut[c * 2 * D + d] = __double2float_rd(mn);
+                boxOut[c * 2 * D + D + d] = __double2float_ru(mx);
+            }
+        }
+    }
+}
+
 int launch_grouping(tblaeq_batch* b, uint64_t mesh) {
     tblaeq_ctx* ctx = b->ctx;
     uint64_t* ctl = b->ctl;
@@ -1353,6 +1403,19 @@ extern "C" int tblaeq_batch_create(tblaeq_ctx* ctx, const tblaeq_level* levels, 
         const uint64_t g = std::min<uint64_t>((n * kWave + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 16);
         hipLaunchKernelGGL(k_aabb, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream, b->ids[L - 1], n,
                            levels[L - 1].col_res, b->coords[L], static_cast<int>(D), b->aabb);
+    }
+    if (L >= 2) {
+        const uint64_t n = b->count[L - 2], cols = std::max<uint64_t>(n, levels[L - 2].cols);
+        b->enclC = balloc<double>(b, cols * D, &rc);
+        if (rc) return fail(rc);
+        b->enclR = balloc<double>(b, cols, &rc);
+        if (rc) return fail(rc);
+        b->enclBox = balloc<float>(b, cols * 2 * D, &rc);
+        if (rc) return fail(rc);
+        const uint64_t g = std::min<uint64_t>((n * kWave + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 16);
+        hipLaunchKernelGGL(k_enclose, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream,
+                           b->ids[L - 2], n, levels[L - 2].col_res, b->coords[L - 2], b->coords[L - 1],
+                           b->radiusPos[L - 1], static_cast<int>(D), b->enclC, b->enclR, b->enclBox);
     }
     b->ctl = balloc<uint64_t>(b, kCtlWords + kQueueSlots, &rc);
     if (rc) return fail(rc);

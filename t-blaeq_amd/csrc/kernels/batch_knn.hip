@@ -731,20 +731,22 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
 }
 
 // --------------------------------------------- cluster-major last level (split)
-// Work item = (cluster p of level L-2, slice of <= 64 queries that kept p);
-// lane = query, p's children (last-level clusters) staged in LDS.  Three
-// passes share the structure:
+// Work item = (cluster p of level L-2, slice of <= 64 queries that kept p),
+// wave w taking items w, w + waves, ...  The slice's queries first test p's
+// enclosing ball and box; those that may still reach a child are staged in LDS
+// (point and thresholds), and the wave then runs lane = child of p, one staged
+// query at a time.  Three passes share the structure:
 //   kCand   records (okey(lb), q, id) with (lb, id) <= tau[q]: a superset of
 //           each query's best M clusters, selected exactly afterwards;
-//   kCount  survivor statistics (lb <= ub) and, per cluster, the number of
-//           queries that must refine it (survivor, not a seed, lb within the
-//           certified radius);
-//   kWrite  the same test, writing those queries into per-cluster lists.
+//   kLCount per cluster, the number of queries that must refine it
+//           (survivor, not a seed, lb within the certified radius); kLStats
+//           also counts the survivors (lb <= ub) and their points per query;
+//   kLWrite the same test, writing those queries into per-cluster lists.
 // lb is computed exactly (sqrt_rd, as everywhere) only where the squared
 // distance does not already decide the test: with t the threshold and
 // B = t + r + 1e-9(|t| + r), d2 > B^2 implies sqrt_rd(d2) - r > t by far more
 // than the rounding of the bounds.
-enum LastPass { kCand = 0, kLCount = 1, kLWrite = 2 };
+enum LastPass { kCand = 0, kLCount = 1, kLWrite = 2, kLStats = 3 };
 
 struct KnnLastArgs {
     int D, L;
@@ -787,6 +789,10 @@ struct KnnLastArgs {
     uint64_t* maskOff;
     uint64_t* maskNext;
     uint64_t maskCap;
+    // (L-2)-level clusters by id: centre and the radius of a ball holding their children's balls
+    const double* enclC;
+    const double* enclR;
+    const float* enclBox;    // lo[D] | hi[D] of the children's balls
 };
 
 constexpr int kRecIdBits = 40;
@@ -818,37 +824,28 @@ __device__ __forceinline__ void write_pairs(const KnnLastArgs& A, uint64_t cid, 
     }
 }
 
+// LDS row of a staged query: point (DM) | G | five pass parameters.
+template <int DM>
+constexpr int last_row() { return DM + 6; }
+
 template <int DM, int PASS>
 __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_last(KnnLastArgs A) {
-    __shared__ double s_x[kStepWaves][kWave * DM];
-    __shared__ double s_r[kStepWaves][kWave];
-    __shared__ uint64_t s_id[kStepWaves][kWave];
-    __shared__ uint64_t s_w[kStepWaves][kWave];
+    constexpr bool COUNT = PASS == kLCount || PASS == kLStats;
+    constexpr bool STATS = PASS == kLStats;
+    constexpr int RS = last_row<DM>();
+    __shared__ double s_q[kStepWaves][kWave * RS];
     __shared__ uint64_t s_rk[kStepWaves][PASS == kCand ? kRecStage : 1];
     __shared__ uint64_t s_rv[kStepWaves][PASS == kCand ? kRecStage : 1];
     const int lane = threadIdx.x & (kWave - 1);
     const uint64_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const uint64_t gw = blockIdx.x * static_cast<uint64_t>(kStepWaves) + wave;
-    const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kStepWaves;
     const int D = A.D;
     uint64_t nItems = *A.nItems;
     if (nItems > A.itemCap) nItems = A.itemCap;
     const uint64_t ltmask = (1ull << lane) - 1ull;
-    double* sx = s_x[wave];
-    if (D != DM)
-        for (int t = lane; t < kWave * DM; t += kWave) sx[t] = 0.0;
+    double* sq = s_q[wave];
+    const ConstF64 encC = (ConstF64)A.enclC;
+    const ConstF32 encB = (ConstF32)A.enclBox;
     uint64_t staged = 0;
-    // kLCount: one reservation of pair-mask rows for all of this wave's items
-    uint64_t maskBase = 0;
-    if (PASS == kLCount && A.masks) {
-        uint64_t rows = 0;
-        for (uint64_t item = gw; item < nItems; item += nw) {
-            const uint64_t p = A.items[item] >> kSliceBits;
-            rows += A.colResPrev[p + 1] - A.colResPrev[p];
-        }
-        if (lane == 0 && rows) maskBase = atomicAdd(reinterpret_cast<unsigned long long*>(A.maskNext), rows);
-        maskBase = uniform_u64(__shfl(maskBase, 0, kWave));
-    }
     auto flushRec = [&]() {
         if (staged == 0) return;
         wave_lds_sync();
@@ -865,6 +862,19 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_last(KnnLastArgs A) 
         wave_lds_sync();
         staged = 0;
     };
+    const uint64_t gw = blockIdx.x * static_cast<uint64_t>(kStepWaves) + wave;
+    const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kStepWaves;
+    // count passes: one reservation of pair-mask rows for all of this wave's items
+    uint64_t maskBase = 0;
+    if (COUNT && A.masks) {
+        uint64_t rows = 0;
+        for (uint64_t item = gw; item < nItems; item += nw) {
+            const uint64_t p = A.items[item] >> kSliceBits;
+            rows += A.colResPrev[p + 1] - A.colResPrev[p];
+        }
+        if (lane == 0 && rows) maskBase = atomicAdd(reinterpret_cast<unsigned long long*>(A.maskNext), rows);
+        maskBase = uniform_u64(__shfl(maskBase, 0, kWave));
+    }
     for (uint64_t item = gw; item < nItems; item += nw) {
         const uint64_t it = A.items[item];
         const uint64_t p = it >> kSliceBits;
@@ -877,33 +887,11 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_last(KnnLastArgs A) 
         const uint64_t ql = inList ? A.qlist[g0 + start + lane] : 0;
         const bool valid = inList && ql < A.nq;
         const uint64_t q = valid ? ql : 0;
-        double qp[DM];
-#pragma unroll
-        for (int d = 0; d < DM; ++d) qp[d] = (d < D) ? A.points[q * D + d] : 0.0;
-        const uint64_t K = valid ? A.Ks[q] : 0;
-        // per-lane thresholds
-        uint64_t tK = 0, tV = 0, bK = 0, bV = 0;
-        double t1 = -__builtin_inf(), ub = -__builtin_inf(), rho = __builtin_inf();
-        bool rhoInf = true;
-        if (PASS == kCand) {
-            if (valid && K > 0) {
-                tK = A.tau[2 * q];
-                tV = A.tau[2 * q + 1];
-            }
-        } else if (valid) {
-            ub = A.ub[q];
-            bK = A.bnd[2 * q];
-            bV = A.bnd[2 * q + 1];
-            const uint64_t rk = A.thr[2 * q];
-            rhoInf = rk == ~0ull;
-            if (!rhoInf) rho = __builtin_sqrt(okey_inv(rk));
-        }
-        if (PASS == kCand && tK != ~0ull) t1 = okey_inv(tK);
         const uint64_t jb = A.colResPrev[p], je = A.colResPrev[p + 1];
         uint64_t mo = 0;
         bool stored = false;
         if (PASS != kCand && A.masks) {
-            if (PASS == kLCount) {
+            if (COUNT) {
                 mo = maskBase;
                 maskBase += je - jb;
                 if (lane == 0) A.maskOff[item] = mo;
@@ -922,86 +910,171 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_last(KnnLastArgs A) 
             }
             continue;
         }
+        double qp[DM];
+#pragma unroll
+        for (int d = 0; d < DM; ++d) qp[d] = (d < D) ? A.points[q * D + d] : 0.0;
+        const uint64_t K = valid ? A.Ks[q] : 0;
+        // per-query thresholds (lane = query of the slice)
+        uint64_t tK = ~0ull, tV = 0, bK = 0, bV = 0;
+        double ub = -__builtin_inf(), rho = __builtin_inf(), tPar = __builtin_inf();
+        bool need = false;
+        const double E = A.enclR[p];
+        if (PASS == kCand) {
+            if (valid && K > 0) {
+                tK = A.tau[2 * q];
+                tV = A.tau[2 * q + 1];
+                need = true;
+                if (tK != ~0ull) tPar = okey_inv(tK);
+            }
+        } else if (valid) {
+            ub = A.ub[q];
+            bK = A.bnd[2 * q];
+            bV = A.bnd[2 * q + 1];
+            const uint64_t rk = A.thr[2 * q];
+            if (rk != ~0ull) rho = __builtin_sqrt(okey_inv(rk));
+            need = STATS || K > 0;
+            // every child radius is at most E, so this bounds each child's threshold
+            tPar = (STATS || rho == __builtin_inf()) ? ub : fmin(ub, rho + 1e-9 * (1.0 + rho + E));
+        }
+        // Whole parent: every child ball lies in the ball (centre C, radius E)
+        // and in the box of the parent, so a child's lb is at least |q - C| - E
+        // and, for q outside the box, at least the distance to the box; when
+        // either exceeds the threshold by far more than the rounding of both
+        // sides, no child of this parent can pass for this query.
+        if (need) {
+            double c[DM];
+            scalar_row<DM, false>(c, encC, static_cast<uint32_t>(p), D);
+            double d2 = 0.0, b2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < DM; ++d) {
+                const double diff = qp[d] - c[d];
+                d2 = __builtin_fma(diff, diff, d2);
+                if (d < D) {
+                    const double lo = encB[p * 2 * D + d], hi = encB[p * 2 * D + D + d];
+                    const double g = fmax(fmax(lo - qp[d], qp[d] - hi), 0.0);
+                    b2 = __builtin_fma(g, g, b2);
+                }
+            }
+            const double dP = __builtin_sqrt(d2), dB = __builtin_sqrt(b2);
+            const double slack = 1e-9 * (1.0 + __builtin_fabs(tPar) + dP + E);
+            if (dP - E > tPar + slack || dB > fmax(tPar, 0.0) + slack) need = false;
+        }
+        const uint64_t act = __ballot(need);
+        if (act == 0) {
+            if (COUNT && stored)
+                for (uint64_t j = jb + lane; j < je; j += kWave) A.masks[mo + (j - jb)] = 0;
+            continue;
+        }
+        // Per test, d2 > (G + r(1 + 2e-9))^2 with G = T + 2e-9|T| rejects
+        // everything the exact guard rejects for any per-child threshold t <= T.
+        {
+            double* row = sq + lane * RS;
+#pragma unroll
+            for (int d = 0; d < DM; ++d) row[d] = qp[d];
+            row[DM] = tPar + 2e-9 * __builtin_fabs(tPar);
+            if (PASS == kCand) {
+                row[DM + 1] = __longlong_as_double(tK);
+                row[DM + 2] = __longlong_as_double(tV);
+                row[DM + 3] = __longlong_as_double(q);
+            } else {
+                row[DM + 1] = ub;
+                row[DM + 2] = rho;
+                row[DM + 3] = __longlong_as_double(bK);
+                row[DM + 4] = __longlong_as_double(bV);
+                row[DM + 5] = __longlong_as_double(K > 0 ? 1 : 0);
+            }
+        }
+        wave_lds_sync();
         uint64_t mine = 0, fine = 0;
         for (uint64_t j0 = jb; j0 < je; j0 += kWave) {
             const int nc = static_cast<int>(je - j0 < static_cast<uint64_t>(kWave) ? je - j0 : kWave);
-            {
-                const double* src = A.coords + j0 * D;
-                const int tot = nc * D;
-                if (D == DM) {
-                    for (int t = lane; t < tot; t += kWave) sx[t] = src[t];
-                } else {
-                    for (int t = lane; t < tot; t += kWave) sx[(t / D) * DM + t % D] = src[t];
-                }
-                if (lane < nc) {
-                    s_r[wave][lane] = A.radius[j0 + lane];
-                    s_id[wave][lane] = A.ids[j0 + lane];
-                    s_w[wave][lane] = A.weight[j0 + lane];
-                }
-            }
-            wave_lds_sync();
-            uint64_t myMask = 0;
-            for (int k = 0; k < nc; ++k) {
-                const double* x = sx + k * DM;
-                double xv[DM];
+            const bool child = lane < nc;
+            // lane = child of the chunk
+            double xv[DM];
+            double r = 0.0, rg = -__builtin_inf();
+            uint64_t id = 0, w = 0;
+            if (child) {
+                load_row<DM>(A.coords + (j0 + lane) * D, D, xv);
+                r = A.radius[j0 + lane];
+                rg = r + 2e-9 * r;
+                id = A.ids[j0 + lane];
+                if (STATS) w = A.weight[j0 + lane];
+            } else {
 #pragma unroll
-                for (int d = 0; d < DM; ++d) xv[d] = x[d];
-                const double d2 = d2_reg<DM>(xv, qp);
-                const double r = s_r[wave][k];
-                const uint64_t id = s_id[wave][k];
+                for (int d = 0; d < DM; ++d) xv[d] = 0.0;
+            }
+            uint64_t myMask = 0;
+            for (uint64_t m = act; m; m &= m - 1) {
+                const int i = __builtin_ctzll(m);
+                const double* qr = sq + i * RS;
+                const double d2 = d2_reg<DM>(xv, qr);
+                const double Bp = qr[DM] + rg;
+                const bool pre = Bp >= 0.0 && d2 <= Bp * Bp;
+                if (__ballot(pre) == 0) continue;
                 if (PASS == kCand) {
+                    const uint64_t tKi = __double_as_longlong(qr[DM + 1]);
+                    const uint64_t tVi = __double_as_longlong(qr[DM + 2]);
                     bool emit = false;
                     uint64_t key = 0;
-                    if (valid && K > 0) {
-                        const double B = guard_bound(t1, r);
-                        if (tK == ~0ull || (B >= 0.0 && d2 <= B * B)) {
+                    if (pre) {
+                        const double B = guard_bound(okey_inv(tKi), r);
+                        if (tKi == ~0ull || (B >= 0.0 && d2 <= B * B)) {
                             key = okey(sqrt_rd(d2) - r);
-                            emit = key < tK || (key == tK && id <= tV);
+                            emit = key < tKi || (key == tKi && id <= tVi);
                         }
                     }
-                    const uint64_t m = __ballot(emit);
-                    if (m) {
-                        const uint64_t c = __popcll(m);
-                        if (staged + c > static_cast<uint64_t>(kRecStage)) flushRec();
+                    const uint64_t em = __ballot(emit);
+                    if (em) {
+                        const uint64_t qi = __double_as_longlong(qr[DM + 3]);
+                        const uint64_t cnt = __popcll(em);
+                        if (staged + cnt > static_cast<uint64_t>(kRecStage)) flushRec();
                         if (emit) {
-                            const uint64_t o = staged + __popcll(m & ltmask);
+                            const uint64_t o = staged + __popcll(em & ltmask);
                             s_rk[wave][o] = key;
-                            s_rv[wave][o] = (q << kRecIdBits) | id;
+                            s_rv[wave][o] = (qi << kRecIdBits) | id;
                         }
-                        staged += c;
+                        staged += cnt;
                     }
                 } else {
+                    const double ubi = qr[DM + 1], rhoi = qr[DM + 2];
+                    const uint64_t bKi = __double_as_longlong(qr[DM + 3]);
+                    const uint64_t bVi = __double_as_longlong(qr[DM + 4]);
+                    const bool pairable = __double_as_longlong(qr[DM + 5]) != 0;
+                    const bool rhoInf = rhoi == __builtin_inf();
                     bool surv = false, pair = false;
-                    if (valid) {
-                        const double tp = rhoInf ? ub : fmin(ub, rho + 1e-9 * (1.0 + rho + r));
-                        const double t = (PASS == kLCount && A.survStat) ? ub : tp;
+                    if (pre) {
+                        const double tp = rhoInf ? ubi : fmin(ubi, rhoi + 1e-9 * (1.0 + rhoi + r));
+                        const double t = STATS ? ubi : tp;
                         const double B = guard_bound(t, r);
                         if (B >= 0.0 && d2 <= B * B) {
                             const double lb = sqrt_rd(d2) - r;
-                            surv = lb <= ub;
-                            const bool seed = !lt_pair(bK, bV, okey(lb), id);
-                            pair = surv && K > 0 && !seed && (rhoInf || lb <= rho + 1e-9 * (1.0 + rho + r));
+                            surv = lb <= ubi;
+                            const bool seed = !lt_pair(bKi, bVi, okey(lb), id);
+                            pair = surv && pairable && !seed && (rhoInf || lb <= rhoi + 1e-9 * (1.0 + rhoi + r));
                         }
                     }
-                    if (PASS == kLCount && surv) {
-                        ++mine;
-                        fine += s_w[wave][k];
+                    if (pair) myMask |= 1ull << i;
+                    if (STATS) {
+                        const uint64_t sm = __ballot(surv);
+                        const uint64_t fw = wave_sum_u64(surv ? w : 0);
+                        if (lane == i) {
+                            mine += __popcll(sm);
+                            fine += fw;
+                        }
                     }
-                    const uint64_t m = __ballot(pair);
-                    if (lane == k) myMask = m;
                 }
             }
-            if (PASS == kLCount) {
-                if (lane < nc && myMask)
-                    atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[s_id[wave][lane]]),
+            if (COUNT) {
+                if (child && myMask)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[id]),
                               static_cast<unsigned long long>(__popcll(myMask)));
-                if (stored && lane < nc) A.masks[mo + (j0 - jb) + lane] = myMask;
+                if (stored && child) A.masks[mo + (j0 - jb) + lane] = myMask;
             } else if (PASS == kLWrite) {
-                write_pairs(A, lane < nc ? s_id[wave][lane] : 0, myMask, nc, q, lane, ltmask);
+                write_pairs(A, id, myMask, nc, q, lane, ltmask);
             }
-            wave_lds_sync();
         }
-        if (PASS == kLCount && valid) {
+        wave_lds_sync();
+        if (STATS && valid) {
             if (A.survStat && mine)
                 atomicAdd(reinterpret_cast<unsigned long long*>(&A.survStat[q * A.L + A.L - 1]), mine);
             if (A.fineStat && fine) atomicAdd(reinterpret_cast<unsigned long long*>(&A.fineStat[q]), fine);
@@ -1736,6 +1809,9 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
         LA.maskOff = b->maskOff;
         LA.maskNext = b->queue + kMaskQueue0;
         LA.maskCap = b->hitCap;
+        LA.enclC = b->enclC;
+        LA.enclR = b->enclR;
+        LA.enclBox = b->enclBox;
         auto launchLast = [&](int pass) {
             TimedScope ts(ctx, kKBatch0 + 9);
 #define TB_LAST(DMV)                                                                             \
@@ -1743,6 +1819,8 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
         if (pass == kCand) hipLaunchKernelGGL((k_knn_last<DMV, kCand>), sg, st, 0, ctx->stream, LA);     \
         else if (pass == kLCount)                                                                \
             hipLaunchKernelGGL((k_knn_last<DMV, kLCount>), sg, st, 0, ctx->stream, LA);          \
+        else if (pass == kLStats)                                                                \
+            hipLaunchKernelGGL((k_knn_last<DMV, kLStats>), sg, st, 0, ctx->stream, LA);          \
         else hipLaunchKernelGGL((k_knn_last<DMV, kLWrite>), sg, st, 0, ctx->stream, LA);         \
     } while (0)
             if (LA.D <= 4) TB_LAST(4);
@@ -1803,7 +1881,7 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
         LA.survStat = d_level_surv;
         LA.fineStat = d_fine_count;
         LA.masks = b->hits;  // free again: the records went to b->pairs / b->recAux
-        launchLast(kLCount);
+        launchLast(d_level_surv || d_fine_count ? kLStats : kLCount);
         TB_HIP(hipGetLastError());
         if ((rc = launch_group_scan(b, b->count[L - 1], b->off2, b->items2, ctl + 4))) return rc;
         LA.survStat = nullptr;
