@@ -639,8 +639,11 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
                         // pairs of this cluster for the grouping scan, one atomic for the workgroup's queries
                         if (kept) atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[id]), kept);
                     }
+                    // every thread reads the count before any queues again
                     __syncthreads();
-                    if (S.listN >= static_cast<uint32_t>(kKnnFlush)) knn_emit_pairs(A, S, tid, q0);
+                    const bool due = S.listN >= static_cast<uint32_t>(kKnnFlush);
+                    __syncthreads();
+                    if (due) knn_emit_pairs(A, S, tid, q0);
                 }
                 knn_emit_pairs(A, S, tid, q0);
                 if (tid < kKnnQB && A.survStat && q0 + tid < A.nq) A.survStat[(q0 + tid) * L + l] = S.surv[tid];
@@ -709,8 +712,11 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_batch(KnnArgs A) {
                     }
                     if (kept) atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[id]), kept);
                 }
+                // every thread reads the count before any queues again
                 __syncthreads();
-                if (S.listN >= static_cast<uint32_t>(kKnnFlush)) {
+                const bool due = S.listN >= static_cast<uint32_t>(kKnnFlush);
+                __syncthreads();
+                if (due) {
                     if (A.pairs) knn_emit_pairs(A, S, tid, q0);
                     else knn_flush<DM>(S, tid, D, A.coords[L], A.ids[L], nullptr, A.work);
                 }
@@ -808,13 +814,15 @@ __device__ __forceinline__ double guard_bound(double t, double r) {
 __device__ __forceinline__ void write_pairs(const KnnLastArgs& A, uint64_t cid, uint64_t myMask, int nc, uint64_t q,
                                             int lane, uint64_t ltmask) {
     uint64_t myBase = 0;
-    if (lane < nc && myMask) {
+    const bool any = lane < nc && myMask;
+    if (any) {
         const uint64_t pc = __popcll(myMask);
         myBase = A.offNext[cid] + (atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[cid]), 0ull - pc) - pc);
     }
-    for (int k = 0; k < nc; ++k) {
+    // only the rows that hold pairs
+    for (uint64_t nz = __ballot(any); nz; nz &= nz - 1) {
+        const int k = __builtin_ctzll(nz);
         const uint64_t m = readlane_u64(myMask, k);
-        if (m == 0) continue;
         const uint64_t base = readlane_u64(myBase, k);
         if ((m >> lane) & 1ull) {
             const uint64_t o = base + __popcll(m & ltmask);
