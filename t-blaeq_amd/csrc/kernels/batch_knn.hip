@@ -795,6 +795,10 @@ struct KnnLastArgs {
     uint64_t* maskOff;
     uint64_t* maskNext;
     uint64_t maskCap;
+    // beside each stored mask row: the row's first slot in its cluster's query
+    // list (the count pass's atomic returns it), so that kLWrite, when every
+    // item's rows were stored, writes the lists without atomics
+    uint64_t* maskPos;
     // (L-2)-level clusters by id: centre and the radius of a ball holding their children's balls
     const double* enclC;
     const double* enclR;
@@ -853,6 +857,9 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_last(KnnLastArgs A) 
     double* sq = s_q[wave];
     const ConstF64 encC = (ConstF64)A.enclC;
     const ConstF32 encB = (ConstF32)A.enclBox;
+    // kLWrite: every item's mask rows were stored (the count pass reserved
+    // within the capacity), so their list slots are known
+    const bool allStored = PASS == kLWrite && A.masks && *A.maskNext <= A.maskCap;
     uint64_t staged = 0;
     auto flushRec = [&]() {
         if (staged == 0) return;
@@ -912,9 +919,23 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_last(KnnLastArgs A) 
             // pair masks of the count pass: only the writes remain
             for (uint64_t j0 = jb; j0 < je; j0 += kWave) {
                 const int nc = static_cast<int>(je - j0 < static_cast<uint64_t>(kWave) ? je - j0 : kWave);
-                const uint64_t cid = lane < nc ? A.ids[j0 + lane] : 0;
                 const uint64_t myMask = lane < nc ? A.masks[mo + (j0 - jb) + lane] : 0;
-                write_pairs(A, cid, myMask, nc, q, lane, ltmask);
+                const bool any = myMask != 0;
+                if (allStored) {
+                    // slots known from the count pass
+                    const uint64_t myBase = any ? A.offNext[A.ids[j0 + lane]] + A.maskPos[mo + (j0 - jb) + lane] : 0;
+                    for (uint64_t nz = __ballot(any); nz; nz &= nz - 1) {
+                        const int k = __builtin_ctzll(nz);
+                        const uint64_t m = readlane_u64(myMask, k);
+                        if ((m >> lane) & 1ull) {
+                            const uint64_t o = readlane_u64(myBase, k) + __popcll(m & ltmask);
+                            if (o < A.cap) A.qlistNext[o] = static_cast<uint32_t>(q);
+                            else atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 1ull);
+                        }
+                    }
+                } else {
+                    write_pairs(A, any ? A.ids[j0 + lane] : 0, myMask, nc, q, lane, ltmask);
+                }
             }
             continue;
         }
@@ -1073,10 +1094,14 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_last(KnnLastArgs A) 
                 }
             }
             if (COUNT) {
+                uint64_t slot = 0;
                 if (child && myMask)
-                    atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[id]),
-                              static_cast<unsigned long long>(__popcll(myMask)));
-                if (stored && child) A.masks[mo + (j0 - jb) + lane] = myMask;
+                    slot = atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[id]),
+                                     static_cast<unsigned long long>(__popcll(myMask)));
+                if (stored && child) {
+                    A.masks[mo + (j0 - jb) + lane] = myMask;
+                    A.maskPos[mo + (j0 - jb) + lane] = slot;
+                }
             } else if (PASS == kLWrite) {
                 write_pairs(A, id, myMask, nc, q, lane, ltmask);
             }
@@ -1814,6 +1839,7 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
         LA.cap = b->pairCap;
         LA.overflow = ctl + 3;
         LA.masks = nullptr;  // kCand writes its records into b->hits
+        LA.maskPos = b->hitsTmp;  // free once the records are grouped
         LA.maskOff = b->maskOff;
         LA.maskNext = b->queue + kMaskQueue0;
         LA.maskCap = b->hitCap;
@@ -1896,6 +1922,9 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
         LA.fineStat = nullptr;
         launchLast(kLWrite);
         TB_HIP(hipGetLastError());
+        // the list writes count the per-cluster pairs back to zero only when
+        // they take their slots by atomics
+        launch_fill(b, b->cnt, 0, b->count[L - 1]);
         rItems = b->items2;
         rN = ctl + 4;
         rOff = b->off2;
