@@ -1181,16 +1181,31 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_select(KnnSelectArgs A) {
             S.qp[i][d] = q < A.nq && d < D ? A.points[q * D + d] : 0.0;
         }
         __syncthreads();
-        int par = 0;
+        // every query slot's records in the same rounds (one per thread and slot)
+        uint64_t r0[kKnnQB], rn[kKnnQB], rmax = 0;
+#pragma unroll
         for (int i = 0; i < kKnnQB; ++i) {
             const uint64_t q = q0 + i;
-            if (q >= A.nq || S.M[i] == 0) continue;
-            const uint64_t r0 = A.offQ[q], rn = A.offQ[q + 1] - r0;
-            for (uint64_t b = 0; b < rn; b += kKnnThreads, par ^= 1) {
-                const uint64_t r = b + tid;
-                const bool have = r < rn;
-                knn_round1(S, tid, i, have, have ? A.gK[r0 + r] : 0, have ? A.gV[r0 + r] : 0, par);
+            r0[i] = 0;
+            rn[i] = 0;
+            if (q < A.nq && S.M[i] > 0) {
+                r0[i] = A.offQ[q];
+                rn[i] = A.offQ[q + 1] - r0[i];
             }
+            rmax = rn[i] > rmax ? rn[i] : rmax;
+        }
+        int par = 0;
+        for (uint64_t b = 0; b < rmax; b += kKnnThreads, par ^= 1) {
+            bool have[kKnnQB];
+            uint64_t key[kKnnQB], val[kKnnQB];
+#pragma unroll
+            for (int i = 0; i < kKnnQB; ++i) {
+                const uint64_t r = b + tid;
+                have[i] = r < rn[i];
+                key[i] = have[i] ? A.gK[r0[i] + r] : 0;
+                val[i] = have[i] ? A.gV[r0[i] + r] : 0;
+            }
+            knn_round(S, tid, have, key, val, par);
         }
         __syncthreads();
         knn_compact(S, tid);
@@ -1478,15 +1493,23 @@ __global__ __launch_bounds__(kKnnThreads) void k_knn_merge(KnnMergeArgs A) {
             if (tid == 0) S.cnt[i] = n0;
         }
         __syncthreads();
+        // every query slot's records in the same rounds (one per thread and slot)
+        uint64_t rmax = 0;
+#pragma unroll
+        for (int i = 0; i < kKnnQB; ++i)
+            if (S.M[i] > 0 && s_rn[i] > rmax) rmax = s_rn[i];
         int par = 0;
-        for (int i = 0; i < kKnnQB; ++i) {
-            if (S.M[i] == 0) continue;
-            const uint64_t r0 = s_r0[i], rn = s_rn[i];
-            for (uint64_t b = 0; b < rn; b += kKnnThreads, par ^= 1) {
+        for (uint64_t b = 0; b < rmax; b += kKnnThreads, par ^= 1) {
+            bool have[kKnnQB];
+            uint64_t key[kKnnQB], val[kKnnQB];
+#pragma unroll
+            for (int i = 0; i < kKnnQB; ++i) {
                 const uint64_t r = b + tid;
-                const bool have = r < rn;
-                knn_round1(S, tid, i, have, have ? A.gK[r0 + r] : 0, have ? A.gV[r0 + r] : 0, par);
+                have[i] = S.M[i] > 0 && r < s_rn[i];
+                key[i] = have[i] ? A.gK[s_r0[i] + r] : 0;
+                val[i] = have[i] ? A.gV[s_r0[i] + r] : 0;
             }
+            knn_round(S, tid, have, key, val, par);
         }
         __syncthreads();
         knn_compact(S, tid);
