@@ -117,6 +117,10 @@ struct RangeStepArgs {
     uint64_t* maskOff;
     uint64_t* maskNext;
     uint64_t maskCap;
+    // beside each stored mask row: the row's first slot in its cluster's query
+    // list (returned by the count pass's atomic), so that the write pass, when
+    // every row of the level was stored, places the queries without atomics
+    uint64_t* maskPos;
 };
 
 // Appends a wave's staged hits to the global hit list (entries past the
@@ -172,6 +176,11 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
     if (!A.level0 && nItems > A.itemCap) nItems = A.itemCap;
     const uint64_t ltmask = (1ull << lane) - 1ull;
     const bool useBox = A.aabb != nullptr;
+    // write pass: every item's mask rows were stored (level 0: item i's rows at
+    // i * count0; else the count pass reserved within the capacity)
+    const bool allStored = MODE == kWrite && A.masks &&
+                           (A.level0 ? ((A.nq + kWave - 1) / kWave) * A.count0 <= A.maskCap
+                                     : ((ConstU64)A.maskNext)[0] <= A.maskCap);
     // count pass: one reservation of keep-mask rows for all of this wave's items
     uint64_t maskBase = 0;
     if (MODE == kCount && A.masks && !A.level0) {
@@ -221,9 +230,23 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
             // masks of the count pass: only the writes remain
             for (uint64_t j0 = jb; j0 < je; j0 += kWave) {
                 const int nc = static_cast<int>(je - j0 < static_cast<uint64_t>(kWave) ? je - j0 : kWave);
-                const uint64_t myId = lane < nc ? A.ids[j0 + lane] : 0;
                 const uint64_t myMask = lane < nc ? A.masks[mo + (j0 - jb) + lane] : 0;
-                write_kept(A, myId, myMask, nc, q, lane, ltmask);
+                if (allStored) {
+                    const bool any = myMask != 0;
+                    const uint64_t myBase = any ? A.offNext[A.ids[j0 + lane]] + A.maskPos[mo + (j0 - jb) + lane] : 0;
+                    for (uint64_t nz = __ballot(any); nz; nz &= nz - 1) {
+                        const int k = __builtin_ctzll(nz);
+                        const uint64_t m = readlane_u64(myMask, k);
+                        if ((m >> lane) & 1ull) {
+                            const uint64_t o = readlane_u64(myBase, k) + __popcll(m & ltmask);
+                            if (o < A.cap) A.qlistNext[o] = static_cast<uint32_t>(q);
+                            else atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 1ull);
+                        }
+                    }
+                } else {
+                    const uint64_t myId = lane < nc ? A.ids[j0 + lane] : 0;
+                    write_kept(A, myId, myMask, nc, q, lane, ltmask);
+                }
             }
             continue;
         }
@@ -307,10 +330,14 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
                         }
                     }
                 }
+                uint64_t slot = 0;
                 if (lane < nc && myMask)
-                    atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[myId]),
-                              static_cast<unsigned long long>(__popcll(myMask)));
-                if (stored && lane < nc) A.masks[mo + (j0 - jb) + lane] = myMask;
+                    slot = atomicAdd(reinterpret_cast<unsigned long long*>(&A.cnt[myId]),
+                                     static_cast<unsigned long long>(__popcll(myMask)));
+                if (stored && lane < nc) {
+                    A.masks[mo + (j0 - jb) + lane] = myMask;
+                    A.maskPos[mo + (j0 - jb) + lane] = slot;
+                }
                 continue;
             }
             write_kept(A, myId, myMask, nc, q, lane, ltmask);
@@ -1548,6 +1575,7 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
         const uint64_t bound = l == 0 ? (nq + kWave - 1) / kWave : b->itemCap;
         // keep masks of the count pass, reused by the write pass
         A.masks = b->hits;
+        A.maskPos = b->hitsTmp;
         A.maskCap = b->hitCap;
         A.maskOff = b->maskOff;
         A.maskNext = b->queue + kMaskQueue0 + l;
@@ -1565,6 +1593,9 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
         A.survStat = nullptr;
         A.fineStat = nullptr;
         if ((rc = launch_range_step<kWrite>(b, A, bound))) return rc;
+        // the writes count the pairs back to zero only when they take their
+        // slots by atomics
+        launch_fill(b, b->cnt, 0, mesh);
         cur = nxt;
     }
     // final step: exact filter of the finest candidates
