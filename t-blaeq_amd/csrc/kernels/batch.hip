@@ -121,6 +121,9 @@ struct RangeStepArgs {
     // list (returned by the count pass's atomic), so that the write pass, when
     // every row of the level was stored, places the queries without atomics
     uint64_t* maskPos;
+    // with a box test (aabb): whether the ball test is also applied (both
+    // passes of a level must agree; statistics runs need it for their counts)
+    int keepBall;
 };
 
 // Appends a wave's staged hits to the global hit list (entries past the
@@ -176,6 +179,10 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
     if (!A.level0 && nItems > A.itemCap) nItems = A.itemCap;
     const uint64_t ltmask = (1ull << lane) - 1ull;
     const bool useBox = A.aabb != nullptr;
+    // Without statistics the last prune keeps (cluster, query) by the box test
+    // alone: the box of a cluster's points is the tighter bound in these
+    // dimensions, and the exact filter decides every point anyway.
+    const bool ballTest = !useBox || A.keepBall;
     // write pass: every item's mask rows were stored (level 0: item i's rows at
     // i * count0; else the count pass reserved within the capacity)
     const bool allStored = MODE == kWrite && A.masks &&
@@ -286,7 +293,8 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
                     rn = crad[j + 1];
                 }
                 double dist = 0.0;
-                if (fastDist) {
+                if (!ballTest) {
+                } else if (fastDist) {
 #pragma unroll
                     for (int d = 0; d < DM; ++d) {
                         const double diff = fmax(fmax(lo[d] - xv[d], xv[d] - hi[d]), 0.0);
@@ -301,7 +309,7 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
                     }
                 }
                 const bool ball = valid && dist <= r * r;
-                bool keep = ball;
+                bool keep = ballTest ? ball : valid;
                 if (useBox) {
 #pragma unroll
                     for (int d = 0; d < DM; ++d) {
@@ -1566,6 +1574,7 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
         A.radiusPos = b->radiusPos[l];
         A.ids = b->ids[l];
         A.aabb = l == L - 1 ? b->aabb : nullptr;
+        A.keepBall = (d_level_surv || d_fine_count) ? 1 : 0;
         A.weightPos = (l == L - 1 && d_fine_count) ? b->weightPos[l] : nullptr;
         A.offNext = offS[nxt];
         A.qlistNext = qS[nxt];
