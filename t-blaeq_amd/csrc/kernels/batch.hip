@@ -164,6 +164,61 @@ __device__ __forceinline__ void write_kept(const RangeStepArgs& A, uint64_t myId
     }
 }
 
+// Lanes of m whose outward-rounded float box [flo, fhi] meets the
+// wave-uniform bounding box [blo, bhi] in eight dimensions: the same v_cmpx
+// chain as box_mask8 (EXEC narrowed from EXEC & m, read back and restored in
+// the one asm block), on floats.  blo <= fhi and bhi >= flo per dimension,
+// false for NaN as the C comparisons.
+__device__ __forceinline__ uint64_t aabb_mask8(uint64_t m, const float* blo, const float* bhi, const float* flo,
+                                               const float* fhi) {
+    uint64_t out, save;
+    asm volatile(
+        "s_mov_b64 %[sv], exec\n\t"
+        "s_and_b64 exec, exec, %[m]\n\t"
+        "v_cmpx_le_f32 vcc, %[a0], %[f0]\n\t"
+        "v_cmpx_ge_f32 vcc, %[b0], %[g0]\n\t"
+        "v_cmpx_le_f32 vcc, %[a1], %[f1]\n\t"
+        "v_cmpx_ge_f32 vcc, %[b1], %[g1]\n\t"
+        "v_cmpx_le_f32 vcc, %[a2], %[f2]\n\t"
+        "v_cmpx_ge_f32 vcc, %[b2], %[g2]\n\t"
+        "v_cmpx_le_f32 vcc, %[a3], %[f3]\n\t"
+        "v_cmpx_ge_f32 vcc, %[b3], %[g3]\n\t"
+        "v_cmpx_le_f32 vcc, %[a4], %[f4]\n\t"
+        "v_cmpx_ge_f32 vcc, %[b4], %[g4]\n\t"
+        "v_cmpx_le_f32 vcc, %[a5], %[f5]\n\t"
+        "v_cmpx_ge_f32 vcc, %[b5], %[g5]\n\t"
+        "v_cmpx_le_f32 vcc, %[a6], %[f6]\n\t"
+        "v_cmpx_ge_f32 vcc, %[b6], %[g6]\n\t"
+        "v_cmpx_le_f32 vcc, %[a7], %[f7]\n\t"
+        "v_cmpx_ge_f32 vcc, %[b7], %[g7]\n\t"
+        "s_mov_b64 %[out], exec\n\t"
+        "s_mov_b64 exec, %[sv]"
+        : [out] "=&s"(out), [sv] "=&s"(save)
+        : [m] "s"(m), [a0] "s"(blo[0]), [a1] "s"(blo[1]), [a2] "s"(blo[2]), [a3] "s"(blo[3]), [a4] "s"(blo[4]),
+          [a5] "s"(blo[5]), [a6] "s"(blo[6]), [a7] "s"(blo[7]), [b0] "s"(bhi[0]), [b1] "s"(bhi[1]), [b2] "s"(bhi[2]),
+          [b3] "s"(bhi[3]), [b4] "s"(bhi[4]), [b5] "s"(bhi[5]), [b6] "s"(bhi[6]), [b7] "s"(bhi[7]), [f0] "v"(fhi[0]),
+          [f1] "v"(fhi[1]), [f2] "v"(fhi[2]), [f3] "v"(fhi[3]), [f4] "v"(fhi[4]), [f5] "v"(fhi[5]), [f6] "v"(fhi[6]),
+          [f7] "v"(fhi[7]), [g0] "v"(flo[0]), [g1] "v"(flo[1]), [g2] "v"(flo[2]), [g3] "v"(flo[3]), [g4] "v"(flo[4]),
+          [g5] "v"(flo[5]), [g6] "v"(flo[6]), [g7] "v"(flo[7])
+        : "vcc");
+    return out;
+}
+template <int DM>
+__device__ __forceinline__ uint64_t aabb_mask(uint64_t m, const float* blo, const float* bhi, const float* flo,
+                                              const float* fhi) {
+    if constexpr (DM == 4) {
+        float a[8] = {blo[0], blo[1], blo[2], blo[3], 0.0f, 0.0f, 0.0f, 0.0f};
+        float b[8] = {bhi[0], bhi[1], bhi[2], bhi[3], 0.0f, 0.0f, 0.0f, 0.0f};
+        float f[8] = {fhi[0], fhi[1], fhi[2], fhi[3], 0.0f, 0.0f, 0.0f, 0.0f};
+        float g[8] = {flo[0], flo[1], flo[2], flo[3], 0.0f, 0.0f, 0.0f, 0.0f};
+        return aabb_mask8(m, a, b, g, f);
+    } else {
+#pragma unroll
+        for (int d = 0; d < DM; d += 8) m = aabb_mask8(m, blo + d, bhi + d, flo + d, fhi + d);
+        return m;
+    }
+}
+
 template <int DM, int MODE, bool EXACT>
 __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs A) {
     static_assert(MODE == kCount || MODE == kWrite, "prune modes only");
@@ -277,6 +332,21 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
             const int nc = static_cast<int>(je - j0 < static_cast<uint64_t>(kWave) ? je - j0 : kWave);
             const uint64_t myId = lane < nc ? A.ids[j0 + lane] : 0;
             uint64_t myMask = 0, myBall = 0;
+            if (!ballTest) {
+                // box test alone: a compare chain per row against its bounding box
+                const uint64_t vm = __ballot(valid);
+                for (int k = 0; k < nc; ++k) {
+                    const uint64_t j = j0 + k;
+                    float blo[DM], bhi[DM];
+#pragma unroll
+                    for (int d = 0; d < DM; ++d) {
+                        blo[d] = d < D ? cab[j * 2 * D + d] : 0.0f;
+                        bhi[d] = d < D ? cab[j * 2 * D + D + d] : 0.0f;
+                    }
+                    const uint64_t m = aabb_mask<DM>(vm, blo, bhi, flo, fhi);
+                    if (lane == k) myMask = m;
+                }
+            } else {
             double xn[DM], rn;
 #pragma unroll
             for (int d = 0; d < DM; ++d) xn[d] = d < D ? cx[j0 * D + d] : 0.0;
@@ -293,8 +363,7 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
                     rn = crad[j + 1];
                 }
                 double dist = 0.0;
-                if (!ballTest) {
-                } else if (fastDist) {
+                if (fastDist) {
 #pragma unroll
                     for (int d = 0; d < DM; ++d) {
                         const double diff = fmax(fmax(lo[d] - xv[d], xv[d] - hi[d]), 0.0);
@@ -309,7 +378,7 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
                     }
                 }
                 const bool ball = valid && dist <= r * r;
-                bool keep = ballTest ? ball : valid;
+                bool keep = ball;
                 if (useBox) {
 #pragma unroll
                     for (int d = 0; d < DM; ++d) {
@@ -324,6 +393,7 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
                     const uint64_t mb = useBox ? __ballot(ball) : m;
                     if (lane == k) myBall = mb;
                 }
+            }
             }
             if (MODE == kCount) {
                 // ball survivors per lane for the statistics; kept pairs per cluster
