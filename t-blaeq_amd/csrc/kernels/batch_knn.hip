@@ -107,18 +107,11 @@ __device__ __forceinline__ bool lt_pair(uint64_t ak, uint64_t av, uint64_t bk, u
     return ak < bk || (ak == bk && av < bv);
 }
 
-// Sorts every query's buffer (entries beyond cnt are treated as sentinels) and
-// keeps its first M entries; updates the insertion threshold.  Wave i sorts
-// query i's buffer in registers (lane l holds entries r * 64 + l, r < 8) with
-// a bitonic network: partners 64 or more apart are in the same lane, nearer
-// ones are exchanged with __shfl_xor; one workgroup barrier at the end.
-static_assert(kKnnQB == kKnnThreads / kWave, "one wave per query buffer");
-__device__ void knn_compact(KnnShared& S, int tid) {
-    constexpr int R = kKnnCap / kWave;
-    const int lane = tid & (kWave - 1), qi = tid / kWave;
-    uint64_t* bk = S.bk[qi];
-    uint64_t* bv = S.bv[qi];
-    const uint32_t cnt = S.cnt[qi];
+// Bitonic sort of the first 64 * R entries of one buffer (entries from cnt
+// on read as sentinels), in registers of the calling wave; written back.
+template <int R>
+__device__ __forceinline__ void knn_sort_regs(uint64_t* bk, uint64_t* bv, uint32_t cnt, int lane) {
+    constexpr int N = R * kWave;
     uint64_t K[R], V[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -127,7 +120,7 @@ __device__ void knn_compact(KnnShared& S, int tid) {
         V[r] = e < cnt ? bv[e] : ~0ull;
     }
 #pragma unroll
-    for (int k = 2; k <= kKnnCap; k <<= 1) {
+    for (int k = 2; k <= N; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
             if (j >= kWave) {
@@ -167,6 +160,25 @@ __device__ void knn_compact(KnnShared& S, int tid) {
         bk[r * kWave + lane] = K[r];
         bv[r * kWave + lane] = V[r];
     }
+}
+
+static_assert(kKnnQB == kKnnThreads / kWave, "one wave per query buffer");
+// Sorts every query's buffer (entries beyond cnt are treated as sentinels) and
+// keeps its first M entries; updates the insertion threshold.  Wave i sorts
+// query i's buffer in registers with a bitonic network over the smallest
+// power of two (at least 64) that holds its cnt entries: lane l holds entries
+// r * 64 + l; partners 64 or more apart are in the same lane, nearer ones are
+// exchanged with __shfl_xor.  One workgroup barrier at the end.
+__device__ void knn_compact(KnnShared& S, int tid) {
+    const int lane = tid & (kWave - 1), qi = tid / kWave;
+    uint64_t* bk = S.bk[qi];
+    uint64_t* bv = S.bv[qi];
+    const uint32_t cnt = S.cnt[qi];
+    static_assert(kKnnCap == 8 * kWave, "sort sizes");
+    if (cnt <= static_cast<uint32_t>(kWave)) knn_sort_regs<1>(bk, bv, cnt, lane);
+    else if (cnt <= 2u * kWave) knn_sort_regs<2>(bk, bv, cnt, lane);
+    else if (cnt <= 4u * kWave) knn_sort_regs<4>(bk, bv, cnt, lane);
+    else knn_sort_regs<8>(bk, bv, cnt, lane);
     wave_lds_sync();
     if (lane == 0) {
         const uint32_t m = S.M[qi];
