@@ -168,27 +168,31 @@ static_assert(kKnnQB == kKnnThreads / kWave, "one wave per query buffer");
 // query i's buffer in registers with a bitonic network over the smallest
 // power of two (at least 64) that holds its cnt entries: lane l holds entries
 // r * 64 + l; partners 64 or more apart are in the same lane, nearer ones are
-// exchanged with __shfl_xor.  One workgroup barrier at the end.
+// exchanged with __shfl_xor.  Workgroup barriers at the start and the end.
 __device__ void knn_compact(KnnShared& S, int tid) {
+    // every wave has read the counts for its own decisions before any is rewritten
+    __syncthreads();
     const int lane = tid & (kWave - 1), qi = tid / kWave;
     uint64_t* bk = S.bk[qi];
     uint64_t* bv = S.bv[qi];
     const uint32_t cnt = S.cnt[qi];
-    static_assert(kKnnCap == 8 * kWave, "sort sizes");
-    if (cnt <= static_cast<uint32_t>(kWave)) knn_sort_regs<1>(bk, bv, cnt, lane);
-    else if (cnt <= 2u * kWave) knn_sort_regs<2>(bk, bv, cnt, lane);
-    else if (cnt <= 4u * kWave) knn_sort_regs<4>(bk, bv, cnt, lane);
-    else knn_sort_regs<8>(bk, bv, cnt, lane);
-    wave_lds_sync();
-    if (lane == 0) {
-        const uint32_t m = S.M[qi];
-        if (S.cnt[qi] > m) S.cnt[qi] = m;
-        if (m > 0 && S.cnt[qi] >= m) {
-            S.thrK[qi] = bk[m - 1];
-            S.thrV[qi] = bv[m - 1];
-        } else {
-            S.thrK[qi] = ~0ull;
-            S.thrV[qi] = ~0ull;
+    {
+        static_assert(kKnnCap == 8 * kWave, "sort sizes");
+        if (cnt <= static_cast<uint32_t>(kWave)) knn_sort_regs<1>(bk, bv, cnt, lane);
+        else if (cnt <= 2u * kWave) knn_sort_regs<2>(bk, bv, cnt, lane);
+        else if (cnt <= 4u * kWave) knn_sort_regs<4>(bk, bv, cnt, lane);
+        else knn_sort_regs<8>(bk, bv, cnt, lane);
+        wave_lds_sync();
+        if (lane == 0) {
+            const uint32_t m = S.M[qi];
+            if (S.cnt[qi] > m) S.cnt[qi] = m;
+            if (m > 0 && S.cnt[qi] >= m) {
+                S.thrK[qi] = bk[m - 1];
+                S.thrV[qi] = bv[m - 1];
+            } else {
+                S.thrK[qi] = ~0ull;
+                S.thrV[qi] = ~0ull;
+            }
         }
     }
     __syncthreads();
