@@ -34,8 +34,9 @@ extern "C" {
 #define TBLAEQ_ERR_UNSUPPORTED 3 /* shape outside what the kernels handle */
 #define TBLAEQ_ERR_CAPACITY 4    /* an output or scratch buffer is too small */
 #define TBLAEQ_ERR_IO 5          /* index file missing or malformed */
-#define TBLAEQ_ERR_NOMEM 6       /* device allocation failed */
+#define TBLAEQ_ERR_NOMEM 6       /* device or host allocation failed */
 #define TBLAEQ_ERR_TIMEOUT 7     /* a wait ended before the work was done */
+#define TBLAEQ_ERR_INTERNAL 8    /* unexpected internal error (message in tblaeq_last_error) */
 
 typedef struct tblaeq_ctx tblaeq_ctx;
 

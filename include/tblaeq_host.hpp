@@ -21,16 +21,42 @@
 #include <string>
 #include <vector>
 
-#include "../../../include/tblaeq.h"
+#include "tblaeq.h"
 
 namespace tblaeq_host {
 
+// A failed HIP call or library entry point; `status` is the TBLAEQ_ERR_* code
+// a C caller receives (TBLAEQ_ERR_NOMEM for a failed device allocation, the
+// entry point's own code when a C-ABI call failed).
 struct HipError : std::runtime_error {
-    explicit HipError(const std::string& m) : std::runtime_error(m) {}
+    explicit HipError(const std::string& m, int s = TBLAEQ_ERR_HIP) : std::runtime_error(m), status(s) {}
+    int status;
+};
+
+// An index, dataset or query file that is missing, unreadable or malformed
+// (the reference throws std::runtime_error for these, File.cu and
+// IndexSerializer.cu); C callers receive TBLAEQ_ERR_IO.
+struct IoError : std::runtime_error {
+    explicit IoError(const std::string& m) : std::runtime_error(m) {}
+};
+
+// A call that does not fit the object's state, e.g. reading a point from a
+// batch of range queries (the reference throws std::runtime_error, File.cu);
+// C callers receive TBLAEQ_ERR_INVALID.
+struct UsageError : std::runtime_error {
+    explicit UsageError(const std::string& m) : std::runtime_error(m) {}
 };
 
 void hipCheck(hipError_t e, const char* what);
 void tbCheck(int rc, const char* what);
+
+// Status code of the exception being handled (call only inside a catch
+// block), with its message recorded as the thread's last error:
+// HipError -> its status, IoError -> TBLAEQ_ERR_IO, std::bad_alloc ->
+// TBLAEQ_ERR_NOMEM, UsageError and std::logic_error (invalid_argument,
+// out_of_range, a strategy used before it was prepared) ->
+// TBLAEQ_ERR_INVALID, anything else -> TBLAEQ_ERR_INTERNAL.
+int statusOfCurrentException(const char* what) noexcept;
 
 // RAII device buffer (Memory.cuh:33-168).
 template <class T>

@@ -703,6 +703,21 @@ void oracle_recon_finest(const OIndex* ix, double* out) {
         std::memcpy(out + g.ids[i] * D, &g.vals[i * D], D * sizeof(double));
 }
 
+// Reconstructed coordinates of every point of level `level` (0 = coarsest
+// mesh, `intervals` = finest), by level id: the coarsest mesh prolonged level by
+// level over all clusters (SpMSpV.cuh:47-67, Refactor.cu:10-16), the values
+// every traversal sees at that level.  Returns the point count.
+uint64_t oracle_recon_level(const OIndex* ix, uint64_t level, double* out) {
+    const uint64_t D = ix->D;
+    Grid g;
+    g.ids = ix->cIds;
+    g.vals = ix->cVals;
+    for (size_t l = 0; l < level && l < ix->intervals; ++l) g = prolong(*ix, l, g);
+    if (out)
+        for (size_t i = 0; i < g.size(); ++i) std::memcpy(out + g.ids[i] * D, &g.vals[i * D], D * sizeof(double));
+    return g.size();
+}
+
 uint64_t oracle_brute_range(const double* pts, uint64_t N, uint64_t D, const double* lo, const double* hi,
                             uint64_t* out, uint64_t cap) {
     uint64_t n = 0;

@@ -95,6 +95,8 @@ uint64_t oracle_refine_range(const OResult* r, const double* lo, const double* h
 uint64_t oracle_refine_knn(const OResult* r, const double* q, uint64_t K, uint64_t* ids, double* d2);
 /* Reconstructed coordinates of every finest point, indexed by original id (N*D). */
 void oracle_recon_finest(const OIndex* idx, double* out);
+/* Reconstructed coordinates of level `level` (0 = coarsest, intervals = finest) by level id; returns the count. */
+uint64_t oracle_recon_level(const OIndex* idx, uint64_t level, double* out);
 uint64_t oracle_brute_range(const double* pts, uint64_t N, uint64_t D, const double* lo, const double* hi,
                             uint64_t* out, uint64_t cap);
 uint64_t oracle_brute_knn(const double* pts, uint64_t N, uint64_t D, const double* q, uint64_t K, uint64_t* ids,

@@ -52,6 +52,7 @@ def lib():
             "oracle_refine_range": (C.c_uint64, [vp, f64p, f64p, u64p, C.c_uint64]),
             "oracle_refine_knn": (C.c_uint64, [vp, f64p, C.c_uint64, u64p, f64p]),
             "oracle_recon_finest": (None, [vp, f64p]),
+            "oracle_recon_level": (C.c_uint64, [vp, C.c_uint64, f64p]),
             "oracle_brute_range": (C.c_uint64, [f64p, C.c_uint64, C.c_uint64, f64p, f64p, u64p, C.c_uint64]),
             "oracle_brute_knn": (C.c_uint64, [f64p, C.c_uint64, C.c_uint64, f64p, C.c_uint64, u64p, f64p]),
             "oracle_sqrt_rd": (C.c_double, [C.c_double]),
@@ -145,6 +146,13 @@ class Index:
     def recon(self):
         out = np.zeros((self.N, self.D))
         lib().oracle_recon_finest(self.h, _p(out, f64p))
+        return out
+
+    def recon_level(self, level):
+        """Reconstructed coordinates of level `level` (0 = coarsest, intervals = finest), by id."""
+        n = lib().oracle_recon_level(self.h, level, None)
+        out = np.zeros((n, self.D))
+        lib().oracle_recon_level(self.h, level, _p(out, f64p))
         return out
 
     def range(self, lo, hi):

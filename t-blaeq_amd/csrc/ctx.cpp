@@ -219,8 +219,9 @@ extern "C" const char* tblaeq_status_string(int s) {
         case TBLAEQ_ERR_UNSUPPORTED: return "unsupported shape";
         case TBLAEQ_ERR_CAPACITY: return "buffer too small";
         case TBLAEQ_ERR_IO: return "index file error";
-        case TBLAEQ_ERR_NOMEM: return "out of device memory";
+        case TBLAEQ_ERR_NOMEM: return "out of memory";
         case TBLAEQ_ERR_TIMEOUT: return "timed out";
+        case TBLAEQ_ERR_INTERNAL: return "internal error";
         default: return "unknown status";
     }
 }

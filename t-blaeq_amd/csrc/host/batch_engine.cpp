@@ -19,20 +19,27 @@ template <class F>
 int guarded(const char* what, F&& f) {
     try {
         return f();
-    } catch (const std::exception& e) {
-        tblaeq::set_error(std::string(what) + ": " + e.what());
-        return TBLAEQ_ERR_HIP;
+    } catch (...) {
+        return statusOfCurrentException(what);
     }
 }
 
 BatchState* stateOf(tblaeq_handler* h) { return h ? h->batch.get() : nullptr; }
+
+// TBLAEQ_ERR_INVALID with a message naming the entry point (a null handle, a
+// batched engine not initialised, no queries uploaded, or a bad argument).
+int invalidArg(const char* fn) {
+    tblaeq::set_error(std::string(fn) + ": invalid argument, or the batched engine is not initialised or holds "
+                      "no queries (batch_init / batch_set_queries first; at most the reserved query count)");
+    return TBLAEQ_ERR_INVALID;
+}
 
 }  // namespace
 
 extern "C" {
 
 int tblaeq_handler_batch_init(tblaeq_handler* h, uint64_t maxQueries, uint64_t pairCap, uint64_t hitCap) {
-    if (!h || maxQueries == 0) return TBLAEQ_ERR_INVALID;
+    if (!h || maxQueries == 0) return invalidArg("tblaeq_handler_batch_init");
     return guarded("tblaeq_handler_batch_init", [&]() -> int {
         IndexData& idx = h->h->index();
         if (!h->h->prepared()) h->h->prepareForQuery(LevelPolicy::L0);
@@ -70,8 +77,11 @@ int tblaeq_handler_batch_init(tblaeq_handler* h, uint64_t maxQueries, uint64_t p
 int tblaeq_handler_batch_set_queries(tblaeq_handler* h, int type, const double* queries, const uint64_t* Ks,
                                      uint64_t nq) {
     BatchState* st = stateOf(h);
-    if (!st || !queries || nq == 0 || nq > st->maxQ || (type != 0 && type != 1) || (type == 1 && !Ks))
+    if (!st || !queries || nq == 0 || nq > st->maxQ || (type != 0 && type != 1) || (type == 1 && !Ks)) {
+        tblaeq::set_error("tblaeq_handler_batch_set_queries: invalid argument (batch_init first; 1 <= nq <= " +
+                          std::to_string(st ? st->maxQ : 0) + " queries; type 0 or 1; KNN needs Ks)");
         return TBLAEQ_ERR_INVALID;
+    }
     return guarded("tblaeq_handler_batch_set_queries", [&]() -> int {
         IndexData& idx = h->h->index();
         const size_t D = idx.D;
@@ -95,7 +105,7 @@ int tblaeq_handler_batch_set_queries(tblaeq_handler* h, int type, const double* 
 // *total = hits (range) or results (KNN).
 int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total) {
     BatchState* st = stateOf(h);
-    if (!st || st->nq == 0) return TBLAEQ_ERR_INVALID;
+    if (!st || st->nq == 0) return invalidArg("tblaeq_handler_batch_run");
     return guarded("tblaeq_handler_batch_run", [&]() -> int {
         IndexData& idx = h->h->index();
         const size_t D = idx.D, L = idx.intervals;
@@ -222,7 +232,7 @@ int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total) {
 
 int tblaeq_handler_batch_results(tblaeq_handler* h, uint64_t* off, uint64_t* ids, double* d2, uint64_t* n) {
     BatchState* st = stateOf(h);
-    if (!st || st->nq == 0) return TBLAEQ_ERR_INVALID;
+    if (!st || st->nq == 0) return invalidArg("tblaeq_handler_batch_results");
     return guarded("tblaeq_handler_batch_results", [&]() -> int {
         if (st->type == 0) {
             std::vector<uint64_t> o(st->nq + 1);
@@ -240,7 +250,7 @@ int tblaeq_handler_batch_results(tblaeq_handler* h, uint64_t* off, uint64_t* ids
 
 int tblaeq_handler_batch_stats(tblaeq_handler* h, uint64_t* levelSurv, uint64_t* fine) {
     BatchState* st = stateOf(h);
-    if (!st || st->nq == 0) return TBLAEQ_ERR_INVALID;
+    if (!st || st->nq == 0) return invalidArg("tblaeq_handler_batch_stats");
     return guarded("tblaeq_handler_batch_stats", [&]() -> int {
         const size_t L = h->h->index().intervals;
         if (levelSurv) hipCheck(hipMemcpy(levelSurv, st->dSurv.data(), st->nq * L * 8, hipMemcpyDeviceToHost), "D2H");
@@ -251,7 +261,7 @@ int tblaeq_handler_batch_stats(tblaeq_handler* h, uint64_t* levelSurv, uint64_t*
 
 int tblaeq_handler_batch_use_graph(tblaeq_handler* h, int on) {
     BatchState* st = stateOf(h);
-    if (!st) return TBLAEQ_ERR_INVALID;
+    if (!st) return invalidArg("tblaeq_handler_batch_use_graph");
     st->useGraph = on != 0;
     return TBLAEQ_OK;
 }
@@ -267,7 +277,7 @@ int tblaeq_handler_batch_set_param(tblaeq_handler* h, int param, uint64_t value)
 
 int tblaeq_handler_batch_work(tblaeq_handler* h, uint64_t* work) {
     BatchState* st = stateOf(h);
-    if (!st || !work) return TBLAEQ_ERR_INVALID;
+    if (!st || !work) return invalidArg("tblaeq_handler_batch_work");
     work[0] = st->work[0];
     work[1] = st->work[1];
     return TBLAEQ_OK;
@@ -307,13 +317,13 @@ int tblaeq_handler_batch_get_param(tblaeq_handler* h, int param, uint64_t* value
 
 int tblaeq_handler_batch_save_params(tblaeq_handler* h, const char* path) {
     BatchState* st = stateOf(h);
-    if (!st) return TBLAEQ_ERR_INVALID;
+    if (!st) return invalidArg("tblaeq_handler_batch_save_params");
     return tblaeq_batch_save_params(st->b, path);
 }
 
 int tblaeq_handler_batch_load_params(tblaeq_handler* h, const char* path) {
     BatchState* st = stateOf(h);
-    if (!st) return TBLAEQ_ERR_INVALID;
+    if (!st) return invalidArg("tblaeq_handler_batch_load_params");
     return tblaeq_batch_load_params(st->b, path);
 }
 

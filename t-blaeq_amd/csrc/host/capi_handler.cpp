@@ -21,15 +21,8 @@ int guarded(const char* what, F&& f) {
     try {
         f();
         return TBLAEQ_OK;
-    } catch (const HipError& e) {
-        tblaeq::set_error(std::string(what) + ": " + e.what());
-        return TBLAEQ_ERR_HIP;
-    } catch (const std::invalid_argument& e) {
-        tblaeq::set_error(std::string(what) + ": " + e.what());
-        return TBLAEQ_ERR_INVALID;
-    } catch (const std::exception& e) {
-        tblaeq::set_error(std::string(what) + ": " + e.what());
-        return TBLAEQ_ERR_IO;
+    } catch (...) {
+        return statusOfCurrentException(what);
     }
 }
 std::vector<size_t> toVec(const uint64_t* r, uint64_t n) { return std::vector<size_t>(r, r + n); }

@@ -2,7 +2,7 @@
 #pragma once
 #include <memory>
 
-#include "tblaeq_host.hpp"
+#include "../../../include/tblaeq_host.hpp"
 
 namespace tblaeq_host {
 

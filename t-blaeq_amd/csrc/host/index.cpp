@@ -11,17 +11,51 @@
 #include <sstream>
 
 #include "../../../include/tblaeq_build.h"
-#include "tblaeq_host.hpp"
+#include "../../../include/tblaeq_host.hpp"
+#include "../ctx.hpp"
 
 namespace tblaeq_host {
 
 void hipCheck(hipError_t e, const char* what) {
-    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+    if (e != hipSuccess)
+        throw HipError(std::string(what) + ": " + hipGetErrorString(e),
+                       e == hipErrorOutOfMemory ? TBLAEQ_ERR_NOMEM : TBLAEQ_ERR_HIP);
 }
 
 void tbCheck(int rc, const char* what) {
     if (rc != TBLAEQ_OK)
-        throw HipError(std::string(what) + ": " + tblaeq_status_string(rc) + " (" + tblaeq_last_error() + ")");
+        throw HipError(std::string(what) + ": " + tblaeq_status_string(rc) + " (" + tblaeq_last_error() + ")", rc);
+}
+
+int statusOfCurrentException(const char* what) noexcept {
+    int rc = TBLAEQ_ERR_INTERNAL;
+    std::string msg = "unknown exception";
+    try {
+        throw;
+    } catch (const HipError& e) {
+        rc = e.status;
+        msg = e.what();
+    } catch (const IoError& e) {
+        rc = TBLAEQ_ERR_IO;
+        msg = e.what();
+    } catch (const UsageError& e) {
+        rc = TBLAEQ_ERR_INVALID;
+        msg = e.what();
+    } catch (const std::bad_alloc& e) {
+        rc = TBLAEQ_ERR_NOMEM;
+        msg = std::string("host allocation failed (") + e.what() + ")";
+    } catch (const std::logic_error& e) {
+        rc = TBLAEQ_ERR_INVALID;
+        msg = e.what();
+    } catch (const std::exception& e) {
+        msg = e.what();
+    } catch (...) {
+    }
+    try {
+        tblaeq::set_error(std::string(what) + ": " + msg);
+    } catch (...) {
+    }
+    return rc;
 }
 
 // ---------------------------------------------------------------- IndexData
@@ -174,7 +208,7 @@ void wr(std::ofstream& f, const T* p, size_t n) {
 template <class T>
 void rd(std::ifstream& f, T* p, size_t n, const std::string& path) {
     f.read(reinterpret_cast<char*>(p), static_cast<std::streamsize>(n * sizeof(T)));
-    if (!f) throw std::runtime_error("IndexSerializer: short read in " + path);
+    if (!f) throw IoError("IndexSerializer: short read in " + path);
 }
 }  // namespace
 
@@ -184,7 +218,7 @@ void IndexSerializer::save(const IndexData& d, const std::string& dir) {
     {
         const std::string p = dir + "/metadata.bin";
         std::ofstream f(p, std::ios::binary);
-        if (!f) throw std::runtime_error("Cannot write: " + p);
+        if (!f) throw IoError("Cannot write: " + p);
         const uint64_t meta[4] = {d.D, d.N, d.height, d.intervals};
         wr(f, meta, 4);
         wr(f, &aos, 1);
@@ -196,7 +230,7 @@ void IndexSerializer::save(const IndexData& d, const std::string& dir) {
     for (size_t i = 0; i < d.intervals; ++i) {
         const std::string p = dir + "/P_Tensor_" + std::to_string(i) + ".bin";
         std::ofstream f(p, std::ios::binary);
-        if (!f) throw std::runtime_error("Cannot write: " + p);
+        if (!f) throw IoError("Cannot write: " + p);
         const auto& t = *d.pTensors[i];
         const uint64_t hdr[4] = {t.D, t.rows, t.cols, t.nnz};
         const uint8_t a = t.isAos ? 1 : 0;
@@ -210,7 +244,7 @@ void IndexSerializer::save(const IndexData& d, const std::string& dir) {
     for (size_t i = 0; i < d.intervals; ++i) {
         const std::string p = dir + "/MaxRadius_" + std::to_string(i) + ".bin";
         std::ofstream f(p, std::ios::binary);
-        if (!f) throw std::runtime_error("Cannot write: " + p);
+        if (!f) throw IoError("Cannot write: " + p);
         const uint64_t n = d.meshSizes[d.intervals - i];
         wr(f, &n, 1);
         wr(f, d.meshMaxRadius[i].data(), n);
@@ -219,7 +253,7 @@ void IndexSerializer::save(const IndexData& d, const std::string& dir) {
         if (d.maps[i].empty()) continue;
         const std::string p = dir + "/Map_" + std::to_string(i) + ".bin";
         std::ofstream f(p, std::ios::binary);
-        if (!f) throw std::runtime_error("Cannot write: " + p);
+        if (!f) throw IoError("Cannot write: " + p);
         const uint64_t n = d.meshSizes[d.intervals - i];
         wr(f, &n, 1);
         wr(f, d.maps[i].data(), n);
@@ -227,7 +261,7 @@ void IndexSerializer::save(const IndexData& d, const std::string& dir) {
     {
         const std::string p = dir + "/Coreast_Mesh.bin";
         std::ofstream f(p, std::ios::binary);
-        if (!f) throw std::runtime_error("Cannot write: " + p);
+        if (!f) throw IoError("Cannot write: " + p);
         const uint64_t hdr[3] = {d.coarsestRows, d.D, d.coarsestIds.size()};
         wr(f, hdr, 3);
         wr(f, &aos, 1);
@@ -241,7 +275,7 @@ IndexData* IndexSerializer::load(const std::string& dir) {
     {
         const std::string p = dir + "/metadata.bin";
         std::ifstream f(p, std::ios::binary);
-        if (!f) throw std::runtime_error("Cannot read: " + p);
+        if (!f) throw IoError("Cannot read: " + p);
         uint64_t meta[4];
         uint8_t aos;
         rd(f, meta, 4, p);
@@ -252,12 +286,12 @@ IndexData* IndexSerializer::load(const std::string& dir) {
         d->intervals = meta[3];
         d->isAosArch = aos != 0;
         if (d->height != d->intervals + 1 || d->D == 0 || d->intervals == 0)
-            throw std::runtime_error("IndexSerializer: inconsistent metadata in " + p);
+            throw IoError("IndexSerializer: inconsistent metadata in " + p);
         d->ratios.resize(d->intervals);
         rd(f, d->ratios.data(), d->intervals, p);
         uint64_t msz;
         rd(f, &msz, 1, p);
-        if (msz != d->height) throw std::runtime_error("IndexSerializer: mesh size count mismatch in " + p);
+        if (msz != d->height) throw IoError("IndexSerializer: mesh size count mismatch in " + p);
         d->meshSizes.resize(msz);
         rd(f, d->meshSizes.data(), msz, p);
     }
@@ -267,7 +301,7 @@ IndexData* IndexSerializer::load(const std::string& dir) {
     for (size_t i = 0; i < d->intervals; ++i) {
         const std::string p = dir + "/P_Tensor_" + std::to_string(i) + ".bin";
         std::ifstream f(p, std::ios::binary);
-        if (!f) throw std::runtime_error("Cannot read: " + p);
+        if (!f) throw IoError("Cannot read: " + p);
         auto t = std::make_unique<SparseTensorCscFormat>();
         uint64_t hdr[4];
         uint8_t a;
@@ -286,16 +320,16 @@ IndexData* IndexSerializer::load(const std::string& dir) {
         rd(f, t->nnzPerCol.data(), t->cols, p);
         rd(f, t->colRes.data(), t->cols + 1, p);
         rd(f, t->vals.data(), t->nnz * t->D, p);
-        if (t->colRes[t->cols] != t->nnz) throw std::runtime_error("IndexSerializer: CSC column pointers in " + p);
+        if (t->colRes[t->cols] != t->nnz) throw IoError("IndexSerializer: CSC column pointers in " + p);
         d->pTensors[i] = std::move(t);
     }
     for (size_t i = 0; i < d->intervals; ++i) {
         const std::string p = dir + "/MaxRadius_" + std::to_string(i) + ".bin";
         std::ifstream f(p, std::ios::binary);
-        if (!f) throw std::runtime_error("Cannot read: " + p);
+        if (!f) throw IoError("Cannot read: " + p);
         uint64_t n;
         rd(f, &n, 1, p);
-        if (n != d->meshSizes[d->intervals - i]) throw std::runtime_error("IndexSerializer: radius count in " + p);
+        if (n != d->meshSizes[d->intervals - i]) throw IoError("IndexSerializer: radius count in " + p);
         d->meshMaxRadius[i].resize(n);
         rd(f, d->meshMaxRadius[i].data(), n, p);
     }
@@ -305,14 +339,14 @@ IndexData* IndexSerializer::load(const std::string& dir) {
         std::ifstream f(p, std::ios::binary);
         uint64_t n;
         rd(f, &n, 1, p);
-        if (n != d->meshSizes[d->intervals - i]) throw std::runtime_error("IndexSerializer: map size in " + p);
+        if (n != d->meshSizes[d->intervals - i]) throw IoError("IndexSerializer: map size in " + p);
         d->maps[i].resize(n);
         rd(f, d->maps[i].data(), n, p);
     }
     {
         const std::string p = dir + "/Coreast_Mesh.bin";
         std::ifstream f(p, std::ios::binary);
-        if (!f) throw std::runtime_error("Cannot read: " + p);
+        if (!f) throw IoError("Cannot read: " + p);
         uint64_t hdr[3];
         uint8_t a;
         rd(f, hdr, 3, p);
@@ -614,66 +648,83 @@ IndexData* IndexBuilder::build(const double* data, size_t N, size_t D, const std
 
 // ------------------------------------------------------------------ text I/O
 
-PointCloud loadFromFile(const std::string& path) {
+// Reads `count dim` and then `count` rows of numbers from a text file (the
+// dataset format of File.cu:8-50; range query files hold 2*dim numbers per
+// row, lower corner then upper corner, File.cu:139-216).  Extra numbers on a
+// row are ignored and rows past `count` are not read, as in the reference; a
+// short row, a missing row, a bad header or an unreadable file is an IoError.
+static std::vector<double> readRows(const std::string& path, const char* what, int64_t* count, int64_t* dim,
+                                    int64_t perDim) {
     std::ifstream f(path);
-    if (!f) throw std::runtime_error("loadFromFile: cannot open '" + path + "'");
+    if (!f) throw IoError(std::string(what) + ": '" + path + "' could not be opened");
     std::string line;
-    if (!std::getline(f, line)) throw std::runtime_error("loadFromFile: empty file '" + path + "'");
-    PointCloud pc;
+    if (!std::getline(f, line)) throw IoError(std::string(what) + ": '" + path + "' has no header line");
     {
         std::istringstream iss(line);
-        if (!(iss >> pc.size >> pc.dim) || pc.size <= 0 || pc.dim <= 0)
-            throw std::runtime_error("loadFromFile: invalid header in '" + path + "'");
+        if (!(iss >> *count >> *dim) || *count <= 0 || *dim <= 0)
+            throw IoError(std::string(what) + ": header of '" + path + "' is not two positive integers");
     }
-    pc.data.resize(static_cast<size_t>(pc.size * pc.dim));
+    const int64_t cols = *dim * perDim;
+    std::vector<double> out(static_cast<size_t>(*count * cols));
     int64_t row = 0;
-    while (row < pc.size && std::getline(f, line)) {
+    for (; row < *count && std::getline(f, line); ++row) {
         std::istringstream iss(line);
-        double v;
         int64_t col = 0;
-        while (col < pc.dim && iss >> v) pc.data[static_cast<size_t>(row * pc.dim + col++)] = v;
-        if (col != pc.dim) throw std::runtime_error("loadFromFile: dimension mismatch at row " + std::to_string(row + 1));
-        ++row;
+        for (double v; col < cols && iss >> v; ++col) out[static_cast<size_t>(row * cols + col)] = v;
+        if (col != cols)
+            throw IoError(std::string(what) + ": line " + std::to_string(row + 2) + " of '" + path + "' holds " +
+                          std::to_string(col) + " numbers, " + std::to_string(cols) + " expected");
     }
-    if (row != pc.size) throw std::runtime_error("loadFromFile: row count mismatch in '" + path + "'");
+    if (row != *count)
+        throw IoError(std::string(what) + ": '" + path + "' ends after " + std::to_string(row) + " of " +
+                      std::to_string(*count) + " rows");
+    return out;
+}
+
+PointCloud loadFromFile(const std::string& path) {
+    PointCloud pc;
+    pc.data = readRows(path, "loadFromFile", &pc.size, &pc.dim, 1);
     return pc;
 }
 
 static Query loadQuery(const std::string& path, QueryType t) {
-    std::ifstream f(path);
-    if (!f) throw std::runtime_error("loadQueryFromFile: cannot open '" + path + "'");
-    std::string line;
-    if (!std::getline(f, line)) throw std::runtime_error("loadQueryFromFile: empty file");
-    int len = 0, dim = 0;
-    {
-        std::istringstream iss(line);
-        if (!(iss >> len >> dim) || len <= 0 || dim <= 0) throw std::runtime_error("loadQueryFromFile: invalid header");
-    }
-    Query q(len, dim, t);
-    const int cols = t == QueryType::RANGE ? 2 * dim : dim;
-    int row = 0;
-    while (row < len && std::getline(f, line)) {
-        std::istringstream iss(line);
-        double v;
-        int col = 0;
-        while (col < cols && iss >> v) q.data[static_cast<size_t>(row) * cols + col++] = v;
-        if (col != cols) throw std::runtime_error("loadQueryFromFile: expected " + std::to_string(cols) + " values at row " + std::to_string(row + 1));
-        ++row;
-    }
-    if (row != len) throw std::runtime_error("loadQueryFromFile: row count mismatch");
+    int64_t n = 0, dim = 0;
+    std::vector<double> rows = readRows(path, "loadQueryFromFile", &n, &dim, t == QueryType::RANGE ? 2 : 1);
+    if (n > INT_MAX || dim > INT_MAX) throw IoError("loadQueryFromFile: '" + path + "' is too large");
+    Query q(static_cast<int>(n), static_cast<int>(dim), t);
+    q.data = std::move(rows);
     return q;
+}
+
+// Label of a range query file for the CSV's "Query Parameter" column
+// (Utils.cu:18-35): the text between the path's last '_' and the next '.';
+// "P-E" (integers) becomes "P%^E=C%" with C = (P/100)^E * 100 to two
+// decimals, anything else gets a '%' appended.
+std::string rangeInfoOfPath(const std::string& path) {
+    const size_t us = path.rfind('_');
+    const size_t b = us == std::string::npos ? 0 : us + 1;
+    const size_t dot = path.find('.', b);
+    const std::string seg = path.substr(b, dot == std::string::npos ? std::string::npos : dot - b);
+    const size_t dash = seg.find('-');
+    if (dash == std::string::npos) return seg + "%";
+    const int pct = std::stoi(seg.substr(0, dash));
+    const int exps = std::stoi(seg.substr(dash + 1));
+    std::ostringstream o;
+    o << pct << "%^" << exps << "=" << std::fixed << std::setprecision(2)
+      << std::pow(pct / 100.0, exps) * 100.0 << "%";
+    return o.str();
 }
 
 Query loadQueryPointFromFile(const std::string& path) { return loadQuery(path, QueryType::POINT); }
 Query loadQueryRangeFromFile(const std::string& path) {
     Query q = loadQuery(path, QueryType::RANGE);
-    q.queryRangeInfo = std::filesystem::path(path).stem().string();
+    q.queryRangeInfo = rangeInfoOfPath(path);
     return q;
 }
 
 void saveQueryToFile(const Query& q, const std::string& path) {
     std::ofstream f(path);
-    if (!f) throw std::runtime_error("saveQueryToFile: cannot create '" + path + "'");
+    if (!f) throw IoError("saveQueryToFile: '" + path + "' could not be created");
     f << q.length << ' ' << q.dim << '\n';
     f << std::setprecision(17);
     const int cols = q.type == QueryType::RANGE ? 2 * q.dim : q.dim;
@@ -687,29 +738,29 @@ void saveQueryToFile(const Query& q, const std::string& path) {
 }
 
 std::vector<double> Query::getQueryPoint(int i) const {
-    if (type != QueryType::POINT) throw std::runtime_error("getQueryPoint: query type is not POINT");
-    if (i < 0 || i >= length) throw std::out_of_range("getQueryPoint: index out of range");
+    if (type != QueryType::POINT) throw UsageError("getQueryPoint: the query batch does not hold point (KNN) queries");
+    if (i < 0 || i >= length) throw std::out_of_range("getQueryPoint: query " + std::to_string(i) + " of " + std::to_string(length));
     return std::vector<double>(data.begin() + static_cast<long>(i) * dim, data.begin() + static_cast<long>(i + 1) * dim);
 }
 
 std::pair<std::vector<double>, std::vector<double>> Query::getQueryRange(int i) const {
-    if (type != QueryType::RANGE) throw std::runtime_error("getQueryRange: query type is not RANGE");
-    if (i < 0 || i >= length) throw std::out_of_range("getQueryRange: index out of range");
+    if (type != QueryType::RANGE) throw UsageError("getQueryRange: the query batch does not hold range queries");
+    if (i < 0 || i >= length) throw std::out_of_range("getQueryRange: query " + std::to_string(i) + " of " + std::to_string(length));
     const size_t off = static_cast<size_t>(i) * 2 * dim;
     return {std::vector<double>(data.begin() + off, data.begin() + off + dim),
             std::vector<double>(data.begin() + off + dim, data.begin() + off + 2 * dim)};
 }
 
 void Query::setQueryPoint(int i, const std::vector<double>& p) {
-    if (type != QueryType::POINT) throw std::runtime_error("setQueryPoint: query type is not POINT");
-    if (i < 0 || i >= length) throw std::out_of_range("setQueryPoint: index out of range");
+    if (type != QueryType::POINT) throw UsageError("setQueryPoint: the query batch does not hold point (KNN) queries");
+    if (i < 0 || i >= length) throw std::out_of_range("setQueryPoint: query " + std::to_string(i) + " of " + std::to_string(length));
     if (static_cast<int>(p.size()) != dim) throw std::invalid_argument("setQueryPoint: dimension mismatch");
     std::copy(p.begin(), p.end(), data.begin() + static_cast<long>(i) * dim);
 }
 
 void Query::setQueryRange(int i, const std::vector<double>& lo, const std::vector<double>& hi) {
-    if (type != QueryType::RANGE) throw std::runtime_error("setQueryRange: query type is not RANGE");
-    if (i < 0 || i >= length) throw std::out_of_range("setQueryRange: index out of range");
+    if (type != QueryType::RANGE) throw UsageError("setQueryRange: the query batch does not hold range queries");
+    if (i < 0 || i >= length) throw std::out_of_range("setQueryRange: query " + std::to_string(i) + " of " + std::to_string(length));
     if (static_cast<int>(lo.size()) != dim || static_cast<int>(hi.size()) != dim)
         throw std::invalid_argument("setQueryRange: dimension mismatch");
     const size_t off = static_cast<size_t>(i) * 2 * dim;

@@ -17,7 +17,7 @@
 #include <iostream>
 #include <numeric>
 
-#include "tblaeq_host.hpp"
+#include "../../../include/tblaeq_host.hpp"
 
 namespace tblaeq_host {
 

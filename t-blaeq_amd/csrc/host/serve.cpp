@@ -89,11 +89,6 @@ struct tblaeq_server {
 
 namespace {
 
-int fail(const char* what, const std::exception& e) {
-    tblaeq::set_error(std::string(what) + ": " + e.what());
-    return TBLAEQ_ERR_HIP;
-}
-
 // Runs requests [a, a+n) of `batch` on the worker's engine, splitting on overflow.
 void run_batch(tblaeq_server* s, Worker& w, std::vector<Request*>& batch, size_t a, size_t n) {
     const size_t D = s->D;
@@ -193,8 +188,8 @@ void worker_loop(tblaeq_server* s, Worker& w) {
         int rc = TBLAEQ_OK;
         try {
             run_batch(s, w, batch, 0, batch.size());
-        } catch (const std::exception& e) {
-            rc = fail("tblaeq_server batch", e);
+        } catch (...) {
+            rc = tblaeq_host::statusOfCurrentException("tblaeq_server batch");
         }
         const auto now = Clock::now();
         {
@@ -309,12 +304,13 @@ int tblaeq_server_create(tblaeq_handler* h, const tblaeq_server_config* cfgIn, t
         s->cfg = cfg;
         tblaeq_server* raw = s.get();
         for (int k = 0; k < 2; ++k) raw->w[k].th = std::thread(worker_loop, raw, std::ref(raw->w[k]));
-    } catch (const std::exception& e) {
+    } catch (...) {
+        const int rc = tblaeq_host::statusOfCurrentException("tblaeq_server_create");
         for (auto& w : s->w) {
             if (w.b) tblaeq_batch_destroy(w.b);
             if (w.ctx) tblaeq_ctx_destroy(w.ctx);
         }
-        return fail("tblaeq_server_create", e);
+        return rc;
     }
     *out = s.release();
     return TBLAEQ_OK;

@@ -121,9 +121,10 @@ struct RangeStepArgs {
     // list (returned by the count pass's atomic), so that the write pass, when
     // every row of the level was stored, places the queries without atomics
     uint64_t* maskPos;
-    // with a box test (aabb): whether the ball test is also applied (both
-    // passes of a level must agree; statistics runs need it for their counts)
-    int keepBall;
+    // with a box test (aabb): evaluate the ball test of every row, also where
+    // the box test already failed for every lane (the statistics count ball
+    // survivors); the kept pairs are ball AND box either way
+    int ballStats;
 };
 
 // Appends a wave's staged hits to the global hit list (entries past the
@@ -233,11 +234,13 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
     uint64_t nItems = A.level0 ? (A.nq + kWave - 1) / kWave : ((ConstU64)A.nItems)[0];
     if (!A.level0 && nItems > A.itemCap) nItems = A.itemCap;
     const uint64_t ltmask = (1ull << lane) - 1ull;
+    // Every level keeps a (cluster, query) pair only if the reference's ball
+    // test passes.  The last prune also needs the box to meet the bounding box
+    // of the cluster's points (no hit otherwise): that compare chain runs
+    // first, and a row whose box test fails for every lane skips the FP64 ball
+    // test unless statistics (which count ball survivors) are being gathered.
     const bool useBox = A.aabb != nullptr;
-    // Without statistics the last prune keeps (cluster, query) by the box test
-    // alone: the box of a cluster's points is the tighter bound in these
-    // dimensions, and the exact filter decides every point anyway.
-    const bool ballTest = !useBox || A.keepBall;
+    const bool ballAlways = !useBox || A.ballStats;
     // write pass: every item's mask rows were stored (level 0: item i's rows at
     // i * count0; else the count pass reserved within the capacity)
     const bool allStored = MODE == kWrite && A.masks &&
@@ -332,21 +335,7 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
             const int nc = static_cast<int>(je - j0 < static_cast<uint64_t>(kWave) ? je - j0 : kWave);
             const uint64_t myId = lane < nc ? A.ids[j0 + lane] : 0;
             uint64_t myMask = 0, myBall = 0;
-            if (!ballTest) {
-                // box test alone: a compare chain per row against its bounding box
-                const uint64_t vm = __ballot(valid);
-                for (int k = 0; k < nc; ++k) {
-                    const uint64_t j = j0 + k;
-                    float blo[DM], bhi[DM];
-#pragma unroll
-                    for (int d = 0; d < DM; ++d) {
-                        blo[d] = d < D ? cab[j * 2 * D + d] : 0.0f;
-                        bhi[d] = d < D ? cab[j * 2 * D + D + d] : 0.0f;
-                    }
-                    const uint64_t m = aabb_mask<DM>(vm, blo, bhi, flo, fhi);
-                    if (lane == k) myMask = m;
-                }
-            } else {
+            const uint64_t vm = __ballot(valid);
             double xn[DM], rn;
 #pragma unroll
             for (int d = 0; d < DM; ++d) xn[d] = d < D ? cx[j0 * D + d] : 0.0;
@@ -362,38 +351,40 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_step(RangeStepArgs
                     for (int d = 0; d < DM; ++d) xn[d] = d < D ? cx[(j + 1) * D + d] : 0.0;
                     rn = crad[j + 1];
                 }
-                double dist = 0.0;
-                if (fastDist) {
-#pragma unroll
-                    for (int d = 0; d < DM; ++d) {
-                        const double diff = fmax(fmax(lo[d] - xv[d], xv[d] - hi[d]), 0.0);
-                        dist = __builtin_fma(diff, diff, dist);
-                    }
-                } else {
-#pragma unroll
-                    for (int d = 0; d < DM; ++d) {
-                        const double below = lo[d] - xv[d], above = xv[d] - hi[d];
-                        const double diff = xv[d] < lo[d] ? below : (xv[d] > hi[d] ? above : 0.0);
-                        dist = __builtin_fma(diff, diff, dist);
-                    }
-                }
-                const bool ball = valid && dist <= r * r;
-                bool keep = ball;
+                uint64_t boxm = vm;
                 if (useBox) {
+                    // compare chain against the row's bounding box in SGPRs
+                    float blo[DM], bhi[DM];
 #pragma unroll
                     for (int d = 0; d < DM; ++d) {
-                        const float blo = d < D ? cab[j * 2 * D + d] : 0.0f;
-                        const float bhi = d < D ? cab[j * 2 * D + D + d] : 0.0f;
-                        keep = keep & (bhi >= flo[d]) & (blo <= fhi[d]);
+                        blo[d] = d < D ? cab[j * 2 * D + d] : 0.0f;
+                        bhi[d] = d < D ? cab[j * 2 * D + D + d] : 0.0f;
                     }
+                    boxm = aabb_mask<DM>(vm, blo, bhi, flo, fhi);
                 }
-                const uint64_t m = __ballot(keep);
-                if (lane == k) myMask = m;
-                if (MODE == kCount) {
-                    const uint64_t mb = useBox ? __ballot(ball) : m;
-                    if (lane == k) myBall = mb;
+                uint64_t ballm = 0;
+                if (ballAlways || boxm != 0) {
+                    double dist = 0.0;
+                    if (fastDist) {
+#pragma unroll
+                        for (int d = 0; d < DM; ++d) {
+                            const double diff = fmax(fmax(lo[d] - xv[d], xv[d] - hi[d]), 0.0);
+                            dist = __builtin_fma(diff, diff, dist);
+                        }
+                    } else {
+#pragma unroll
+                        for (int d = 0; d < DM; ++d) {
+                            const double below = lo[d] - xv[d], above = xv[d] - hi[d];
+                            const double diff = xv[d] < lo[d] ? below : (xv[d] > hi[d] ? above : 0.0);
+                            dist = __builtin_fma(diff, diff, dist);
+                        }
+                    }
+                    ballm = __ballot(valid && dist <= r * r);
                 }
-            }
+                if (lane == k) {
+                    myMask = ballm & boxm;
+                    myBall = ballm;
+                }
             }
             if (MODE == kCount) {
                 // ball survivors per lane for the statistics; kept pairs per cluster
@@ -1644,7 +1635,7 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
         A.radiusPos = b->radiusPos[l];
         A.ids = b->ids[l];
         A.aabb = l == L - 1 ? b->aabb : nullptr;
-        A.keepBall = (d_level_surv || d_fine_count) ? 1 : 0;
+        A.ballStats = (d_level_surv || d_fine_count) ? 1 : 0;
         A.weightPos = (l == L - 1 && d_fine_count) ? b->weightPos[l] : nullptr;
         A.offNext = offS[nxt];
         A.qlistNext = qS[nxt];

@@ -3,8 +3,10 @@
 The library is built in-tree (``make -C t-blaeq_amd``).  There is no fallback:
 if the shared library is missing, importing any entry point raises.
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -166,14 +168,51 @@ def _p(a, t):
     return a.ctypes.data_as(t)
 
 
+# Native objects still open when the interpreter exits are closed by an atexit
+# hook, while the HIP runtime and every Python object they use still exist:
+# servers first (their worker threads use the handler's index), then learned
+# models and results, then handlers.  Without it, objects kept alive until
+# shutdown (e.g. by a failed test's traceback) would be finalised in arbitrary
+# order, possibly with server threads still running during runtime teardown.
+_LIVE = {"server": weakref.WeakSet(), "mlp": weakref.WeakSet(), "result": weakref.WeakSet(),
+         "handler": weakref.WeakSet()}
+
+
+def _track(kind, obj):
+    _LIVE[kind].add(obj)
+
+
+@atexit.register
+def close_all():
+    """Closes every native object still open (servers, models, results, then handlers)."""
+    for kind in ("server", "mlp", "result", "handler"):
+        for obj in list(_LIVE[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
+class _Closing:
+    """Context-manager support: `with Handler.random(...) as h:` closes on exit."""
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
+
 LEVEL_POLICIES = {"L0": 0, "L1": 1, "L2": 2, "L3": 3}
 
 
-class Handler:
+class Handler(_Closing):
     """Mirror of the reference QueryHandler (src/core/QueryHandler.cuh:50-247)."""
 
     def __init__(self, h):
         self.h = h
+        _track("handler", self)
         meta = np.zeros(4, np.uint64)
         ms = np.zeros(16, np.uint64)
         check(lib().tblaeq_handler_meta(self.h, _p(meta, u64p), _p(ms, u64p), 16), "meta")
@@ -396,7 +435,11 @@ class Handler:
         return c
 
     def close(self):
+        """Destroys the index (servers still serving it are closed first)."""
         if self.h:
+            for srv in list(_LIVE["server"]):
+                if srv.handler is self:
+                    srv.close()
             lib().tblaeq_handler_destroy(self.h)
             self.h = None
 
@@ -406,13 +449,15 @@ class Handler:
             self.h = None
 
 
-class Mlp:
-    """The learned-radius network (tblaeq_learn.h) on a handler's stream."""
+class Mlp(_Closing):
+    """The learned-radius network (tblaeq_learn.h) on the handler's device."""
 
     def __init__(self, handler, seed=7):
         self.m = vp()
+        self.handler = handler  # the handler outlives the network
         check(lib().tblaeq_mlp_create(handler.ctx(), int(seed), C.byref(self.m)), "mlp_create")
         self.n_params = int(lib().tblaeq_mlp_param_count())
+        _track("mlp", self)
 
     def get_params(self):
         w = np.zeros(self.n_params, np.float32)
@@ -472,7 +517,7 @@ class ServerConfig(C.Structure):
                 ("knn_request_bytes", C.c_uint64), ("max_wait_us", C.c_double)]
 
 
-class Server:
+class Server(_Closing):
     """Continuous-batching serving loop over a handler's index (tblaeq_serve.h).
     Close it before the handler."""
 
@@ -485,6 +530,7 @@ class Server:
         self.D = handler.D
         self.s = vp()
         check(lib().tblaeq_server_create(handler.h, C.byref(c), C.byref(self.s)), "server_create")
+        _track("server", self)
 
     def submit(self, qtype, q, K=0):
         q = np.ascontiguousarray(q, np.float64)
@@ -532,7 +578,9 @@ class Server:
             self.s = None
 
 
-class Result:
+class Result(_Closing):
+    """QueryResult of the plain traversal (host copies of the per-query results)."""
+
     def __init__(self, h, D, L):
         self.h = h
         self.D = D
@@ -540,6 +588,12 @@ class Result:
         nq = C.c_uint64(0)
         check(lib().tblaeq_result_info(self.h, C.byref(nq), None, None), "result_info")
         self.nq = nq.value
+        _track("result", self)
+
+    def close(self):
+        if self.h:
+            lib().tblaeq_result_destroy(self.h)
+            self.h = None
 
     def info(self, i):
         lv = np.zeros(self.L, np.uint64)

@@ -19,3 +19,14 @@ def gpu_available():
         return torch.cuda.is_available()
     except Exception:
         return False
+
+
+@pytest.fixture(autouse=True)
+def _close_test_handlers():
+    """Handlers a test makes through batch_checks.make are destroyed when the test ends,
+    passed or failed (module fixtures take theirs off the list themselves)."""
+    import batch_checks
+
+    n = len(batch_checks._OPEN)
+    yield
+    batch_checks.close_new(n)

@@ -5,8 +5,12 @@ superset sizes equal the oracle's."""
 import numpy as np
 import pytest
 
-import oracle_py as O
+import oracle_cases as OC
 import tblaeq
+from batch_checks import _OPEN
+from batch_checks import check_knn_batch as _check_knn_batch
+from batch_checks import check_range_batch as _check_range_batch
+from batch_checks import make as _make
 from tblaeq import workload as W
 
 pytestmark = pytest.mark.gpu
@@ -14,40 +18,12 @@ pytestmark = pytest.mark.gpu
 CFG = dict(N=60000, D=6, vmin=1.0, vmax=100.0, is_int=False, ratios=(30, 10, 8), seed=21, sigma_div=2.5)
 
 
-def _make(cfg, max_q=512, pair_cap=1 << 22, hit_cap=1 << 22):
-    h = tblaeq.Handler.random(cfg["N"], cfg["D"], cfg["vmin"], cfg["vmax"], cfg["is_int"], cfg["ratios"], cfg["seed"],
-                              cfg["sigma_div"])
-    o = O.Index.generate(cfg["N"], cfg["D"], cfg["vmin"], cfg["vmax"], cfg["is_int"], cfg["ratios"], cfg["seed"],
-                         cfg["sigma_div"])
-    h.prepare("L0")
-    h.batch_init(max_q, pair_cap, hit_cap)
-    return h, o
-
-
 @pytest.fixture(scope="module")
 def idx():
-    return _make(CFG)
-
-
-def _check_range_batch(h, o, boxes):
-    """Results without statistics (the timed path) and with them, against the oracle."""
-    h.batch_set("range", boxes)
-    h.batch_run()
-    plain = h.batch_results()
-    h.batch_run(stats=True)
-    res = h.batch_results()
-    surv, fine = h.batch_stats()
-    D = h.D
-    for i, row in enumerate(boxes):
-        lo, hi = row[:D], row[D:]
-        r = o.range(lo, hi)
-        exp = r.refine_range(lo, hi)
-        assert np.array_equal(plain[i], exp), "query %d hits differ (no statistics)" % i
-        assert np.array_equal(res[i], exp), "query %d hits differ" % i
-        _, s, out = r.levels()
-        assert np.array_equal(surv[i], s), "query %d survivors per level differ" % i
-        assert fine[i] == out[-1]
-    return res
+    h, o = _make(CFG)
+    yield h, o
+    _OPEN.remove(h)
+    h.close()
 
 
 def test_batch_range_matches_oracle(idx):
@@ -69,6 +45,44 @@ def test_batch_range_edges(idx):
     ])
     res = _check_range_batch(h, o, boxes)
     assert len(res[1]) == h.N and len(res[0]) == 0
+
+
+def test_batch_range_points_outside_stored_radius(idx):
+    """Degenerate boxes on reconstructed finest points that lie outside their last-level
+    cluster's stored radius (radii come from the original coordinates): the reference's ball
+    test drops the cluster, so none of these boxes may return the point, on any path."""
+    h, o = idx
+    ids, pts = OC.points_outside_radius(o, limit=256)
+    assert len(ids) >= 100
+    boxes = OC.degenerate_boxes(pts)
+    res = _check_range_batch(h, o, boxes)
+    assert sum(len(x) for x in res) == 0
+
+
+def test_batch_range_integer_grid():
+    """Integer index: degenerate boxes on points outside their cluster's radius and boxes with
+    integer corners (points on the box faces, equal coordinates everywhere)."""
+    h, o = _make(dict(CFG, N=20000, is_int=True, seed=8), max_q=512)
+    _, pts = OC.points_outside_radius(o, limit=128)
+    assert len(pts) >= 50
+    boxes = np.concatenate([OC.degenerate_boxes(pts), OC.integer_boxes(h.data(), 200, 3)])
+    res = _check_range_batch(h, o, boxes)
+    assert sum(len(x) for x in res) > 0
+
+
+@pytest.mark.parametrize("blocks,stage,radix", [(1, 256, 1), (2, 1024, 32), (64, 512, 2), (7, 256, 5),
+                                                (16, 1024, 4)])
+def test_batch_range_launch_params_vs_oracle(blocks, stage, radix):
+    """Range launch parameters (prune and filter grid, LDS hit stage, radix grid) change only
+    how the work is spread: every setting's answers equal the oracle's."""
+    h, o = _make(dict(CFG, N=30000, seed=19), max_q=256)
+    h.batch_set_param(tblaeq.PARAM_RANGE_BLOCKS_PER_CU, blocks)
+    h.batch_set_param(tblaeq.PARAM_HIT_STAGE, stage)
+    h.batch_set_param(tblaeq.PARAM_RADIX_BLOCKS_PER_CU, radix)
+    data = h.data()
+    boxes, _ = W.range_boxes(data, 160, 23, 1e-4, 0.2)
+    _, pts = OC.points_outside_radius(o, limit=40)
+    _check_range_batch(h, o, np.concatenate([boxes, OC.degenerate_boxes(pts)]))
 
 
 def test_batch_range_single_and_uneven(idx):
@@ -98,26 +112,6 @@ def test_batch_range_hierarchy_heights(ratios):
     data = h.data()
     boxes, _ = W.range_boxes(data, 80, 15, 1e-4, 5e-2)
     _check_range_batch(h, o, boxes)
-
-
-def _check_knn_batch(h, o, pts, Ks):
-    """Results without statistics (the timed path) and with them, against the oracle."""
-    h.batch_set("knn", pts, Ks)
-    h.batch_run()
-    plain = h.batch_results()
-    h.batch_run(stats=True)
-    res = h.batch_results()
-    surv, fine = h.batch_stats()
-    for i, q in enumerate(pts):
-        K = int(Ks[i])
-        r = o.knn(q, K)
-        eid, ed2 = r.refine_knn(q, K)
-        for gid, gd2 in (plain[i], res[i]):
-            assert np.array_equal(gid, eid), "query %d ids differ" % i
-            assert np.array_equal(gd2.view(np.uint64), ed2.view(np.uint64)), "query %d distances differ" % i
-        _, s, out = r.levels()
-        assert np.array_equal(surv[i], s), "query %d survivors per level differ" % i
-        assert fine[i] == out[-1]
 
 
 @pytest.mark.parametrize("K", [1, 10, 100])

@@ -12,6 +12,7 @@ from fractions import Fraction
 import numpy as np
 import pytest
 
+import oracle_cases as OC
 import oracle_py as O
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -304,3 +305,46 @@ def test_policy_recommend_hand_computed():
     assert O.policy_recommend(nnz, cols, D, 200, 300, free).tolist() == [0, 0]
     # not even the permanent data
     assert O.policy_recommend(nnz, cols, D, 200, 300, 100).tolist() == [3, 3]
+
+
+# ---------------------------------------------------------- reconstructed points outside their radius
+
+def test_recon_level_matches_traversal_values():
+    """Per-level reconstruction equals what a traversal with every radius infinite returns at
+    that level, and its finest level equals recon()."""
+    ix = O.Index.generate(3000, 3, 1.0, 100.0, False, (10, 5), 77, 3.0)
+    assert np.array_equal(ix.recon_level(ix.intervals).view(np.uint64), ix.recon().view(np.uint64))
+    coarse = ix.recon_level(0)
+    assert coarse.shape == (ix.mesh_sizes[-1], 3)
+    # level 1 values are P[0] ratios times their parent's coarsest values (SpMSpV.cuh:64)
+    lvl1 = ix.recon_level(1)
+    par = OC.parent_ids(ix, 0).astype(np.int64)
+    col_res, fmap, pv = ix.array(0, 0), ix.array(0, 3), ix.array(0, 4).reshape(-1, 3)
+    for c in range(len(col_res) - 1):
+        for j in range(int(col_res[c]), int(col_res[c + 1])):
+            assert par[fmap[j]] == c
+            assert np.array_equal(lvl1[fmap[j]], pv[j] * coarse[c])
+
+
+def test_points_outside_stored_radius_are_dropped_by_the_reference_traversal():
+    """The batched parity index holds reconstructed finest points outside their last-level
+    cluster's stored radius.  A degenerate box on one of them returns nothing from the
+    reference traversal although brute force over the reconstructed coordinates finds the
+    point; for many the cluster reaches the last level and its ball test drops it there --
+    the case a box-only last prune gets wrong, so the GPU tests on these boxes have teeth."""
+    ix = O.Index.generate(60000, 6, 1.0, 100.0, False, (30, 10, 8), 21, 2.5)
+    ids, pts = OC.points_outside_radius(ix)
+    assert len(ids) == 692
+    L = ix.intervals
+    fine = ix.recon_level(L)
+    par = OC.last_level_parent(ix).astype(np.int64)
+    grand = OC.parent_ids(ix, L - 2).astype(np.int64)
+    at_last = 0
+    for i, x in zip(ids[:200], pts[:200]):
+        r = ix.range(x, x)
+        assert len(r.refine_range(x, x)) == 0
+        assert int(i) in O.brute_range(fine, x, x).tolist()
+        if grand[par[int(i)]] in r.survivors(L - 2).tolist():
+            assert par[int(i)] not in r.survivors(L - 1).tolist()
+            at_last += 1
+    assert at_last >= 20, at_last
