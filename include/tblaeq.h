@@ -92,6 +92,22 @@ int tblaeq_range_prune_compact(tblaeq_ctx* ctx, const tblaeq_level* lv, const ui
                                uint64_t* surv_ids, double* surv_vals, uint64_t* surv_off, uint64_t* d_surv_count,
                                uint64_t* d_out_count);
 
+/* Allocation-free variant of tblaeq_range_prune_compact.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same kernel, arguments and results, but it takes the caller's stream (a
+ * hipStream_t, null = the default stream) instead of a context and keeps its
+ * look-back status words in a fixed device array of the library (512 KiB per
+ * device, held until the process ends), so it calls no hipMalloc, hipFree or
+ * allocating library routine.  count_bound may be at most 16 777 216 grid points
+ * (else TBLAEQ_ERR_CAPACITY).  All calls on one device share that array:
+ * calls that overlap on one device (concurrent streams or threads) race and
+ * must be serialised by the caller. */
+int tblaeq_range_prune_compact_noalloc(void* stream, const tblaeq_level* lv, const uint64_t* ids, const double* vals,
+                                       const uint64_t* d_count, uint64_t count_bound, const double* d_lo,
+                                       const double* d_hi, uint64_t* surv_ids, double* surv_vals, uint64_t* surv_off,
+                                       uint64_t* d_surv_count, uint64_t* d_out_count);
+
 /* KNN lower bound, selection of the closest clusters and STEP, then compaction.
  * Replaces calculateClusterDistanceKernel + gpuSort + host STEP loop + mask
  * upload + compaction (KNN.cu:9-110, MergeSort.cuh:628-696,

@@ -8,6 +8,9 @@
 // publishes its survivor count and survivor child-count through a decoupled
 // look-back, so survivors land in grid order together with their prolongation
 // offsets, and the element count is read from device memory (no host sync).
+#include <mutex>
+#include <string>
+
 #include "../ctx.hpp"
 #include "../scan_common.hpp"
 
@@ -335,5 +338,134 @@ extern "C" int tblaeq_knn_lower_bounds(tblaeq_ctx* ctx, uint64_t D, const uint64
     hipLaunchKernelGGL(k_knn_lb, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, ctx->stream,
                        static_cast<int>(D), ids, vals, count, d_q, radius, out_lb);
     TB_HIP(hipGetLastError());
+    return TBLAEQ_OK;
+}
+
+// --------------------------------------------- allocation-free range compaction
+//
+// tblaeq_range_prune_compact_noalloc: the same kernel as
+// tblaeq_range_prune_compact, with its look-back status words and scan control
+// block in a file-scope device array instead of a context's allocations.  The
+// array's device address is looked up once per device (hipGetSymbolAddress,
+// that device current) and passed to the kernel; host code never uses the
+// array's name as an address.  Every call on one device shares that device's
+// array: the status words carry the launch epoch (advanced on the device by the
+// last workgroup to leave), so consecutive calls need no clearing, and a fill
+// kernel clears them once every 2^26 calls, before the 27-bit epoch could wrap
+// onto a stale word.  Calls that overlap on one device race on the array.
+
+namespace tblaeq {
+
+constexpr uint64_t kNoAllocMaxTiles = 16384;  // grid points per call <= 16384 * 1024
+constexpr uint64_t kNoAllocCtrlWords = sizeof(ScanCtrl) / sizeof(uint64_t);
+constexpr uint64_t kNoAllocWords = kNoAllocCtrlWords + kNoAllocMaxTiles * 4;  // 512 KiB + 64 B
+
+static __device__ __align__(256) uint64_t scratch[kNoAllocWords];
+
+constexpr int kNoAllocMaxDevices = 64;
+
+struct NoAllocDevice {
+    uint64_t* ptr = nullptr;  // device address of `scratch` on this device
+    int cuCount = 0;
+    uint64_t calls = 0;
+};
+
+__global__ void k_noalloc_clear(uint64_t* p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        p[i] = 0;
+}
+
+// Per-device set-up (first call on a device): the array's address on the
+// current device and its CU count.  Returns null with the error recorded.
+static NoAllocDevice* noalloc_device(int* rc) {
+    static std::mutex mu;
+    static NoAllocDevice devs[kNoAllocMaxDevices];
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        *rc = hip_fail(e, "hipGetDevice");
+        return nullptr;
+    }
+    if (dev < 0 || dev >= kNoAllocMaxDevices) {
+        set_error("tblaeq_range_prune_compact_noalloc: device index out of range");
+        *rc = TBLAEQ_ERR_UNSUPPORTED;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    NoAllocDevice& d = devs[dev];
+    if (!d.ptr) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            *rc = hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+            return nullptr;
+        }
+        int cus = 0;
+        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) {
+            *rc = hip_fail(e, "hipDeviceGetAttribute");
+            return nullptr;
+        }
+        d.cuCount = cus > 0 ? cus : 1;
+        d.ptr = static_cast<uint64_t*>(p);
+    }
+    return &d;
+}
+
+}  // namespace tblaeq
+
+extern "C" int tblaeq_range_prune_compact_noalloc(void* stream, const tblaeq_level* lv, const uint64_t* ids,
+                                                  const double* vals, const uint64_t* d_count, uint64_t count_bound,
+                                                  const double* d_lo, const double* d_hi, uint64_t* surv_ids,
+                                                  double* surv_vals, uint64_t* surv_off, uint64_t* d_surv_count,
+                                                  uint64_t* d_out_count) {
+    if (!lv || !d_count || !d_lo || !d_hi || !surv_ids || !surv_vals || !surv_off || !d_surv_count ||
+        !d_out_count || lv->D == 0 || lv->D > 64 || !lv->radius || !lv->col_res) {
+        set_error("tblaeq_range_prune_compact_noalloc: invalid argument");
+        return TBLAEQ_ERR_INVALID;
+    }
+    const uint64_t tiles = count_bound == 0 ? 1 : (count_bound + kScanTile - 1) / kScanTile;
+    if (tiles > kNoAllocMaxTiles) {
+        set_error("tblaeq_range_prune_compact_noalloc: count_bound " + std::to_string(count_bound) +
+                  " exceeds the fixed scratch (at most " + std::to_string(kNoAllocMaxTiles * kScanTile) +
+                  " grid points per call)");
+        return TBLAEQ_ERR_CAPACITY;
+    }
+    int rc = TBLAEQ_OK;
+    NoAllocDevice* dev = noalloc_device(&rc);
+    if (!dev) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    uint64_t calls;
+    {
+        static std::mutex mu;
+        std::lock_guard<std::mutex> lock(mu);
+        calls = ++dev->calls;
+    }
+    if (calls % (1ull << 26) == 0)
+        hipLaunchKernelGGL(k_noalloc_clear, dim3(64), dim3(256), 0, s, dev->ptr + kNoAllocCtrlWords,
+                           kNoAllocMaxTiles * 4);
+    CompactArgs A{};
+    A.D = static_cast<int>(lv->D);
+    A.ids = ids;
+    A.vals = vals;
+    A.dCount = d_count;
+    A.a = d_lo;
+    A.b = d_hi;
+    A.radius = lv->radius;
+    A.colRes = lv->col_res;
+    A.outIds = surv_ids;
+    A.outVals = surv_vals;
+    A.outOff = surv_off;
+    A.dOutCount = d_surv_count;
+    A.dOutNnz = d_out_count;
+    A.ctrl = reinterpret_cast<ScanCtrl*>(dev->ptr);
+    A.status = dev->ptr + kNoAllocCtrlWords;
+    A.maxTiles = tiles;
+    const uint64_t cap = static_cast<uint64_t>(dev->cuCount) * 8;
+    const unsigned grid = static_cast<unsigned>(tiles < cap ? tiles : cap);
+    hipLaunchKernelGGL(k_compact<kModeRange>, dim3(grid), dim3(kScanThreads), 0, s, A);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "tblaeq_range_prune_compact_noalloc launch");
     return TBLAEQ_OK;
 }

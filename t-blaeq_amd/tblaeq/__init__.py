@@ -34,6 +34,7 @@ SIGNATURES = {
     "tblaeq_ctx_timing": (C.c_int, [vp, C.c_int, C.c_int, u64p, f64p]),
     "tblaeq_ctx_timing_reset": (C.c_int, [vp]),
     "tblaeq_range_prune_compact": (C.c_int, [vp] * 13),
+    "tblaeq_range_prune_compact_noalloc": (C.c_int, [vp, vp, vp, vp, vp, C.c_uint64, vp, vp, vp, vp, vp, vp, vp]),
     "tblaeq_knn_lower_bound_sort_step": (C.c_int, [vp, vp, vp, vp, vp, C.c_uint64, vp, C.c_uint64, vp, vp, vp, vp,
                                                    vp]),
     "tblaeq_prolong_refactor": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_uint64, vp, vp]),
