@@ -9,10 +9,10 @@ exact answers (range: sorted id sets; KNN: top-K ids and squared distances).
 Reported:
   value           queries/s of the fastest path measured (whole job, 1 GPU)
   plain           the per-query GPU traversal (QueryHandler -> QueryEngine -> L0)
-  roofline        dominant batched kernel (the exact box filter): FP64 compare
-                  operations per launch / average launch time (HIP events on
-                  the library's stream) against the FP64 peak; the plain
-                  path's dominant HBM kernel is reported beside it
+  roofline        the batched kernel with the largest time per step (the exact
+                  box filter against VALU compare issue, or the hit sort
+                  against HBM), from HIP events on the library's stream; the
+                  other one and the plain path's dominant HBM kernel beside it
   cpu_baseline    the OpenMP CPU traversal (the oracle) on the host cores
 """
 import argparse
@@ -31,6 +31,9 @@ import numpy as np  # noqa: E402
 METRIC = "range+KNN queries/s at N=10M D=8, batched/learned vs plain; p99 latency"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# VALU lane-operations per second (non-packed FP64/FP32 compares): 256 CUs x 4
+# SIMDs x 16 lanes x 2.4 GHz; half the FP64 FMA flop rate of 78.6 TFLOP/s
+VALU_PEAK_TOPS = 39.3
 
 
 def final_traffic(cfg):
@@ -43,6 +46,20 @@ def final_traffic(cfg):
     except (OSError, ValueError):
         return None
     if t.get("N") != cfg["N"] or t.get("D") != cfg["D"]:
+        return None
+    return t.get("bytes_per_launch")
+
+
+def sort_traffic(cfg, passes):
+    """HBM bytes per hit sort (counts + passes), from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary for this workload (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "c2_hit_sort_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("N") != cfg["N"] or t.get("D") != cfg["D"] or t.get("passes") != passes:
         return None
     return t.get("bytes_per_launch")
 
@@ -401,6 +418,7 @@ def main():
                  ("knn_refine", 15), ("knn_merge", 16))
         kacc = {n: [0, 0.0] for n, _ in kinds}
         work_r = work_k = (0, 0)
+        leaf_tests = 0
         for it in range(1 + args.steps):
             h.batch_set("range", boxes)
             timing.reset()
@@ -408,6 +426,7 @@ def main():
             hits_p = h.batch_run()
             t1 = time.perf_counter()
             work_r = h.batch_work()
+            leaf_tests = h.batch_leaf_tests()
             h.batch_set("knn", knn_q, bK)
             t2 = time.perf_counter()
             nn_p = h.batch_run()
@@ -432,6 +451,7 @@ def main():
                    "knn_plain_ms": round(float(np.mean(tp_k)) * 1e3, 3),
                    "kernels_ms_per_step": per_step,
                    "final_filter_tests": work_r[0], "final_filter_rows": work_r[1],
+                   "final_filter_leaf_tests": leaf_tests,
                    "knn_refine_evals": work_k[0], "knn_refine_rows": work_k[1],
                    "speedup_vs_plain": round(2 * args.nq / step_s / plain_qps, 2),
                    "autotune": None if tuned is None else {
@@ -439,24 +459,55 @@ def main():
                        "knn_ms_before_after": [round(float(x), 3) for x in tuned["knn_ms"]],
                        "params": tuned_params}}
         log("batched", batched)
-        # roofline of the dominant batched kernel: the exact box filter (one
-        # launch per range batch).  Its work is 2D FP64 compares per candidate
-        # row x box (work_r[0] tests); it is bounded by FP64 compare issue, not
-        # by HBM (each child row is read once per pair of 64-box slices, as a
-        # scalar load).
+        # rooflines of the two largest batched kernels, per launch (one launch
+        # per range batch, averaged over the timed plain-launch steps):
+        #  - the exact box filter: bounded by VALU compare issue (each compare
+        #    is one VALU lane-operation; 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+        #    = 39.3 T lane-ops/s for FP64 and FP32 alike), with its algorithmic
+        #    HBM bytes (candidate rows read once per work item, hits written)
+        #    beside it;
+        #  - the hit sort: HBM-bound, algorithmic bytes = a key read for the
+        #    counts plus a key read and write per pass.
+        # The dominant one (largest time per step) is the bench line's roofline.
+        D = cfg["D"]
+        rl = {}
         n_f, ms_f = kacc["range_final"]
         if n_f:
             avg_s = ms_f / 1e3 / n_f
-            ops = 2.0 * cfg["D"] * work_r[0]
+            ops = 2.0 * D * (work_r[0] + leaf_tests)
             ach = ops / avg_s / 1e12
-            roofline = {"bound": "mfma", "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": final_traffic(cfg),
-                        "kernel": "k_range_final<8> (exact box filter of the batched range path)",
-                        "launches": n_f, "avg_launch_us": round(avg_s * 1e6, 3), "ops_per_launch": ops,
-                        "note": "FP64 vector compares (2D per candidate row x box, a v_cmpx chain), counted "
-                                "against MI355X's FP64 peak (vector and matrix both 78.6 TFLOP/s); each child row is "
-                                "one scalar load per pair of 64-box slices, so HBM is not its bound",
-                        "plain": plain_roofline}
+            alg = work_r[1] * (8 * D + 8) + 8 * tot_hits
+            rl["range_final"] = {
+                "bound": "fp64-valu", "achieved": round(ach, 3), "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+                "frac": round(ach / VALU_PEAK_TOPS, 4), "traffic": final_traffic(cfg),
+                "kernel": "k_range_final<%d> (exact box filter of the batched range path, kd leaves)" % D,
+                "launches": n_f, "avg_launch_us": round(avg_s * 1e6, 3), "ops_per_launch": ops,
+                "row_tests": int(work_r[0]), "leaf_tests": int(leaf_tests), "rows": int(work_r[1]),
+                "hits": int(tot_hits), "algorithmic_bytes": int(alg),
+                "hbm_frac_algorithmic": round(alg / avg_s / 1e9 / HBM_PEAK_GBS, 4),
+                "note": "compare lane-operations (2D per row x box test in FP64, 2D per leaf x box test in FP32, "
+                        "v_cmpx chains) against the VALU issue rate; HBM is not its bound"}
+        n_s, ms_s = kacc["radix_sort"]
+        if n_s:
+            avg_s = ms_s / 1e3 / n_s
+            id_bits = max(1, int(cfg["N"] - 1).bit_length())
+            q_bits = max(1, int(args.nq - 1).bit_length())
+            nbits = id_bits + q_bits
+            rb = 9 if (nbits + 8) // 9 < (nbits + 7) // 8 else 8
+            passes = (nbits + rb - 1) // rb
+            alg = tot_hits * (8 + 16 * passes)
+            ach = alg / avg_s / 1e9
+            rl["radix_sort"] = {
+                "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": sort_traffic(cfg, passes),
+                "kernel": "hit sort (k_rs_counts + %d x k_rs_onesweep<%d>, %d-bit keys)" % (passes, rb, nbits),
+                "launches": n_s, "avg_launch_us": round(avg_s * 1e6, 3), "bytes_per_launch": int(alg),
+                "keys": int(tot_hits)}
+        if rl:
+            dom = max(rl, key=lambda k: rl[k]["avg_launch_us"])
+            roofline = dict(rl[dom])
+            roofline["others"] = {k: v for k, v in rl.items() if k != dom}
+            roofline["plain"] = plain_roofline
 
     learned = None
     if args.learn_traces and not args.no_batch:

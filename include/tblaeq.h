@@ -217,6 +217,10 @@ int tblaeq_batch_reserve(tblaeq_batch* b, uint64_t max_queries, uint64_t pair_ca
  * Asynchronous on the context stream; no host synchronisation inside. */
 int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64_t nq, uint64_t* d_out_off, uint64_t* d_out_ids,
                        uint64_t out_cap, uint64_t* d_status, uint64_t* d_level_surv, uint64_t* d_fine_count);
+/* Leaf x box tests of the exact filter in the engine's last range batch (the
+ * float tests of the kd leaves' boxes that decide which rows are tested);
+ * synchronises the context stream.  No reference counterpart (measurement). */
+int tblaeq_batch_leaf_tests(tblaeq_batch* b, uint64_t* out);
 /* tblaeq_batch_range as one hipGraph launch: the first call with a given set
  * of arguments captures the whole batch (level loop, grouping, hit sort) on
  * the context stream; later calls with the same arguments replay it (the

@@ -89,6 +89,8 @@ int tblaeq_handler_batch_use_graph(tblaeq_handler* h, int on);
  * distance evaluations of the cluster-major refine), work[1] = candidate rows
  * scanned. */
 int tblaeq_handler_batch_work(tblaeq_handler* h, uint64_t* work);
+/* tblaeq_batch_leaf_tests of the handler's engine (the last range sub-batch). */
+int tblaeq_handler_batch_leaf_tests(tblaeq_handler* h, uint64_t* out);
 /* tblaeq_batch_autotune on the handler's engine with host sample queries
  * (boxes rows lo|hi, KNN points and K per point); out_ms as there. */
 int tblaeq_handler_batch_autotune(tblaeq_handler* h, const double* boxes, uint64_t nBoxes, const double* points,

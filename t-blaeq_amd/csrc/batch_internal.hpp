@@ -2,7 +2,8 @@
 //
 // The engine keeps, for every level l = 0..L, the level's points in "position"
 // order: level 0 in coarsest-mesh order, level l >= 1 in the CSC row order of
-// P[l-1].  Their reconstructed coordinates are produced once, at creation, by
+// P[l-1] (at the finest level, the rows of each column reordered into kd
+// order).  Their reconstructed coordinates are produced once, at creation, by
 // running the prolongation over the whole hierarchy (every cluster a survivor),
 // so they are bit-identical to what the per-query prolongation produces.  A
 // cluster c of level l owns the contiguous positions colRes_l[c]..colRes_l[c+1]
@@ -37,6 +38,10 @@ struct tblaeq_batch {
     uint64_t* off2 = nullptr;
     uint64_t* items2 = nullptr;
     float* aabb = nullptr;       // [count[L-1] * 2D] bounding box of each last-level position's points
+    uint64_t* leafOff = nullptr; // [cols of P[L-1] + 1] first kd leaf of every last-level cluster, by id
+    float* leafBox = nullptr;    // [leaves * 2D] outward-rounded float box lo[D] | hi[D] of every kd leaf
+    uint64_t nLeaves = 0;
+    float* fbox = nullptr;       // [maxQ * 2 * dm] outward float boxes of the current range batch
     double* enclC = nullptr;     // [count[L-2] * D] (L >= 2) centre of each level-(L-2) cluster, by id
     double* enclR = nullptr;     // [count[L-2]] radius of a ball around it holding its children's balls
     float* enclBox = nullptr;    // [count[L-2] * 2D] box lo[D] | hi[D] holding its children's balls, by id
@@ -94,6 +99,11 @@ namespace tblaeq {
 constexpr int kSliceBits = 20;
 constexpr int kQueryBits = 24;
 constexpr int kStepWaves = 4;  // waves per workgroup of the cluster-major kernels
+// The finest rows of every last-level cluster are stored in kd order (engine
+// creation): each run of kLeafRows consecutive rows of a cluster is a leaf with
+// its own bounding box, tested before its rows by the range exact filter.
+constexpr int kLeafRows = 16;
+constexpr int kFilterPadRows = 4;  // spare rows after the finest level's last row
 
 // chunked counting scatters: kAggPer elements per thread, targets ranked in
 // LDS when there are at most kAggLds of them

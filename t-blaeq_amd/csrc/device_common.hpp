@@ -123,6 +123,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 typedef const __attribute__((address_space(4))) double* ConstF64;
 typedef const __attribute__((address_space(4))) uint64_t* ConstU64;
 typedef const __attribute__((address_space(4))) float* ConstF32;
+// LDS pointer (its integer value is the LDS byte address).
+typedef __attribute__((address_space(3))) uint64_t* LdsU64;
 
 // Uniform copy of a value every lane holds (keeps it in SGPRs).
 __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {

@@ -283,6 +283,12 @@ int tblaeq_handler_batch_work(tblaeq_handler* h, uint64_t* work) {
     return TBLAEQ_OK;
 }
 
+int tblaeq_handler_batch_leaf_tests(tblaeq_handler* h, uint64_t* out) {
+    BatchState* st = stateOf(h);
+    if (!st || !out) return invalidArg("tblaeq_handler_batch_leaf_tests");
+    return tblaeq_batch_leaf_tests(st->b, out);
+}
+
 int tblaeq_handler_batch_autotune(tblaeq_handler* h, const double* boxes, uint64_t nBoxes, const double* points,
                                   const uint64_t* Ks, uint64_t nPoints, int reps, double* outMs) {
     BatchState* st = stateOf(h);

@@ -13,8 +13,11 @@
 // path (RangePruning.cu:5-25: fma chain of the clamped differences against
 // r*r); the last step is the exact point-in-box filter of the candidates.
 #include <algorithm>
+#include <atomic>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
+#include <vector>
 
 #include "../batch_internal.hpp"
 #include "../scan_common.hpp"
@@ -107,7 +110,7 @@ struct RangeStepArgs {
     uint64_t* survStat;  // [nq * L]
     uint64_t* fineStat;  // [nq]
     int level, L;
-    uint64_t* work;      // final: [0] box tests, [1] child rows scanned
+    uint64_t* work;      // final: [0] row x box tests, [1] child rows, [2] leaf x box tests
     uint64_t* queue;     // final: work-queue counter (zero at launch)
     // prune: the count pass stores each item's per-row keep masks here when
     // they fit in maskCap (item i's rows from maskOff[i], each wave reserving
@@ -121,6 +124,13 @@ struct RangeStepArgs {
     // list (returned by the count pass's atomic), so that the write pass, when
     // every row of the level was stored, places the queries without atomics
     uint64_t* maskPos;
+    // final filter: first leaf of every last-level cluster (by id) and the
+    // leaves' outward-rounded float boxes lo[D] | hi[D]
+    const uint64_t* leafOff;
+    const float* leafBox;
+    // final filter: outward-rounded float copy of every box, lo | hi with
+    // stride 2 * dm_of(D) (padding dimensions 0)
+    const float* fboxes;
     // with a box test (aabb): evaluate the ball test of every row, also where
     // the box test already failed for every lane (the statistics count ball
     // survivors); the kept pairs are ball AND box either way
@@ -141,6 +151,25 @@ __device__ __forceinline__ void flush_hits(const RangeStepArgs& A, const uint64_
     wave_lds_sync();
 }
 
+
+// flush_hits for code that keeps few scalar registers free: the output
+// pointers and capacity are read from the kernel arguments at the flush (a
+// volatile read, so they are not held in registers across the caller's loop).
+__device__ __forceinline__ void flush_hits_lean(const RangeStepArgs& A, const uint64_t* stage, uint64_t n, int lane) {
+    if (n == 0) return;
+    uint64_t* const out = *reinterpret_cast<uint64_t* const volatile*>(&A.out);
+    uint64_t* const outCount = *reinterpret_cast<uint64_t* const volatile*>(&A.outCount);
+    uint64_t* const overflow = *reinterpret_cast<uint64_t* const volatile*>(&A.overflow);
+    const uint64_t cap = *reinterpret_cast<const volatile uint64_t*>(&A.cap);
+    wave_lds_sync();
+    uint64_t base = 0;
+    if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(outCount), n);
+    base = __shfl(base, 0, kWave);
+    for (uint64_t o = lane; o < n; o += kWave)
+        if (base + o < cap) out[base + o] = stage[o];
+    if (lane == 0 && base + n > cap) atomicOr(reinterpret_cast<unsigned long long*>(overflow), 4ull);
+    wave_lds_sync();
+}
 
 // Write pass of a chunk of nc child rows: lane k holds row k's keep mask
 // (queries of the slice that keep child myId); each kept (child, query) pair
@@ -482,6 +511,31 @@ __device__ __forceinline__ uint64_t box_mask8(uint64_t m, const double* x, const
     return out;
 }
 
+// Writes key | id of every lane of m into LDS at byte address
+// ldsAddr + 8 * (rank of the lane among m): EXEC is set to m (skipping the
+// writes when it is empty), restored in the same asm block.  The key's and id's
+// halves are OR-ed into the two words of a ds_write2_b32.
+__device__ __forceinline__ void stage_write(uint64_t m, uint32_t ldsAddr, uint32_t klo, uint32_t khi, uint64_t id) {
+    uint64_t save;
+    uint32_t t, w0, w1;
+    asm volatile(
+        "s_mov_b64 %[sv], exec\n\t"
+        "s_mov_b64 exec, %[m]\n\t"
+        "s_cbranch_execz .Lstage_skip%=\n\t"
+        "v_mbcnt_lo_u32_b32 %[t], exec_lo, 0\n\t"
+        "v_mbcnt_hi_u32_b32 %[t], exec_hi, %[t]\n\t"
+        "v_lshl_add_u32 %[t], %[t], 3, %[a]\n\t"
+        "v_or_b32 %[w0], %[il], %[kl]\n\t"
+        "v_or_b32 %[w1], %[ih], %[kh]\n\t"
+        "ds_write2_b32 %[t], %[w0], %[w1] offset1:1\n\t"
+        ".Lstage_skip%=:\n\t"
+        "s_mov_b64 exec, %[sv]"
+        : [sv] "=&s"(save), [t] "=&v"(t), [w0] "=&v"(w0), [w1] "=&v"(w1)
+        : [m] "s"(m), [a] "s"(ldsAddr), [kl] "v"(klo), [kh] "v"(khi), [il] "s"(static_cast<uint32_t>(id)),
+          [ih] "s"(static_cast<uint32_t>(id >> 32))
+        : "memory");
+}
+
 template <int DM>
 __device__ __forceinline__ uint64_t box_mask(uint64_t m, const double* x, const double* lo, const double* hi) {
     if constexpr (DM == 4) {
@@ -506,107 +560,217 @@ __device__ __forceinline__ bool slice_query(const RangeStepArgs& A, uint64_t c, 
 }
 
 // The exact point-in-box filter of the last level (RangePruning.cu:5-25 with
-// r = 0 reduces to x in [lo, hi] per dimension).  A wave takes the work items
-// of an even slice 2s of a cluster together with its odd successor 2s + 1 (odd
-// items are skipped): every lane holds up to two query boxes in registers, so
-// each child row -- a scalar load into SGPRs, the next row in flight while the
-// current one is compared -- is tested against up to 128 boxes.  The hit mask of
-// a row and box set is box_mask.  Hits go straight into the wave's LDS stage and
-// are appended to the global list in runs of up to STAGE (one atomic per run).
-// Items come from a work queue (chunks of consecutive items per grab).
+// r = 0 reduces to x in [lo, hi] per dimension), over kd leaves.  The rows of
+// every last-level cluster are stored in kd order at engine creation, so each
+// run of kLeafRows consecutive rows (a leaf) is spatially tight and carries an
+// outward-rounded float bounding box.  A wave takes a work item (cluster, 64-
+// query slice) whose slice is a multiple of S = final_sets(DM) together with
+// the next S - 1 slices of the same cluster (the other items are skipped): up
+// to 64 S queries, every lane holding S outward-rounded float boxes.
+// Per leaf, a float v_cmpx chain per set gives the queries whose box meets the
+// leaf's box; those queries (about a fifth at the headline workload) are
+// compacted through LDS into full 64-lane chunks, each lane loads its query's
+// FP64 box, and the leaf's rows -- scalar loads into SGPRs, the next row in
+// flight while the current one is compared -- are tested with the FP64 v_cmpx
+// chain of box_mask.  A query whose box misses a leaf's box has no hit among
+// its rows, so the hits are exactly those of the whole cluster.  Hits go to the
+// wave's LDS stage and are appended to the global list in runs (one atomic per
+// run).  Items come from a work queue.
+// Box sets per lane: 4 up to D = 8, 2 up to 16, 1 above (register budget).
+constexpr int final_sets(int DM) { return DM <= 8 ? 4 : (DM <= 16 ? 2 : 1); }
+
 template <int DM, bool EXACT, int STAGE>
 __global__ __launch_bounds__(kStepWaves * kWave) void k_range_final(RangeStepArgs A) {
+    constexpr int SETS = final_sets(DM);
     __shared__ uint64_t s_hit[kStepWaves][STAGE];
+    __shared__ uint32_t s_q[kStepWaves][SETS * kWave];
     const int lane = threadIdx.x & (kWave - 1);
     const uint64_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kStepWaves;
     const int D = EXACT ? DM : A.D;
-    uint64_t nItems = A.level0 ? (A.nq + kWave - 1) / kWave : ((ConstU64)A.nItems)[0];
-    if (!A.level0 && nItems > A.itemCap) nItems = A.itemCap;
+    uint64_t nItems = ((ConstU64)A.nItems)[0];
+    if (nItems > A.itemCap) nItems = A.itemCap;
     uint64_t* stage = s_hit[wave];
+    const uint32_t stageAddr = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((LdsU64)stage));
+    uint32_t* qs = s_q[wave];
     uint32_t staged = 0;
-    uint64_t wTests = 0, wRows = 0;
+    uint64_t wTests = 0, wRows = 0, wLeafTests = 0;
     const uint64_t chunk = queue_chunk(nItems, nw);
-    // appends the hits of one row (lanes of m) to the stage
-    auto emit = [&](uint64_t m, uint64_t key) {
-        const uint32_t pc = static_cast<uint32_t>(__popcll(m));
-        if (staged + pc > static_cast<uint32_t>(STAGE)) {
-            flush_hits(A, stage, staged, lane);
-            staged = 0;
-        }
-        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
-                                                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
-        if ((m >> lane) & 1ull) stage[staged + rank] = key;
-        staged += pc;
-    };
+    const ConstF32 lbox = (ConstF32)A.leafBox;
+    const uint64_t rowBytes = static_cast<uint64_t>(D) * 8;
     uint64_t ib, ie;
     while (next_items(A.queue, chunk, nItems, lane, &ib, &ie))
     for (uint64_t item = ib; item < ie; ++item) {
-        uint64_t jb, je, q0 = 0, q1 = 0;
-        bool v0, v1 = false;
-        if (A.level0) {
-            const uint64_t qq = item * kWave + lane;
-            v0 = qq < A.nq;
-            q0 = v0 ? qq : 0;
-            jb = 0;
-            je = A.count0;
-        } else {
-            const uint64_t it = ((ConstU64)A.items)[item];
-            const uint64_t c = it >> kSliceBits;
-            const uint64_t slice = it & ((1ull << kSliceBits) - 1);
-            if (slice & 1) continue;  // done with its even partner
-            v0 = slice_query(A, c, slice, lane, &q0);
-            v1 = slice_query(A, c, slice + 1, lane, &q1);
-            jb = ((ConstU64)A.colResPrev)[c];
-            je = ((ConstU64)A.colResPrev)[c + 1];
-        }
-        const uint64_t vm0 = __ballot(v0), vm1 = __ballot(v1);
-        wRows += je - jb;
-        wTests += (je - jb) * static_cast<uint64_t>(__popcll(vm0) + __popcll(vm1));
-        if ((vm0 | vm1) == 0 || jb >= je) continue;
-        double lo0[DM], hi0[DM], lo1[DM], hi1[DM];
+        const uint64_t it = ((ConstU64)A.items)[item];
+        const uint64_t c = it >> kSliceBits;
+        const uint64_t slice = it & ((1ull << kSliceBits) - 1);
+        if (slice % SETS) continue;  // done with the first slice of its group
+        // the lane's query of every set (bit s of vbits: set s holds one)
+        uint32_t q[SETS];
+        uint32_t vbits = 0;
 #pragma unroll
-        for (int d = 0; d < DM; ++d) {
-            lo0[d] = d < D ? A.boxes[q0 * 2 * D + d] : 0.0;
-            hi0[d] = d < D ? A.boxes[q0 * 2 * D + D + d] : 0.0;
-            lo1[d] = d < D ? A.boxes[q1 * 2 * D + d] : 0.0;
-            hi1[d] = d < D ? A.boxes[q1 * 2 * D + D + d] : 0.0;
+        for (int s = 0; s < SETS; ++s) {
+            uint64_t qq;
+            vbits |= static_cast<uint32_t>(slice_query(A, c, slice + s, lane, &qq)) << s;
+            q[s] = static_cast<uint32_t>(qq);
         }
-        const uint64_t k0 = q0 << A.idBits, k1 = q1 << A.idBits;
-        const ConstF64 rows = (ConstF64)A.coords + jb * D;
-        const ConstU64 rid = (ConstU64)A.ids + jb;
-        const uint32_t n = static_cast<uint32_t>(je - jb);
-        double xa[DM], xb[DM];
-        uint64_t ida = rid[0], idb = 0;
-        scalar_row<DM, EXACT>(xa, rows, 0, D);
-        // one row: compare against both box sets, emit
-        auto test = [&](const double* x, uint64_t id) {
-            const uint64_t m0 = box_mask<DM>(vm0, x, lo0, hi0);
-            if (m0) emit(m0, k0 | id);
-            if (vm1) {
-                const uint64_t m1 = box_mask<DM>(vm1, x, lo1, hi1);
-                if (m1) emit(m1, k1 | id);
+        // every set's float boxes in one batch of loads (invalid lanes read box 0)
+        float flo[SETS][DM], fhi[SETS][DM];
+#pragma unroll
+        for (int s = 0; s < SETS; ++s) {
+            const float* fb = A.fboxes + static_cast<uint64_t>(q[s]) * 2 * DM;
+#pragma unroll
+            for (int d = 0; d < DM; ++d) {
+                flo[s][d] = fb[d];
+                fhi[s][d] = fb[DM + d];
             }
-        };
-        for (uint32_t r = 0; r < n; r += 2) {
-            if (r + 1 < n) {
-                scalar_row<DM, EXACT>(xb, rows, r + 1, D);
-                idb = rid[r + 1];
-            }
-            test(xa, ida);
-            if (r + 1 >= n) break;
-            if (r + 2 < n) {
-                scalar_row<DM, EXACT>(xa, rows, r + 2, D);
-                ida = rid[r + 2];
-            }
-            test(xb, idb);
         }
+        const uint64_t jb = ((ConstU64)A.colResPrev)[c], je = ((ConstU64)A.colResPrev)[c + 1];
+        if (__ballot(vbits != 0) == 0 || jb >= je) continue;
+        wRows += je - jb;
+        const uint64_t leaf0 = ((ConstU64)A.leafOff)[c];
+        const uint32_t nLeaves = static_cast<uint32_t>((je - jb + kLeafRows - 1) / kLeafRows);
+        // one vector load per 64-byte line of the cluster's rows, ids and leaf
+        // boxes pulls them into L2 at once, ahead of their scalar loads (the
+        // loaded words are consumed at the end of the item)
+        uint32_t pf = 0;
+        {
+            const uint32_t* pr = reinterpret_cast<const uint32_t*>(A.coords + jb * D);
+            const uint64_t nlr = ((je - jb) * D * 8 + 63) / 64;
+            for (uint64_t l = lane; l < nlr; l += kWave) pf ^= pr[l * 16];
+            const uint32_t* pi = reinterpret_cast<const uint32_t*>(A.ids + jb);
+            const uint64_t nli = ((je - jb) * 8 + 63) / 64;
+            for (uint64_t l = lane; l < nli; l += kWave) pf ^= pi[l * 16];
+            const uint32_t* pb = reinterpret_cast<const uint32_t*>(A.leafBox + leaf0 * 2 * D);
+            const uint64_t nlb = (static_cast<uint64_t>(nLeaves) * 2 * D * 4 + 63) / 64;
+            for (uint64_t l = lane; l < nlb; l += kWave) pf ^= pb[l * 16];
+        }
+        for (uint32_t lf = 0; lf < nLeaves; ++lf) {
+            // queries whose box meets the leaf's box, per set
+            float llo[DM], lhi[DM];
+#pragma unroll
+            for (int d = 0; d < DM; ++d) {
+                llo[d] = d < D ? lbox[(leaf0 + lf) * 2 * D + d] : 0.0f;
+                lhi[d] = d < D ? lbox[(leaf0 + lf) * 2 * D + D + d] : 0.0f;
+            }
+            // compact those queries into the wave's list
+            uint32_t total = 0;
+#pragma unroll
+            for (int s = 0; s < SETS; ++s) {
+                const uint64_t vm = __ballot((vbits >> s) & 1u);
+                if (vm == 0) continue;
+                const uint64_t m = aabb_mask<DM>(vm, llo, lhi, flo[s], fhi[s]);
+                wLeafTests += __popcll(vm);
+                if ((m >> lane) & 1ull) {
+                    const uint32_t r = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+                    qs[total + r] = q[s];
+                }
+                total += static_cast<uint32_t>(__popcll(m));
+            }
+            if (total == 0) continue;
+            wave_lds_sync();
+            const uint64_t r0 = jb + static_cast<uint64_t>(lf) * kLeafRows;
+            const uint32_t nr = static_cast<uint32_t>(je - r0 < static_cast<uint64_t>(kLeafRows) ? je - r0
+                                                                                                   : kLeafRows);
+            for (uint32_t k0 = 0; k0 < total; k0 += kWave) {
+                const bool valid = k0 + lane < total;
+                const uint64_t qq = valid ? qs[k0 + lane] : 0;
+                const uint64_t vmask = __ballot(valid);
+                double lo[DM], hi[DM];
+#pragma unroll
+                for (int d = 0; d < DM; ++d) {
+                    lo[d] = d < D ? A.boxes[qq * 2 * D + d] : 0.0;
+                    hi[d] = d < D ? A.boxes[qq * 2 * D + D + d] : 0.0;
+                }
+                const uint64_t key = qq << A.idBits;
+                const uint32_t klo = static_cast<uint32_t>(key), khi = static_cast<uint32_t>(key >> 32);
+                wTests += static_cast<uint64_t>(nr) * static_cast<uint64_t>(__popcll(vmask));
+                // wait for the boxes here, once, rather than inside the row loop
+                asm volatile("" : : "v"(lo[0]), "v"(hi[DM - 1]));
+                // Rows in pairs, ping-pong between the register sets A and B:
+                // a pair is compared while the next pair's scalar loads are in
+                // flight.  Scalar loads complete out of order, so a wait on the
+                // pair in hand would also wait on any load issued after it: the
+                // empty asm consumes a word of each row in hand (the wait lands
+                // before it) and yields the addresses the next loads use, so
+                // they are issued after that wait.  Loads run up to three rows
+                // past the leaf (the arrays have spare rows at the end); those
+                // are never tested.  A row's hits go straight into the LDS
+                // stage, which is flushed before a pair whenever it could not
+                // take 128 more.
+                uint64_t pa = (uint64_t)((ConstF64)A.coords + r0 * D);
+                uint64_t pi = (uint64_t)((ConstU64)A.ids + r0);
+                double a0[DM], a1[DM], b0[DM], b1[DM];
+                uint64_t ia0, ia1, ib0, ib1;
+                auto load2 = [&](double* x0, double* x1, uint64_t& i0, uint64_t& i1) {
+                    scalar_row<DM, EXACT>(x0, (ConstF64)pa, 0, D);
+                    scalar_row<DM, EXACT>(x1, (ConstF64)(pa + rowBytes), 0, D);
+                    i0 = ((ConstU64)pi)[0];
+                    i1 = ((ConstU64)pi)[1];
+                };
+                auto test1 = [&](const double* x, uint64_t id) {
+                    const uint64_t m = box_mask<DM>(vmask, x, lo, hi);
+                    if (m) {
+                        stage_write(m, stageAddr + staged * 8, klo, khi, id);
+                        staged += static_cast<uint32_t>(__popcll(m));
+                    }
+                };
+                auto room = [&]() {
+                    if (staged > static_cast<uint32_t>(STAGE - 2 * kWave)) {
+                        flush_hits_lean(A, stage, staged, lane);
+                        staged = 0;
+                    }
+                };
+                load2(a0, a1, ia0, ia1);
+                for (uint32_t r = 0; r < nr; r += 4) {
+                    pa += 2 * rowBytes;
+                    pi += 16;
+                    asm volatile("" : "+s"(pa), "+s"(pi) : "s"(a0[0]), "s"(a1[0]), "s"(ia0), "s"(ia1));
+                    load2(b0, b1, ib0, ib1);
+                    room();
+                    test1(a0, ia0);
+                    if (r + 1 < nr) test1(a1, ia1);
+                    if (r + 2 >= nr) break;
+                    pa += 2 * rowBytes;
+                    pi += 16;
+                    asm volatile("" : "+s"(pa), "+s"(pi) : "s"(b0[0]), "s"(b1[0]), "s"(ib0), "s"(ib1));
+                    load2(a0, a1, ia0, ia1);
+                    room();
+                    test1(b0, ib0);
+                    if (r + 3 < nr) test1(b1, ib1);
+                }
+            }
+            wave_lds_sync();  // the list is rewritten by the next leaf
+        }
+        asm volatile("" : : "v"(pf));
     }
     flush_hits(A, stage, staged, lane);
-    // work done by this wave: box tests (child rows x queries) and child rows
+    // work done by this wave: row x box tests, child rows, leaf x box tests
     if (lane == 0 && A.work) {
         atomicAdd(reinterpret_cast<unsigned long long*>(A.work), wTests);
         atomicAdd(reinterpret_cast<unsigned long long*>(A.work + 1), wRows);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.work + 2), wLeafTests);
+    }
+}
+
+// Dimensions of the kernel instantiation that handles D (4, 8, 16 or 32).
+inline int dm_of(int D) { return D <= 4 ? 4 : (D <= 8 ? 8 : (D <= 16 ? 16 : 32)); }
+
+// Outward-rounded float copy of the boxes for the exact filter's leaf tests:
+// out[q * 2DM + d] = rd(lo_d), out[q * 2DM + DM + d] = ru(hi_d), padding 0.
+__global__ void k_box_f32(const double* __restrict__ boxes, uint64_t nq, int D, int DM, float* __restrict__ out) {
+    const uint64_t n = nq * 2 * DM;
+    for (uint64_t e = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; e < n;
+         e += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t q = e / (2 * DM);
+        const int k = static_cast<int>(e - q * 2 * DM), hi = k >= DM, d = hi ? k - DM : k;
+        float v = 0.0f;
+        if (d < D) {
+            const double x = boxes[q * 2 * D + (hi ? D : 0) + d];
+            v = hi ? __double2float_ru(x) : __double2float_rd(x);
+        }
+        out[e] = v;
     }
 }
 
@@ -1319,6 +1483,83 @@ __global__ void k_enclose(const uint64_t* __restrict__ ids, uint64_t n, const ui
     }
 }
 
+// Outward-rounded float box of every kd leaf: leaf k of cluster c (by id) is
+// rows colRes[c] + k*kLeafRows .. (at most kLeafRows, within the column) of the
+// finest level; one thread per cluster.
+__global__ void k_leaf_box(const uint64_t* __restrict__ colRes, uint64_t cols, const uint64_t* __restrict__ leafOff,
+                           const double* __restrict__ coords, int D, float* __restrict__ out) {
+    for (uint64_t c = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; c < cols;
+         c += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t b = colRes[c], e = colRes[c + 1];
+        uint64_t leaf = leafOff[c];
+        for (uint64_t r0 = b; r0 < e; r0 += kLeafRows, ++leaf) {
+            const uint64_t r1 = r0 + kLeafRows < e ? r0 + kLeafRows : e;
+            for (int d = 0; d < D; ++d) {
+                double mn = __builtin_inf(), mx = -__builtin_inf();
+                for (uint64_t r = r0; r < r1; ++r) {
+                    const double v = coords[r * D + d];
+                    mn = fmin(mn, v);
+                    mx = fmax(mx, v);
+                }
+                out[leaf * 2 * D + d] = __double2float_rd(mn);
+                out[leaf * 2 * D + D + d] = __double2float_ru(mx);
+            }
+        }
+    }
+}
+
+// kd order of the rows of every column (rows colRes[c]..colRes[c+1] of the
+// point array x, D doubles each): a node of more than kLeafRows rows splits
+// along the dimension of its widest extent at a multiple of kLeafRows near its
+// middle (nth_element), recursively, so that each run of kLeafRows consecutive
+// rows is a spatially tight leaf.  perm[i] = the original row now at i.
+static void kd_leaf_order(const std::vector<double>& x, int D, const std::vector<uint64_t>& colRes,
+                          std::vector<uint64_t>& perm) {
+    const uint64_t cols = colRes.size() - 1;
+    perm.resize(colRes.back());
+    for (uint64_t i = 0; i < perm.size(); ++i) perm[i] = i;
+    auto split = [&](auto& self, uint64_t b, uint64_t e) -> void {
+        const uint64_t n = e - b;
+        if (n <= static_cast<uint64_t>(kLeafRows)) return;
+        int best = 0;
+        double wbest = -1.0;
+        for (int d = 0; d < D; ++d) {
+            double mn = x[perm[b] * D + d], mx = mn;
+            for (uint64_t i = b + 1; i < e; ++i) {
+                const double v = x[perm[i] * D + d];
+                mn = std::min(mn, v);
+                mx = std::max(mx, v);
+            }
+            if (mx - mn > wbest) {
+                wbest = mx - mn;
+                best = d;
+            }
+        }
+        const uint64_t leaves = (n + kLeafRows - 1) / kLeafRows;
+        const uint64_t m = b + (leaves / 2) * kLeafRows;
+        std::nth_element(perm.begin() + static_cast<long>(b), perm.begin() + static_cast<long>(m),
+                         perm.begin() + static_cast<long>(e), [&](uint64_t u, uint64_t v) {
+                             const double a = x[u * D + best], c = x[v * D + best];
+                             return a < c || (a == c && u < v);
+                         });
+        self(self, b, m);
+        self(self, m, e);
+    };
+    const unsigned hw = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    std::atomic<uint64_t> next{0};
+    for (unsigned t = 0; t < hw; ++t)
+        th.emplace_back([&]() {
+            for (;;) {
+                const uint64_t c0 = next.fetch_add(1024);
+                if (c0 >= cols) return;
+                const uint64_t c1 = std::min<uint64_t>(cols, c0 + 1024);
+                for (uint64_t c = c0; c < c1; ++c) split(split, colRes[c], colRes[c + 1]);
+            }
+        });
+    for (auto& t : th) t.join();
+}
+
 int launch_grouping(tblaeq_batch* b, uint64_t mesh) {
     tblaeq_ctx* ctx = b->ctx;
     uint64_t* ctl = b->ctl;
@@ -1392,6 +1633,14 @@ void launch_copy_status(tblaeq_batch* b, uint64_t* status) {
 void launch_fill(tblaeq_batch* b, uint64_t* p, uint64_t v, uint64_t n) {
     const uint64_t g = std::min<uint64_t>((n + 255) / 256, static_cast<uint64_t>(b->ctx->cuCount) * 8);
     hipLaunchKernelGGL(k_fill_u64, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, b->ctx->stream, p, v, n);
+}
+
+// Frees one of the engine's own allocations.
+void bfree(tblaeq_batch* b, void* p) {
+    auto it = std::find(b->owned.begin(), b->owned.end(), p);
+    if (it == b->owned.end()) return;
+    (void)hipFree(p);
+    b->owned.erase(it);
 }
 
 template <class T>
@@ -1493,6 +1742,49 @@ extern "C" int tblaeq_batch_create(tblaeq_ctx* ctx, const tblaeq_level* levels, 
         }
     }
     {
+        // kd order of every last-level cluster's rows and the leaves' boxes
+        const uint64_t cols = levels[L - 1].cols, nf = b->count[L];
+        std::vector<uint64_t> hcr(cols + 1), perm;
+        std::vector<double> hx(nf * D);
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
+            hipMemcpy(hcr.data(), levels[L - 1].col_res, (cols + 1) * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(hx.data(), b->coords[L], nf * D * 8, hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(hip_fail(hipGetLastError(), "batch create: kd order copy"));
+        kd_leaf_order(hx, static_cast<int>(D), hcr, perm);
+        std::vector<uint64_t> hlo(cols + 1, 0);
+        for (uint64_t c = 0; c < cols; ++c) hlo[c + 1] = hlo[c] + (hcr[c + 1] - hcr[c] + kLeafRows - 1) / kLeafRows;
+        b->nLeaves = hlo[cols];
+        uint64_t* dperm = balloc<uint64_t>(b, nf, &rc);
+        if (rc) return fail(rc);
+        // kFilterPadRows spare rows after the last one: the exact filter loads
+        // rows in pairs ahead of their tests and may read up to three rows
+        // past a leaf's end
+        double* xk = balloc<double>(b, (nf + kFilterPadRows) * D, &rc);
+        if (rc) return fail(rc);
+        uint64_t* ik = balloc<uint64_t>(b, nf + kFilterPadRows, &rc);
+        if (rc) return fail(rc);
+        TB_HIP(hipMemsetAsync(xk + nf * D, 0, kFilterPadRows * D * 8, ctx->stream));
+        TB_HIP(hipMemsetAsync(ik + nf, 0, kFilterPadRows * 8, ctx->stream));
+        b->leafOff = balloc<uint64_t>(b, cols + 1, &rc);
+        if (rc) return fail(rc);
+        b->leafBox = balloc<float>(b, b->nLeaves * 2 * D, &rc);
+        if (rc) return fail(rc);
+        if (hipMemcpy(dperm, perm.data(), nf * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(b->leafOff, hlo.data(), (cols + 1) * 8, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(hip_fail(hipGetLastError(), "batch create: kd order upload"));
+        move_rows(ctx, false, b->coords[L], dperm, nf, D, xk);
+        move_rows(ctx, false, b->ids[L], dperm, nf, 1, ik);
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(hip_fail(hipGetLastError(), "kd gather"));
+        bfree(b, b->coords[L]);
+        bfree(b, b->ids[L]);
+        bfree(b, dperm);
+        b->coords[L] = xk;
+        b->ids[L] = ik;
+        const uint64_t g = std::min<uint64_t>((cols + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 16);
+        hipLaunchKernelGGL(k_leaf_box, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream,
+                           levels[L - 1].col_res, cols, b->leafOff, b->coords[L], static_cast<int>(D), b->leafBox);
+    }
+    {
         const uint64_t n = b->count[L - 1];
         b->aabb = balloc<float>(b, n * 2 * D, &rc);
         if (rc) return fail(rc);
@@ -1573,6 +1865,10 @@ extern "C" int tblaeq_batch_reserve(tblaeq_batch* b, uint64_t max_queries, uint6
         b->recAux = balloc<uint64_t>(b, hit_cap, &rc);
         if (rc) return rc;
         b->hitCap = hit_cap;
+    }
+    if (max_queries > b->maxQ || !b->fbox) {
+        b->fbox = balloc<float>(b, std::max(max_queries, b->maxQ) * 2 * dm_of(b->D), &rc);
+        if (rc) return rc;
     }
     b->maxQ = std::max(b->maxQ, max_queries);
     if (radix_hist_words(b->hitCap, b->maxQ) > b->histCap) {
@@ -1693,6 +1989,15 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     A.outCount = ctl + 2;
     A.idBits = idBits;
     A.work = ctl + 8;
+    A.leafOff = b->leafOff;
+    A.leafBox = b->leafBox;
+    A.fboxes = b->fbox;
+    {
+        const uint64_t n = nq * 2 * dm_of(b->D);
+        const uint64_t g = std::min<uint64_t>((n + 255) / 256, static_cast<uint64_t>(ctx->cuCount) * 4);
+        hipLaunchKernelGGL(k_box_f32, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, ctx->stream, d_boxes, nq,
+                           b->D, dm_of(b->D), b->fbox);
+    }
     A.queue = b->queue;
     A.masks = nullptr;
     if ((rc = launch_range_step<kFinal>(b, A, b->itemCap))) return rc;
@@ -1710,6 +2015,16 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
         return rc;
     hipLaunchKernelGGL(k_copy_status, dim3(1), dim3(64), 0, ctx->stream, ctl, d_status);
     TB_HIP(hipGetLastError());
+    return TBLAEQ_OK;
+}
+
+extern "C" int tblaeq_batch_leaf_tests(tblaeq_batch* b, uint64_t* out) {
+    if (!b || !out) {
+        set_error("tblaeq_batch_leaf_tests: invalid argument");
+        return TBLAEQ_ERR_INVALID;
+    }
+    TB_HIP(hipStreamSynchronize(b->ctx->stream));
+    TB_HIP(hipMemcpy(out, b->ctl + 10, 8, hipMemcpyDeviceToHost));
     return TBLAEQ_OK;
 }
 

@@ -93,6 +93,8 @@ SIGNATURES = {
     "tblaeq_batch_get_param": (C.c_int, [vp, C.c_int, u64p]),
     "tblaeq_handler_batch_set_param": (C.c_int, [vp, C.c_int, C.c_uint64]),
     "tblaeq_handler_batch_work": (C.c_int, [vp, u64p]),
+    "tblaeq_handler_batch_leaf_tests": (C.c_int, [vp, u64p]),
+    "tblaeq_batch_leaf_tests": (C.c_int, [vp, u64p]),
     "tblaeq_batch_autotune": (C.c_int, [vp, vp, C.c_uint64, vp, vp, C.c_uint64, C.c_uint64, C.c_int, f64p]),
     "tblaeq_batch_save_params": (C.c_int, [vp, C.c_char_p]),
     "tblaeq_batch_load_params": (C.c_int, [vp, C.c_char_p]),
@@ -396,6 +398,12 @@ class Handler(_Closing):
         w = np.zeros(2, np.uint64)
         check(lib().tblaeq_handler_batch_work(self.h, _p(w, u64p)), "batch_work")
         return int(w[0]), int(w[1])
+
+    def batch_leaf_tests(self):
+        """Leaf x box tests of the exact filter in the last range (sub-)batch."""
+        v = C.c_uint64(0)
+        check(lib().tblaeq_handler_batch_leaf_tests(self.h, C.byref(v)), "batch_leaf_tests")
+        return v.value
 
     def batch_set_param(self, param, value):
         """Launch parameter of the batched engine (PARAM_*); results do not depend on it."""
