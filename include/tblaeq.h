@@ -242,6 +242,10 @@ int tblaeq_batch_range_graph(tblaeq_batch* b, const double* d_boxes, uint64_t nq
 int tblaeq_batch_knn(tblaeq_batch* b, const double* d_points, const uint64_t* d_K, uint64_t nq, uint64_t Kmax,
                      uint64_t* d_out_ids, double* d_out_d2, uint64_t* d_out_n, uint64_t* d_status,
                      uint64_t* d_level_surv, uint64_t* d_fine_count);
+/* Largest K the batched KNN kernels take for this index (256 - empty
+ * clusters - 1); the handler and the server route queries with a larger K to
+ * the per-query traversal. */
+int tblaeq_batch_knn_max_k(const tblaeq_batch* b, uint64_t* out);
 /* tblaeq_batch_knn as one hipGraph launch: the first call with a given set of
  * arguments captures the whole batch (levels, grouping, last-level passes,
  * selection, refine, merge) on the context stream; later calls with the same

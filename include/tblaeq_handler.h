@@ -55,6 +55,27 @@ int tblaeq_handler_set_l3_tile_bytes(uint64_t bytes);
  * candidate grids. */
 int tblaeq_handler_query(tblaeq_handler* h, int type, const double* queries, uint64_t nq, uint64_t K, int refine,
                          int saveFineMesh, tblaeq_result** out);
+/* QueryHandler::performQuery(queryPath, qType, saveFineMesh, maxQueryCount, K)
+ * (QueryHandler.cu:198-218): the query file's first max_queries queries
+ * (type 0 = range, 1 = KNN) through the per-query traversal, timed the
+ * reference's way (QueryEngine.cu:168-177: min and max dropped when more than
+ * two queries run); no refine.  A file whose dimension differs from the index
+ * gives TBLAEQ_ERR_INVALID (the reference returns a result with errorCode 1);
+ * a missing or malformed file TBLAEQ_ERR_IO. */
+int tblaeq_handler_query_file(tblaeq_handler* h, const char* path, int type, int save_fine_mesh, int max_queries,
+                              uint64_t K, tblaeq_result** out);
+/* Dataset text file (loadFromFile, File.cu:8-50): a line "N D", then N rows of
+ * D numbers.  *n and *dim receive the sizes; min(N*D, cap) values are copied to
+ * out (out may be NULL).  Missing or malformed file: TBLAEQ_ERR_IO. */
+int tblaeq_dataset_load(const char* path, uint64_t* n, uint64_t* dim, double* out, uint64_t cap);
+/* Query text file (loadQueryPointFromFile / loadQueryRangeFromFile,
+ * File.cu:139-216): a line "N D", then N rows of D numbers (type 1, points) or
+ * of 2D numbers, the lower corner then the upper one (type 0, ranges).  *n,
+ * *dim and out as for tblaeq_dataset_load (out receives N rows); for a range
+ * file, info (info_cap bytes, NUL-terminated) receives the label its name
+ * gives in the CSV's "Query Parameter" column (Utils.cu:18-35). */
+int tblaeq_query_file_load(const char* path, int type, uint64_t* n, uint64_t* dim, double* out, uint64_t cap,
+                           char* info, uint64_t info_cap);
 int tblaeq_result_destroy(tblaeq_result* r);
 int tblaeq_result_info(tblaeq_result* r, uint64_t* nq, uint64_t* levels, double* totalUs);
 int tblaeq_result_query(tblaeq_result* r, uint64_t i, uint64_t* levelSizes, double* us, uint64_t* fineSize,

@@ -37,7 +37,9 @@ typedef struct tblaeq_server tblaeq_server;
 
 typedef struct {
     uint64_t max_batch;          /* requests per batch and kind (default 4096) */
-    uint64_t max_k;              /* largest K accepted (default 256 - empty clusters - 1) */
+    uint64_t max_k;              /* largest K served by the batched engine (default and upper limit:
+                                    256 - empty clusters - 1); requests with a larger K (any K is
+                                    accepted) run through the per-query traversal on the KNN worker */
     uint64_t pair_cap;           /* engine scratch per kind (entries), 0 = sized from free HBM (<= 2^24) */
     uint64_t hit_cap;
     double hbm_fraction;         /* share of free HBM used as the in-flight budget (default 0.5) */

@@ -34,7 +34,129 @@ int invalidArg(const char* fn) {
     return TBLAEQ_ERR_INVALID;
 }
 
+// A KNN batch holding queries with K above the batched kernels' limit
+// (their LDS selection holds K + empty clusters + 1 <= 256 entries): those
+// queries run through the per-query traversal, which takes any K (the
+// reference puts no limit on K, KNN.cu:75-96), the rest through the batched
+// engine; results, statistics and *total are assembled in the batch's rows.
+int runKnnMixed(tblaeq_handler* h, BatchState* st, uint64_t kcap, bool stats, uint64_t* total) {
+    IndexData& idx = h->h->index();
+    const size_t D = idx.D, L = idx.intervals;
+    const uint64_t nq = st->nq, Kmax = st->Kmax;
+    hipStream_t s = streamOf(idx);
+    std::vector<uint64_t> small, large;
+    for (uint64_t i = 0; i < nq; ++i) (st->hK[i] <= kcap ? small : large).push_back(i);
+    std::vector<uint64_t> hIds(nq * Kmax, 0), hN(nq, 0), hSurv(stats ? nq * L : 0, 0), hFine(stats ? nq : 0, 0);
+    std::vector<double> hD2(nq * Kmax, 0.0);
+    st->work[0] = st->work[1] = 0;
+    if (!small.empty()) {
+        const uint64_t ns = small.size();
+        uint64_t ks = 0;
+        std::vector<double> q(ns * D);
+        std::vector<uint64_t> k(ns);
+        for (uint64_t j = 0; j < ns; ++j) {
+            std::copy_n(st->hQ.data() + small[j] * D, D, q.data() + j * D);
+            k[j] = st->hK[small[j]];
+            ks = std::max(ks, k[j]);
+        }
+        if (ks == 0) ks = 1;
+        DeviceBuffer<double> dQ(ns * D), dD2(ns * ks);
+        DeviceBuffer<uint64_t> dK(ns), dIds(ns * ks), dN(ns), dSurv(stats ? ns * L : 1), dFine(stats ? ns : 1);
+        hipCheck(hipMemcpy(dQ.data(), q.data(), ns * D * 8, hipMemcpyHostToDevice), "H2D");
+        hipCheck(hipMemcpy(dK.data(), k.data(), ns * 8, hipMemcpyHostToDevice), "H2D");
+        std::vector<std::pair<uint64_t, uint64_t>> todo{{0, ns}};
+        while (!todo.empty()) {
+            auto [a, n] = todo.back();
+            todo.pop_back();
+            int rc = tblaeq_batch_knn(st->b, dQ.data() + a * D, dK.data() + a, n, ks, dIds.data() + a * ks,
+                                      dD2.data() + a * ks, dN.data() + a, st->dStatus.data(),
+                                      stats ? dSurv.data() + a * L : nullptr, stats ? dFine.data() + a : nullptr);
+            if (rc) return rc;
+            uint64_t status[4];
+            hipCheck(hipMemcpyAsync(status, st->dStatus.data(), 32, hipMemcpyDeviceToHost, s), "D2H");
+            hipCheck(hipStreamSynchronize(s), "sync");
+            if (status[1] != 0) {
+                if (n == 1) {
+                    tblaeq::set_error("batched KNN: one query exceeds the engine capacities");
+                    return TBLAEQ_ERR_CAPACITY;
+                }
+                todo.push_back({a + n / 2, n - n / 2});
+                todo.push_back({a, n / 2});
+                continue;
+            }
+            st->work[0] += status[2];
+            st->work[1] += status[3];
+        }
+        std::vector<uint64_t> ids(ns * ks), n(ns), sv(stats ? ns * L : 0), fn(stats ? ns : 0);
+        std::vector<double> d2(ns * ks);
+        hipCheck(hipMemcpy(ids.data(), dIds.data(), ns * ks * 8, hipMemcpyDeviceToHost), "D2H");
+        hipCheck(hipMemcpy(d2.data(), dD2.data(), ns * ks * 8, hipMemcpyDeviceToHost), "D2H");
+        hipCheck(hipMemcpy(n.data(), dN.data(), ns * 8, hipMemcpyDeviceToHost), "D2H");
+        if (stats) {
+            hipCheck(hipMemcpy(sv.data(), dSurv.data(), ns * L * 8, hipMemcpyDeviceToHost), "D2H");
+            hipCheck(hipMemcpy(fn.data(), dFine.data(), ns * 8, hipMemcpyDeviceToHost), "D2H");
+        }
+        for (uint64_t j = 0; j < ns; ++j) {
+            const uint64_t i = small[j];
+            hN[i] = n[j];
+            std::copy_n(ids.data() + j * ks, n[j], hIds.data() + i * Kmax);
+            std::copy_n(d2.data() + j * ks, n[j], hD2.data() + i * Kmax);
+            if (stats) {
+                std::copy_n(sv.data() + j * L, L, hSurv.data() + i * L);
+                hFine[i] = fn[j];
+            }
+        }
+    }
+    for (uint64_t i : large) {
+        std::vector<uint64_t> ids, lv;
+        std::vector<double> d2;
+        uint64_t fine = 0;
+        plainKnn(h, st->hQ.data() + i * D, st->hK[i], ids, d2, stats ? &lv : nullptr, stats ? &fine : nullptr);
+        hN[i] = ids.size();
+        std::copy(ids.begin(), ids.end(), hIds.begin() + static_cast<long>(i * Kmax));
+        std::copy(d2.begin(), d2.end(), hD2.begin() + static_cast<long>(i * Kmax));
+        if (stats) {
+            std::copy(lv.begin(), lv.end(), hSurv.begin() + static_cast<long>(i * L));
+            hFine[i] = fine;
+        }
+    }
+    hipCheck(hipMemcpy(st->dIds.data(), hIds.data(), nq * Kmax * 8, hipMemcpyHostToDevice), "H2D");
+    hipCheck(hipMemcpy(st->dD2.data(), hD2.data(), nq * Kmax * 8, hipMemcpyHostToDevice), "H2D");
+    hipCheck(hipMemcpy(st->dN.data(), hN.data(), nq * 8, hipMemcpyHostToDevice), "H2D");
+    if (stats) {
+        hipCheck(hipMemcpy(st->dSurv.data(), hSurv.data(), nq * L * 8, hipMemcpyHostToDevice), "H2D");
+        hipCheck(hipMemcpy(st->dFine.data(), hFine.data(), nq * 8, hipMemcpyHostToDevice), "H2D");
+    }
+    if (total) {
+        uint64_t t = 0;
+        for (uint64_t v : hN) t += v;
+        *total = t;
+    }
+    return TBLAEQ_OK;
+}
+
 }  // namespace
+
+namespace tblaeq_host {
+
+void plainKnn(tblaeq_handler* h, const double* point, uint64_t K, std::vector<uint64_t>& ids,
+              std::vector<double>& d2, std::vector<uint64_t>* levelSurv, uint64_t* fine) {
+    std::lock_guard<std::mutex> lock(h->plainMu);
+    const int D = static_cast<int>(h->h->getDim());
+    Query q(1, D, QueryType::POINT);
+    std::copy_n(point, D, q.data.data());
+    QueryEngine::RunConfig cfg;
+    cfg.K = K;
+    cfg.refine = true;
+    QueryResult r = h->h->performQuery(q, cfg);
+    if (r.errorCode) throw std::invalid_argument("query dimension mismatch");
+    ids = r.refinedIds[0];
+    d2 = r.refinedDist2[0];
+    if (levelSurv) levelSurv->assign(r.levelSurvivors[0].begin(), r.levelSurvivors[0].end());
+    if (fine) *fine = r.fineMeshSize[0];
+}
+
+}  // namespace tblaeq_host
 
 extern "C" {
 
@@ -116,6 +238,9 @@ int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total) {
             hipCheck(hipMemsetAsync(st->dFine.data(), 0, st->nq * 8, s), "memset");
         }
         if (st->type == 1) {
+            uint64_t kcap = 0;
+            tbCheck(tblaeq_batch_knn_max_k(st->b, &kcap), "batch_knn_max_k");
+            if (st->Kmax > kcap) return runKnnMixed(h, st, kcap, stats != 0, total);
             // split in halves while a capacity is exceeded; KNN rows are fixed-size
             std::vector<std::pair<uint64_t, uint64_t>> todo{{0, st->nq}};
             while (!todo.empty()) {

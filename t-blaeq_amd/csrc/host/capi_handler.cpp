@@ -1,4 +1,5 @@
 // C facade over QueryHandler for foreign callers (ctypes, cgo, JNI).
+#include <algorithm>
 #include <cstring>
 #include <iostream>
 
@@ -170,6 +171,7 @@ int tblaeq_handler_query(tblaeq_handler* h, int type, const double* queries, uin
                          int saveFineMesh, tblaeq_result** out) {
     if (!h || !out || !queries || nq == 0 || (type != 0 && type != 1)) return TBLAEQ_ERR_INVALID;
     return guarded("tblaeq_handler_query", [&] {
+        std::lock_guard<std::mutex> lock(h->plainMu);
         const int D = static_cast<int>(h->h->getDim());
         Query q(static_cast<int>(nq), D, type == 0 ? QueryType::RANGE : QueryType::POINT);
         std::memcpy(q.data.data(), queries, q.data.size() * 8);
@@ -191,6 +193,7 @@ int tblaeq_handler_query_learned(tblaeq_handler* h, const double* points, uint64
                                  const double* radii, uint64_t* fallbacks, tblaeq_result** out) {
     if (!h || !out || !points || !radii || nq == 0 || K == 0) return TBLAEQ_ERR_INVALID;
     return guarded("tblaeq_handler_query_learned", [&] {
+        std::lock_guard<std::mutex> lock(h->plainMu);
         const int D = static_cast<int>(h->h->getDim());
         Query q(static_cast<int>(nq), D, QueryType::POINT);
         std::memcpy(q.data.data(), points, q.data.size() * 8);
@@ -210,13 +213,80 @@ int tblaeq_handler_query_learned(tblaeq_handler* h, const double* points, uint64
     });
 }
 
+int tblaeq_handler_query_file(tblaeq_handler* h, const char* path, int type, int save_fine_mesh, int max_queries,
+                              uint64_t K, tblaeq_result** out) {
+    if (!h || !path || !out || (type != 0 && type != 1)) {
+        tblaeq::set_error("tblaeq_handler_query_file: invalid argument");
+        return TBLAEQ_ERR_INVALID;
+    }
+    return guarded("tblaeq_handler_query_file", [&] {
+        std::lock_guard<std::mutex> lock(h->plainMu);
+        auto* r = new tblaeq_result{h->h->performQuery(std::string(path), type == 0 ? QueryType::RANGE : QueryType::POINT,
+                                                       save_fine_mesh != 0, max_queries, K),
+                                    {}};
+        if (r->r.errorCode) {
+            delete r;
+            throw std::invalid_argument("the query file's dimension differs from the index's");
+        }
+        *out = r;
+    });
+}
+
+int tblaeq_dataset_load(const char* path, uint64_t* n, uint64_t* dim, double* out, uint64_t cap) {
+    if (!path || !n || !dim) {
+        tblaeq::set_error("tblaeq_dataset_load: invalid argument");
+        return TBLAEQ_ERR_INVALID;
+    }
+    return guarded("tblaeq_dataset_load", [&] {
+        const PointCloud pc = loadFromFile(path);
+        *n = static_cast<uint64_t>(pc.size);
+        *dim = static_cast<uint64_t>(pc.dim);
+        if (out) std::memcpy(out, pc.data.data(), std::min<uint64_t>(cap, pc.data.size()) * 8);
+    });
+}
+
+int tblaeq_query_file_load(const char* path, int type, uint64_t* n, uint64_t* dim, double* out, uint64_t cap,
+                           char* info, uint64_t info_cap) {
+    if (!path || !n || !dim || (type != 0 && type != 1)) {
+        tblaeq::set_error("tblaeq_query_file_load: invalid argument");
+        return TBLAEQ_ERR_INVALID;
+    }
+    return guarded("tblaeq_query_file_load", [&] {
+        const Query q = type == 0 ? loadQueryRangeFromFile(path) : loadQueryPointFromFile(path);
+        *n = static_cast<uint64_t>(q.length);
+        *dim = static_cast<uint64_t>(q.dim);
+        if (out) std::memcpy(out, q.data.data(), std::min<uint64_t>(cap, q.data.size()) * 8);
+        if (info && info_cap) {
+            const size_t c = std::min<size_t>(info_cap - 1, q.queryRangeInfo.size());
+            std::memcpy(info, q.queryRangeInfo.data(), c);
+            info[c] = '\0';
+        }
+    });
+}
+
+namespace {
+// Rejects a missing result or a query index past the queries it holds.
+bool resultArgOk(const tblaeq_result* r, uint64_t i, size_t n, const char* fn, const char* what) {
+    if (!r) {
+        tblaeq::set_error(std::string(fn) + ": null result");
+        return false;
+    }
+    if (i >= n) {
+        tblaeq::set_error(std::string(fn) + ": query " + std::to_string(i) + " out of range (" + std::to_string(n) +
+                          " " + what + ")");
+        return false;
+    }
+    return true;
+}
+}  // namespace
+
 int tblaeq_result_destroy(tblaeq_result* r) {
     delete r;
     return TBLAEQ_OK;
 }
 
 int tblaeq_result_info(tblaeq_result* r, uint64_t* nq, uint64_t* levels, double* totalUs) {
-    if (!r) return TBLAEQ_ERR_INVALID;
+    if (!resultArgOk(r, 0, 1, "tblaeq_result_info", "results")) return TBLAEQ_ERR_INVALID;
     if (nq) *nq = r->r.fineMeshSize.size();
     if (levels) *levels = r->r.levelFineMeshSize.empty() ? 0 : r->r.levelFineMeshSize[0].size();
     if (totalUs) *totalUs = static_cast<double>(r->r.totalTimeUs);
@@ -225,7 +295,8 @@ int tblaeq_result_info(tblaeq_result* r, uint64_t* nq, uint64_t* levels, double*
 
 int tblaeq_result_query(tblaeq_result* r, uint64_t i, uint64_t* levelSizes, double* us, uint64_t* fineSize,
                         uint64_t* refinedCount) {
-    if (!r || i >= r->r.fineMeshSize.size()) return TBLAEQ_ERR_INVALID;
+    if (!resultArgOk(r, i, r ? r->r.fineMeshSize.size() : 0, "tblaeq_result_query", "queries"))
+        return TBLAEQ_ERR_INVALID;
     if (levelSizes)
         for (size_t l = 0; l < r->r.levelFineMeshSize[i].size(); ++l) levelSizes[l] = r->r.levelFineMeshSize[i][l];
     if (us) *us = r->r.queryTimeUsExact[i];
@@ -235,7 +306,8 @@ int tblaeq_result_query(tblaeq_result* r, uint64_t i, uint64_t* levelSizes, doub
 }
 
 int tblaeq_result_levels(tblaeq_result* r, uint64_t i, uint64_t* survivors, uint64_t* outputs) {
-    if (!r || i >= r->r.fineMeshSize.size()) return TBLAEQ_ERR_INVALID;
+    if (!resultArgOk(r, i, r ? r->r.fineMeshSize.size() : 0, "tblaeq_result_levels", "queries"))
+        return TBLAEQ_ERR_INVALID;
     for (size_t l = 0; l < r->r.levelFineMeshSize[i].size(); ++l) {
         if (survivors) survivors[l] = r->r.levelSurvivors[i][l];
         if (outputs) outputs[l] = r->r.levelFineMeshSize[i][l];
@@ -244,7 +316,9 @@ int tblaeq_result_levels(tblaeq_result* r, uint64_t i, uint64_t* survivors, uint
 }
 
 int tblaeq_result_fine_mesh(tblaeq_result* r, uint64_t i, uint64_t* ids, double* vals) {
-    if (!r || i >= r->r.fineMesh.size()) return TBLAEQ_ERR_INVALID;
+    if (!resultArgOk(r, i, r ? r->r.fineMesh.size() : 0, "tblaeq_result_fine_mesh",
+                     "fine meshes kept; the query ran with save_fine_mesh off"))
+        return TBLAEQ_ERR_INVALID;
     const SparseGrid* g = r->r.fineMesh[i];
     const size_t n = g->get_nnz_nums(), D = g->get_dimensions();
     if (ids) std::memcpy(ids, g->get_ids_(), n * 8);
@@ -253,7 +327,9 @@ int tblaeq_result_fine_mesh(tblaeq_result* r, uint64_t i, uint64_t* ids, double*
 }
 
 int tblaeq_result_refined(tblaeq_result* r, uint64_t i, uint64_t* ids, double* d2) {
-    if (!r || i >= r->r.refinedIds.size()) return TBLAEQ_ERR_INVALID;
+    if (!resultArgOk(r, i, r ? r->r.refinedIds.size() : 0, "tblaeq_result_refined",
+                     "refined results kept; the queries ran without the exact refine"))
+        return TBLAEQ_ERR_INVALID;
     const auto& v = r->r.refinedIds[i];
     if (ids) std::memcpy(ids, v.data(), v.size() * 8);
     if (d2 && i < r->r.refinedDist2.size()) std::memcpy(d2, r->r.refinedDist2[i].data(), r->r.refinedDist2[i].size() * 8);
@@ -261,7 +337,10 @@ int tblaeq_result_refined(tblaeq_result* r, uint64_t i, uint64_t* ids, double* d
 }
 
 int tblaeq_result_save_csv(tblaeq_result* r, const char* path) {
-    if (!r || !path) return TBLAEQ_ERR_INVALID;
+    if (!r || !path) {
+        tblaeq::set_error("tblaeq_result_save_csv: invalid argument");
+        return TBLAEQ_ERR_INVALID;
+    }
     return guarded("tblaeq_result_save_csv", [&] { saveQueryResult(r->r, path); });
 }
 

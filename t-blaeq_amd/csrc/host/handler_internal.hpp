@@ -1,6 +1,8 @@
 // Internal definition of the C handle behind tblaeq_handler.h.
 #pragma once
 #include <memory>
+#include <mutex>
+#include <vector>
 
 #include "../../../include/tblaeq_host.hpp"
 
@@ -36,4 +38,15 @@ struct BatchState {
 struct tblaeq_handler {
     std::unique_ptr<tblaeq_host::QueryHandler> h;
     std::unique_ptr<tblaeq_host::BatchState> batch;  // destroyed before the index it uses
+    // serialises the per-query traversal (its level buffers are shared): the
+    // C entry points and a server's large-K requests take it
+    std::mutex plainMu;
 };
+
+namespace tblaeq_host {
+// Exact top-K (ids, squared distances) of one KNN query through the per-query
+// traversal (any K), with its per-level survivors and candidate-superset size;
+// takes the handler's plainMu.
+void plainKnn(tblaeq_handler* h, const double* point, uint64_t K, std::vector<uint64_t>& ids,
+              std::vector<double>& d2, std::vector<uint64_t>* levelSurv, uint64_t* fine);
+}  // namespace tblaeq_host

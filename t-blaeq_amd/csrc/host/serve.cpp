@@ -187,7 +187,24 @@ void worker_loop(tblaeq_server* s, Worker& w) {
         }
         int rc = TBLAEQ_OK;
         try {
-            run_batch(s, w, batch, 0, batch.size());
+            if (w.type == 1) {
+                // K above the batched kernels' limit: the per-query traversal
+                std::vector<Request*> small;
+                for (Request* r : batch) {
+                    if (r->K <= s->maxK) {
+                        small.push_back(r);
+                        continue;
+                    }
+                    try {
+                        plainKnn(s->h, r->q.data(), r->K, r->ids, r->d2, nullptr, nullptr);
+                    } catch (...) {
+                        r->status = tblaeq_host::statusOfCurrentException("tblaeq_server large-K request");
+                    }
+                }
+                if (!small.empty()) run_batch(s, w, small, 0, small.size());
+            } else {
+                run_batch(s, w, batch, 0, batch.size());
+            }
         } catch (...) {
             rc = tblaeq_host::statusOfCurrentException("tblaeq_server batch");
         }
@@ -350,10 +367,6 @@ int tblaeq_server_submit(tblaeq_server* s, int type, const double* q, uint64_t K
     if (!s || !q || !ticket || (type != 0 && type != 1)) {
         tblaeq::set_error("tblaeq_server_submit: invalid argument");
         return TBLAEQ_ERR_INVALID;
-    }
-    if (type == 1 && K > s->maxK) {
-        tblaeq::set_error("tblaeq_server_submit: K above the server's maximum");
-        return TBLAEQ_ERR_UNSUPPORTED;
     }
     auto r = std::make_unique<Request>();
     r->type = type;

@@ -1607,6 +1607,17 @@ const ParamSpec* find_param(int id) {
 }
 }  // namespace
 
+extern "C" int tblaeq_batch_knn_max_k(const tblaeq_batch* b, uint64_t* out) {
+    if (!b || !out) {
+        set_error("tblaeq_batch_knn_max_k: invalid argument");
+        return TBLAEQ_ERR_INVALID;
+    }
+    uint64_t maxEmpty = 0;
+    for (int l = 0; l < b->L; ++l) maxEmpty = std::max(maxEmpty, b->emptyCols[l]);
+    *out = static_cast<uint64_t>(kKnnMaxM) > maxEmpty + 1 ? static_cast<uint64_t>(kKnnMaxM) - maxEmpty - 1 : 0;
+    return TBLAEQ_OK;
+}
+
 extern "C" int tblaeq_batch_set_param(tblaeq_batch* b, int param, uint64_t value) {
     if (!b) {
         set_error("tblaeq_batch_set_param: null engine");

@@ -68,6 +68,10 @@ SIGNATURES = {
     "tblaeq_handler_ctx": (C.c_int, [vp, C.POINTER(vp)]),
     "tblaeq_handler_set_l3_tile_bytes": (C.c_int, [C.c_uint64]),
     "tblaeq_handler_query": (C.c_int, [vp, C.c_int, f64p, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.POINTER(vp)]),
+    "tblaeq_handler_query_file": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]),
+    "tblaeq_dataset_load": (C.c_int, [C.c_char_p, u64p, u64p, f64p, C.c_uint64]),
+    "tblaeq_query_file_load": (C.c_int, [C.c_char_p, C.c_int, u64p, u64p, f64p, C.c_uint64, C.c_char_p,
+                                         C.c_uint64]),
     "tblaeq_result_destroy": (C.c_int, [vp]),
     "tblaeq_result_info": (C.c_int, [vp, u64p, u64p, f64p]),
     "tblaeq_result_query": (C.c_int, [vp, C.c_uint64, u64p, f64p, u64p, u64p]),
@@ -95,6 +99,7 @@ SIGNATURES = {
     "tblaeq_handler_batch_work": (C.c_int, [vp, u64p]),
     "tblaeq_handler_batch_leaf_tests": (C.c_int, [vp, u64p]),
     "tblaeq_batch_leaf_tests": (C.c_int, [vp, u64p]),
+    "tblaeq_batch_knn_max_k": (C.c_int, [vp, u64p]),
     "tblaeq_batch_autotune": (C.c_int, [vp, vp, C.c_uint64, vp, vp, C.c_uint64, C.c_uint64, C.c_int, f64p]),
     "tblaeq_batch_save_params": (C.c_int, [vp, C.c_char_p]),
     "tblaeq_batch_load_params": (C.c_int, [vp, C.c_char_p]),
@@ -315,6 +320,14 @@ class Handler(_Closing):
                                          C.byref(r)), "query")
         return Result(r, self.D, self.intervals)
 
+    def query_file(self, path, qtype, max_queries=2 ** 31 - 1, K=0, save_fine_mesh=False):
+        """QueryHandler::performQuery(queryPath, ...): a query file through the per-query traversal."""
+        r = vp()
+        check(lib().tblaeq_handler_query_file(self.h, str(path).encode(), 0 if qtype == "range" else 1,
+                                              int(save_fine_mesh), int(max_queries), int(K), C.byref(r)),
+              "query_file")
+        return Result(r, self.D, self.intervals)
+
     def range(self, lo, hi, refine=True, save_fine_mesh=True):
         return self.query("range", np.concatenate([np.asarray(lo, float), np.asarray(hi, float)]), 0, refine,
                           save_fine_mesh)
@@ -456,6 +469,30 @@ class Handler(_Closing):
         if getattr(self, "h", None) and _LIB is not None:
             _LIB.tblaeq_handler_destroy(self.h)
             self.h = None
+
+
+def load_dataset(path):
+    """loadFromFile (File.cu:8-50): the dataset text file as an N x D array."""
+    n, d = C.c_uint64(0), C.c_uint64(0)
+    check(lib().tblaeq_dataset_load(str(path).encode(), C.byref(n), C.byref(d), None, 0), "dataset_load")
+    out = np.zeros((n.value, d.value))
+    check(lib().tblaeq_dataset_load(str(path).encode(), C.byref(n), C.byref(d), _p(out, f64p), out.size),
+          "dataset_load")
+    return out
+
+
+def load_query_file(path, qtype):
+    """loadQueryRangeFromFile / loadQueryPointFromFile (File.cu:139-216): rows (lo|hi or points) and, for
+    range files, the CSV label of the file name."""
+    t = 0 if qtype == "range" else 1
+    n, d = C.c_uint64(0), C.c_uint64(0)
+    info = C.create_string_buffer(256)
+    check(lib().tblaeq_query_file_load(str(path).encode(), t, C.byref(n), C.byref(d), None, 0, info, 256),
+          "query_file_load")
+    out = np.zeros((n.value, d.value * (2 if t == 0 else 1)))
+    check(lib().tblaeq_query_file_load(str(path).encode(), t, C.byref(n), C.byref(d), _p(out, f64p), out.size, info,
+                                       256), "query_file_load")
+    return out, info.value.decode()
 
 
 class Mlp(_Closing):

@@ -124,8 +124,16 @@ def test_batch_knn_matches_oracle(idx, K):
 def test_batch_knn_mixed_k(idx):
     h, o = idx
     pts = W.knn_points(h.data(), 97, 31, h.mesh_sizes, CFG["vmin"], CFG["vmax"], noise=0.5)
-    Ks = np.random.default_rng(4).choice([0, 1, 5, 10, 37, 100, 200], 97)
+    Ks = np.random.default_rng(4).choice([0, 1, 5, 10, 37, 100, 200, 256, 1000], 97)
+    assert (Ks == 1000).any() and (Ks == 256).any()
     _check_knn_batch(h, o, pts, Ks)
+
+
+def test_batch_knn_large_k_only(idx):
+    """Every query above the batched kernels' K limit: the whole batch runs per query."""
+    h, o = idx
+    pts = W.knn_points(h.data(), 12, 33, h.mesh_sizes, CFG["vmin"], CFG["vmax"], noise=0.5)
+    _check_knn_batch(h, o, pts, np.full(12, 2000))
 
 
 def test_batch_knn_far_and_ties():
@@ -238,3 +246,38 @@ def test_autotune_keeps_results_and_params_roundtrip(tmp_path):
     bad.write_text("hit_stage 300\n")
     with pytest.raises(tblaeq.TblaeqError):
         h.batch_load_params(bad)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_knn_many_workgroups(mode):
+    """4096 KNN queries in one batch -- over a thousand query-major workgroups, several resident
+    per CU, in both refine modes.  Each workgroup's LDS queue counters must start from zero (a
+    counter not reset per workgroup once made a workgroup read past its queue); every answer of
+    the captured graph and of plain launches equals the oracle's."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    h, o = _make(dict(CFG, N=60000, seed=23), max_q=4096, pair_cap=1 << 24, hit_cap=1 << 24)
+    h.batch_set_param(tblaeq.PARAM_KNN_MODE, mode)
+    n = 4096
+    pts = W.knn_points(h.data(), n, 40 + mode, h.mesh_sizes, CFG["vmin"], CFG["vmax"], noise=0.2)
+    Ks = np.random.default_rng(41 + mode).choice([1, 10, 100, 200], n).astype(np.uint64)
+    h.batch_set("knn", pts, Ks)
+    h.batch_run()
+    graph = h.batch_results()
+    h.batch_use_graph(False)
+    try:
+        h.batch_run()
+        plain = h.batch_results()
+    finally:
+        h.batch_use_graph(True)
+
+    def expect(i):
+        return o.knn(pts[i], int(Ks[i])).refine_knn(pts[i], int(Ks[i]))
+
+    with ThreadPoolExecutor(16) as ex:
+        exp = list(ex.map(expect, range(n)))
+    for i in range(n):
+        eid, ed2 = exp[i]
+        for gid, gd2 in (graph[i], plain[i]):
+            assert np.array_equal(gid, eid), "query %d ids differ" % i
+            assert np.array_equal(gd2.view(np.uint64), ed2.view(np.uint64)), "query %d distances differ" % i

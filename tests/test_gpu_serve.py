@@ -32,7 +32,7 @@ def _requests(h, n, seed):
     data = h.data()
     boxes, _ = W.range_boxes(data, n, seed, 1e-4, 2e-2)
     pts = W.knn_points(data, n, seed + 1, h.mesh_sizes, CFG["vmin"], CFG["vmax"], noise=0.3)
-    Ks = np.random.default_rng(seed).choice([0, 1, 5, 10, 40, 120], n)
+    Ks = np.random.default_rng(seed).choice([0, 1, 5, 10, 40, 120, 256, 1000], n)
     reqs = [("range", b, 0) for b in boxes] + [("knn", p, int(k)) for p, k in zip(pts, Ks)]
     random.Random(seed).shuffle(reqs)
     return reqs
