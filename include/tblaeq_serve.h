@@ -70,6 +70,33 @@ int tblaeq_server_result(tblaeq_server* s, uint64_t ticket, uint64_t* ids, uint6
 int tblaeq_server_stats(tblaeq_server* s, double* out);
 int tblaeq_server_reset_stats(tblaeq_server* s);
 
+/* The reference's POST /query request and response (HTTPServer.cu:414-553)
+ * over this server, without the HTTP transport: request is the JSON body
+ * ("dataset_name", "query_type" KNN|RANGE (any case), "query_point" and "k"
+ * (default 10) or "lower_bound" and "upper_bound"), checked as the reference
+ * checks it.  The request is queued and admitted like any other.  By default
+ * it runs the per-query traversal with the fine mesh kept, as the reference
+ * does, and the response carries the reference's fields: "success",
+ * "dataset_name", "query_type", "result_count", "result" (fine-mesh point ids
+ * in traversal order), "memory_policy", "level_original_sizes",
+ * "level_pruned_sizes" (points per level) and "query_runtime" ("query_time_us",
+ * "query_time_ms", "total_time_us", "avg_time_us", "query_count").  With the
+ * extra field "exact": true it runs on the batched engine instead: "result"
+ * is the exact answer (range hits ascending; KNN top-K by (squared distance,
+ * id) with "squared_distances"), "query_runtime" is the request's latency
+ * through the server and "level_pruned_sizes" is absent.  "dataset_name" must
+ * name the served index (its directory name up to the first '_').
+ * *response receives a malloc'd NUL-terminated JSON text (release it with
+ * tblaeq_free_string) and *http_status the status the reference's front end
+ * would send: 200, 400 (a rejected request; {"success": false, "message"}),
+ * 500 (the query failed), 503 (the server is stopping) or 504 (not finished
+ * within timeout_s seconds; the request still completes and is then dropped;
+ * timeout_s < 0 waits without limit).  Returns TBLAEQ_OK whenever a response
+ * was produced.  Thread-safe. */
+int tblaeq_server_query_json(tblaeq_server* s, const char* request, double timeout_s, char** response,
+                             int* http_status);
+void tblaeq_free_string(char* p);
+
 #ifdef __cplusplus
 }
 #endif

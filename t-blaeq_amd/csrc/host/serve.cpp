@@ -11,6 +11,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cctype>
+#include <cmath>
+#include <cstdlib>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -23,6 +26,7 @@
 #include "../../../include/tblaeq_serve.h"
 #include "../ctx.hpp"
 #include "handler_internal.hpp"
+#include "json.hpp"
 
 using namespace tblaeq_host;
 using Clock = std::chrono::steady_clock;
@@ -40,6 +44,14 @@ struct Request {
     int status = TBLAEQ_OK;
     std::vector<uint64_t> ids;
     std::vector<double> d2;
+    // traversal requests run the per-query traversal on the worker and keep its
+    // fine mesh (ids) and per-level sizes instead of the exact answer
+    bool traversal = false;
+    std::vector<uint64_t> levelSizes, levelOriginal;
+    uint64_t queryUs = 0, totalUs = 0, queryCount = 0;
+    std::string policy;
+    std::string error;  // message of a failed request
+    bool orphan = false;  // its caller stopped waiting: dropped when it completes
 };
 
 template <class T>
@@ -82,6 +94,9 @@ struct tblaeq_server {
     uint64_t nextTicket = 1;
     bool stop = false;
     uint64_t budget = 0, inflight = 0, peakInflight = 0;
+    // what the JSON front end reports about the served index
+    std::string datasetName, policy;
+    std::vector<uint64_t> levelOriginal;
     // statistics
     std::vector<double> latUs;
     uint64_t batches = 0, batchedReqs = 0, deferred = 0;
@@ -147,6 +162,31 @@ void run_batch(tblaeq_server* s, Worker& w, std::vector<Request*>& batch, size_t
     }
 }
 
+// One request through the per-query traversal (performSingleRangeQuery /
+// performSingleKNNQuery with the fine mesh kept, QueryHandler.cu:220-247), as
+// the reference's front end runs it: the fine mesh's ids in traversal order,
+// per-level output sizes and the query's time.
+void run_traversal(tblaeq_server* s, Request* r) {
+    std::lock_guard<std::mutex> lock(s->h->plainMu);
+    const size_t D = s->D;
+    std::vector<double> lo, hi;
+    if (r->type == 0) {
+        lo.assign(r->q.begin(), r->q.begin() + static_cast<long>(D));
+        hi.assign(r->q.begin() + static_cast<long>(D), r->q.end());
+    }
+    const QueryResult qr = r->type == 0 ? s->h->h->performSingleRangeQuery(hi, lo, true)
+                                        : s->h->h->performSingleKNNQuery(r->q, r->K, true);
+    if (qr.errorCode || qr.fineMesh.size() != 1) throw std::runtime_error("the traversal returned no fine mesh");
+    const SparseGrid* g = qr.fineMesh[0];
+    r->ids.assign(g->get_ids_(), g->get_ids_() + g->get_nnz_nums());
+    r->levelSizes.assign(qr.levelFineMeshSize[0].begin(), qr.levelFineMeshSize[0].end());
+    r->levelOriginal.assign(qr.levelOriginalSize.begin(), qr.levelOriginalSize.end());
+    r->queryUs = static_cast<uint64_t>(qr.queryTimeUs[0]);
+    r->totalUs = static_cast<uint64_t>(qr.totalTimeUs);
+    r->queryCount = qr.queryCount;
+    r->policy = qr.memoryPolicy;
+}
+
 void worker_loop(tblaeq_server* s, Worker& w) {
     if (hipSetDevice(w.ctx->device) != hipSuccess) return;
     const double maxWait = s->cfg.max_wait_us;
@@ -186,37 +226,43 @@ void worker_loop(tblaeq_server* s, Worker& w) {
             s->peakInflight = std::max(s->peakInflight, s->inflight);
         }
         int rc = TBLAEQ_OK;
+        std::string rcMsg;
         try {
-            if (w.type == 1) {
-                // K above the batched kernels' limit: the per-query traversal
-                std::vector<Request*> small;
-                for (Request* r : batch) {
-                    if (r->K <= s->maxK) {
-                        small.push_back(r);
-                        continue;
-                    }
-                    try {
-                        plainKnn(s->h, r->q.data(), r->K, r->ids, r->d2, nullptr, nullptr);
-                    } catch (...) {
-                        r->status = tblaeq_host::statusOfCurrentException("tblaeq_server large-K request");
-                    }
+            // traversal requests, and KNN above the batched kernels' K limit,
+            // run the per-query traversal; the rest form the engine's batch
+            std::vector<Request*> batched;
+            for (Request* r : batch) {
+                if (!r->traversal && !(w.type == 1 && r->K > s->maxK)) {
+                    batched.push_back(r);
+                    continue;
                 }
-                if (!small.empty()) run_batch(s, w, small, 0, small.size());
-            } else {
-                run_batch(s, w, batch, 0, batch.size());
+                try {
+                    if (r->traversal) run_traversal(s, r);
+                    else plainKnn(s->h, r->q.data(), r->K, r->ids, r->d2, nullptr, nullptr);
+                } catch (...) {
+                    r->status = tblaeq_host::statusOfCurrentException("tblaeq_server request");
+                    r->error = tblaeq_last_error();
+                }
             }
+            if (!batched.empty()) run_batch(s, w, batched, 0, batched.size());
         } catch (...) {
             rc = tblaeq_host::statusOfCurrentException("tblaeq_server batch");
+            rcMsg = tblaeq_last_error();
         }
         const auto now = Clock::now();
         {
             std::lock_guard<std::mutex> lk(s->mu);
             for (Request* r : batch) {
-                if (rc) r->status = rc;
+                if (rc && r->status == TBLAEQ_OK) {
+                    r->status = rc;
+                    r->error = rcMsg;
+                }
                 r->finished = now;
                 r->done = true;
                 s->latUs.push_back(std::chrono::duration<double, std::micro>(now - r->submitted).count());
             }
+            for (Request* r : batch)
+                if (r->orphan) s->reqs.erase(r->ticket);
             s->inflight -= est;
             s->batches += 1;
             s->batchedReqs += batch.size();
@@ -233,6 +279,196 @@ double percentile(std::vector<double> v, double p) {
     const size_t i = static_cast<size_t>(pos);
     const double f = pos - static_cast<double>(i);
     return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
+}
+
+// Queues a request (admission estimate by kind) and wakes the workers.
+int enqueue(tblaeq_server* s, std::unique_ptr<Request> r, uint64_t* ticket) {
+    r->est = r->type == 0 ? s->cfg.range_request_bytes : s->cfg.knn_request_bytes;
+    r->submitted = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) {
+            tblaeq::set_error("tblaeq_server_submit: server is stopping");
+            return TBLAEQ_ERR_INVALID;
+        }
+        r->ticket = s->nextTicket++;
+        *ticket = r->ticket;
+        s->queue[r->type].push_back(r.get());
+        s->reqs.emplace(r->ticket, std::move(r));
+    }
+    s->cvWork.notify_all();
+    return TBLAEQ_OK;
+}
+
+// ---------------------------------------------------------------- JSON front end
+
+struct Reply {
+    int http = 200;
+    std::string body;
+};
+
+Reply failure(int http, const std::string& msg) {
+    tblaeq_json::Writer w;
+    w.beginObject();
+    w.key("success");
+    w.boolean(false);
+    w.key("message");
+    w.str(msg);
+    w.endObject();
+    return {http, w.out};
+}
+
+// Dataset names the reference's front end accepts (HTTPServer.cu:94-97): one
+// or more of [A-Za-z0-9._-].
+bool validDatasetName(const std::string& n) {
+    if (n.empty()) return false;
+    for (const char c : n)
+        if (!(std::isalnum(static_cast<unsigned char>(c)) || c == '.' || c == '_' || c == '-')) return false;
+    return true;
+}
+
+bool numericArray(const tblaeq_json::Value* v, std::vector<double>& out) {
+    if (!v || !v->isArray()) return false;
+    out.clear();
+    for (const auto& e : v->arr) {
+        if (!e.isNumber()) return false;
+        out.push_back(e.num);
+    }
+    return true;
+}
+
+// The POST /query contract of the reference's front end (HTTPServer.cu:414-553):
+// the same request fields and checks, the same response fields.  Requests run
+// through this server's queue; by default as the per-query traversal the
+// reference runs (result = fine-mesh ids in traversal order), with "exact":
+// true on the batched engine (result = the exact answer).
+Reply handle_query(tblaeq_server* s, const std::string& text, double timeout_s) {
+    using tblaeq_json::Value;
+    Value body;
+    try {
+        body = tblaeq_json::Reader(text).parse();
+    } catch (const std::exception& e) {
+        return failure(400, std::string("request is not valid JSON: ") + e.what());
+    }
+    if (!body.isObject()) return failure(400, "the request body must be a JSON object");
+    const Value* nameV = body.find("dataset_name");
+    if (!nameV || !nameV->isString()) return failure(400, "'dataset_name' is required and must be a string");
+    const std::string name = nameV->str;
+    if (!validDatasetName(name)) return failure(400, "'dataset_name' may hold only letters, digits, '.', '_' and '-'");
+    // an index's dataset name is its directory name up to the first '_'
+    // (Utils.cu:6-10); an index with no name answers to any dataset_name
+    if (!s->datasetName.empty() && name.substr(0, name.find('_')) != s->datasetName)
+        return failure(400, "dataset '" + name + "' is not served here (this server holds '" + s->datasetName + "')");
+    const Value* typeV = body.find("query_type");
+    if (!typeV || !typeV->isString()) return failure(400, "'query_type' is required and must be 'KNN' or 'RANGE'");
+    std::string qtype = typeV->str;
+    for (char& c : qtype) c = static_cast<char>(std::toupper(static_cast<unsigned char>(c)));
+    if (qtype != "KNN" && qtype != "RANGE") return failure(400, "'query_type' must be 'KNN' or 'RANGE'");
+    bool exact = false;
+    if (const Value* ex = body.find("exact")) {
+        if (!ex->isBool()) return failure(400, "'exact' must be a boolean");
+        exact = ex->b;
+    }
+    const size_t D = s->D;
+    auto r = std::make_unique<Request>();
+    r->traversal = !exact;
+    if (qtype == "KNN") {
+        r->type = 1;
+        const Value* pt = body.find("query_point");
+        if (!pt) return failure(400, "'query_point' is required for a KNN query");
+        if (!numericArray(pt, r->q)) return failure(400, "'query_point' must be an array of numbers");
+        if (r->q.size() != D) return failure(400, "'query_point' must have " + std::to_string(D) + " numbers");
+        r->K = 10;
+        if (const Value* kv = body.find("k")) {
+            if (!kv->isNumber() || !(kv->num > 0) || std::floor(kv->num) != kv->num || kv->num > 9007199254740992.0)
+                return failure(400, "'k' must be a positive integer");
+            r->K = static_cast<uint64_t>(kv->num);
+        }
+    } else {
+        r->type = 0;
+        std::vector<double> lo, hi;
+        const Value* up = body.find("upper_bound");
+        const Value* low = body.find("lower_bound");
+        if (!up || !low) return failure(400, "'upper_bound' and 'lower_bound' are required for a RANGE query");
+        if (!numericArray(up, hi) || !numericArray(low, lo))
+            return failure(400, "'upper_bound' and 'lower_bound' must be arrays of numbers");
+        if (hi.size() != D || lo.size() != D)
+            return failure(400, "'upper_bound' and 'lower_bound' must have " + std::to_string(D) + " numbers each");
+        for (size_t d = 0; d < D; ++d)
+            if (lo[d] > hi[d])
+                return failure(400, "empty range: lower_bound[" + std::to_string(d) + "] > upper_bound[" +
+                                        std::to_string(d) + "]");
+        r->q = lo;
+        r->q.insert(r->q.end(), hi.begin(), hi.end());
+    }
+    uint64_t ticket = 0;
+    if (enqueue(s, std::move(r), &ticket) != TBLAEQ_OK) return failure(503, "the server is stopping");
+    std::unique_lock<std::mutex> lk(s->mu);
+    Request* req = s->reqs.at(ticket).get();
+    if (timeout_s < 0) {
+        s->cvDone.wait(lk, [&] { return req->done; });
+    } else if (!s->cvDone.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] { return req->done; })) {
+        req->orphan = true;
+        return failure(504, "the query did not finish within the timeout");
+    }
+    std::unique_ptr<Request> done = std::move(s->reqs.at(ticket));
+    s->reqs.erase(ticket);
+    lk.unlock();
+    if (done->status != TBLAEQ_OK)
+        return failure(500, "the query failed: " + (done->error.empty() ? std::string(tblaeq_status_string(done->status))
+                                                                         : done->error));
+    const double latencyUs =
+        std::floor(std::chrono::duration<double, std::micro>(done->finished - done->submitted).count());
+    tblaeq_json::Writer w;
+    w.beginObject();
+    w.key("success");
+    w.boolean(true);
+    w.key("dataset_name");
+    w.str(name);
+    w.key("query_type");
+    w.str(qtype);
+    w.key("result_count");
+    w.integer(done->ids.size());
+    w.key("result");
+    w.integers(done->ids);
+    if (exact && done->type == 1) {
+        w.key("squared_distances");
+        w.beginArray();
+        for (const double v : done->d2) w.number(v);
+        w.endArray();
+    }
+    w.key("memory_policy");
+    w.str(exact ? s->policy : done->policy);
+    w.key("level_original_sizes");
+    w.integers(exact ? s->levelOriginal : done->levelOriginal);
+    if (!exact) {
+        w.key("level_pruned_sizes");
+        w.integers(done->levelSizes);
+    }
+    // traversal: the reference's timings of the one query; exact: the
+    // request's latency through the server (queue, batch, copies)
+    const double qUs = exact ? latencyUs : static_cast<double>(done->queryUs);
+    const double tUs = exact ? latencyUs : static_cast<double>(done->totalUs);
+    const double cnt = exact ? 1.0 : static_cast<double>(done->queryCount);
+    w.key("query_runtime");
+    w.beginObject();
+    w.key("query_time_us");
+    w.number(qUs);
+    w.key("query_time_ms");
+    w.number(qUs / 1000.0);
+    w.key("total_time_us");
+    w.number(tUs);
+    w.key("avg_time_us");
+    w.number(cnt > 0 ? tUs / cnt : 0.0);
+    w.key("query_count");
+    w.number(cnt);
+    w.endObject();
+    if (exact) {
+        w.key("exact");
+        w.boolean(true);
+    }
+    w.endObject();
+    return {200, w.out};
 }
 
 }  // namespace
@@ -270,6 +506,12 @@ int tblaeq_server_create(tblaeq_handler* h, const tblaeq_server_config* cfgIn, t
         if (!h->h->prepared()) h->h->prepareForQuery(LevelPolicy::L0);
         for (size_t l = 0; l < idx.intervals; ++l) idx.uploadLevelValues(l);
         s->D = idx.D;
+        s->datasetName = idx.datasetName;
+        for (size_t l = 0; l < idx.intervals; ++l) {
+            if (l) s->policy += "-";
+            s->policy += levelPolicyName(idx.activePolicy[l]);
+            s->levelOriginal.push_back(idx.meshSizes[idx.intervals - (l + 1)]);
+        }
         std::vector<tblaeq_level> lv(idx.intervals);
         std::vector<double> maxR(idx.intervals, 0.0);
         uint64_t maxEmpty = 0;
@@ -372,21 +614,7 @@ int tblaeq_server_submit(tblaeq_server* s, int type, const double* q, uint64_t K
     r->type = type;
     r->q.assign(q, q + (type == 0 ? 2 * s->D : s->D));
     r->K = K;
-    r->est = type == 0 ? s->cfg.range_request_bytes : s->cfg.knn_request_bytes;
-    r->submitted = Clock::now();
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (s->stop) {
-            tblaeq::set_error("tblaeq_server_submit: server is stopping");
-            return TBLAEQ_ERR_INVALID;
-        }
-        r->ticket = s->nextTicket++;
-        *ticket = r->ticket;
-        s->queue[type].push_back(r.get());
-        s->reqs.emplace(r->ticket, std::move(r));
-    }
-    s->cvWork.notify_all();
-    return TBLAEQ_OK;
+    return enqueue(s, std::move(r), ticket);
 }
 
 int tblaeq_server_wait(tblaeq_server* s, uint64_t ticket, double timeout_s) {
@@ -438,6 +666,28 @@ int tblaeq_server_stats(tblaeq_server* s, double* out) {
     out[7] = static_cast<double>(s->budget);
     return TBLAEQ_OK;
 }
+
+int tblaeq_server_query_json(tblaeq_server* s, const char* request, double timeout_s, char** response,
+                             int* http_status) {
+    if (!s || !request || !response || !http_status) {
+        tblaeq::set_error("tblaeq_server_query_json: invalid argument");
+        return TBLAEQ_ERR_INVALID;
+    }
+    *response = nullptr;
+    try {
+        const Reply rep = handle_query(s, request, timeout_s);
+        char* out = static_cast<char*>(std::malloc(rep.body.size() + 1));
+        if (!out) throw std::bad_alloc();
+        std::memcpy(out, rep.body.c_str(), rep.body.size() + 1);
+        *response = out;
+        *http_status = rep.http;
+        return TBLAEQ_OK;
+    } catch (...) {
+        return tblaeq_host::statusOfCurrentException("tblaeq_server_query_json");
+    }
+}
+
+void tblaeq_free_string(char* p) { std::free(p); }
 
 int tblaeq_server_reset_stats(tblaeq_server* s) {
     if (!s) return TBLAEQ_ERR_INVALID;

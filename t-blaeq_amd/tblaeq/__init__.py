@@ -5,6 +5,7 @@ if the shared library is missing, importing any entry point raises.
 """
 import atexit
 import ctypes as C
+import json
 import os
 import weakref
 
@@ -135,6 +136,8 @@ SIGNATURES = {
     "tblaeq_server_result": (C.c_int, [vp, C.c_uint64, u64p, C.c_uint64, f64p, u64p, C.c_int]),
     "tblaeq_server_stats": (C.c_int, [vp, f64p]),
     "tblaeq_server_reset_stats": (C.c_int, [vp]),
+    "tblaeq_server_query_json": (C.c_int, [vp, C.c_char_p, C.c_double, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]),
+    "tblaeq_free_string": (None, [C.c_void_p]),
     "tblaeq_handler_batch_load_params": (C.c_int, [vp, C.c_char_p]),
 }
 
@@ -612,6 +615,23 @@ class Server(_Closing):
 
     def reset_stats(self):
         check(lib().tblaeq_server_reset_stats(self.s), "server_reset_stats")
+
+    def query_json(self, request, timeout=-1.0):
+        """The reference's POST /query body (a JSON text or a dict) through the server: (HTTP status, response
+        dict)."""
+        if not isinstance(request, (str, bytes)):
+            request = json.dumps(request)
+        if isinstance(request, str):
+            request = request.encode()
+        out = C.c_void_p()
+        status = C.c_int(0)
+        check(lib().tblaeq_server_query_json(self.s, request, float(timeout), C.byref(out), C.byref(status)),
+              "server_query_json")
+        try:
+            text = C.string_at(out.value).decode()
+        finally:
+            lib().tblaeq_free_string(out)
+        return status.value, json.loads(text)
 
     def close(self):
         if self.s:
