@@ -295,6 +295,116 @@ def run_learned(h, data, cfg, knn_q, Ks, n_traces, epochs, plain_sample, log):
     return out
 
 
+def run_c3(n_q, n_traces, epochs, plain_sample, log):
+    """The learned-radius configuration: its own index (N = 10M FP64 points, D = 16, height 4,
+    ratios 100,50,20), n_q KNN points (K 10 | 100) at N(0, 0.01 spacing) from data points, the
+    16+32 -> 64 -> 64 -> 4 BF16-MFMA network trained on n_traces batched-engine traces.  Reports
+    the miss rate (first proposed radius below the K-th neighbour distance), the fallback rates,
+    and the speedups against the plain per-query traversal and the batched seed method; every
+    learned answer is compared with the plain one."""
+    import tblaeq
+    from tblaeq import learn
+    from tblaeq import workload as W
+
+    cfg = W.C3
+    log("C3: generating index", cfg)
+    t0 = time.perf_counter()
+    h = tblaeq.Handler.random(cfg["N"], cfg["D"], cfg["vmin"], cfg["vmax"], cfg["is_int"], cfg["ratios"],
+                              cfg["seed"], cfg["sigma_div"])
+    mlp = None
+    try:
+        h.prepare("L0")
+        gen_s = time.perf_counter() - t0
+        log("C3: index ready in %.1fs, mesh sizes %s" % (gen_s, h.mesh_sizes))
+        data = h.data()
+        pts = W.knn_points(data, n_q, cfg["seed"] + 12, h.mesh_sizes, cfg["vmin"], cfg["vmax"], noise=0.01)
+        Ks = np.where(np.arange(n_q) % 2 == 0, 10, 100).astype(np.uint64)
+        h.batch_init(max(n_q, 4096), 1 << 27, 1 << 27)
+        t0 = time.perf_counter()
+        X, T = learn.traces(h, n_traces, cfg["seed"] + 13, cfg["vmin"], cfg["vmax"])
+        t1 = time.perf_counter()
+        log("C3: %d traces in %.1fs" % (n_traces, t1 - t0))
+        mlp = tblaeq.Mlp(h, seed=7)
+        mlp.set_norm(cfg["vmin"], cfg["vmax"] - cfg["vmin"])
+        losses = learn.fit(mlp, X, T, epochs=epochs, batch=1024, lr=2e-3)
+        t2 = time.perf_counter()
+        log("C3: trained %d epochs in %.1fs" % (epochs, t2 - t1))
+        Y = mlp.forward(X[:20000])
+        factor = float(np.exp(np.quantile(T[:20000, 0] - Y[:, 0], 0.9)))
+        # held-out traces: radius and survivor-budget errors of the network
+        Xh, Th = learn.traces(h, 8192, cfg["seed"] + 14, cfg["vmin"], cfg["vmax"])
+        Yh = mlp.forward(Xh)
+        out = {"workload": "hierarchy N=%d D=%d ratios %s sigma/%.1f seed %d; %d KNN points (K 10|100, "
+                           "noise 0.01 spacing)" % (cfg["N"], cfg["D"], ",".join(map(str, cfg["ratios"])),
+                                                    cfg["sigma_div"], cfg["seed"], n_q),
+               "index_s": round(gen_s, 1), "traces": n_traces, "trace_s": round(t1 - t0, 2),
+               "train_s": round(t2 - t1, 2), "epochs": epochs,
+               "loss_first_last": [round(float(losses[0]), 5), round(float(losses[-1]), 5)],
+               "radius_factor": round(factor, 4),
+               "heldout_log_dk_rms": round(float(np.sqrt(np.mean((Th[:, 0] - Yh[:, 0]) ** 2))), 4),
+               "heldout_budget_rms": [round(float(v), 4) for v in np.sqrt(np.mean((Th[:, 1:4] - Yh[:, 1:4]) ** 2, 0))],
+               "budgets_note": "outputs 1..3 (log1p survivors of levels 0..2, /10) are trained and reported; "
+                               "no traversal reads them"}
+        # batched engine: seed method vs learned radii, the same answers
+        h.batch_set("knn", pts, Ks)
+        h.batch_run()
+        ref = h.batch_results()
+        tb, tl = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            h.batch_run()
+            tb.append(time.perf_counter() - t0)
+        fbb = 0
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fbb = h.batch_knn_learned(mlp, factor=factor)
+            tl.append(time.perf_counter() - t0)
+        res = h.batch_results()
+        same_b = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1].view(np.uint64), b[1].view(np.uint64))
+                     for a, b in zip(ref, res))
+        radii = learn.propose_radii(h, mlp, pts, Ks, cfg["vmin"], cfg["vmax"], factor=factor)
+        dk = np.array([np.sqrt(r[1][-1]) if len(r[1]) else 0.0 for r in ref])
+        miss = float(np.mean(dk > radii))
+        # per-query path: plain vs learned on a sample (half K=10, half K=100); the
+        # learned traversal tries the radius x 1, 1.25, 1.25^2, then the full traversal
+        P = plain_sample
+        tp = tq = 0.0
+        retries = 0
+        same_p = True
+        for K, sl in ((10, slice(0, 2 * P, 2)), (100, slice(1, 2 * P, 2))):
+            sp, sr = pts[sl], radii[sl]
+            h.query("knn", sp[:2], K=K, refine=2)
+            t0 = time.perf_counter()
+            plain = h.query("knn", sp, K=K, refine=True)
+            t1 = time.perf_counter()
+            lr, fb = h.query_learned(sp, K, sr)
+            t2 = time.perf_counter()
+            tp += t1 - t0
+            tq += t2 - t1
+            retries += fb
+            same_p &= all(np.array_equal(plain.refined(i)[0], lr.refined(i)[0]) for i in range(len(sp)))
+            plain.close()
+            lr.close()
+        plain_qps = 2 * P / tp
+        learned_b_qps = n_q / min(tl)
+        out["batched"] = {"queries": n_q, "seed_ms": round(min(tb) * 1e3, 3), "learned_ms": round(min(tl) * 1e3, 3),
+                          "fallback_rate": round(fbb / n_q, 4), "answers_identical": bool(same_b),
+                          "learned_qps": round(learned_b_qps, 1), "seed_qps": round(n_q / min(tb), 1)}
+        out["miss_rate"] = round(miss, 4)
+        out["per_query"] = {"queries": 2 * P, "plain_qps": round(plain_qps, 1), "learned_qps": round(2 * P / tq, 1),
+                            "speedup": round(tp / tq, 2), "retries_per_query": round(retries / (2 * P), 4),
+                            "full_traversal_rate": round(float(np.mean(dk[:2 * P] > radii[:2 * P] * 1.5625)), 4),
+                            "answers_identical": bool(same_p)}
+        out["speedup_vs_plain"] = {"batched_learned": round(learned_b_qps / plain_qps, 1),
+                                   "per_query_learned": round(tp / tq, 2)}
+        log("C3", out)
+        return out
+    finally:
+        if mlp is not None:
+            mlp.close()
+        h.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -314,6 +424,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--learn-traces", type=int, default=200000, help="0 skips the learned-radius run")
     ap.add_argument("--learn-epochs", type=int, default=20)
+    ap.add_argument("--c3-queries", type=int, default=8192,
+                    help="KNN points of the learned-radius run on its own N=10M D=16 index (0 skips it)")
+    ap.add_argument("--c3-traces", type=int, default=200000)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch parameters (profiling runs)")
     ap.add_argument("--build-n", type=int, default=1_000_000, help="points of the GPU index build run (0 skips it)")
     ap.add_argument("--build-d", type=int, default=32)
@@ -514,6 +627,10 @@ def main():
         learned = run_learned(h, data, cfg, knn_q, Ks.astype(np.uint64), args.learn_traces, args.learn_epochs,
                               min(args.plain_sample, 96), log)
 
+    c3 = None
+    if args.c3_queries and not args.no_batch:
+        c3 = run_c3(args.c3_queries, args.c3_traces, args.learn_epochs, min(args.plain_sample, 48), log)
+
     build = None
     if args.build_n:
         build = run_build(args.build_n, args.build_d, (100, 50, 20), args.build_cpu_n, log)
@@ -561,6 +678,7 @@ def main():
                   "p99_us": round(float(np.percentile(us, 99)), 1), "queries": nq},
         "batched": batched,
         "learned": learned,
+        "learned_d16": c3,
         "serving": serving,
         "build": build,
         "roofline": roofline,

@@ -12,6 +12,10 @@ import numpy as np
 # ratios 100,50,20, sigma divisor 2.2, seed 20260330 in [1, 100]^8.
 C2 = dict(N=10_000_000, D=8, vmin=1.0, vmax=100.0, is_int=False, ratios=(100, 50, 20), seed=20260330,
           sigma_div=2.2)
+# The learned-radius index: N = 10M FP64 points, D = 16, height 4, ratios
+# 100,50,20 (generator constants as for the headline index).
+C3 = dict(N=10_000_000, D=16, vmin=1.0, vmax=100.0, is_int=False, ratios=(100, 50, 20), seed=20260330,
+          sigma_div=2.2)
 C1 = dict(N=500_000, D=5, vmin=1.0, vmax=100.0, is_int=False, ratios=(100, 50, 20), seed=12345, sigma_div=3.0)
 
 
