@@ -116,64 +116,159 @@ def run_plain(h, boxes, knn_q, Ks, steps, warmup, timing):
     return nq, wall, np.asarray(us), totals
 
 
-def run_serving(h, data, cfg, rates, seconds, log):
-    """Open-loop Poisson request stream through the continuous-batching server:
-    70% range (selectivity log-uniform 1e-6..1e-3), 30% KNN (K log-uniform 1..256;
-    K above the batched kernels' limit runs per query on the KNN worker), at each
-    offered rate for `seconds`."""
-    import random
-
-    import tblaeq
+def serving_mix(data, seed, mesh_sizes, vmin, vmax, n=2048, subsample=200_000):
+    """The serving request mix over an index whose data (or a row sample of it) is `data`: n range
+    boxes with selectivity log-uniform in 1e-6..1e-3, n KNN points with K log-uniform in 1..256."""
     from tblaeq import workload as W
 
-    boxes, _ = W.range_boxes(data, 2048, cfg["seed"] + 3, 1e-6, 1e-3)
-    pts = W.knn_points(data, 2048, cfg["seed"] + 4, h.mesh_sizes, cfg["vmin"], cfg["vmax"])
-    rng = np.random.default_rng(cfg["seed"] + 5)
+    boxes, _ = W.range_boxes(data, n, seed + 3, 1e-6, 1e-3, subsample=subsample)
+    pts = W.knn_points(data, n, seed + 4, mesh_sizes, vmin, vmax)
+    rng = np.random.default_rng(seed + 5)
+    Ks = np.clip(np.round(np.exp(rng.uniform(0.0, np.log(256.0), n))), 1, 256).astype(np.int64)
+    return boxes, pts, Ks
+
+
+def run_serving(h, boxes, pts, Ks, rates, seconds, log, tag="serving"):
+    """Open-loop Poisson request stream through the continuous-batching server: 70% range, 30%
+    KNN (K above the batched kernels' limit runs per query on the KNN worker), at each offered
+    rate for `seconds`.  A consumer thread waits for the requests in submission order and
+    releases each result as it completes, so results do not pile up on the host."""
+    import queue
+    import random
+    import threading
+
+    import tblaeq
+
+    n = len(boxes)
     srv = tblaeq.Server(h, max_batch=4096)
     out = []
     try:
-        Ks = np.clip(np.round(np.exp(rng.uniform(0.0, np.log(256.0), 2048))), 1, 256).astype(np.int64)
-        # warm-up
         for i in range(64):
-            srv.wait(srv.submit("range", boxes[i]))
-            srv.wait(srv.submit("knn", pts[i], int(Ks[i])))
+            srv.wait(srv.submit("range", boxes[i % n]))
+            srv.wait(srv.submit("knn", pts[i % n], int(Ks[i % n])))
         for rate in rates:
             r = random.Random(int(rate))
             srv.reset_stats()
-            tickets = []
+            pending = queue.Queue()
+            failures = []
+
+            def consume():
+                while True:
+                    t = pending.get()
+                    if t is None:
+                        return
+                    try:
+                        srv.wait(t, 300.0)
+                        srv.release(t)
+                    except Exception as e:  # reported after the run
+                        failures.append(e)
+
+            th = threading.Thread(target=consume)
+            th.start()
+            sent = 0
             start = time.perf_counter()
             t_next = start
             i = 0
-            while True:
-                now = time.perf_counter()
-                if now - start >= seconds:
-                    break
-                if now < t_next:
-                    if t_next - now > 2e-4:
-                        time.sleep(t_next - now - 1e-4)
-                    continue
-                if r.random() < 0.7:
-                    tickets.append(srv.submit("range", boxes[i % 2048]))
-                else:
-                    tickets.append(srv.submit("knn", pts[i % 2048], int(Ks[i % 2048])))
-                i += 1
-                t_next += r.expovariate(rate)
-            offered_s = time.perf_counter() - start
-            for t in tickets:
-                srv.wait(t, 120.0)
+            try:
+                while True:
+                    now = time.perf_counter()
+                    if now - start >= seconds:
+                        break
+                    if now < t_next:
+                        if t_next - now > 2e-4:
+                            time.sleep(t_next - now - 1e-4)
+                        continue
+                    if r.random() < 0.7:
+                        pending.put(srv.submit("range", boxes[i % n]))
+                    else:
+                        pending.put(srv.submit("knn", pts[i % n], int(Ks[i % n])))
+                    sent += 1
+                    i += 1
+                    t_next += r.expovariate(rate)
+                offered_s = time.perf_counter() - start
+            finally:
+                pending.put(None)
+                th.join()
             done_s = time.perf_counter() - start
-            for t in tickets:
-                srv.release(t)
+            if failures:
+                raise failures[0]
             st = srv.stats()
-            rec = {"offered_rps": round(len(tickets) / offered_s, 1), "completed_rps": round(len(tickets) / done_s, 1),
-                   "requests": len(tickets), "p50_us": round(st["p50_us"], 1), "p99_us": round(st["p99_us"], 1),
+            rec = {"offered_rps": round(sent / offered_s, 1), "completed_rps": round(sent / done_s, 1),
+                   "requests": sent, "p50_us": round(st["p50_us"], 1), "p99_us": round(st["p99_us"], 1),
                    "mean_batch": round(st["mean_batch"], 1), "batches": int(st["batches"]),
                    "deferred_by_hbm": int(st["deferred"])}
-            log("serving", rec)
+            for kind in ("range", "knn"):
+                ks = srv.kind_stats(kind)
+                rec[kind] = {"completed": int(ks["completed"]), "batches": int(ks["batches"]),
+                             "reruns": int(ks["reruns"]), "per_query": int(ks["traversals"]),
+                             "busy_s": round(ks["busy_us"] / 1e6, 3)}
+            log(tag, rec)
             out.append(rec)
     finally:
         srv.close()
     return out
+
+
+def run_plain_mix(h, boxes, pts, Ks, n, seed):
+    """The serving mix one request at a time through the per-query traversal with the exact refine
+    (the reference's front end serves requests one by one): queries/s and service-time percentiles."""
+    rng = np.random.default_rng(seed)
+    D = h.D
+    lat = []
+    t0 = time.perf_counter()
+    for i in range(n):
+        j = int(rng.integers(len(boxes)))
+        s = time.perf_counter()
+        if rng.random() < 0.7:
+            r = h.range(boxes[j][:D], boxes[j][D:], save_fine_mesh=False)
+        else:
+            r = h.knn(pts[j], int(Ks[j]), save_fine_mesh=False)
+        r.close()
+        lat.append(time.perf_counter() - s)
+    wall = time.perf_counter() - t0
+    lat = np.array(lat) * 1e6
+    return {"requests": n, "qps": round(n / wall, 1), "p50_us": round(float(np.percentile(lat, 50)), 1),
+            "p99_us": round(float(np.percentile(lat, 99)), 1)}
+
+
+def run_c5(n, rates, seconds, plain_n, log):
+    """The serving configuration: N FP64 points (500M by default), D = 8, height 5, ratios
+    100,50,20,10, every level resident under policy L0, built by the parallel synthetic builder
+    (the generator's shape from per-chunk streams).  The serving mix comes from a 1M-row sample of
+    the data; the plain per-query traversal serves a sample of it one request at a time, then the
+    server takes the open-loop Poisson sweep."""
+    import tblaeq
+
+    cfg = dict(N=n, D=8, vmin=1.0, vmax=100.0, is_int=False, ratios=(100, 50, 20, 10), seed=20260330,
+               sigma_div=2.2)
+    log("C5: building the synthetic index on host threads", cfg)
+    t0 = time.perf_counter()
+    h = tblaeq.Handler.synthetic(cfg["N"], cfg["D"], cfg["vmin"], cfg["vmax"], cfg["is_int"], cfg["ratios"],
+                                 cfg["seed"], cfg["sigma_div"], threads=0)
+    try:
+        build_s = time.perf_counter() - t0
+        log("C5: built in %.1fs, mesh sizes %s" % (build_s, h.mesh_sizes))
+        t0 = time.perf_counter()
+        h.prepare("L0")
+        prep_s = time.perf_counter() - t0
+        log("C5: resident under L0 in %.1fs" % prep_s)
+        rng = np.random.default_rng(cfg["seed"] + 20)
+        sample = h.data_rows(np.sort(rng.integers(0, n, min(n, 1_000_000))))
+        boxes, pts, Ks = serving_mix(sample, cfg["seed"], h.mesh_sizes, cfg["vmin"], cfg["vmax"], n=1024,
+                                     subsample=sample.shape[0])
+        log("C5: request mix ready")
+        plain = run_plain_mix(h, boxes, pts, Ks, plain_n, cfg["seed"] + 23)
+        log("C5: plain per-query service", plain)
+        runs = run_serving(h, boxes, pts, Ks, rates, seconds, log, tag="C5 serving")
+        out = {"workload": "synthetic hierarchy N=%d D=8 height 5 ratios 100,50,20,10 sigma/2.2 seed %d (parallel "
+                           "builder), policy L0; open-loop Poisson stream, 70%% range (selectivity log-uniform "
+                           "1e-6..1e-3), 30%% KNN (K log-uniform 1..256)" % (n, cfg["seed"]),
+               "build_s": round(build_s, 1), "prepare_s": round(prep_s, 1), "plain": plain, "runs": runs,
+               "note": "the server runs the batched engine with the seed method (no learned radii)"}
+        log("C5", out)
+        return out
+    finally:
+        h.close()
 
 
 FP32_MFMA_PEAK_TFLOPS = 157.3
@@ -427,6 +522,12 @@ def main():
     ap.add_argument("--c3-queries", type=int, default=8192,
                     help="KNN points of the learned-radius run on its own N=10M D=16 index (0 skips it)")
     ap.add_argument("--c3-traces", type=int, default=200000)
+    ap.add_argument("--c5-n", type=int, default=0,
+                    help="points of the serving run on its own height-5 index (500000000 for the serving "
+                         "configuration; 0 skips it: it takes several minutes)")
+    ap.add_argument("--c5-rates", default="1000,5000,20000,50000,100000,200000")
+    ap.add_argument("--c5-seconds", type=float, default=2.0)
+    ap.add_argument("--c5-only", action="store_true", help="run only the serving configuration")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch parameters (profiling runs)")
     ap.add_argument("--build-n", type=int, default=1_000_000, help="points of the GPU index build run (0 skips it)")
     ap.add_argument("--build-d", type=int, default=32)
@@ -442,6 +543,12 @@ def main():
 
     import tblaeq
     from tblaeq import workload as W
+
+    c5_rates = [float(x) for x in args.c5_rates.split(",") if x]
+    if args.c5_only:
+        c5 = run_c5(args.c5_n or 500_000_000, c5_rates, args.c5_seconds, 64, log)
+        print(json.dumps({"serving_c5": c5}), flush=True)
+        return 0
 
     cfg = dict(W.C1 if args.small else W.C2)
     log("generating index", cfg)
@@ -641,7 +748,12 @@ def main():
         serving = {"workload": "open-loop Poisson stream on the same index: 70% range (selectivity log-uniform "
                                "1e-6..1e-3), 30% KNN (K log-uniform 1..256, K above the batched kernels' limit per query); continuous batching, admission by "
                                "free HBM; latency = submit to result on the host",
-                   "runs": run_serving(h, data, cfg, rates, args.serve_seconds, log)}
+                   "runs": run_serving(h, *serving_mix(data, cfg["seed"], h.mesh_sizes, cfg["vmin"], cfg["vmax"]),
+                                       rates, args.serve_seconds, log)}
+
+    c5 = None
+    if args.c5_n and not args.no_batch:
+        c5 = run_c5(args.c5_n, c5_rates, args.c5_seconds, 64, log)
 
     cpu = None
     if not args.no_cpu:
@@ -680,6 +792,7 @@ def main():
         "learned": learned,
         "learned_d16": c3,
         "serving": serving,
+        "serving_c5": c5,
         "build": build,
         "roofline": roofline,
         "cpu_baseline": cpu,

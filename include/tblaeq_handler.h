@@ -24,6 +24,18 @@ typedef struct tblaeq_result tblaeq_result;
  * (QueryHandler.cuh:95-101, QueryHandler.cu:101-124): synthetic hierarchy. */
 int tblaeq_handler_create_random(uint64_t N, uint64_t D, double vmin, double vmax, int isInt, uint64_t height,
                                  const uint64_t* ratios, uint64_t seed, double sigmaDivisor, tblaeq_handler** out);
+/* No reference counterpart: the generator's hierarchy shape (coarsest layer
+ * uniform in [vmin, vmax]^D, child i of a layer under parent i % parents,
+ * Gaussian with sigma = spacing / sigmaDivisor, clamped, rounded when isInt)
+ * built on `threads` host threads (0 = all): each chunk of 65536 values or
+ * children draws from its own mt19937_64 seeded by SplitMix64 of (seed, level,
+ * chunk), so the index does not depend on the thread count, and each P-tensor
+ * column lists its rows in ascending order.  For indexes too large for the
+ * single-stream generator above (hundreds of millions of points); its points
+ * are not the reference's sequence.  Dataset name "Syn<N>D<D>". */
+int tblaeq_handler_create_synthetic(uint64_t N, uint64_t D, double vmin, double vmax, int isInt, uint64_t height,
+                                    const uint64_t* ratios, uint64_t seed, double sigmaDivisor, uint64_t threads,
+                                    tblaeq_handler** out);
 /* QueryHandler(forceCPU, PointCloud, name, height, ratios) (QueryHandler.cu:79-99):
  * k-means build of an in-memory dataset (data: N*D doubles, AoS). */
 int tblaeq_handler_create_from_data(const double* data, uint64_t N, uint64_t D, uint64_t height,
@@ -37,6 +49,10 @@ int tblaeq_handler_destroy(tblaeq_handler* h);
 int tblaeq_handler_meta(tblaeq_handler* h, uint64_t* meta, uint64_t* meshSizes, uint64_t cap);
 /* Finest generated data layer (generator-built indexes only). */
 int tblaeq_handler_data(tblaeq_handler* h, double* out, uint64_t cap);
+/* Rows rows[0..n) of the generated finest data (n*D doubles into out); for
+ * callers that need a sample of a large index without copying all of it.  No
+ * reference counterpart. */
+int tblaeq_handler_data_rows(tblaeq_handler* h, const uint64_t* rows, uint64_t n, double* out);
 /* prepareForQuery (QueryHandler.cu:160-182): levels != NULL forces a per-level
  * policy (0..3 = L0..L3); otherwise the scheduler picks one from the device's
  * free memory, or from freeMemOverride when it is non-zero. */

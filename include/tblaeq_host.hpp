@@ -317,6 +317,16 @@ public:
     static IndexData* assemble(size_t N, size_t D, const std::vector<size_t>& ratios,
                                std::vector<std::vector<double>>& layers,
                                const std::vector<std::vector<uint64_t>>& labels);
+    // No reference counterpart: the generator's hierarchy shape built on
+    // `threads` host threads (0 = all) from per-chunk streams, for indexes too
+    // large for the single-stream generator; not the reference's sequence.
+    static IndexData* buildSynthetic(size_t N, size_t D, double valMin, double valMax, bool isInt,
+                                     const std::string& name, size_t height, const std::vector<size_t>& ratios,
+                                     uint64_t seed, double sigmaDivisor, unsigned threads);
+    // assemble() on host threads, each column's rows in ascending point order.
+    static IndexData* assembleStable(size_t N, size_t D, const std::vector<size_t>& ratios,
+                                     std::vector<std::vector<double>>& layers,
+                                     const std::vector<std::vector<uint64_t>>& labels, unsigned threads);
 };
 
 // ---- strategies (IQueryStrategy.cuh) ----------------------------------------
@@ -453,6 +463,9 @@ public:
                  const std::vector<size_t>& ratios = IndexBuilder::kDefaultRatios, uint64_t seed = 12345,
                  double sigmaDivisor = 3.0, const std::string& name = "random");
     QueryHandler(const std::string& indexPath, bool loadFromIndex);
+    // Takes over an index built elsewhere (e.g. IndexBuilder::buildSynthetic);
+    // no reference counterpart.
+    explicit QueryHandler(std::unique_ptr<IndexData> index);
 
     void saveIndex(const std::string& dir) const;
     void prepareForQuery();

@@ -40,8 +40,11 @@ typedef struct {
     uint64_t max_k;              /* largest K served by the batched engine (default and upper limit:
                                     256 - empty clusters - 1); requests with a larger K (any K is
                                     accepted) run through the per-query traversal on the KNN worker */
-    uint64_t pair_cap;           /* engine scratch per kind (entries), 0 = sized from free HBM (<= 2^24) */
-    uint64_t hit_cap;
+    uint64_t pair_cap;           /* (cluster, query) pairs per level, 0 = sized from free HBM (<= 2^26) */
+    uint64_t hit_cap;            /* range hits per batch, 0 = sized from free HBM (<= 2^27); range
+                                    batches are sized from the running mean of hits per request so
+                                    that they fit, and a batch that overflows is split (into parts
+                                    sized from its hit count) and rerun */
     double hbm_fraction;         /* share of free HBM used as the in-flight budget (default 0.5) */
     uint64_t range_request_bytes; /* admission estimate per range request (default 4 MiB) */
     uint64_t knn_request_bytes;   /* admission estimate per KNN request (default 64 KiB) */
@@ -69,6 +72,12 @@ int tblaeq_server_result(tblaeq_server* s, uint64_t ticket, uint64_t* ids, uint6
  * budget (admissions postponed), [6] peak in-flight bytes, [7] budget bytes. */
 int tblaeq_server_stats(tblaeq_server* s, double* out);
 int tblaeq_server_reset_stats(tblaeq_server* s);
+/* Per request kind (type 0 range, 1 KNN) since the last reset: out[0]
+ * completed requests, [1] batches, [2] engine runs repeated after a capacity
+ * overflow, [3] requests run through the per-query traversal, [4] worker busy
+ * time in us (batch formation excluded), [5] running mean of hits per range
+ * request (the batch-size estimate). */
+int tblaeq_server_kind_stats(tblaeq_server* s, int type, double* out);
 
 /* The reference's POST /query request and response (HTTPServer.cu:414-553)
  * over this server, without the HTTP transport: request is the JSON body

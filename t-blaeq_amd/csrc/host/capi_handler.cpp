@@ -42,6 +42,40 @@ int tblaeq_handler_create_random(uint64_t N, uint64_t D, double vmin, double vma
     });
 }
 
+int tblaeq_handler_create_synthetic(uint64_t N, uint64_t D, double vmin, double vmax, int isInt, uint64_t height,
+                                    const uint64_t* ratios, uint64_t seed, double sigmaDivisor, uint64_t threads,
+                                    tblaeq_handler** out) {
+    if (!out || !ratios || height < 2) {
+        tblaeq::set_error("tblaeq_handler_create_synthetic: invalid argument");
+        return TBLAEQ_ERR_INVALID;
+    }
+    return guarded("tblaeq_handler_create_synthetic", [&] {
+        const std::vector<size_t> r = toVec(ratios, height - 1);
+        std::unique_ptr<IndexData> idx(IndexBuilder::buildSynthetic(
+            N, D, vmin, vmax, isInt != 0, "Syn" + std::to_string(N) + "D" + std::to_string(D), height, r, seed,
+            sigmaDivisor, static_cast<unsigned>(std::min<uint64_t>(threads, 1024))));
+        auto* h = new tblaeq_handler();
+        h->h = std::make_unique<QueryHandler>(std::move(idx));
+        *out = h;
+    });
+}
+
+int tblaeq_handler_data_rows(tblaeq_handler* h, const uint64_t* rows, uint64_t n, double* out) {
+    if (!h || (n && (!rows || !out))) {
+        tblaeq::set_error("tblaeq_handler_data_rows: invalid argument");
+        return TBLAEQ_ERR_INVALID;
+    }
+    return guarded("tblaeq_handler_data_rows", [&] {
+        const IndexData& idx = h->h->index();
+        if (idx.finestData.empty()) throw UsageError("tblaeq_handler_data_rows: the index holds no generated data");
+        const size_t D = idx.D;
+        for (uint64_t i = 0; i < n; ++i) {
+            if (rows[i] >= idx.N) throw std::out_of_range("tblaeq_handler_data_rows: row " + std::to_string(rows[i]));
+            std::memcpy(out + i * D, idx.finestData.data() + rows[i] * D, D * 8);
+        }
+    });
+}
+
 int tblaeq_handler_create_from_data(const double* data, uint64_t N, uint64_t D, uint64_t height,
                                     const uint64_t* ratios, const char* name, tblaeq_handler** out) {
     if (!out || !data || !ratios) return TBLAEQ_ERR_INVALID;

@@ -9,6 +9,7 @@
 #include <numeric>
 #include <random>
 #include <sstream>
+#include <thread>
 
 #include "../../../include/tblaeq_build.h"
 #include "../../../include/tblaeq_host.hpp"
@@ -478,6 +479,193 @@ IndexData* IndexBuilder::buildRandom(size_t N, size_t D, double valMin, double v
     idx->datasetName = name;
     idx->finestData = std::move(layers[0]);
     return idx;
+}
+
+// ------------------------------------------------ parallel synthetic hierarchy
+
+namespace {
+
+// SplitMix64 (Steele, Lea and Flood, 2014): decorrelated seeds for the
+// per-chunk generators of the parallel builder.
+uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Runs f(begin, end, t) over [0, n) split into `threads` contiguous ranges.
+template <class F>
+void parallelRanges(size_t n, unsigned threads, F f) {
+    threads = static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(threads, n)));
+    std::vector<std::thread> pool;
+    const size_t per = (n + threads - 1) / threads;
+    for (unsigned t = 0; t < threads; ++t) {
+        const size_t b = std::min(n, t * per), e = std::min(n, b + per);
+        pool.emplace_back([&f, b, e, t] { f(b, e, t); });
+    }
+    for (auto& th : pool) th.join();
+}
+
+constexpr size_t kSynthChunk = size_t{1} << 16;  // children per generator stream
+
+}  // namespace
+
+// The generator's hierarchy shape (coarsest layer uniform in [valMin,
+// valMax]^D, child ci of a layer under parent ci % nParents, drawn from a
+// normal of sigma = ((valMax-valMin)/nParents^(1/D))/sigmaDivisor, clamped,
+// rounded for integer data) built on `threads` host threads: every chunk of
+// kSynthChunk values or children draws from its own mt19937_64 seeded by
+// SplitMix64 of (seed, level, chunk), so the output does not depend on the
+// thread count.  No reference counterpart: it is not the reference's
+// single-stream sequence (RandomKmeans.cuh:176-250), and each column's rows are
+// in ascending point order (a stable argsort of the labels).
+IndexData* IndexBuilder::buildSynthetic(size_t N, size_t D, double valMin, double valMax, bool isInt,
+                                        const std::string& name, size_t height, const std::vector<size_t>& ratios,
+                                        uint64_t seed, double sigmaDivisor, unsigned threads) {
+    if (!(valMin > 0.0) || !(valMin < valMax)) throw std::invalid_argument("synthetic: need 0 < valMin < valMax");
+    if (!(sigmaDivisor > 0.0)) throw std::invalid_argument("synthetic: sigmaDivisor must be > 0");
+    if (height < 2 || ratios.size() != height - 1) throw std::invalid_argument("synthetic: bad height/ratios");
+    if (N == 0 || D == 0) throw std::invalid_argument("synthetic: N and D must be positive");
+    if (threads == 0) threads = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+    const size_t L = ratios.size();
+    std::vector<size_t> sizes(L + 1);
+    sizes[0] = N;
+    for (size_t i = 0; i < L; ++i) sizes[i + 1] = std::max<size_t>(1, sizes[i] / ratios[i]);
+    std::vector<std::vector<double>> layers(L + 1);
+    std::vector<std::vector<uint64_t>> labels(L);
+    auto rngFor = [seed](uint64_t level, uint64_t chunk) {
+        return std::mt19937_64(splitmix64(seed ^ splitmix64((level << 40) + chunk + 1)));
+    };
+    auto round = [isInt](double v) { return isInt ? std::round(v) : v; };
+    {
+        auto& top = layers[L];
+        top.resize(sizes[L] * D);
+        const size_t chunks = (top.size() + kSynthChunk - 1) / kSynthChunk;
+        parallelRanges(chunks, threads, [&](size_t b, size_t e, unsigned) {
+            for (size_t c = b; c < e; ++c) {
+                std::mt19937_64 rng = rngFor(L, c);
+                std::uniform_real_distribution<double> uni(valMin, valMax);
+                const size_t hi = std::min(top.size(), (c + 1) * kSynthChunk);
+                for (size_t i = c * kSynthChunk; i < hi; ++i) top[i] = round(uni(rng));
+            }
+        });
+    }
+    for (size_t lvl = L; lvl > 0; --lvl) {
+        const size_t nP = sizes[lvl], nC = sizes[lvl - 1];
+        const double sigma =
+            (valMax - valMin) / std::pow(static_cast<double>(nP), 1.0 / static_cast<double>(D)) / sigmaDivisor;
+        const auto& par = layers[lvl];
+        auto& ch = layers[lvl - 1];
+        auto& lab = labels[lvl - 1];
+        ch.resize(nC * D);
+        lab.resize(nC);
+        const size_t chunks = (nC + kSynthChunk - 1) / kSynthChunk;
+        parallelRanges(chunks, threads, [&](size_t b, size_t e, unsigned) {
+            for (size_t c = b; c < e; ++c) {
+                std::mt19937_64 rng = rngFor(lvl - 1, c);
+                std::normal_distribution<double> noise(0.0, sigma);
+                const size_t hi = std::min(nC, (c + 1) * kSynthChunk);
+                for (size_t ci = c * kSynthChunk; ci < hi; ++ci) {
+                    const size_t p = ci % nP;
+                    lab[ci] = p;
+                    for (size_t d = 0; d < D; ++d)
+                        ch[ci * D + d] = round(std::clamp(par[p * D + d] + noise(rng), valMin, valMax));
+                }
+            }
+        });
+    }
+    IndexData* idx = assembleStable(N, D, ratios, layers, labels, threads);
+    idx->datasetName = name;
+    idx->finestData = std::move(layers[0]);
+    return idx;
+}
+
+// assemble() on `threads` host threads, with each column's rows in ascending
+// point order (a parallel stable counting sort of the labels).
+IndexData* IndexBuilder::assembleStable(size_t N, size_t D, const std::vector<size_t>& ratios,
+                                        std::vector<std::vector<double>>& layers,
+                                        const std::vector<std::vector<uint64_t>>& labels, unsigned threads) {
+    auto idx = std::make_unique<IndexData>();
+    idx->D = D;
+    idx->N = N;
+    idx->height = ratios.size() + 1;
+    idx->intervals = ratios.size();
+    idx->ratios = ratios;
+    idx->pTensors.resize(idx->intervals);
+    idx->meshMaxRadius.resize(idx->intervals);
+    idx->maps.resize(idx->height);
+    for (size_t i = 0; i < idx->intervals; ++i) {
+        const size_t fineIdx = idx->intervals - i, pIdx = fineIdx - 1;
+        const auto& fine = layers[i];
+        const auto& coarse = layers[i + 1];
+        const auto& lab = labels[i];
+        const size_t nF = lab.size(), nC = coarse.size() / D;
+        idx->meshSizes.push_back(nF);
+        auto P = std::make_unique<SparseTensorCscFormat>();
+        P->D = D;
+        P->rows = nF;
+        P->cols = nC;
+        P->nnz = nF;
+        // per-thread column counts over contiguous row ranges, then each
+        // thread's first slot per column: a stable scatter
+        const unsigned T = static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>(threads, nF)));
+        std::vector<std::vector<uint64_t>> cnt(T, std::vector<uint64_t>(nC, 0));
+        parallelRanges(nF, T, [&](size_t b, size_t e, unsigned t) {
+            for (size_t r = b; r < e; ++r) ++cnt[t][lab[r]];  // labels come from the builder: < nC
+        });
+        P->nnzPerCol.assign(nC, 0);
+        P->colRes.assign(nC + 1, 0);
+        for (size_t c = 0; c < nC; ++c) {
+            uint64_t run = P->colRes[c];
+            for (unsigned t = 0; t < T; ++t) {
+                const uint64_t k = cnt[t][c];
+                cnt[t][c] = run;
+                run += k;
+            }
+            P->nnzPerCol[c] = run - P->colRes[c];
+            P->colRes[c + 1] = run;
+        }
+        auto& map = idx->maps[fineIdx];
+        map.resize(nF);
+        P->rowIds.resize(nF);
+        parallelRanges(nF, T, [&](size_t b, size_t e, unsigned t) {
+            for (size_t r = b; r < e; ++r) {
+                P->rowIds[r] = r;
+                map[cnt[t][lab[r]]++] = r;
+            }
+        });
+        cnt.clear();
+        P->vals.resize(nF * D);
+        auto& rad = idx->meshMaxRadius[pIdx];
+        rad.assign(nC, 0.0);
+        parallelRanges(nC, threads, [&](size_t b, size_t e, unsigned) {
+            for (size_t c = b; c < e; ++c) {
+                const double* cv = &coarse[c * D];
+                double best = 0.0;
+                for (uint64_t pos = P->colRes[c]; pos < P->colRes[c + 1]; ++pos) {
+                    const double* fv = &fine[map[pos] * D];
+                    double s = 0.0;
+                    for (size_t d = 0; d < D; ++d) {
+                        P->vals[pos * D + d] = fv[d] / cv[d];
+                        const double df = cv[d] - fv[d];
+                        s = s + df * df;
+                    }
+                    best = std::max(best, std::sqrt(s));
+                }
+                rad[c] = best;
+            }
+        });
+        idx->pTensors[pIdx] = std::move(P);
+        if (i + 2 == idx->height) {
+            idx->meshSizes.push_back(nC);
+            idx->coarsestRows = nC;
+            idx->coarsestIds.resize(nC);
+            std::iota(idx->coarsestIds.begin(), idx->coarsestIds.end(), uint64_t{0});
+            idx->coarsestVals = coarse;
+        }
+    }
+    return idx.release();
 }
 
 // IndexBuilder.cu:44-151 on the GPU (the reference's cuVS path when

@@ -77,7 +77,38 @@ struct Worker {
     Pinned<double> hQ, hD2;
     Pinned<uint64_t> hK, hOff, hIds, hN;
     std::thread th;
+    double hitsPerRequest = 0.0;  // range worker: running mean of hits per request (batch sizing)
+    // statistics (under the server lock after each batch)
+    uint64_t completed = 0, batches = 0, reruns = 0, traversals = 0;
+    double busyUs = 0.0;
 };
+
+// Copies out[i] = src[off[i] .. off[i+1]) for n requests, on several threads
+// when there are many ids.
+void scatter_results(const uint64_t* src, const uint64_t* off, size_t n, std::vector<Request*>& batch, size_t a) {
+    const uint64_t total = off[n] - off[0];
+    const unsigned threads = total < (4ull << 20) ? 1u : 8u;
+    auto work = [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i) batch[a + i]->ids.assign(src + off[i], src + off[i + 1]);
+    };
+    if (threads == 1) {
+        work(0, n);
+        return;
+    }
+    // request ranges of about equal id counts
+    std::vector<size_t> cut(threads + 1, n);
+    cut[0] = 0;
+    size_t i = 0;
+    for (unsigned t = 1; t < threads; ++t) {
+        const uint64_t target = off[0] + total * t / threads;
+        while (i < n && off[i] < target) ++i;
+        cut[t] = i;
+    }
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < threads; ++t)
+        if (cut[t] < cut[t + 1]) pool.emplace_back(work, cut[t], cut[t + 1]);
+    for (auto& th : pool) th.join();
+}
 
 }  // namespace
 
@@ -138,8 +169,18 @@ void run_batch(tblaeq_server* s, Worker& w, std::vector<Request*>& batch, size_t
             batch[a]->status = TBLAEQ_ERR_CAPACITY;
             return;
         }
-        run_batch(s, w, batch, a, n / 2);
-        run_batch(s, w, batch, a + n / 2, n - n / 2);
+        // only the hit list overflowed: status[0] still counts every hit, so
+        // split once into parts that each fit with a margin; otherwise halve
+        ++w.reruns;
+        uint64_t parts = 2;
+        if (w.type == 0 && status[1] == 4) {
+            const double need = static_cast<double>(status[0]) * 1.25 / static_cast<double>(s->cfg.hit_cap);
+            parts = std::min<uint64_t>(n, std::max<uint64_t>(2, static_cast<uint64_t>(std::ceil(need))));
+        }
+        for (uint64_t k = 0; k < parts; ++k) {
+            const size_t b0 = a + n * k / parts, b1 = a + n * (k + 1) / parts;
+            if (b1 > b0) run_batch(s, w, batch, b0, b1 - b0);
+        }
         return;
     }
     if (w.type == 0) {
@@ -147,8 +188,10 @@ void run_batch(tblaeq_server* s, Worker& w, std::vector<Request*>& batch, size_t
         hipCheck(hipMemcpyAsync(w.hOff.p, w.dOff.data(), (n + 1) * 8, hipMemcpyDeviceToHost, st), "D2H");
         hipCheck(hipMemcpyAsync(w.hIds.p, w.dIds.data(), hits * 8, hipMemcpyDeviceToHost, st), "D2H");
         hipCheck(hipStreamSynchronize(st), "sync");
-        for (size_t i = 0; i < n; ++i)
-            batch[a + i]->ids.assign(w.hIds.p + w.hOff.p[i], w.hIds.p + w.hOff.p[i + 1]);
+        scatter_results(w.hIds.p, w.hOff.p, n, batch, a);
+        // running mean of hits per request sizes the next batches
+        const double per = static_cast<double>(hits) / static_cast<double>(n);
+        w.hitsPerRequest = w.hitsPerRequest > 0.0 ? 0.8 * w.hitsPerRequest + 0.2 * per : per;
     } else {
         hipCheck(hipMemcpyAsync(w.hN.p, w.dN.data(), n * 8, hipMemcpyDeviceToHost, st), "D2H");
         hipCheck(hipMemcpyAsync(w.hIds.p, w.dIds.data(), n * Kmax * 8, hipMemcpyDeviceToHost, st), "D2H");
@@ -205,7 +248,13 @@ void worker_loop(tblaeq_server* s, Worker& w) {
                                    [&] { return s->stop || s->queue[w.type].size() >= s->cfg.max_batch; });
             }
             auto& q = s->queue[w.type];
-            while (!q.empty() && batch.size() < s->cfg.max_batch) {
+            // range batches are sized so that their expected hits fit the
+            // hit list (a batch that overflows is split and rerun)
+            size_t limit = s->cfg.max_batch;
+            if (w.type == 0 && w.hitsPerRequest > 0.0)
+                limit = static_cast<size_t>(std::clamp(0.7 * static_cast<double>(s->cfg.hit_cap) / w.hitsPerRequest,
+                                                       1.0, static_cast<double>(s->cfg.max_batch)));
+            while (!q.empty() && batch.size() < limit) {
                 Request* r = q.front();
                 if (!batch.empty() || s->inflight > 0) {
                     if (s->inflight + est + r->est > s->budget) {
@@ -227,6 +276,8 @@ void worker_loop(tblaeq_server* s, Worker& w) {
         }
         int rc = TBLAEQ_OK;
         std::string rcMsg;
+        const auto busy0 = Clock::now();
+        uint64_t travs = 0;
         try {
             // traversal requests, and KNN above the batched kernels' K limit,
             // run the per-query traversal; the rest form the engine's batch
@@ -236,6 +287,7 @@ void worker_loop(tblaeq_server* s, Worker& w) {
                     batched.push_back(r);
                     continue;
                 }
+                ++travs;
                 try {
                     if (r->traversal) run_traversal(s, r);
                     else plainKnn(s->h, r->q.data(), r->K, r->ids, r->d2, nullptr, nullptr);
@@ -266,6 +318,10 @@ void worker_loop(tblaeq_server* s, Worker& w) {
             s->inflight -= est;
             s->batches += 1;
             s->batchedReqs += batch.size();
+            w.completed += batch.size();
+            w.batches += 1;
+            w.traversals += travs;
+            w.busyUs += std::chrono::duration<double, std::micro>(now - busy0).count();
         }
         s->cvDone.notify_all();
         s->cvWork.notify_all();
@@ -525,9 +581,11 @@ int tblaeq_server_create(tblaeq_handler* h, const tblaeq_server_config* cfgIn, t
         size_t freeB = 0, totalB = 0;
         hipCheck(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
         // engine scratch from free HBM when not given: 1/256 of it per buffer, at most 16M entries
-        const uint64_t autoCap = std::min<uint64_t>(std::max<uint64_t>(freeB / 256 / 8, 1ull << 20), 1ull << 24);
-        if (!cfg.pair_cap) cfg.pair_cap = autoCap;
-        if (!cfg.hit_cap) cfg.hit_cap = autoCap;
+        // pairs: 1/256 of free HBM (16 bytes an entry), at most 64M entries;
+        // hits: 1/128 of it, at most 128M (the range worker holds four
+        // hit-sized buffers)
+        if (!cfg.pair_cap) cfg.pair_cap = std::min<uint64_t>(std::max<uint64_t>(freeB / 256 / 16, 1ull << 20), 1ull << 26);
+        if (!cfg.hit_cap) cfg.hit_cap = std::min<uint64_t>(std::max<uint64_t>(freeB / 128 / 8, 1ull << 20), 1ull << 27);
         const int device = idx.ctx->device;
         for (int k = 0; k < 2; ++k) {
             Worker& w = s->w[k];
@@ -536,7 +594,10 @@ int tblaeq_server_create(tblaeq_handler* h, const tblaeq_server_config* cfgIn, t
             tbCheck(tblaeq_batch_create(w.ctx, lv.data(), lv.size(), idx.dCoarsestIds.data(), idx.dCoarsestVals.data(),
                                         idx.coarsestIds.size(), maxR.data(), &w.b),
                     "serve: engine");
-            tbCheck(tblaeq_batch_reserve(w.b, cfg.max_batch, cfg.pair_cap, cfg.hit_cap), "serve: reserve");
+            // the KNN engine's candidate records use min(pair_cap, hit_cap) entries
+            tbCheck(tblaeq_batch_reserve(w.b, cfg.max_batch, cfg.pair_cap,
+                                         k == 0 ? cfg.hit_cap : std::min(cfg.pair_cap, cfg.hit_cap)),
+                    "serve: reserve");
             const size_t qw = k == 0 ? 2 * s->D : s->D;
             w.dQ.resize(cfg.max_batch * qw);
             w.hQ.alloc(cfg.max_batch * qw);
@@ -689,11 +750,31 @@ int tblaeq_server_query_json(tblaeq_server* s, const char* request, double timeo
 
 void tblaeq_free_string(char* p) { std::free(p); }
 
+int tblaeq_server_kind_stats(tblaeq_server* s, int type, double* out) {
+    if (!s || !out || (type != 0 && type != 1)) {
+        tblaeq::set_error("tblaeq_server_kind_stats: invalid argument");
+        return TBLAEQ_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    const Worker& w = s->w[type];
+    out[0] = static_cast<double>(w.completed);
+    out[1] = static_cast<double>(w.batches);
+    out[2] = static_cast<double>(w.reruns);
+    out[3] = static_cast<double>(w.traversals);
+    out[4] = w.busyUs;
+    out[5] = w.hitsPerRequest;
+    return TBLAEQ_OK;
+}
+
 int tblaeq_server_reset_stats(tblaeq_server* s) {
     if (!s) return TBLAEQ_ERR_INVALID;
     std::lock_guard<std::mutex> lk(s->mu);
     s->latUs.clear();
     s->batches = s->batchedReqs = s->deferred = 0;
+    for (auto& w : s->w) {
+        w.completed = w.batches = w.reruns = w.traversals = 0;
+        w.busyUs = 0.0;
+    }
     s->peakInflight = s->inflight;
     return TBLAEQ_OK;
 }

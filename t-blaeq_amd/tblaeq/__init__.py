@@ -48,6 +48,10 @@ SIGNATURES = {
     # tblaeq_handler.h
     "tblaeq_handler_create_random": (C.c_int, [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_int, C.c_uint64,
                                                u64p, C.c_uint64, C.c_double, C.POINTER(vp)]),
+    "tblaeq_handler_create_synthetic": (C.c_int, [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_int,
+                                                  C.c_uint64, u64p, C.c_uint64, C.c_double, C.c_uint64,
+                                                  C.POINTER(vp)]),
+    "tblaeq_handler_data_rows": (C.c_int, [vp, u64p, C.c_uint64, f64p]),
     "tblaeq_handler_create_from_data": (C.c_int, [f64p, C.c_uint64, C.c_uint64, C.c_uint64, u64p, C.c_char_p,
                                                   C.POINTER(vp)]),
     "tblaeq_handler_load": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
@@ -136,6 +140,7 @@ SIGNATURES = {
     "tblaeq_server_result": (C.c_int, [vp, C.c_uint64, u64p, C.c_uint64, f64p, u64p, C.c_int]),
     "tblaeq_server_stats": (C.c_int, [vp, f64p]),
     "tblaeq_server_reset_stats": (C.c_int, [vp]),
+    "tblaeq_server_kind_stats": (C.c_int, [vp, C.c_int, f64p]),
     "tblaeq_server_query_json": (C.c_int, [vp, C.c_char_p, C.c_double, C.POINTER(C.c_void_p), C.POINTER(C.c_int)]),
     "tblaeq_free_string": (None, [C.c_void_p]),
     "tblaeq_handler_batch_load_params": (C.c_int, [vp, C.c_char_p]),
@@ -239,6 +244,17 @@ class Handler(_Closing):
         return cls(h)
 
     @classmethod
+    def synthetic(cls, N, D, vmin=1.0, vmax=100.0, is_int=False, ratios=(100, 50, 20), seed=12345, sigma_div=3.0,
+                  threads=0):
+        """The generator's hierarchy shape on host threads, from per-chunk streams (not the reference's
+        sequence; independent of the thread count); for very large indexes."""
+        r = np.asarray(ratios, np.uint64)
+        h = vp()
+        check(lib().tblaeq_handler_create_synthetic(N, D, vmin, vmax, int(is_int), len(r) + 1, _p(r, u64p), seed,
+                                                    sigma_div, int(threads), C.byref(h)), "create_synthetic")
+        return cls(h)
+
+    @classmethod
     def from_data(cls, data, ratios=(100, 50, 20), name="in-memory"):
         data = np.ascontiguousarray(data, np.float64)
         r = np.asarray(ratios, np.uint64)
@@ -290,6 +306,13 @@ class Handler(_Closing):
     def data(self):
         out = np.zeros((self.N, self.D))
         check(lib().tblaeq_handler_data(self.h, _p(out, f64p), out.size), "data")
+        return out
+
+    def data_rows(self, rows):
+        """Rows of the generated finest data (a sample of a large index without copying all of it)."""
+        r = np.ascontiguousarray(rows, np.uint64)
+        out = np.zeros((r.size, self.D))
+        check(lib().tblaeq_handler_data_rows(self.h, _p(r, u64p), r.size, _p(out, f64p)), "data_rows")
         return out
 
     def prepare(self, policy=None, free_mem=0):
@@ -615,6 +638,13 @@ class Server(_Closing):
 
     def reset_stats(self):
         check(lib().tblaeq_server_reset_stats(self.s), "server_reset_stats")
+
+    def kind_stats(self, qtype):
+        """Per request kind: completed, batches, reruns after overflow, per-query traversals, busy us, hits per
+        range request."""
+        o = np.zeros(6)
+        check(lib().tblaeq_server_kind_stats(self.s, 0 if qtype == "range" else 1, _p(o, f64p)), "server_kind_stats")
+        return dict(zip(("completed", "batches", "reruns", "traversals", "busy_us", "hits_per_request"), o.tolist()))
 
     def query_json(self, request, timeout=-1.0):
         """The reference's POST /query body (a JSON text or a dict) through the server: (HTTP status, response
