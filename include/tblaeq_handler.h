@@ -111,6 +111,18 @@ int tblaeq_result_save_csv(tblaeq_result* r, const char* path);
  * batches that exceed a capacity are split and rerun); *total = number of hits
  * or neighbours.  Results and per-query statistics stay on the device until
  * batch_results / batch_stats copy them out. */
+/* Batch slots (no reference counterpart): the handler holds up to two batched
+ * engines, each on its own HIP stream; batch_slot selects the one every
+ * tblaeq_handler_batch_* call acts on (slot 0 runs on the index's stream and is
+ * the default; batch_init on slot 1 creates its own context).  run_slots runs
+ * the uploaded batches of the slots in mask (bit i = slot i) concurrently --
+ * e.g. a range batch in slot 0 beside a KNN batch in slot 1 -- each replayed as
+ * its captured graph, and waits for all; totals[i] = hits or neighbours of slot
+ * i.  A slot whose batch exceeds its capacities (or holds K above the batched
+ * KNN limit, or has graphs turned off) is then run as batch_run runs it.
+ * Results as for a batch_run of each slot. */
+int tblaeq_handler_batch_slot(tblaeq_handler* h, int slot);
+int tblaeq_handler_batch_run_slots(tblaeq_handler* h, uint64_t mask, uint64_t* totals);
 int tblaeq_handler_batch_init(tblaeq_handler* h, uint64_t maxQueries, uint64_t pairCap, uint64_t hitCap);
 int tblaeq_handler_batch_set_queries(tblaeq_handler* h, int type, const double* queries, const uint64_t* Ks,
                                      uint64_t nq);

@@ -13,7 +13,6 @@ using namespace tblaeq_host;
 
 namespace {
 
-hipStream_t streamOf(IndexData& idx) { return static_cast<hipStream_t>(tblaeq_ctx_stream(idx.ctx)); }
 
 template <class F>
 int guarded(const char* what, F&& f) {
@@ -24,7 +23,7 @@ int guarded(const char* what, F&& f) {
     }
 }
 
-BatchState* stateOf(tblaeq_handler* h) { return h ? h->batch.get() : nullptr; }
+BatchState* stateOf(tblaeq_handler* h) { return h ? h->slots[h->slot].get() : nullptr; }
 
 // TBLAEQ_ERR_INVALID with a message naming the entry point (a null handle, a
 // batched engine not initialised, no queries uploaded, or a bad argument).
@@ -43,7 +42,7 @@ int runKnnMixed(tblaeq_handler* h, BatchState* st, uint64_t kcap, bool stats, ui
     IndexData& idx = h->h->index();
     const size_t D = idx.D, L = idx.intervals;
     const uint64_t nq = st->nq, Kmax = st->Kmax;
-    hipStream_t s = streamOf(idx);
+    hipStream_t s = st->stream();
     std::vector<uint64_t> small, large;
     for (uint64_t i = 0; i < nq; ++i) (st->hK[i] <= kcap ? small : large).push_back(i);
     std::vector<uint64_t> hIds(nq * Kmax, 0), hN(nq, 0), hSurv(stats ? nq * L : 0, 0), hFine(stats ? nq : 0, 0);
@@ -166,15 +165,22 @@ int tblaeq_handler_batch_init(tblaeq_handler* h, uint64_t maxQueries, uint64_t p
         IndexData& idx = h->h->index();
         if (!h->h->prepared()) h->h->prepareForQuery(LevelPolicy::L0);
         for (size_t l = 0; l < idx.intervals; ++l) idx.uploadLevelValues(l);
-        h->batch.reset();
+        h->slots[h->slot].reset();
         auto st = std::make_unique<BatchState>();
+        if (h->slot == 0) {
+            st->ctx = idx.ctx;
+        } else {
+            tbCheck(tblaeq_ctx_create(idx.ctx->device, nullptr, &st->ctx), "batch slot context");
+            st->ownsCtx = true;
+        }
+        hipCheck(hipHostMalloc(reinterpret_cast<void**>(&st->hStatus), 8 * sizeof(uint64_t), 0), "hipHostMalloc");
         std::vector<tblaeq_level> lv(idx.intervals);
         std::vector<double> maxR(idx.intervals, 0.0);
         for (size_t l = 0; l < idx.intervals; ++l) {
             lv[l] = idx.levelDesc(l);
             for (double r : idx.meshMaxRadius[l]) maxR[l] = std::max(maxR[l], r);
         }
-        int rc = tblaeq_batch_create(idx.ctx, lv.data(), lv.size(), idx.dCoarsestIds.data(), idx.dCoarsestVals.data(),
+        int rc = tblaeq_batch_create(st->ctx, lv.data(), lv.size(), idx.dCoarsestIds.data(), idx.dCoarsestVals.data(),
                                      idx.coarsestIds.size(), maxR.data(), &st->b);
         if (rc) return rc;
         rc = tblaeq_batch_reserve(st->b, maxQueries, pairCap, hitCap);
@@ -191,7 +197,7 @@ int tblaeq_handler_batch_init(tblaeq_handler* h, uint64_t maxQueries, uint64_t p
         st->dFine.resize(maxQueries);
         st->dN.resize(maxQueries);
         st->dIds.resize(std::max<uint64_t>(hitCap, 1));
-        h->batch = std::move(st);
+        h->slots[h->slot] = std::move(st);
         return TBLAEQ_OK;
     });
 }
@@ -231,7 +237,7 @@ int tblaeq_handler_batch_run(tblaeq_handler* h, int stats, uint64_t* total) {
     return guarded("tblaeq_handler_batch_run", [&]() -> int {
         IndexData& idx = h->h->index();
         const size_t D = idx.D, L = idx.intervals;
-        hipStream_t s = streamOf(idx);
+        hipStream_t s = st->stream();
         st->work[0] = st->work[1] = 0;
         if (stats) {
             hipCheck(hipMemsetAsync(st->dSurv.data(), 0, st->nq * L * 8, s), "memset");
@@ -471,7 +477,7 @@ int tblaeq_handler_knn_features(tblaeq_handler* h, const double* points, const u
         DeviceBuffer<uint64_t> dK(Ks, n);
         DeviceBuffer<float> dX(n * 48);
         tbCheck(tblaeq_knn_features(st->b, dP.data(), dK.data(), n, vmin, scale, dX.data()), "features");
-        hipStream_t s = streamOf(h->h->index());
+        hipStream_t s = st->stream();
         hipCheck(hipMemcpyAsync(X, dX.data(), n * 48 * 4, hipMemcpyDeviceToHost, s), "D2H");
         hipCheck(hipStreamSynchronize(s), "sync");
         return TBLAEQ_OK;
@@ -490,7 +496,7 @@ int tblaeq_handler_batch_knn_learned(tblaeq_handler* h, tblaeq_mlp* m, double fa
     return guarded("tblaeq_handler_batch_knn_learned", [&]() -> int {
         IndexData& idx = h->h->index();
         const size_t D = idx.D, L = idx.intervals;
-        hipStream_t s = streamOf(idx);
+        hipStream_t s = st->stream();
         const uint64_t nq = st->nq, Kmax = st->Kmax;
         double vmin = 0, scale = 1;
         tblaeq_mlp_get_norm(m, &vmin, &scale);
@@ -597,8 +603,87 @@ int tblaeq_handler_batch_knn_learned(tblaeq_handler* h, tblaeq_mlp* m, double fa
     });
 }
 
+int tblaeq_handler_batch_slot(tblaeq_handler* h, int slot) {
+    if (!h || slot < 0 || slot >= kBatchSlots) {
+        tblaeq::set_error("tblaeq_handler_batch_slot: slot must be 0 or 1");
+        return TBLAEQ_ERR_INVALID;
+    }
+    h->slot = slot;
+    return TBLAEQ_OK;
+}
+
+int tblaeq_handler_batch_run_slots(tblaeq_handler* h, uint64_t mask, uint64_t* totals) {
+    if (!h || mask == 0 || mask >= (1ull << kBatchSlots)) {
+        tblaeq::set_error("tblaeq_handler_batch_run_slots: mask must select slots 0 and/or 1");
+        return TBLAEQ_ERR_INVALID;
+    }
+    for (int i = 0; i < kBatchSlots; ++i)
+        if (((mask >> i) & 1) && (!h->slots[i] || h->slots[i]->nq == 0)) {
+            tblaeq::set_error("tblaeq_handler_batch_run_slots: slot " + std::to_string(i) +
+                              " is not initialised or holds no queries");
+            return TBLAEQ_ERR_INVALID;
+        }
+    const int saved = h->slot;
+    const int rc = guarded("tblaeq_handler_batch_run_slots", [&]() -> int {
+        // launch every selected slot's captured batch on its own stream first
+        bool launched[kBatchSlots] = {};
+        for (int i = 0; i < kBatchSlots; ++i) {
+            if (!((mask >> i) & 1)) continue;
+            BatchState* st = h->slots[i].get();
+            if (!st->useGraph) continue;
+            int r = TBLAEQ_OK;
+            if (st->type == 0) {
+                r = tblaeq_batch_range_graph(st->b, st->dQ.data(), st->nq, st->dOff.data(), st->dIds.data(),
+                                             st->dIds.size(), st->dStatus.data());
+            } else {
+                uint64_t kcap = 0;
+                tbCheck(tblaeq_batch_knn_max_k(st->b, &kcap), "batch_knn_max_k");
+                if (st->Kmax > kcap) continue;  // mixed K: the sequential path below
+                r = tblaeq_batch_knn_graph(st->b, st->dQ.data(), st->dK.data(), st->nq, st->Kmax, st->dIds.data(),
+                                           st->dD2.data(), st->dN.data(), st->dStatus.data());
+            }
+            if (r) return r;
+            hipCheck(hipMemcpyAsync(st->hStatus, st->dStatus.data(), 32, hipMemcpyDeviceToHost, st->stream()), "D2H");
+            launched[i] = true;
+        }
+        // then wait for each; a slot whose capacities overflowed, or that was
+        // not launched, runs through batch_run (which splits)
+        for (int i = 0; i < kBatchSlots; ++i) {
+            if (!((mask >> i) & 1)) continue;
+            BatchState* st = h->slots[i].get();
+            uint64_t t = 0;
+            if (launched[i]) {
+                hipCheck(hipStreamSynchronize(st->stream()), "sync");
+                const uint64_t* status = st->hStatus;
+                if (status[1] == 0) {
+                    st->work[0] = status[2];
+                    st->work[1] = status[3];
+                    if (st->type == 0) {
+                        t = status[0];
+                        st->lastTotal = t;
+                    } else {
+                        std::vector<uint64_t> n(st->nq);
+                        hipCheck(hipMemcpy(n.data(), st->dN.data(), st->nq * 8, hipMemcpyDeviceToHost), "D2H");
+                        for (auto v : n) t += v;
+                    }
+                    if (totals) totals[i] = t;
+                    continue;
+                }
+            }
+            h->slot = i;
+            const int r = tblaeq_handler_batch_run(h, 0, &t);
+            h->slot = saved;
+            if (r) return r;
+            if (totals) totals[i] = t;
+        }
+        return TBLAEQ_OK;
+    });
+    h->slot = saved;
+    return rc;
+}
+
 int tblaeq_handler_batch_free(tblaeq_handler* h) {
-    if (h) h->batch.reset();
+    if (h) h->slots[h->slot].reset();
     return TBLAEQ_OK;
 }
 

@@ -11,6 +11,12 @@ namespace tblaeq_host {
 // Device-resident batched engine state of a handler.
 struct BatchState {
     tblaeq_batch* b = nullptr;
+    // context (stream) the engine runs on: the index's for slot 0, a context of
+    // its own (owned) for the other slots, so that slots run concurrently
+    tblaeq_ctx* ctx = nullptr;
+    bool ownsCtx = false;
+    uint64_t* hStatus = nullptr;  // pinned status words of concurrent runs
+    hipStream_t stream() const { return static_cast<hipStream_t>(tblaeq_ctx_stream(ctx)); }
     uint64_t maxQ = 0, pairCap = 0, hitCap = 0;
     int type = 0;
     uint64_t nq = 0, Kmax = 0;
@@ -30,14 +36,22 @@ struct BatchState {
     DeviceBuffer<uint64_t> lIdx, lK, lIds, lN, lSurv, lFine;
     ~BatchState() {
         if (b) tblaeq_batch_destroy(b);
+        if (hStatus) (void)hipHostFree(hStatus);
+        // the device buffers are freed after this body; hipFree needs no context
+        if (ownsCtx && ctx) tblaeq_ctx_destroy(ctx);
     }
 };
+
+constexpr int kBatchSlots = 2;
 
 }  // namespace tblaeq_host
 
 struct tblaeq_handler {
     std::unique_ptr<tblaeq_host::QueryHandler> h;
-    std::unique_ptr<tblaeq_host::BatchState> batch;  // destroyed before the index it uses
+    // batched engines (slots, each on its own stream), destroyed before the
+    // index they use; the batch entry points act on slots[slot]
+    std::unique_ptr<tblaeq_host::BatchState> slots[tblaeq_host::kBatchSlots];
+    int slot = 0;
     // serialises the per-query traversal (its level buffers are shared): the
     // C entry points and a server's large-K requests take it
     std::mutex plainMu;

@@ -92,6 +92,8 @@ SIGNATURES = {
     "tblaeq_handler_batch_init": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.c_uint64]),
     "tblaeq_handler_batch_set_queries": (C.c_int, [vp, C.c_int, f64p, u64p, C.c_uint64]),
     "tblaeq_handler_batch_run": (C.c_int, [vp, C.c_int, u64p]),
+    "tblaeq_handler_batch_slot": (C.c_int, [vp, C.c_int]),
+    "tblaeq_handler_batch_run_slots": (C.c_int, [vp, C.c_uint64, u64p]),
     "tblaeq_handler_batch_results": (C.c_int, [vp, u64p, u64p, f64p, u64p]),
     "tblaeq_handler_batch_stats": (C.c_int, [vp, u64p, u64p]),
     "tblaeq_handler_batch_free": (C.c_int, [vp]),
@@ -365,16 +367,34 @@ class Handler(_Closing):
     def batch_init(self, max_queries, pair_cap, hit_cap):
         check(lib().tblaeq_handler_batch_init(self.h, max_queries, pair_cap, hit_cap), "batch_init")
 
+    # the uploaded batch of each slot: (type, count, largest K)
+    @property
+    def _bq(self):
+        return self.__dict__.setdefault("_slots", {}).get(self.__dict__.get("_slot", 0))
+
+    def batch_slot(self, slot):
+        """Selects the batched engine (slot 0 or 1, each on its own stream) the batch calls act on."""
+        check(lib().tblaeq_handler_batch_slot(self.h, int(slot)), "batch_slot")
+        self._slot = int(slot)
+
+    def batch_run_slots(self, slots=(0, 1)):
+        """Runs the uploaded batches of several slots concurrently; returns their totals."""
+        mask = sum(1 << int(i) for i in slots)
+        t = np.zeros(2, np.uint64)
+        check(lib().tblaeq_handler_batch_run_slots(self.h, mask, _p(t, u64p)), "batch_run_slots")
+        return [int(t[i]) for i in slots]
+
     def batch_set(self, qtype, queries, Ks=None):
         q = np.ascontiguousarray(queries, np.float64)
-        self._bq = (qtype, q.shape[0])
         if qtype == "range":
             check(lib().tblaeq_handler_batch_set_queries(self.h, 0, _p(q, f64p), None, q.shape[0]), "batch_set")
+            kmax = 0
         else:
             k = np.ascontiguousarray(np.broadcast_to(np.asarray(Ks, np.uint64), (q.shape[0],)))
-            self._bK = int(k.max())
+            kmax = int(k.max())
             check(lib().tblaeq_handler_batch_set_queries(self.h, 1, _p(q, f64p), _p(k, u64p), q.shape[0]),
                   "batch_set")
+        self.__dict__.setdefault("_slots", {})[self.__dict__.get("_slot", 0)] = (qtype, q.shape[0], kmax)
 
     def batch_use_graph(self, on=True):
         check(lib().tblaeq_handler_batch_use_graph(self.h, int(on)), "batch_use_graph")
@@ -454,7 +474,7 @@ class Handler(_Closing):
         return t.value
 
     def batch_results(self):
-        qtype, nq = self._bq
+        qtype, nq, K = self._bq
         if qtype == "range":
             off = np.zeros(nq + 1, np.uint64)
             check(lib().tblaeq_handler_batch_results(self.h, _p(off, u64p), None, None, None), "batch_results")
@@ -462,7 +482,6 @@ class Handler(_Closing):
             check(lib().tblaeq_handler_batch_results(self.h, _p(off, u64p), _p(ids, u64p), None, None),
                   "batch_results")
             return [ids[int(off[i]):int(off[i + 1])] for i in range(nq)]
-        K = self._bK
         n = np.zeros(nq, np.uint64)
         ids = np.zeros(nq * K, np.uint64)
         d2 = np.zeros(nq * K)

@@ -281,3 +281,55 @@ def test_batch_knn_many_workgroups(mode):
         for gid, gd2 in (graph[i], plain[i]):
             assert np.array_equal(gid, eid), "query %d ids differ" % i
             assert np.array_equal(gd2.view(np.uint64), ed2.view(np.uint64)), "query %d distances differ" % i
+
+
+@pytest.mark.parametrize("caps", [(1 << 22, 1 << 22, 1 << 22, 1 << 22, (1, 10, 50, 200)),
+                                  (3000, 40000, 1500, 1500, (1, 10, 20))])
+def test_batch_slots_run_concurrently(caps):
+    """A range batch in slot 0 and a KNN batch in slot 1 replayed concurrently on two streams:
+    every answer equals the oracle's, also when small capacities force both slots back onto the
+    splitting path and when the KNN batch holds K above the batched limit (per-query route)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    rp, rh, kp, kh, kchoice = caps
+    h, o = _make(dict(CFG, N=40000, seed=25), max_q=512, pair_cap=rp, hit_cap=rh)
+    try:
+        h.batch_slot(1)
+        h.batch_init(512, kp, kh)
+        data = h.data()
+        boxes, _ = W.range_boxes(data, 200, 26, 1e-4, 2e-2)
+        pts = W.knn_points(data, 150, 27, h.mesh_sizes, CFG["vmin"], CFG["vmax"], noise=0.3)
+        for Ks in (np.random.default_rng(28).choice(kchoice, 150),
+                   np.random.default_rng(29).choice([5, 300], 150)):
+            h.batch_slot(0)
+            h.batch_set("range", boxes)
+            h.batch_slot(1)
+            h.batch_set("knn", pts, Ks)
+            for _ in range(2):  # capture, then replay
+                hits, nn = h.batch_run_slots((0, 1))
+            h.batch_slot(0)
+            rres = h.batch_results()
+            h.batch_slot(1)
+            kres = h.batch_results()
+            D = h.D
+
+            def exp_range(i):
+                lo, hi = boxes[i][:D], boxes[i][D:]
+                return o.range(lo, hi).refine_range(lo, hi)
+
+            def exp_knn(i):
+                return o.knn(pts[i], int(Ks[i])).refine_knn(pts[i], int(Ks[i]))
+
+            with ThreadPoolExecutor(16) as ex:
+                er = list(ex.map(exp_range, range(len(boxes))))
+                ek = list(ex.map(exp_knn, range(len(pts))))
+            assert hits == sum(len(x) for x in er) and nn == sum(len(x[0]) for x in ek)
+            for i in range(len(boxes)):
+                assert np.array_equal(rres[i], er[i]), "box %d" % i
+            for i in range(len(pts)):
+                assert np.array_equal(kres[i][0], ek[i][0]), "point %d ids" % i
+                assert np.array_equal(kres[i][1].view(np.uint64), ek[i][1].view(np.uint64)), "point %d d2" % i
+    finally:
+        h.batch_slot(0)
+        _OPEN.remove(h)
+        h.close()
