@@ -66,6 +66,16 @@ def sort_traffic(cfg, passes):
     return t.get("bytes_per_launch")
 
 
+def _json_default(o, path=None):
+    """NumPy values in the result record become plain numbers and lists."""
+    if isinstance(o, np.ndarray):
+        log("note: an array in the result record was written as a list, shape %s" % (o.shape,))
+        return o.tolist()
+    if isinstance(o, np.generic):
+        return o.item()
+    raise TypeError("not serialisable: %r" % type(o))
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -751,6 +761,22 @@ def main():
             roofline["others"] = {k: v for k, v in rl.items() if k != dom}
             roofline["plain"] = plain_roofline
 
+    # serving first after the batched step: it measures latency, so it runs before the legs that
+    # leave large host and device state behind.  Slot 1's engine goes first: its stream would
+    # share one of the process's four hardware queues with a server worker's stream.
+    if batched:
+        h.batch_slot(1)
+        h.batch_free()
+        h.batch_slot(0)
+    serving = None
+    if args.serve_rates and not args.no_batch:
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        serving = {"workload": "open-loop Poisson stream on the same index: 70% range (selectivity log-uniform "
+                               "1e-6..1e-3), 30% KNN (K log-uniform 1..256, K above the batched kernels' limit per query); continuous batching, admission by "
+                               "free HBM; latency = submit to result on the host",
+                   "runs": run_serving(h, *serving_mix(data, cfg["seed"], h.mesh_sizes, cfg["vmin"], cfg["vmax"]),
+                                       rates, args.serve_seconds, log)}
+
     learned = None
     if args.learn_traces and not args.no_batch:
         learned = run_learned(h, data, cfg, knn_q, Ks.astype(np.uint64), args.learn_traces, args.learn_epochs,
@@ -764,15 +790,6 @@ def main():
     if args.build_n:
         build = run_build(args.build_n, args.build_d, (100, 50, 20), args.build_cpu_n, log)
 
-    serving = None
-    if args.serve_rates and not args.no_batch:
-        rates = [float(x) for x in args.serve_rates.split(",") if x]
-        serving = {"workload": "open-loop Poisson stream on the same index: 70% range (selectivity log-uniform "
-                               "1e-6..1e-3), 30% KNN (K log-uniform 1..256, K above the batched kernels' limit per query); continuous batching, admission by "
-                               "free HBM; latency = submit to result on the host",
-                   "runs": run_serving(h, *serving_mix(data, cfg["seed"], h.mesh_sizes, cfg["vmin"], cfg["vmax"]),
-                                       rates, args.serve_seconds, log)}
-
     c5 = None
     if args.c5_n and not args.no_batch:
         c5 = run_c5(args.c5_n, c5_rates, args.c5_seconds, 64, log)
@@ -785,9 +802,9 @@ def main():
         o = O.Index.generate(cfg["N"], cfg["D"], cfg["vmin"], cfg["vmax"], cfg["is_int"], cfg["ratios"], cfg["seed"],
                              cfg["sigma_div"])
         nr, nk = args.cpu_range, args.cpu_knn
-        w1, us1, c1, th = o.cpu_batch(0, boxes[:nr])
-        w2, us2, c2, _ = o.cpu_batch(1, knn_q[:nk], K=10)
-        w3, us3, c3, _ = o.cpu_batch(1, knn_q[args.nq // 2: args.nq // 2 + nk], K=100)
+        w1, us1, _, th = o.cpu_batch(0, boxes[:nr])
+        w2, us2, _, _ = o.cpu_batch(1, knn_q[:nk], K=10)
+        w3, us3, _, _ = o.cpu_batch(1, knn_q[args.nq // 2: args.nq // 2 + nk], K=100)
         tot_q = nr + 2 * nk
         cpu = {"value": round(tot_q / (w1 + w2 + w3), 3), "unit": "queries/s", "cores": th, "kind": "port",
                "sample": "%d range + %d KNN(K=10) + %d KNN(K=100) queries of the same batch, OpenMP over queries, "
@@ -819,7 +836,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(out, default=_json_default), flush=True)
     if args.calib_copy_mb:
         # device-to-device hipMemcpy: the runtime's copy kernel, n bytes read and n written
         import ctypes

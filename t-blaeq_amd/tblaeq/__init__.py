@@ -377,6 +377,11 @@ class Handler(_Closing):
         check(lib().tblaeq_handler_batch_slot(self.h, int(slot)), "batch_slot")
         self._slot = int(slot)
 
+    def batch_free(self):
+        """Releases the selected slot's engine, its stream and scratch."""
+        check(lib().tblaeq_handler_batch_free(self.h), "batch_free")
+        self.__dict__.setdefault("_slots", {}).pop(self.__dict__.get("_slot", 0), None)
+
     def batch_run_slots(self, slots=(0, 1)):
         """Runs the uploaded batches of several slots concurrently; returns their totals."""
         mask = sum(1 << int(i) for i in slots)
