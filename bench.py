@@ -338,6 +338,68 @@ def run_build(n, D, ratios, cpu_n, log):
     return out
 
 
+def run_c4(n, D, ratios, iters, seed, sub_n, log):
+    """The index-build configuration: N points of the generator's hierarchy shape (the parallel
+    synthetic builder, D dimensions, the given ratios) as FP32 values, built on the GPU with k-means||
+    seeding and `iters` Lloyd iterations from `seed` (FP32 matrix-core assignment with the exact FP64
+    re-score of near ties, CSC P-tensors on the device).  Beside it, the first sub_n points are built
+    on the GPU with the CPU build's k-means++ and on the host with the same settings (their inertia
+    must agree), and on the GPU with k-means|| (its inertia against k-means++'s)."""
+    import tblaeq
+
+    log("C4: generating %d x %d points" % (n, D))
+    syn = tblaeq.Handler.synthetic(n, D, 1.0, 100.0, False, ratios, seed=20260331, sigma_div=3.0, threads=0)
+    x = syn.data().astype(np.float32)
+    syn.close()
+    log("C4: data ready (FP32, %.1f GB)" % (x.nbytes / 1e9))
+    tblaeq.set_progress(lambda m: log("C4:", m))
+    try:
+        t0 = time.perf_counter()
+        hb = tblaeq.Handler.build(x, ratios, "c4", max_iter=iters, seed=seed, seeding=1)
+        build_s = time.perf_counter() - t0
+    finally:
+        tblaeq.set_progress(None)
+    st = hb.build_stats()
+    inertia = hb.build_inertia()
+    hb.close()
+    dp = 4
+    while dp < D:
+        dp *= 2
+    lv = []
+    for r, iv in zip(st, inertia):
+        ms = float(r[3])
+        fl = float(r[2]) * 2 * dp
+        lv.append({"iterations": int(r[0]), "rescored_points": int(r[1]), "score_evals": int(r[2]),
+                   "assign_ms": int(ms), "candidates": int(r[5]), "inertia": float(iv),
+                   "mfma_tflops": round(fl / (ms / 1e3) / 1e12, 2) if ms else None,
+                   "mfma_frac": round(fl / (ms / 1e3) / 1e12 / FP32_MFMA_PEAK_TFLOPS, 4) if ms else None})
+    xs = np.ascontiguousarray(x[:sub_n])
+    t0 = time.perf_counter()
+    g = tblaeq.Handler.build(xs, ratios, "c4-sub", max_iter=iters, seed=seed)
+    g_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    c = tblaeq.Handler.build(xs, ratios, "c4-sub", force_cpu=True, max_iter=iters, seed=seed)
+    c_s = time.perf_counter() - t0
+    p = tblaeq.Handler.build(xs, ratios, "c4-sub", max_iter=iters, seed=seed, seeding=1)
+    gi, ci, pi = g.build_inertia()[0], c.build_inertia()[0], p.build_inertia()[0]
+    same_iters = bool(np.array_equal(g.build_stats()[:, 0], c.build_stats()[:, 0]))
+    for hh in (g, c, p):
+        hh.close()
+    out = {"workload": "k-means index build of N=%d FP32 points, D=%d (the generator's hierarchy shape, parallel "
+                       "builder), ratios %s, %d Lloyd iterations from seed %d, k-means|| seeding (oversampling "
+                       "0.5 k, 4 rounds)" % (n, D, ",".join(map(str, ratios)), iters, seed),
+           "gpu_s": round(build_s, 1), "levels": lv,
+           "finest_level_ranking": {"bound": "mfma", "achieved": lv[0]["mfma_tflops"], "peak": FP32_MFMA_PEAK_TFLOPS,
+                                    "unit": "TFLOP/s", "frac": lv[0]["mfma_frac"],
+                                    "note": "FP32 matrix-core score ||c||^2 - 2x.c (2 x padded D flops per point x "
+                                            "centre) over the Lloyd iterations, assignment + FP64 re-score time"},
+           "subsample": {"N": sub_n, "gpu_kmeanspp_s": round(g_s, 2), "cpu_s": round(c_s, 2),
+                         "relative_inertia_gpu_vs_cpu": abs(gi - ci) / ci, "same_iterations": same_iters,
+                         "kmeans_parallel_over_kmeanspp_inertia": round(pi / gi, 5)}}
+    log("C4", out)
+    return out
+
+
 def run_learned(h, data, cfg, knn_q, Ks, n_traces, epochs, plain_sample, log):
     """Learned KNN radius on this index: traces from the batched engine, BF16-MFMA
     training on the GPU, then the per-query traversal with and without the
@@ -540,6 +602,11 @@ def main():
     ap.add_argument("--c5-rates", default="1000,5000,20000,50000,100000,200000")
     ap.add_argument("--c5-seconds", type=float, default=2.0)
     ap.add_argument("--c5-only", action="store_true", help="run only the serving configuration")
+    ap.add_argument("--c4-n", type=int, default=0,
+                    help="points of the index-build configuration's run (50000000 for it; 0 skips it: minutes)")
+    ap.add_argument("--c4-iters", type=int, default=20)
+    ap.add_argument("--c4-sub", type=int, default=20000, help="subsample built on the GPU and on the host")
+    ap.add_argument("--c4-only", action="store_true", help="run only the index-build configuration")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch parameters (profiling runs)")
     ap.add_argument("--build-n", type=int, default=1_000_000, help="points of the GPU index build run (0 skips it)")
     ap.add_argument("--build-d", type=int, default=32)
@@ -557,6 +624,10 @@ def main():
     from tblaeq import workload as W
 
     c5_rates = [float(x) for x in args.c5_rates.split(",") if x]
+    if args.c4_only:
+        c4 = run_c4(args.c4_n or 50_000_000, 32, (100, 50, 20), args.c4_iters, 7, args.c4_sub, log)
+        print(json.dumps({"build_c4": c4}, default=_json_default), flush=True)
+        return 0
     if args.c5_only:
         c5 = run_c5(args.c5_n or 500_000_000, c5_rates, args.c5_seconds, 64, log)
         print(json.dumps({"serving_c5": c5}), flush=True)
@@ -790,6 +861,10 @@ def main():
     if args.build_n:
         build = run_build(args.build_n, args.build_d, (100, 50, 20), args.build_cpu_n, log)
 
+    c4 = None
+    if args.c4_n:
+        c4 = run_c4(args.c4_n, 32, (100, 50, 20), args.c4_iters, 7, args.c4_sub, log)
+
     c5 = None
     if args.c5_n and not args.no_batch:
         c5 = run_c5(args.c5_n, c5_rates, args.c5_seconds, 64, log)
@@ -832,6 +907,7 @@ def main():
         "learned_d16": c3,
         "serving": serving,
         "serving_c5": c5,
+        "build_c4": c4,
         "build": build,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -75,6 +75,12 @@ int tblaeq_ctx_set_tag(tblaeq_ctx* ctx, int tag);
 int tblaeq_ctx_timing(tblaeq_ctx* ctx, int kind, int tag, uint64_t* launches, double* ms);
 int tblaeq_ctx_timing_reset(tblaeq_ctx* ctx);
 const char* tblaeq_last_error(void);
+/* Progress messages of long-running calls (the GPU index build reports its
+ * seeding rounds, seeding picks and Lloyd iterations): fn(message, user) is
+ * called on the calling thread; fn = NULL turns them off (the default).
+ * Process-wide.  No reference counterpart. */
+typedef void (*tblaeq_progress_fn)(const char* message, void* user);
+void tblaeq_set_progress(tblaeq_progress_fn fn, void* user);
 
 /* --- plain per-query level kernels ---------------------------------------- */
 

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "tblaeq.h"
+#include "tblaeq_build.h"
 
 namespace tblaeq_host {
 
@@ -261,7 +262,7 @@ struct IndexData {
     std::vector<uint64_t> coarsestIds;
     std::vector<double> coarsestVals;
     std::vector<double> finestData;  // generator output (finest layer), when generated
-    std::vector<uint64_t> buildStats;  // GPU build: per level (finest first) tblaeq_kmeans stats[4]
+    std::vector<uint64_t> buildStats;  // builds: per level (finest first) the six tblaeq_kmeans_ex stats words
 
     // device side
     tblaeq_ctx* ctx = nullptr;
@@ -305,10 +306,13 @@ public:
     static const std::vector<size_t> kDefaultRatios;
     static constexpr size_t kDefaultHeight = 4;
     static size_t computeCentroidCount(size_t n, size_t ratio);
+    // GPU build; data32 (when data is null) holds FP32 values, converted to
+    // FP64 on the device; opt null = the CPU build's settings.
     static IndexData* buildGpu(const double* data, size_t N, size_t D, const std::string& name, size_t height,
-                               const std::vector<size_t>& ratios);
+                               const std::vector<size_t>& ratios, const tblaeq_kmeans_options* opt = nullptr,
+                               const float* data32 = nullptr);
     static IndexData* build(const double* data, size_t N, size_t D, const std::string& name, bool forceUseCPU,
-                            size_t height, const std::vector<size_t>& ratios);
+                            size_t height, const std::vector<size_t>& ratios, const tblaeq_kmeans_options* opt = nullptr);
     static IndexData* buildRandom(size_t N, size_t D, double valMin, double valMax, bool isInt,
                                   const std::string& name, size_t height, const std::vector<size_t>& ratios,
                                   uint64_t seed, double sigmaDivisor);
@@ -317,6 +321,13 @@ public:
     static IndexData* assemble(size_t N, size_t D, const std::vector<size_t>& ratios,
                                std::vector<std::vector<double>>& layers,
                                const std::vector<std::vector<uint64_t>>& labels);
+    // build / buildGpu after the reference's plan adjustment (a level that
+    // would collapse to one centroid ends the hierarchy, QueryHandler.cu:23-45),
+    // with FP64 data or, when data is null, FP32 data32; no reference
+    // counterpart for the options.
+    static IndexData* buildPlanned(const double* data, const float* data32, size_t N, size_t D,
+                                   const std::string& name, bool forceUseCPU, size_t height,
+                                   const std::vector<size_t>& ratios, const tblaeq_kmeans_options* opt);
     // No reference counterpart: the generator's hierarchy shape built on
     // `threads` host threads (0 = all) from per-chunk streams, for indexes too
     // large for the single-stream generator; not the reference's sequence.

@@ -329,10 +329,10 @@ struct KM {
         }
         return s;
     }
-    std::vector<double> seed(uint64_t k) const {
+    std::vector<double> seed(uint64_t k, uint64_t streamSeed = 42) const {
         std::vector<double> c;
         c.reserve(k * D);
-        std::mt19937_64 rng(42);
+        std::mt19937_64 rng(streamSeed);
         std::uniform_int_distribution<size_t> uni(0, N - 1);
         const size_t first = uni(rng);
         c.insert(c.end(), x + first * D, x + first * D + D);
@@ -828,8 +828,15 @@ double oracle_cpu_batch(const OIndex* ix, int type, const double* qs, uint64_t n
 
 uint64_t oracle_kmeans(const double* data, uint64_t N, uint64_t D, uint64_t k, uint64_t maxIter, uint64_t* labels,
                        double* centroids) {
+    return oracle_kmeans_seeded(data, N, D, k, maxIter, 42, labels, centroids);
+}
+
+// oracle_kmeans with the random stream's seed as a parameter (the CPU build
+// seeds mt19937_64 with 42; the index-build configuration asks for 7).
+uint64_t oracle_kmeans_seeded(const double* data, uint64_t N, uint64_t D, uint64_t k, uint64_t maxIter,
+                              uint64_t streamSeed, uint64_t* labels, double* centroids) {
     KM km{data, N, D};
-    std::vector<double> c = km.seed(k);
+    std::vector<double> c = km.seed(k, streamSeed);
     for (uint64_t i = 0; i < N; ++i) labels[i] = 0;
     uint64_t it = 0;
     for (; it < maxIter; ++it) {

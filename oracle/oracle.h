@@ -128,6 +128,8 @@ double oracle_cpu_batch(const OIndex* idx, int type, const double* qs, uint64_t 
 /* Full CPU k-means of CPUKmeans.cuh; labels[N], centroids[k*D]; returns iterations run. */
 uint64_t oracle_kmeans(const double* data, uint64_t N, uint64_t D, uint64_t k, uint64_t maxIter, uint64_t* labels,
                        double* centroids);
+uint64_t oracle_kmeans_seeded(const double* data, uint64_t N, uint64_t D, uint64_t k, uint64_t maxIter,
+                              uint64_t streamSeed, uint64_t* labels, double* centroids);
 /* Lloyd assignment (strict <, lowest index wins) and its inertia. */
 double oracle_kmeans_assign(const double* data, uint64_t N, uint64_t D, const double* cent, uint64_t k,
                             uint64_t* labels);

@@ -65,6 +65,8 @@ def lib():
             "oracle_cpu_batch": (C.c_double, [vp, C.c_int, f64p, C.c_uint64, C.c_uint64, C.c_int, f64p, u64p,
                                               C.POINTER(C.c_int)]),
             "oracle_kmeans": (C.c_uint64, [f64p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, u64p, f64p]),
+            "oracle_kmeans_seeded": (C.c_uint64, [f64p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
+                                                  u64p, f64p]),
             "oracle_kmeans_assign": (C.c_double, [f64p, C.c_uint64, C.c_uint64, f64p, C.c_uint64, u64p]),
         }
         for name, (res, args) in sig.items():
@@ -276,12 +278,12 @@ def policy_recommend(nnz, cols, D, coarsest_bytes, maps_bytes, free_mem):
     return out
 
 
-def kmeans(data, k, max_iter):
+def kmeans(data, k, max_iter, seed=42):
     data = f64(data)
     N, D = data.shape
     lab = np.zeros(N, np.uint64)
     cent = np.zeros((k, D))
-    it = lib().oracle_kmeans(_p(data, f64p), N, D, k, max_iter, _p(lab, u64p), _p(cent, f64p))
+    it = lib().oracle_kmeans_seeded(_p(data, f64p), N, D, k, max_iter, seed, _p(lab, u64p), _p(cent, f64p))
     return lab, cent, it
 
 

@@ -4,13 +4,37 @@
 #include <cstring>
 #include <string>
 
+#include <cstdarg>
+#include <cstdio>
+#include <mutex>
+
 namespace {
 thread_local std::string g_lastError;
+std::mutex g_progressMu;
+tblaeq_progress_fn g_progress = nullptr;
+void* g_progressUser = nullptr;
 }
 
 namespace tblaeq {
 
 void set_error(const std::string& msg) { g_lastError = msg; }
+
+void progress(const char* fmt, ...) {
+    tblaeq_progress_fn fn;
+    void* user;
+    {
+        std::lock_guard<std::mutex> lk(g_progressMu);
+        fn = g_progress;
+        user = g_progressUser;
+    }
+    if (!fn) return;
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    fn(buf, user);
+}
 
 int hip_fail(hipError_t e, const char* what) {
     g_lastError = std::string(what) + ": " + hipGetErrorString(e);
@@ -227,3 +251,9 @@ extern "C" const char* tblaeq_status_string(int s) {
 }
 
 extern "C" const char* tblaeq_last_error(void) { return g_lastError.c_str(); }
+
+extern "C" void tblaeq_set_progress(tblaeq_progress_fn fn, void* user) {
+    std::lock_guard<std::mutex> lk(g_progressMu);
+    g_progress = fn;
+    g_progressUser = user;
+}

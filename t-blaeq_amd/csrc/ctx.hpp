@@ -62,6 +62,9 @@ constexpr int kSortTile = 2048;                         // pairs per LDS tile so
 constexpr int kSortThreads = 256;
 
 void set_error(const std::string& msg);
+// Sends a printf-style progress message to the callback set by
+// tblaeq_set_progress (nothing when none is set).
+void progress(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 int hip_fail(hipError_t e, const char* what);
 // Ensures the look-back status array can hold `tiles` tiles (epochs advance on the device).
 int scan_prepare(tblaeq_ctx* ctx, uint64_t tiles);

@@ -105,6 +105,30 @@ int tblaeq_handler_build(const double* data, uint64_t N, uint64_t D, uint64_t he
     });
 }
 
+int tblaeq_handler_build_ex(const void* data, int dtype, uint64_t N, uint64_t D, uint64_t height,
+                            const uint64_t* ratios, const char* name, int forceCPU, const tblaeq_kmeans_options* opt,
+                            tblaeq_handler** out) {
+    if (!out || !data || !ratios || N == 0 || D == 0 || height < 2 || (dtype != 0 && dtype != 1)) {
+        tblaeq::set_error("tblaeq_handler_build_ex: invalid argument (data, ratios, N, D > 0, height >= 2, "
+                          "dtype 0 (FP64) or 1 (FP32))");
+        return TBLAEQ_ERR_INVALID;
+    }
+    if (forceCPU && opt && opt->seeding != 0) {
+        tblaeq::set_error("tblaeq_handler_build_ex: k-means|| seeding runs on the GPU only");
+        return TBLAEQ_ERR_UNSUPPORTED;
+    }
+    return guarded("tblaeq_handler_build_ex", [&] {
+        const std::vector<size_t> r = toVec(ratios, height - 1);
+        std::unique_ptr<IndexData> idx(IndexBuilder::buildPlanned(
+            dtype == 0 ? static_cast<const double*>(data) : nullptr,
+            dtype == 1 ? static_cast<const float*>(data) : nullptr, N, D, name ? name : "in-memory", forceCPU != 0,
+            height, r, opt));
+        auto* h = new tblaeq_handler();
+        h->h = std::make_unique<QueryHandler>(std::move(idx));
+        *out = h;
+    });
+}
+
 int tblaeq_handler_build_stats(tblaeq_handler* h, uint64_t* out, uint64_t cap, uint64_t* n) {
     if (!h || !n) return TBLAEQ_ERR_INVALID;
     return guarded("tblaeq_handler_build_stats", [&] {

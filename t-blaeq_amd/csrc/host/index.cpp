@@ -674,9 +674,13 @@ IndexData* IndexBuilder::assembleStable(size_t N, size_t D, const std::vector<si
 // build's labels and centroids) and tblaeq_build_level (CSC P-tensor, map,
 // radius; rows of a column in point order).
 IndexData* IndexBuilder::buildGpu(const double* data, size_t N, size_t D, const std::string& name, size_t height,
-                                  const std::vector<size_t>& ratios) {
+                                  const std::vector<size_t>& ratios, const tblaeq_kmeans_options* optIn,
+                                  const float* data32) {
     if (height < 2 || ratios.size() != height - 1) throw std::invalid_argument("build: bad height/ratios");
-    constexpr size_t kMaxIter = 7;
+    if (!data && !data32) throw std::invalid_argument("build: no data");
+    tblaeq_kmeans_options opt;
+    tblaeq_kmeans_options_init(&opt);
+    if (optIn) opt = *optIn;
     const size_t L = ratios.size();
     auto idx = std::make_unique<IndexData>();
     idx->D = D;
@@ -691,7 +695,14 @@ IndexData* IndexBuilder::buildGpu(const double* data, size_t N, size_t D, const 
     idx->ensureCtx();
     tblaeq_ctx* ctx = idx->ctx;
     DeviceBuffer<double> dx;
-    dx.reset(data, N * D);
+    if (data) {
+        dx.reset(data, N * D);
+    } else {
+        DeviceBuffer<float> d32(data32, N * D);
+        dx.resize(N * D);
+        tbCheck(tblaeq_convert_f32_f64(ctx, d32.data(), N * D, dx.data()), "tblaeq_convert_f32_f64");
+        hipCheck(hipStreamSynchronize(static_cast<hipStream_t>(tblaeq_ctx_stream(ctx))), "sync");
+    }
     size_t n = N;
     for (size_t i = 0; i < L; ++i) {
         const size_t fineIdx = L - i, pIdx = fineIdx - 1;
@@ -706,12 +717,13 @@ IndexData* IndexBuilder::buildGpu(const double* data, size_t N, size_t D, const 
         dMap.resize(n);
         dVals.resize(n * D);
         dRad.resize(k);
-        uint64_t st[4] = {0, 0, 0, 0};
-        tbCheck(tblaeq_kmeans(ctx, dx.data(), n, D, k, kMaxIter, 42, dc.data(), dlab.data(), st), "tblaeq_kmeans");
+        uint64_t st[6] = {0, 0, 0, 0, 0, 0};
+        tblaeq::progress("GPU build: level %zu of %zu, %zu points into %zu clusters", i + 1, L, n, k);
+        tbCheck(tblaeq_kmeans_ex(ctx, dx.data(), n, D, k, &opt, dc.data(), dlab.data(), st), "tblaeq_kmeans");
         tbCheck(tblaeq_build_level(ctx, dx.data(), n, D, dc.data(), k, dlab.data(), dColRes.data(), dNnz.data(),
                                    dMap.data(), dVals.data(), dRad.data()),
                 "tblaeq_build_level");
-        idx->buildStats.insert(idx->buildStats.end(), st, st + 4);
+        idx->buildStats.insert(idx->buildStats.end(), st, st + 6);
         idx->meshSizes.push_back(n);
         auto P = std::make_unique<SparseTensorCscFormat>();
         P->D = D;
@@ -748,11 +760,16 @@ IndexData* IndexBuilder::buildGpu(const double* data, size_t N, size_t D, const 
 // IndexBuilder.cu:44-151 with Lloyd k-means + k-means++ (CPUKmeans.cuh:67-205)
 // on the host, or on the GPU (buildGpu) when forceUseCPU is false.
 IndexData* IndexBuilder::build(const double* data, size_t N, size_t D, const std::string& name, bool forceUseCPU,
-                               size_t height, const std::vector<size_t>& ratios) {
-    if (!forceUseCPU) return buildGpu(data, N, D, name, height, ratios);
+                               size_t height, const std::vector<size_t>& ratios, const tblaeq_kmeans_options* optIn) {
+    if (!forceUseCPU) return buildGpu(data, N, D, name, height, ratios, optIn);
     if (height < 2 || ratios.size() != height - 1) throw std::invalid_argument("build: bad height/ratios");
-    constexpr size_t kMaxIter = 7;
+    tblaeq_kmeans_options opt;
+    tblaeq_kmeans_options_init(&opt);
+    if (optIn) opt = *optIn;
+    if (opt.seeding != 0) throw std::invalid_argument("build: k-means|| seeding runs on the GPU only");
+    const size_t maxIter = opt.max_iter;
     const size_t L = ratios.size();
+    std::vector<uint64_t> stats;
     std::vector<std::vector<double>> layers(L + 1);
     std::vector<std::vector<uint64_t>> labels(L);
     layers[0].assign(data, data + N * D);
@@ -771,7 +788,7 @@ IndexData* IndexBuilder::build(const double* data, size_t N, size_t D, const std
         };
         std::vector<double> c;
         c.reserve(k * D);
-        std::mt19937_64 rng(42);
+        std::mt19937_64 rng(opt.seed);
         std::uniform_int_distribution<size_t> uni(0, n - 1);
         const size_t first = uni(rng);
         c.insert(c.end(), &x[first * D], &x[first * D] + D);
@@ -799,7 +816,9 @@ IndexData* IndexBuilder::build(const double* data, size_t N, size_t D, const std
         }
         auto& lab = labels[i];
         lab.assign(n, 0);
-        for (size_t it = 0; it < kMaxIter; ++it) {
+        size_t iters = 0;
+        for (size_t it = 0; it < maxIter; ++it) {
+            ++iters;
             bool changed = false;
             for (size_t j = 0; j < n; ++j) {
                 double best = std::numeric_limits<double>::max();
@@ -827,10 +846,18 @@ IndexData* IndexBuilder::build(const double* data, size_t N, size_t D, const std
                     for (size_t d = 0; d < D; ++d) c[q * D + d] = sums[q * D + d] / static_cast<double>(cnt[q]);
             if (!changed) break;
         }
+        // inertia: squared distances to the final centroids, summed in point order
+        double inertia = 0.0;
+        for (size_t j = 0; j < n; ++j) inertia += sq(&x[j * D], &c[lab[j] * D]);
+        uint64_t bits = 0;
+        std::memcpy(&bits, &inertia, sizeof bits);
+        const uint64_t st[6] = {iters, 0, 0, 0, bits, 0};
+        stats.insert(stats.end(), st, st + 6);
         layers[i + 1] = std::move(c);
     }
     IndexData* idx = assemble(N, D, ratios, layers, labels);
     idx->datasetName = name;
+    idx->buildStats = std::move(stats);
     return idx;
 }
 

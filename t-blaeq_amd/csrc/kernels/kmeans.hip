@@ -27,6 +27,7 @@
 // and their cumulative sums live on the device (per 1024-point tile sums, a
 // block scan over tiles, a scan inside the chosen tile).
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <random>
 #include <vector>
@@ -42,7 +43,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int kKmThreads = 256;
 constexpr int kKmPoints = 256;   // points per assignment workgroup
-constexpr int kKmChunk = 64;     // centers per LDS chunk
+constexpr int kKmChunk = 128;    // centers per LDS chunk
 constexpr int kPpTile = 1024;    // points per seeding tile sum
 
 // Squared distance with the CPU build's arithmetic (no contraction: the
@@ -102,14 +103,14 @@ struct AssignArgs {
     uint64_t* changed;
 };
 
+// Best and second-best score with the best's index, centres arriving in
+// increasing index order (strict <: the lowest index keeps a tie).  With
+// b1 <= b2: b2' = min(b2, max(b1, v)) and b1' = min(b1, v) -- one compare, one
+// select and three min / max, the same values as the branching form.
 __device__ __forceinline__ void keep_best(float v, uint32_t j, float& b1, float& b2, uint32_t& i1) {
-    if (v < b1) {
-        b2 = b1;
-        b1 = v;
-        i1 = j;
-    } else if (v < b2) {
-        b2 = v;
-    }
+    i1 = v < b1 ? j : i1;
+    b2 = fminf(b2, fmaxf(b1, v));
+    b1 = fminf(b1, v);
 }
 
 template <int DP>
@@ -274,10 +275,12 @@ __global__ void k_km_update(const double* __restrict__ x, int D, const uint64_t*
 }
 
 // Seeding: minD[i] = min(minD[i], |x_i - last|^2); tileSum[t] = sum of minD over
-// tile t (256 threads x 4 consecutive points, then a fixed-order tree).
+// tile t (256 threads x 4 consecutive points, then a fixed-order tree), each
+// term times w[i] when weights are given (weighted k-means++ on k-means||
+// candidates).
 __global__ __launch_bounds__(256) void k_kmpp_update(const double* __restrict__ x, uint64_t n, int D,
                                                      const double* __restrict__ last, double* __restrict__ minD,
-                                                     double* __restrict__ tileSum) {
+                                                     double* __restrict__ tileSum, const double* __restrict__ w) {
     __shared__ double s_w[4];
     __shared__ double s_last[64];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
@@ -295,7 +298,7 @@ __global__ __launch_bounds__(256) void k_kmpp_update(const double* __restrict__ 
                     m = dd;
                     minD[i] = m;
                 }
-                s = s + m;
+                s = s + (w ? w[i] * m : m);
             }
         }
         for (int o = 1; o < kWave; o <<= 1) s = s + __shfl_xor(s, o, kWave);
@@ -312,7 +315,8 @@ __global__ __launch_bounds__(256) void k_kmpp_update(const double* __restrict__ 
 __global__ __launch_bounds__(1024) void k_kmpp_pick(const double* __restrict__ x, uint64_t n, int D,
                                                     const double* __restrict__ minD,
                                                     const double* __restrict__ tileSum, const double* __restrict__ u,
-                                                    uint64_t cc, double* __restrict__ centers) {
+                                                    uint64_t cc, double* __restrict__ centers,
+                                                    const double* __restrict__ w) {
     __shared__ double s_part[1024];
     __shared__ uint64_t s_tile, s_pick;
     __shared__ double s_thr, s_base;
@@ -363,7 +367,7 @@ __global__ __launch_bounds__(1024) void k_kmpp_pick(const double* __restrict__ x
         __syncthreads();
         // scan of the tile's minD (one point per thread)
         const uint64_t i = tile * kPpTile + tid;
-        s_part[tid] = i < n ? minD[i] : 0.0;
+        s_part[tid] = i < n ? (w ? w[i] * minD[i] : minD[i]) : 0.0;
         __syncthreads();
         for (int o = 1; o < 1024; o <<= 1) {
             const double v = tid >= o ? s_part[tid - o] : 0.0;
@@ -383,6 +387,124 @@ __global__ __launch_bounds__(1024) void k_kmpp_pick(const double* __restrict__ x
     }
     const uint64_t pick = s_pick;
     if (tid < D) centers[cc * D + tid] = x[pick * D + tid];
+}
+
+// ---- k-means|| (Bahmani et al., 2012) --------------------------------------
+
+// Uniform draw in [0, 1) of point i in round r: SplitMix64 of (seed, r, i).
+__device__ __forceinline__ double unit_draw(uint64_t seed, uint64_t r, uint64_t i) {
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (r * 0x100000001B3ull + i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return static_cast<double>(z >> 11) * 0x1.0p-53;
+}
+
+// minD[i] = |x_i - c0|^2, nearest[i] = 0.
+__global__ void k_kmpar_init(const double* __restrict__ x, uint64_t n, int D, const double* __restrict__ c0,
+                             double* __restrict__ minD, uint64_t* __restrict__ nearest) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        minD[i] = sq64(x + i * D, c0, D);
+        nearest[i] = 0;
+    }
+}
+
+// Per 1024-point tile, the sum of v[i] (256 threads x 4 consecutive points,
+// then a fixed-order tree), and one workgroup's fixed-order sum of the tiles.
+__global__ __launch_bounds__(256) void k_tile_sum(const double* __restrict__ v, uint64_t n,
+                                                  double* __restrict__ tileSum) {
+    __shared__ double s_w[4];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const uint64_t ntiles = (n + kPpTile - 1) / kPpTile;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        double s = 0.0;
+        for (int r = 0; r < 4; ++r) {
+            const uint64_t i = t * kPpTile + static_cast<uint64_t>(tid) * 4 + r;
+            if (i < n) s = s + v[i];
+        }
+        for (int o = 1; o < kWave; o <<= 1) s = s + __shfl_xor(s, o, kWave);
+        if (lane == 0) s_w[wid] = s;
+        __syncthreads();
+        if (tid == 0) tileSum[t] = ((s_w[0] + s_w[1]) + s_w[2]) + s_w[3];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_sum_tiles(const double* __restrict__ tileSum, uint64_t ntiles,
+                                                    double* __restrict__ out) {
+    __shared__ double s_part[1024];
+    const int tid = threadIdx.x;
+    double s = 0.0;
+    for (uint64_t t = tid; t < ntiles; t += 1024) s = s + tileSum[t];
+    s_part[tid] = s;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (tid < o) s_part[tid] = s_part[tid] + s_part[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) *out = s_part[0];
+}
+
+// Round r's sample: point i is drawn when unit_draw < ell * minD[i] / phi.
+__global__ void k_kmpar_mark(const double* __restrict__ minD, uint64_t n, const double* __restrict__ phi, double ell,
+                             uint64_t seed, uint64_t round, uint64_t* __restrict__ flag) {
+    const double p = *phi;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        flag[i] = (p > 0.0 && unit_draw(seed, round, i) * p < ell * minD[i]) ? 1 : 0;
+}
+
+// Drawn points, in point order, appended to the candidates (off = exclusive
+// scan of the flags).
+__global__ void k_kmpar_gather(const double* __restrict__ x, uint64_t n, int D, const uint64_t* __restrict__ flag,
+                               const uint64_t* __restrict__ off, double* __restrict__ cand) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        if (flag[i])
+            for (int d = 0; d < D; ++d) cand[off[i] * D + d] = x[i * D + d];
+}
+
+// After the nearest of a round's new candidates is found (lab, exact): keep
+// it when strictly closer than the point's current candidate.
+__global__ void k_kmpar_improve(const double* __restrict__ x, uint64_t n, int D, const double* __restrict__ cand,
+                                const uint64_t* __restrict__ lab, uint64_t base, double* __restrict__ minD,
+                                uint64_t* __restrict__ nearest) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const double d = sq64(x + i * D, cand + lab[i] * D, D);
+        if (d < minD[i]) {
+            minD[i] = d;
+            nearest[i] = base + lab[i];
+        }
+    }
+}
+
+// Candidate weights: the points whose nearest candidate each one is.
+__global__ void k_kmpar_weights(const uint64_t* __restrict__ nearest, uint64_t n, uint64_t* __restrict__ wcnt) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        atomicAdd(reinterpret_cast<unsigned long long*>(&wcnt[nearest[i]]), 1ull);
+}
+
+__global__ void k_u64_to_f64(const uint64_t* __restrict__ a, uint64_t n, double* __restrict__ b) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        b[i] = static_cast<double>(a[i]);
+}
+
+// v[i] = |x_i - c[lab[i]]|^2 (the inertia terms).
+__global__ void k_km_sqdist(const double* __restrict__ x, uint64_t n, int D, const double* __restrict__ c,
+                            const uint64_t* __restrict__ lab, double* __restrict__ v) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        v[i] = sq64(x + i * D, c + lab[i] * D, D);
+}
+
+__global__ void k_f32_to_f64(const float* __restrict__ a, uint64_t n, double* __restrict__ b) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        b[i] = static_cast<double>(a[i]);
 }
 
 // CSC build: map[pos], vals[pos], per-cluster max squared radius (as u64 bits).
@@ -474,13 +596,38 @@ int padded_dim(uint64_t D) {
 
 using namespace tblaeq;
 
+extern "C" void tblaeq_kmeans_options_init(tblaeq_kmeans_options* o) {
+    if (!o) return;
+    o->max_iter = 7;
+    o->seed = 42;
+    o->seeding = 0;
+    o->oversampling = 0.5;
+    o->rounds = 4;
+}
+
 extern "C" int tblaeq_kmeans(tblaeq_ctx* ctx, const double* d_x, uint64_t n, uint64_t D, uint64_t k,
                              uint64_t max_iter, uint64_t seed, double* d_centers, uint64_t* d_labels,
                              uint64_t* stats) {
-    if (!ctx || !d_x || !d_centers || !d_labels || n == 0 || D == 0 || k == 0 || k > n) {
-        set_error("tblaeq_kmeans: invalid argument");
+    tblaeq_kmeans_options o;
+    tblaeq_kmeans_options_init(&o);
+    o.max_iter = max_iter;
+    o.seed = seed;
+    uint64_t st6[6] = {0, 0, 0, 0, 0, 0};
+    const int rc = tblaeq_kmeans_ex(ctx, d_x, n, D, k, &o, d_centers, d_labels, st6);
+    if (rc == TBLAEQ_OK && stats) std::copy(st6, st6 + 4, stats);
+    return rc;
+}
+
+extern "C" int tblaeq_kmeans_ex(tblaeq_ctx* ctx, const double* d_x, uint64_t n, uint64_t D, uint64_t k,
+                                const tblaeq_kmeans_options* opt, double* d_centers, uint64_t* d_labels,
+                                uint64_t* stats) {
+    if (!ctx || !d_x || !d_centers || !d_labels || !opt || n == 0 || D == 0 || k == 0 || k > n ||
+        (opt->seeding != 0 && opt->seeding != 1) || (opt->seeding == 1 && (!(opt->oversampling > 0.0) || opt->rounds == 0))) {
+        set_error("tblaeq_kmeans: invalid argument (1 <= k <= n; seeding 0 or 1; k-means|| needs oversampling > 0 "
+                  "and rounds >= 1)");
         return TBLAEQ_ERR_INVALID;
     }
+    const uint64_t max_iter = opt->max_iter, seed = opt->seed;
     const int DP = padded_dim(D);
     if (DP == 0 || k >= (1ull << 32)) {
         set_error("tblaeq_kmeans: D above 64 or k above 2^32 is not supported");
@@ -502,8 +649,6 @@ extern "C" int tblaeq_kmeans(tblaeq_ctx* ctx, const double* d_x, uint64_t n, uin
     double* u = S.get<double>(k, &rc);
     float* x32 = S.get<float>(n * DP, &rc);
     float* xn = S.get<float>(n, &rc);
-    float* c32 = S.get<float>(k * DP, &rc);
-    float* cn = S.get<float>(k, &rc);
     uint64_t* ctl = S.get<uint64_t>(8, &rc);  // [0] changed [1] nAmb [2] cmax bits [3..4] scan scratch
     uint64_t* amb = S.get<uint64_t>(n, &rc);
     uint64_t* keysA = S.get<uint64_t>(n, &rc);
@@ -533,7 +678,35 @@ extern "C" int tblaeq_kmeans(tblaeq_ctx* ctx, const double* d_x, uint64_t n, uin
         TB_HIP(hipMemcpyAsync(mid, m64.data(), 64 * sizeof(double), hipMemcpyHostToDevice, st));
         TB_HIP(hipStreamSynchronize(st));
     }
-    // random draws of the CPU build's k-means++ (CPUKmeans.cuh:67-118)
+    hipLaunchKernelGGL(k_km_points, dim3(grid_for(ctx, n, 256)), dim3(256), 0, st, d_x, n, static_cast<int>(D), DP, mid,
+                       x32, xn);
+    const float g = 2.f * static_cast<float>(DP + 8) * 5.9604645e-8f;  // 2 (D + 8) 2^-24
+    // Nearest of `m` centres (FP64, cent) for every point: the FP32 matrix-core
+    // ranking with the FP64 re-score of near ties, labels into lab (the
+    // assignment of the Lloyd iterations); c32 / cn hold at least m rows.
+    auto assign = [&](const double* cent, uint64_t m, float* c32, float* cn, uint64_t* lab) {
+        hipLaunchKernelGGL(k_km_zero, dim3(1), dim3(64), 0, st, ctl, 8);
+        hipLaunchKernelGGL(k_km_centers, dim3(grid_for(ctx, m, 256)), dim3(256), 0, st, cent, m, static_cast<int>(D),
+                           DP, mid, c32, cn, reinterpret_cast<uint32_t*>(ctl + 2));
+        AssignArgs A{};
+        A.n = n;
+        A.k = m;
+        A.x32 = x32;
+        A.xn = xn;
+        A.c32 = c32;
+        A.cn = cn;
+        A.cmax = reinterpret_cast<const uint32_t*>(ctl + 2);
+        A.g = g;
+        A.labels = lab;
+        A.amb = amb;
+        A.nAmb = ctl + 1;
+        A.changed = ctl;
+        launch_assign(DP, dim3(static_cast<unsigned>((n + kKmPoints - 1) / kKmPoints)), ctx, A);
+        hipLaunchKernelGGL(k_km_rescore, dim3(static_cast<unsigned>(ctx->cuCount) * 4), dim3(256), 0, st, d_x,
+                           static_cast<int>(D), cent, m, amb, ctl + 1, lab, ctl);
+    };
+    // random draws of the CPU build's k-means++ (CPUKmeans.cuh:67-118): the
+    // first centre, then one uniform draw per further pick
     std::mt19937_64 rng(seed);
     std::uniform_int_distribution<size_t> uni(0, n - 1);
     const uint64_t first = uni(rng);
@@ -546,21 +719,110 @@ extern "C" int tblaeq_kmeans(tblaeq_ctx* ctx, const double* d_x, uint64_t n, uin
     TB_HIP(hipMemcpyAsync(u, hu.data(), k * sizeof(double), hipMemcpyHostToDevice, st));
     TB_HIP(hipMemcpyAsync(dn, &hn, sizeof(uint64_t), hipMemcpyHostToDevice, st));
     TB_HIP(hipStreamSynchronize(st));
-    TB_HIP(hipMemcpyAsync(d_centers, d_x + first * D, D * sizeof(double), hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_km_fill_f64, dim3(grid_for(ctx, n, 256)), dim3(256), 0, st, minD, n, __DBL_MAX__);
-    const unsigned ppGrid = static_cast<unsigned>(std::min<uint64_t>(ntiles, static_cast<uint64_t>(ctx->cuCount) * 8));
-    for (uint64_t cc = 1; cc < k; ++cc) {
-        hipLaunchKernelGGL(k_kmpp_update, dim3(ppGrid), dim3(256), 0, st, d_x, n, static_cast<int>(D),
-                           d_centers + (cc - 1) * D, minD, tileSum);
-        hipLaunchKernelGGL(k_kmpp_pick, dim3(1), dim3(1024), 0, st, d_x, n, static_cast<int>(D), minD, tileSum, u, cc,
-                           d_centers);
+    // Seeding k-means++ over `m` points `pts` (optionally weighted) into
+    // d_centers, starting from the point `start`.
+    auto kmeanspp = [&](const double* pts, uint64_t m, const double* w, uint64_t start, double* mD, double* tS) {
+        TB_HIP(hipMemcpyAsync(d_centers, pts + start * D, D * sizeof(double), hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_km_fill_f64, dim3(grid_for(ctx, m, 256)), dim3(256), 0, st, mD, m, __DBL_MAX__);
+        const uint64_t mt = (m + kPpTile - 1) / kPpTile;
+        const unsigned ppGrid = static_cast<unsigned>(std::min<uint64_t>(mt, static_cast<uint64_t>(ctx->cuCount) * 8));
+        const uint64_t report = std::max<uint64_t>(1, k / 8);
+        for (uint64_t cc = 1; cc < k; ++cc) {
+            hipLaunchKernelGGL(k_kmpp_update, dim3(ppGrid), dim3(256), 0, st, pts, m, static_cast<int>(D),
+                               d_centers + (cc - 1) * D, mD, tS, w);
+            hipLaunchKernelGGL(k_kmpp_pick, dim3(1), dim3(1024), 0, st, pts, m, static_cast<int>(D), mD, tS, u, cc,
+                               d_centers, w);
+            if (cc % report == 0 && k >= 4096) {
+                TB_HIP(hipStreamSynchronize(st));
+                progress("k-means seeding: %llu of %llu centres picked", static_cast<unsigned long long>(cc),
+                         static_cast<unsigned long long>(k));
+            }
+        }
+        TB_HIP(hipGetLastError());
+        return TBLAEQ_OK;
+    };
+    uint64_t candidates = 0;
+    if (opt->seeding == 1) {
+        // k-means||: rounds of independent D^2-weighted draws (expected
+        // oversampling * k per round) from a counter-based stream, each
+        // round's draws in point order; the nearest candidate of every point
+        // is tracked, the candidates weighted by the points nearest to them,
+        // then weighted k-means++ on the candidates picks the k centres
+        const double ell = opt->oversampling * static_cast<double>(k);
+        uint64_t capC = 1 + opt->rounds * (static_cast<uint64_t>(2.0 * ell) + 1024);
+        double* cand = S.get<double>(capC * D, &rc);
+        uint64_t* nearest = S.get<uint64_t>(n, &rc);
+        uint64_t* flag = S.get<uint64_t>(n, &rc);
+        uint64_t* foff = S.get<uint64_t>(n + 1, &rc);
+        uint64_t* lab = S.get<uint64_t>(n, &rc);
+        double* phi = S.get<double>(1, &rc);
+        uint64_t* scr = S.get<uint64_t>(2, &rc);
+        if (rc) return rc;
+        TB_HIP(hipMemcpyAsync(cand, d_x + first * D, D * sizeof(double), hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_kmpar_init, dim3(grid_for(ctx, n, 256)), dim3(256), 0, st, d_x, n, static_cast<int>(D),
+                           cand, minD, nearest);
+        uint64_t m = 1;
+        const unsigned tGrid = static_cast<unsigned>(std::min<uint64_t>(ntiles, static_cast<uint64_t>(ctx->cuCount) * 8));
+        float* c32r = nullptr;
+        float* cnr = nullptr;
+        uint64_t capR = 0;
+        for (uint64_t r = 0; r < opt->rounds; ++r) {
+            hipLaunchKernelGGL(k_tile_sum, dim3(tGrid), dim3(256), 0, st, minD, n, tileSum);
+            hipLaunchKernelGGL(k_sum_tiles, dim3(1), dim3(1024), 0, st, tileSum, ntiles, phi);
+            hipLaunchKernelGGL(k_kmpar_mark, dim3(grid_for(ctx, n, 256)), dim3(256), 0, st, minD, n, phi, ell, seed,
+                               r, flag);
+            if ((rc = scan_counts(ctx, flag, n, foff, scr))) return rc;
+            uint64_t drawn = 0;
+            TB_HIP(hipMemcpyAsync(&drawn, foff + n, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+            TB_HIP(hipStreamSynchronize(st));
+            if (drawn == 0) break;
+            if (m + drawn > capC) {
+                // more draws than the expected bound: grow the candidate array
+                const uint64_t nc = std::max(capC * 2, m + drawn);
+                double* bigger = S.get<double>(nc * D, &rc);
+                if (rc) return rc;
+                TB_HIP(hipMemcpyAsync(bigger, cand, m * D * sizeof(double), hipMemcpyDeviceToDevice, st));
+                cand = bigger;
+                capC = nc;
+            }
+            hipLaunchKernelGGL(k_kmpar_gather, dim3(grid_for(ctx, n, 256)), dim3(256), 0, st, d_x, n,
+                               static_cast<int>(D), flag, foff, cand + m * D);
+            if (drawn > capR) {
+                c32r = S.get<float>(drawn * DP, &rc);
+                cnr = S.get<float>(drawn, &rc);
+                if (rc) return rc;
+                capR = drawn;
+            }
+            hipLaunchKernelGGL(k_km_zero, dim3(grid_for(ctx, n, 256)), dim3(256), 0, st, lab, n);
+            assign(cand + m * D, drawn, c32r, cnr, lab);
+            hipLaunchKernelGGL(k_kmpar_improve, dim3(grid_for(ctx, n, 256)), dim3(256), 0, st, d_x, n,
+                               static_cast<int>(D), cand + m * D, lab, m, minD, nearest);
+            TB_HIP(hipGetLastError());
+            m += drawn;
+            TB_HIP(hipStreamSynchronize(st));
+            progress("k-means|| round %llu of %llu: %llu candidates drawn, %llu in all",
+                     static_cast<unsigned long long>(r + 1), static_cast<unsigned long long>(opt->rounds),
+                     static_cast<unsigned long long>(drawn), static_cast<unsigned long long>(m));
+        }
+        if (m > k) {
+            uint64_t* wcnt = S.get<uint64_t>(m, &rc);
+            double* w = S.get<double>(m, &rc);
+            double* cMinD = S.get<double>(m, &rc);
+            double* cTile = S.get<double>((m + kPpTile - 1) / kPpTile, &rc);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_km_zero, dim3(grid_for(ctx, m, 256)), dim3(256), 0, st, wcnt, m);
+            hipLaunchKernelGGL(k_kmpar_weights, dim3(grid_for(ctx, n, 256)), dim3(256), 0, st, nearest, n, wcnt);
+            hipLaunchKernelGGL(k_u64_to_f64, dim3(grid_for(ctx, m, 256)), dim3(256), 0, st, wcnt, m, w);
+            if ((rc = kmeanspp(cand, m, w, 0, cMinD, cTile))) return rc;
+            candidates = m;
+        }
     }
-    TB_HIP(hipGetLastError());
+    if (candidates == 0 && (rc = kmeanspp(d_x, n, nullptr, first, minD, tileSum))) return rc;
     // Lloyd
-    hipLaunchKernelGGL(k_km_points, dim3(grid_for(ctx, n, 256)), dim3(256), 0, st, d_x, n, static_cast<int>(D), DP, mid,
-                       x32, xn);
+    float* c32 = S.get<float>(k * DP, &rc);
+    float* cn = S.get<float>(k, &rc);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_km_zero, dim3(grid_for(ctx, n, 256)), dim3(256), 0, st, d_labels, n);
-    const float g = 2.f * static_cast<float>(DP + 8) * 5.9604645e-8f;  // 2 (D + 8) 2^-24
     const uint64_t idxMask = (1ull << idxBits) - 1;
     uint64_t it = 0, rescored = 0;
     double assignMs = 0.0;
@@ -568,26 +830,8 @@ extern "C" int tblaeq_kmeans(tblaeq_ctx* ctx, const double* d_x, uint64_t n, uin
     TB_HIP(hipEventCreate(&e0));
     TB_HIP(hipEventCreate(&e1));
     for (; it < max_iter;) {
-        hipLaunchKernelGGL(k_km_zero, dim3(1), dim3(64), 0, st, ctl, 8);
-        hipLaunchKernelGGL(k_km_centers, dim3(grid_for(ctx, k, 256)), dim3(256), 0, st, d_centers, k,
-                           static_cast<int>(D), DP, mid, c32, cn, reinterpret_cast<uint32_t*>(ctl + 2));
-        AssignArgs A{};
-        A.n = n;
-        A.k = k;
-        A.x32 = x32;
-        A.xn = xn;
-        A.c32 = c32;
-        A.cn = cn;
-        A.cmax = reinterpret_cast<const uint32_t*>(ctl + 2);
-        A.g = g;
-        A.labels = d_labels;
-        A.amb = amb;
-        A.nAmb = ctl + 1;
-        A.changed = ctl;
         TB_HIP(hipEventRecord(e0, st));
-        launch_assign(DP, dim3(static_cast<unsigned>((n + kKmPoints - 1) / kKmPoints)), ctx, A);
-        hipLaunchKernelGGL(k_km_rescore, dim3(static_cast<unsigned>(ctx->cuCount) * 4), dim3(256), 0, st, d_x,
-                           static_cast<int>(D), d_centers, k, amb, ctl + 1, d_labels, ctl);
+        assign(d_centers, k, c32, cn, d_labels);
         TB_HIP(hipEventRecord(e1, st));
         // update: sort (label, point) keys by label, sum each cluster in point order
         hipLaunchKernelGGL(k_km_zero, dim3(grid_for(ctx, k, 256)), dim3(256), 0, st, cnt, k);
@@ -606,17 +850,43 @@ extern "C" int tblaeq_kmeans(tblaeq_ctx* ctx, const double* d_x, uint64_t n, uin
         assignMs += ms;
         rescored += hc[1];
         ++it;
+        if (k >= 4096 || n >= (1ull << 24))
+            progress("k-means Lloyd iteration %llu of %llu: %llu labels changed, assignment %.1f ms",
+                     static_cast<unsigned long long>(it), static_cast<unsigned long long>(max_iter),
+                     static_cast<unsigned long long>(hc[0]), static_cast<double>(ms));
         if (hc[0] == 0) break;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     if (stats) {
+        // inertia: per point squared distance to its centroid, summed per
+        // 1024-point tile and then over the tiles in a fixed order
+        hipLaunchKernelGGL(k_km_sqdist, dim3(grid_for(ctx, n, 256)), dim3(256), 0, st, d_x, n, static_cast<int>(D),
+                           d_centers, d_labels, minD);
+        const unsigned tGrid = static_cast<unsigned>(std::min<uint64_t>(ntiles, static_cast<uint64_t>(ctx->cuCount) * 8));
+        hipLaunchKernelGGL(k_tile_sum, dim3(tGrid), dim3(256), 0, st, minD, n, tileSum);
+        hipLaunchKernelGGL(k_sum_tiles, dim3(1), dim3(1024), 0, st, tileSum, ntiles, u);
+        double inertia = 0.0;
+        TB_HIP(hipMemcpyAsync(&inertia, u, sizeof(double), hipMemcpyDeviceToHost, st));
+        TB_HIP(hipStreamSynchronize(st));
         stats[0] = it;
         stats[1] = rescored;
         stats[2] = it * n * k;
         stats[3] = static_cast<uint64_t>(assignMs);
+        std::memcpy(&stats[4], &inertia, sizeof(double));
+        stats[5] = candidates;
     }
+    return TBLAEQ_OK;
+}
+
+extern "C" int tblaeq_convert_f32_f64(tblaeq_ctx* ctx, const float* d_in, uint64_t n, double* d_out) {
+    if (!ctx || (n && (!d_in || !d_out))) {
+        set_error("tblaeq_convert_f32_f64: invalid argument");
+        return TBLAEQ_ERR_INVALID;
+    }
+    if (n) hipLaunchKernelGGL(k_f32_to_f64, dim3(grid_for(ctx, n, 256)), dim3(256), 0, ctx->stream, d_in, n, d_out);
+    TB_HIP(hipGetLastError());
     return TBLAEQ_OK;
 }
 

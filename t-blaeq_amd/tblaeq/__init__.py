@@ -60,6 +60,12 @@ SIGNATURES = {
     "tblaeq_kmeans_host": (C.c_int, [vp, f64p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, f64p,
                                      u64p, u64p]),
     "tblaeq_build_level": (C.c_int, [vp, vp, C.c_uint64, C.c_uint64, vp, C.c_uint64, vp, vp, vp, vp, vp, vp]),
+    "tblaeq_kmeans_options_init": (None, [vp]),
+    "tblaeq_kmeans_ex": (C.c_int, [vp, vp, C.c_uint64, C.c_uint64, C.c_uint64, vp, vp, vp, u64p]),
+    "tblaeq_convert_f32_f64": (C.c_int, [vp, vp, C.c_uint64, vp]),
+    "tblaeq_handler_build_ex": (C.c_int, [vp, C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, u64p, C.c_char_p,
+                                          C.c_int, vp, C.POINTER(vp)]),
+    "tblaeq_set_progress": (None, [vp, vp]),
     "tblaeq_handler_build": (C.c_int, [f64p, C.c_uint64, C.c_uint64, C.c_uint64, u64p, C.c_char_p, C.c_int,
                                        C.POINTER(vp)]),
     "tblaeq_handler_build_stats": (C.c_int, [vp, u64p, C.c_uint64, u64p]),
@@ -266,24 +272,38 @@ class Handler(_Closing):
         return cls(h)
 
     @classmethod
-    def build(cls, data, ratios=(100, 50, 20), name="in-memory", force_cpu=False):
+    def build(cls, data, ratios=(100, 50, 20), name="in-memory", force_cpu=False, **options):
         """QueryHandler(forceCPU, PointCloud, name, height, ratios): k-means build, on the GPU unless
-        force_cpu (IndexBuilder.cu:44-151)."""
-        data = np.ascontiguousarray(data, np.float64)
+        force_cpu (IndexBuilder.cu:44-151).  FP32 data, or k-means options (max_iter, seed, seeding 0 =
+        k-means++ / 1 = k-means||, oversampling, rounds), go through tblaeq_handler_build_ex."""
         r = np.asarray(ratios, np.uint64)
         h = vp()
+        if np.asarray(data).dtype == np.float32 or options:
+            f32 = np.asarray(data).dtype == np.float32
+            data = np.ascontiguousarray(data, np.float32 if f32 else np.float64)
+            opt = KmeansOptions.make(**options)
+            check(lib().tblaeq_handler_build_ex(data.ctypes.data_as(C.c_void_p), int(f32), data.shape[0],
+                                                data.shape[1], len(r) + 1, _p(r, u64p), name.encode(),
+                                                int(bool(force_cpu)), C.byref(opt), C.byref(h)), "handler_build_ex")
+            return cls(h)
+        data = np.ascontiguousarray(data, np.float64)
         check(lib().tblaeq_handler_build(_p(data, f64p), data.shape[0], data.shape[1], len(r) + 1, _p(r, u64p),
                                          name.encode(), int(bool(force_cpu)), C.byref(h)), "handler_build")
         return cls(h)
 
     def build_stats(self):
-        """Per level of a GPU build (finest first): Lloyd iterations, points re-scored in FP64, FP32
-        matrix-core score evaluations, milliseconds in the assignment kernels."""
+        """Per level of a build (finest first): Lloyd iterations, points re-scored in FP64, FP32 matrix-core
+        score evaluations, milliseconds in the assignment kernels, inertia bits, k-means|| candidates (the
+        host build fills iterations and inertia)."""
         n = C.c_uint64(0)
         check(lib().tblaeq_handler_build_stats(self.h, None, 0, C.byref(n)), "build_stats")
         out = np.zeros(n.value, np.uint64)
         check(lib().tblaeq_handler_build_stats(self.h, _p(out, u64p), n.value, C.byref(n)), "build_stats")
-        return out.reshape(-1, 4)
+        return out.reshape(-1, 6)
+
+    def build_inertia(self):
+        """Per level of a build (finest first): the k-means inertia."""
+        return self.build_stats()[:, 4].copy().view(np.float64)
 
     def kmeans(self, x, k, max_iter=7, seed=42):
         """tblaeq_kmeans on host arrays through this handler's context: (labels, centers, stats)."""
@@ -604,6 +624,36 @@ class Mlp(_Closing):
         if getattr(self, "m", None) and _LIB is not None:
             _LIB.tblaeq_mlp_destroy(self.m)
             self.m = None
+
+
+class KmeansOptions(C.Structure):
+    """tblaeq_kmeans_options (tblaeq_build.h)."""
+    _fields_ = [("max_iter", C.c_uint64), ("seed", C.c_uint64), ("seeding", C.c_int), ("oversampling", C.c_double),
+                ("rounds", C.c_uint64)]
+
+    @classmethod
+    def make(cls, **kw):
+        o = cls()
+        lib().tblaeq_kmeans_options_init(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+
+_PROGRESS_FN = C.CFUNCTYPE(None, C.c_char_p, C.c_void_p)
+_progress_keep = None
+
+
+def set_progress(fn):
+    """fn(message) receives the library's progress messages (long builds); None turns them off."""
+    global _progress_keep
+    if fn is None:
+        lib().tblaeq_set_progress(None, None)
+        _progress_keep = None
+        return
+    cb = _PROGRESS_FN(lambda msg, user: fn(msg.decode()))
+    _progress_keep = cb
+    lib().tblaeq_set_progress(C.cast(cb, C.c_void_p), None)
 
 
 class ServerConfig(C.Structure):

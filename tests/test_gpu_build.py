@@ -101,7 +101,7 @@ def test_gpu_build_index_equals_cpu_build(gen, tmp_path):
     hg = tblaeq.Handler.build(data, (30, 10), "g", force_cpu=False)
     hc = tblaeq.Handler.build(data, (30, 10), "c", force_cpu=True)
     st = hg.build_stats()
-    assert st.shape == (2, 4) and np.all(st[:, 0] >= 1)
+    assert st.shape == (2, 6) and np.all(st[:, 0] >= 1)
     hg.save(str(tmp_path / "g"))
     hc.save(str(tmp_path / "c"))
     g, c = _read_index(str(tmp_path / "g")), _read_index(str(tmp_path / "c"))
@@ -156,4 +156,74 @@ def test_gpu_built_index_answers_like_the_oracle(gen, tmp_path):
     for i, (gid, gd2) in enumerate(h.batch_results()):
         eid, ed2 = o.knn(pts[i], 10).refine_knn(pts[i], 10)
         assert np.array_equal(gid, eid) and np.array_equal(gd2.view(np.uint64), ed2.view(np.uint64))
+    h.close()
+
+
+def _clustered(n, D, seed, sigma=1.5):
+    rng = np.random.default_rng(seed)
+    cen = rng.uniform(1.0, 100.0, (max(1, n // 60), D))
+    return np.ascontiguousarray(cen[rng.integers(0, cen.shape[0], n)] + rng.normal(0.0, sigma, (n, D)))
+
+
+def test_gpu_build_options_and_fp32_input_match_host_build(tmp_path):
+    """The index-build configuration's settings (20 iterations, seed 7) on FP32 input at D = 32: the
+    GPU build (FP32 values converted to FP64 on the device) gives the host build's mesh sizes,
+    coarsest mesh and radii, and the same inertia to within the summation order."""
+    x32 = _clustered(20000, 32, 5).astype(np.float32)
+    g = tblaeq.Handler.build(x32, (50, 10), "c4", force_cpu=False, max_iter=20, seed=7)
+    c = tblaeq.Handler.build(x32, (50, 10), "c4", force_cpu=True, max_iter=20, seed=7)
+    gs, cs = g.build_stats(), c.build_stats()
+    assert np.array_equal(gs[:, 0], cs[:, 0])  # Lloyd iterations per level
+    np.testing.assert_allclose(g.build_inertia(), c.build_inertia(), rtol=1e-12)
+    g.save(str(tmp_path / "g"))
+    c.save(str(tmp_path / "c"))
+    ig, ic = _read_index(str(tmp_path / "g")), _read_index(str(tmp_path / "c"))
+    assert np.array_equal(ig["mesh"], ic["mesh"])
+    assert np.array_equal(ig["cvals"].view(np.uint64), ic["cvals"].view(np.uint64))
+    for l in range(len(ig["R"])):
+        assert np.array_equal(ig["R"][l].view(np.uint64), ic["R"][l].view(np.uint64)), l
+    g.close()
+    c.close()
+
+
+def test_gpu_kmeans_parallel_seeding(tmp_path):
+    """k-means|| seeding: deterministic (two builds give the same index), more candidates than
+    centres, an inertia close to k-means++'s on the same data, and an index that answers queries
+    exactly like the oracle over its saved files."""
+    x = _clustered(60000, 8, 6)
+    opts = dict(max_iter=20, seed=7, seeding=1, oversampling=0.5, rounds=4)
+    a = tblaeq.Handler.build(x, (60, 10), "kpar", **opts)
+    b = tblaeq.Handler.build(x, (60, 10), "kpar", **opts)
+    pp = tblaeq.Handler.build(x, (60, 10), "kpp", max_iter=20, seed=7)
+    sa, sb = a.build_stats(), b.build_stats()
+    assert np.array_equal(sa, sb)
+    assert sa[0, 5] > 1000  # candidates of the first level (k = 1000)
+    assert pp.build_stats()[0, 5] == 0
+    ia, ip = a.build_inertia()[0], pp.build_inertia()[0]
+    assert ia <= 1.1 * ip, (ia, ip)
+    a.save(str(tmp_path / "a"))
+    o = O.Index.load(str(tmp_path / "a"))
+    a.prepare("L0")
+    rng = np.random.default_rng(9)
+    for i in range(6):
+        c = x[rng.integers(x.shape[0])]
+        lo, hi = c - 3.0, c + 3.0
+        r = a.range(lo, hi)
+        assert np.array_equal(r.refined()[0], o.range(lo, hi).refine_range(lo, hi)), i
+        k = a.knn(c + 0.3, 15)
+        eid, ed2 = o.knn(c + 0.3, 15).refine_knn(c + 0.3, 15)
+        rid, rd2 = k.refined()
+        assert np.array_equal(rid, eid) and np.array_equal(rd2.view(np.uint64), ed2.view(np.uint64)), i
+        r.close()
+        k.close()
+    for hh in (a, b, pp):
+        hh.close()
+
+
+def test_kmeans_inertia_stat(gen):
+    x = _clustered(30000, 6, 7)
+    h = tblaeq.Handler.build(x, (100,), "inertia", max_iter=10, seed=3)
+    lab, cen, _ = O.kmeans(x, 300, 10, seed=3)
+    d = x - cen[lab.astype(np.int64)]
+    np.testing.assert_allclose(h.build_inertia()[0], float(np.sum(d * d)), rtol=1e-12)
     h.close()

@@ -79,3 +79,43 @@ def test_scheduler_counts_work_buffers():
         if prev is not None:
             assert all(c <= p or p == 2 for c, p in zip(cur, prev))
         prev = cur
+
+
+def _clustered(n, D, seed):
+    rng = np.random.default_rng(seed)
+    cen = rng.uniform(1.0, 100.0, (max(1, n // 50), D))
+    return np.ascontiguousarray(cen[rng.integers(0, cen.shape[0], n)] + rng.normal(0.0, 1.5, (n, D)))
+
+
+def test_host_build_options_match_oracle_kmeans():
+    """The host build with other iteration and seed settings (the index-build configuration's 20
+    iterations from seed 7) equals the oracle's k-means with the same settings, and its reported
+    inertia is that clustering's."""
+    x = _clustered(6000, 5, 1)
+    h = tblaeq.Handler.build(x, (40,), "opt", force_cpu=True, max_iter=20, seed=7)
+    lab, cen, it = O.kmeans(x, 150, 20, seed=7)
+    st = h.build_stats()
+    assert st.shape == (1, 6) and int(st[0, 0]) == it
+    d = x - cen[lab.astype(np.int64)]
+    np.testing.assert_allclose(h.build_inertia()[0], float(np.sum(d * d)), rtol=1e-12)
+    # another seed gives another clustering
+    lab2, _, _ = O.kmeans(x, 150, 20, seed=42)
+    assert not np.array_equal(lab, lab2)
+    h.close()
+
+
+def test_host_build_fp32_input_equals_fp64_of_it(tmp_path):
+    x32 = _clustered(3000, 4, 2).astype(np.float32)
+    a = tblaeq.Handler.build(x32, (30,), "f32", force_cpu=True)
+    b = tblaeq.Handler.build(x32.astype(np.float64), (30,), "f32", force_cpu=True, max_iter=7)
+    a.save(str(tmp_path / "a"))
+    b.save(str(tmp_path / "b"))
+    for f in sorted(os.listdir(tmp_path / "a")):
+        assert filecmp.cmp(str(tmp_path / "a" / f), str(tmp_path / "b" / f), shallow=False), f
+    a.close()
+    b.close()
+
+
+def test_host_build_rejects_kmeans_parallel_seeding():
+    with pytest.raises(tblaeq.TblaeqError, match="GPU only"):
+        tblaeq.Handler.build(_clustered(500, 3, 3), (10,), force_cpu=True, seeding=1)
