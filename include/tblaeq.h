@@ -272,7 +272,11 @@ int tblaeq_batch_knn_graph(tblaeq_batch* b, const double* d_points, const uint64
  *   TBLAEQ_PARAM_STEP_BLOCKS_PER_CU   cluster-major KNN kernels, 1..64 (16)
  *   TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU  hit radix sort, 1..32 (4)
  *   TBLAEQ_PARAM_HIT_STAGE          LDS hit stage per wave of the final range
- *                                   filter: 256, 512 or 1024 (512) */
+ *                                   filter: 256, 512 or 1024 (512)
+ *   TBLAEQ_PARAM_RANGE_ROW_QUERIES  last-level clusters kept by at most this
+ *                                   many queries are filtered row-major (lane =
+ *                                   point, one query at a time), the others
+ *                                   query-major, 0..64 (16; 0: all query-major) */
 enum {
     TBLAEQ_PARAM_KNN_SEED = 1,
     TBLAEQ_PARAM_KNN_WG_PER_CU = 2,
@@ -280,7 +284,8 @@ enum {
     TBLAEQ_PARAM_RANGE_BLOCKS_PER_CU = 4,
     TBLAEQ_PARAM_STEP_BLOCKS_PER_CU = 5,
     TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU = 6,
-    TBLAEQ_PARAM_HIT_STAGE = 7
+    TBLAEQ_PARAM_HIT_STAGE = 7,
+    TBLAEQ_PARAM_RANGE_ROW_QUERIES = 8
 };
 int tblaeq_batch_set_param(tblaeq_batch* b, int param, uint64_t value);
 int tblaeq_batch_get_param(const tblaeq_batch* b, int param, uint64_t* value);

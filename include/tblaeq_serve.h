@@ -20,6 +20,11 @@
  * until memory frees up.  At least one request is always admitted so the
  * queue drains.
  *
+ * Launch parameters: the server's engines take those of the handler's batched
+ * engine (batch slot 0) when it has one, e.g. after
+ * tblaeq_handler_batch_autotune on a sample of the expected requests;
+ * otherwise the engine defaults.
+ *
  * Destroy the server before the handler it was created from.
  */
 #ifndef TBLAEQ_SERVE_H

@@ -42,6 +42,7 @@ struct tblaeq_batch {
     float* leafBox = nullptr;    // [leaves * 2D] outward-rounded float box lo[D] | hi[D] of every kd leaf
     uint64_t nLeaves = 0;
     float* fbox = nullptr;       // [maxQ * 2 * dm] outward float boxes of the current range batch
+    uint64_t* sparse = nullptr;  // last-level clusters of the row-major exact filter, per scan tile, and counts
     double* enclC = nullptr;     // [count[L-2] * D] (L >= 2) centre of each level-(L-2) cluster, by id
     double* enclR = nullptr;     // [count[L-2]] radius of a ball around it holding its children's balls
     float* enclBox = nullptr;    // [count[L-2] * 2D] box lo[D] | hi[D] holding its children's balls, by id
@@ -91,6 +92,7 @@ struct tblaeq_batch {
     uint64_t stepBlocksPerCu = 16;    // cluster-major KNN passes and refine
     uint64_t radixBlocksPerCu = 4;    // hit radix sort
     uint64_t hitStage = 512;          // LDS hits staged per wave by the final filter (256, 512, 1024)
+    uint64_t rangeRowMax = 16;        // final filter: clusters kept by at most this many queries go row-major
     uint64_t paramVersion = 0;        // bumped by every parameter change (captured graphs re-capture)
     uint64_t gParamVersion = 0;
 };

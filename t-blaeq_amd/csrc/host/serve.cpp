@@ -619,6 +619,19 @@ int tblaeq_server_create(tblaeq_handler* h, const tblaeq_server_config* cfgIn, t
                 w.hD2.alloc(rows);
             }
         }
+        // launch parameters: those of the handler's batched engine when it has
+        // one (tuned with tblaeq_handler_batch_autotune, or set by hand)
+        if (h->slots[0] && h->slots[0]->b) {
+            static const int kParamIds[] = {TBLAEQ_PARAM_KNN_SEED,           TBLAEQ_PARAM_KNN_WG_PER_CU,
+                                            TBLAEQ_PARAM_KNN_MODE,           TBLAEQ_PARAM_RANGE_BLOCKS_PER_CU,
+                                            TBLAEQ_PARAM_STEP_BLOCKS_PER_CU, TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU,
+                                            TBLAEQ_PARAM_HIT_STAGE,          TBLAEQ_PARAM_RANGE_ROW_QUERIES};
+            for (int id : kParamIds) {
+                uint64_t v = 0;
+                tbCheck(tblaeq_batch_get_param(h->slots[0]->b, id, &v), "serve: launch parameter");
+                for (int k = 0; k < 2; ++k) tbCheck(tblaeq_batch_set_param(s->w[k].b, id, v), "serve: launch parameter");
+            }
+        }
         hipCheck(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
         s->budget = static_cast<uint64_t>(static_cast<double>(freeB) * cfg.hbm_fraction);
         s->cfg = cfg;

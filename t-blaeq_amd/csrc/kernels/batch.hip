@@ -135,6 +135,13 @@ struct RangeStepArgs {
     // the box test already failed for every lane (the statistics count ball
     // survivors); the kept pairs are ball AND box either way
     int ballStats;
+    // final filter: clusters kept by at most rowMax queries are listed in
+    // sparse (*nSparse of them) for the row-major kernel (k_range_rows, items
+    // from queue + 1) and get no query-major work items
+    uint64_t rowMax;
+    const uint64_t* sparse;     // [sparseTiles * kScanThreads] (tile t's clusters from t * kScanThreads)
+    const uint64_t* sparseCnt;  // [sparseTiles] clusters in each tile's slots
+    uint64_t sparseTiles;
 };
 
 // Appends a wave's staged hits to the global hit list (entries past the
@@ -754,6 +761,218 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_final(RangeStepArg
     }
 }
 
+// Lanes of m whose row x lies in the box [lo, hi] (all in VGPRs; the box the
+// same in every lane) in eight dimensions: the v_cmpx chain of box_mask8 with
+// the operands' roles swapped (lo <= x and hi >= x, false for NaN as x >= lo
+// && x <= hi).
+__device__ __forceinline__ uint64_t rows_in_box8(uint64_t m, const double* x, const double* lo, const double* hi) {
+    uint64_t out, save;
+    asm volatile(
+        "s_mov_b64 %[sv], exec\n\t"
+        "s_and_b64 exec, exec, %[m]\n\t"
+        "v_cmpx_le_f64 vcc, %[l0], %[x0]\n\t"
+        "v_cmpx_ge_f64 vcc, %[h0], %[x0]\n\t"
+        "v_cmpx_le_f64 vcc, %[l1], %[x1]\n\t"
+        "v_cmpx_ge_f64 vcc, %[h1], %[x1]\n\t"
+        "v_cmpx_le_f64 vcc, %[l2], %[x2]\n\t"
+        "v_cmpx_ge_f64 vcc, %[h2], %[x2]\n\t"
+        "v_cmpx_le_f64 vcc, %[l3], %[x3]\n\t"
+        "v_cmpx_ge_f64 vcc, %[h3], %[x3]\n\t"
+        "v_cmpx_le_f64 vcc, %[l4], %[x4]\n\t"
+        "v_cmpx_ge_f64 vcc, %[h4], %[x4]\n\t"
+        "v_cmpx_le_f64 vcc, %[l5], %[x5]\n\t"
+        "v_cmpx_ge_f64 vcc, %[h5], %[x5]\n\t"
+        "v_cmpx_le_f64 vcc, %[l6], %[x6]\n\t"
+        "v_cmpx_ge_f64 vcc, %[h6], %[x6]\n\t"
+        "v_cmpx_le_f64 vcc, %[l7], %[x7]\n\t"
+        "v_cmpx_ge_f64 vcc, %[h7], %[x7]\n\t"
+        "s_mov_b64 %[out], exec\n\t"
+        "s_mov_b64 exec, %[sv]"
+        : [out] "=&s"(out), [sv] "=&s"(save)
+        : [m] "s"(m), [x0] "v"(x[0]), [x1] "v"(x[1]), [x2] "v"(x[2]), [x3] "v"(x[3]), [x4] "v"(x[4]),
+          [x5] "v"(x[5]), [x6] "v"(x[6]), [x7] "v"(x[7]), [l0] "v"(lo[0]), [l1] "v"(lo[1]), [l2] "v"(lo[2]),
+          [l3] "v"(lo[3]), [l4] "v"(lo[4]), [l5] "v"(lo[5]), [l6] "v"(lo[6]), [l7] "v"(lo[7]), [h0] "v"(hi[0]),
+          [h1] "v"(hi[1]), [h2] "v"(hi[2]), [h3] "v"(hi[3]), [h4] "v"(hi[4]), [h5] "v"(hi[5]), [h6] "v"(hi[6]),
+          [h7] "v"(hi[7])
+        : "vcc");
+    return out;
+}
+template <int DM>
+__device__ __forceinline__ uint64_t rows_in_box(uint64_t m, const double* x, const double* lo, const double* hi) {
+    if constexpr (DM == 4) {
+        const double xx[8] = {x[0], x[1], x[2], x[3], 0.0, 0.0, 0.0, 0.0};
+        const double ll[8] = {lo[0], lo[1], lo[2], lo[3], 0.0, 0.0, 0.0, 0.0};
+        const double hh[8] = {hi[0], hi[1], hi[2], hi[3], 0.0, 0.0, 0.0, 0.0};
+        return rows_in_box8(m, xx, ll, hh);
+    } else {
+#pragma unroll
+        for (int d = 0; d < DM; d += 8) m = rows_in_box8(m, x + d, lo + d, hi + d);
+        return m;
+    }
+}
+
+// The exact filter, row-major, for last-level clusters kept by at most rowMax
+// queries (the k_range_final kernel skips them).  Lane = query wastes most of
+// a wave on such clusters, and their rows are read for few tests, so here one
+// wave takes the whole cluster (its one slice of queries): first lane = leaf,
+// every query's float box (scalar) against up to 64 leaf boxes, giving each
+// query a mask of the leaves that meet it; then for every run of 64 rows (four
+// leaves) that meets some query, lane = row, the rows loaded once with vector
+// loads and tested against each query whose mask holds one of the four leaves
+// (only the rows of those leaves, by the FP64 v_cmpx chain with the box in
+// SGPRs).  Rows of leaves no query meets are never read.  Same predicate and
+// hits as k_range_final.
+// Queries per cluster the row-major filter holds (their FP64 boxes in LDS,
+// 4 KB per wave).
+constexpr int rows_max_queries(int DM) { return 256 / DM; }
+// Slots of the row-major filter's cluster list for a level of n clusters (one
+// group-scan tile of kScanThreads clusters per kScanThreads slots); the
+// tiles' counts follow the slots.
+constexpr uint64_t sparse_slots(uint64_t n) { return (n + kScanThreads - 1) / kScanThreads * kScanThreads; }
+
+template <int DM, bool EXACT, int STAGE>
+__global__ __launch_bounds__(kStepWaves * kWave) void k_range_rows(RangeStepArgs A) {
+    constexpr int kMaxQ = rows_max_queries(DM);
+    __shared__ uint64_t s_hit[kStepWaves][STAGE];
+    __shared__ uint64_t s_lm[kStepWaves][kWave];
+    __shared__ double s_box[kStepWaves][kMaxQ][2 * DM];
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kStepWaves;
+    const int D = EXACT ? DM : A.D;
+    // work units of kRowsUnit list slots (a tile's slots split in kScanThreads / kRowsUnit)
+    constexpr uint64_t kRowsUnit = 16;
+    const uint64_t nItems = A.sparseTiles * (kScanThreads / kRowsUnit);
+    uint64_t* stage = s_hit[wave];
+    uint64_t* lm = s_lm[wave];
+    double(*bx)[2 * DM] = s_box[wave];
+    uint32_t staged = 0;
+    uint64_t wTests = 0, wRows = 0, wLeafTests = 0;
+    const uint64_t chunk = queue_chunk(nItems, nw);
+    uint64_t ib, ie;
+    while (next_items(A.queue + 1, chunk, nItems, lane, &ib, &ie))
+    for (uint64_t item = ib * kRowsUnit, itemEnd = ie * kRowsUnit; item < itemEnd; ++item) {
+        if ((item & (kScanThreads - 1)) >= ((ConstU64)A.sparseCnt)[item / kScanThreads]) {
+            item |= kRowsUnit - 1;  // the rest of this unit's slots are empty
+            continue;
+        }
+        const uint64_t c = ((ConstU64)A.sparse)[item];
+        const uint64_t g0 = ((ConstU64)A.off)[c];
+        const uint64_t gn = ((ConstU64)A.off)[c + 1] - g0;
+        if (gn == 0 || gn > A.rowMax) continue;
+        // queries whose list entry was stored (an overflowed list drops the rest;
+        // the batch is then rerun)
+        const uint32_t nQ = static_cast<uint32_t>(g0 >= A.qcap ? 0 : (A.qcap - g0 < gn ? A.qcap - g0 : gn));
+        const uint64_t jb = ((ConstU64)A.colResPrev)[c], je = ((ConstU64)A.colResPrev)[c + 1];
+        if (nQ == 0 || jb >= je) continue;
+        // lane k < nQ: query k of the cluster; its FP64 box into LDS
+        const uint32_t qv = static_cast<uint32_t>(lane) < nQ ? A.qlist[g0 + lane] : 0u;
+        if (static_cast<uint32_t>(lane) < nQ) {
+            const double* b = A.boxes + static_cast<uint64_t>(qv) * 2 * D;
+#pragma unroll
+            for (int d = 0; d < DM; ++d) {
+                bx[lane][d] = (EXACT || d < D) ? b[d] : 0.0;
+                bx[lane][DM + d] = (EXACT || d < D) ? b[D + d] : 0.0;
+            }
+        }
+        const uint64_t leaf0 = ((ConstU64)A.leafOff)[c];
+        const uint32_t nLeaves = static_cast<uint32_t>((je - jb + kLeafRows - 1) / kLeafRows);
+        for (uint32_t lc = 0; lc < nLeaves; lc += kWave) {
+            const uint32_t nl = nLeaves - lc < static_cast<uint32_t>(kWave) ? nLeaves - lc : kWave;
+            // (query, leaf) pairs, lane = pair: the query's float box against the
+            // leaf's (outward-rounded both, so a miss is exact); lane k ends with
+            // query k's mask of the chunk's leaves that meet it
+            lm[lane] = 0;
+            wave_lds_sync();
+            const uint32_t pairs = nl * nQ;
+            for (uint32_t p0 = 0; p0 < pairs; p0 += kWave) {
+                const uint32_t p = p0 + lane;
+                const bool pv = p < pairs;
+                const uint32_t k = pv ? p / nl : 0, l = pv ? p - k * nl : 0;
+                const uint64_t q = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(static_cast<int>(k) << 2,
+                                                                                      static_cast<int>(qv)));
+                const uint64_t lb = (leaf0 + lc + l) * 2 * D;
+                float llo[DM], lhi[DM], qlo[DM], qhi[DM];
+#pragma unroll
+                for (int d = 0; d < DM; ++d) {
+                    llo[d] = d < D ? A.leafBox[lb + d] : 0.0f;
+                    lhi[d] = d < D ? A.leafBox[lb + D + d] : 0.0f;
+                    qlo[d] = A.fboxes[q * 2 * DM + d];
+                    qhi[d] = A.fboxes[q * 2 * DM + DM + d];
+                }
+                bool meet = pv;
+#pragma unroll
+                for (int d = 0; d < DM; ++d) meet = meet & (qlo[d] <= lhi[d]) & (qhi[d] >= llo[d]);
+                if (meet) atomicOr(reinterpret_cast<unsigned long long*>(&lm[k]), 1ull << l);
+            }
+            wLeafTests += pairs;
+            wave_lds_sync();
+            const uint64_t lmv = static_cast<uint32_t>(lane) < nQ ? lm[lane] : 0ull;
+            uint64_t uni = lmv;
+            for (int o = 1; o < kWave; o <<= 1) uni |= __shfl_xor(uni, o, kWave);
+            uni = uniform_u64(uni);
+            // runs of 64 rows (four leaves) that meet some query
+            for (uint32_t g = 0; 4 * g < nl; ++g) {
+                const uint32_t gm = static_cast<uint32_t>(uni >> (4 * g)) & 15u;
+                if (gm == 0) continue;
+                const uint64_t r0 = jb + (static_cast<uint64_t>(lc) + 4 * g) * kLeafRows;
+                const bool rv = r0 + lane < je;
+                const uint64_t row = rv ? r0 + lane : r0;
+                double x[DM];
+#pragma unroll
+                for (int d = 0; d < DM; ++d) x[d] = (EXACT || d < D) ? A.coords[row * D + d] : 0.0;
+                const uint64_t id = A.ids[row];
+                const uint64_t rvm = __ballot(rv);
+                uint64_t leafRows = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    if ((gm >> b) & 1u) leafRows |= 0xFFFFull << (16 * b);
+                wRows += __popcll(rvm & leafRows);
+                // queries meeting one of the run's leaves, each one's box read from
+                // LDS (the same address in every lane) and tested
+                const uint32_t qmv = static_cast<uint32_t>(lmv >> (4 * g)) & 15u;
+                for (uint64_t act = __ballot(qmv != 0); act; act &= act - 1) {
+                    const int k = __builtin_ctzll(act);
+                    const uint32_t qm = __builtin_amdgcn_readlane(qmv, k);
+                    const uint64_t q = __builtin_amdgcn_readlane(qv, k);
+                    uint64_t m = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        if ((qm >> b) & 1u) m |= 0xFFFFull << (16 * b);
+                    m &= rvm;
+                    double lo[DM], hi[DM];
+#pragma unroll
+                    for (int d = 0; d < DM; ++d) {
+                        lo[d] = bx[k][d];
+                        hi[d] = bx[k][DM + d];
+                    }
+                    if (staged > static_cast<uint32_t>(STAGE - kWave)) {
+                        flush_hits(A, stage, staged, lane);
+                        staged = 0;
+                    }
+                    const uint64_t hit = rows_in_box<DM>(m, x, lo, hi);
+                    wTests += __popcll(m);
+                    if (hit) {
+                        if ((hit >> lane) & 1ull) {
+                            const uint32_t r = __builtin_amdgcn_mbcnt_hi(
+                                static_cast<uint32_t>(hit >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(hit), 0u));
+                            stage[staged + r] = (q << A.idBits) | id;
+                        }
+                        staged += static_cast<uint32_t>(__popcll(hit));
+                    }
+                }
+            }
+            wave_lds_sync();  // the leaf masks are rewritten by the next chunk
+        }
+    }
+    flush_hits(A, stage, staged, lane);
+    if (lane == 0 && A.work) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.work), wTests);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.work + 1), wRows);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.work + 2), wLeafTests);
+    }
+}
+
 // Dimensions of the kernel instantiation that handles D (4, 8, 16 or 32).
 inline int dm_of(int D) { return D <= 4 ? 4 : (D <= 8 ? 8 : (D <= 16 ? 16 : 32)); }
 
@@ -779,9 +998,12 @@ __global__ void k_box_f32(const double* __restrict__ boxes, uint64_t nq, int D, 
 __global__ __launch_bounds__(kScanThreads) void k_group_scan(const uint64_t* __restrict__ cnt, uint64_t n,
                                                              uint64_t* __restrict__ off, uint64_t* __restrict__ items,
                                                              uint64_t itemCap, uint64_t* nItems, uint64_t* overflow,
-                                                             uint64_t* status, ScanCtrl* ctrl) {
+                                                             uint64_t* status, ScanCtrl* ctrl, uint64_t rowMax,
+                                                             uint64_t* __restrict__ sparse,
+                                                             uint64_t* __restrict__ sparseCnt) {
     __shared__ uint64_t s_tile;
     __shared__ uint64_t s_c[kScanThreads / kWave], s_s[kScanThreads / kWave];
+    __shared__ uint32_t s_p[kScanThreads / kWave];
     const uint32_t epoch = launch_epoch(ctrl);
     __shared__ uint64_t s_ex0, s_ex1;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
@@ -793,7 +1015,12 @@ __global__ __launch_bounds__(kScanThreads) void k_group_scan(const uint64_t* __r
         if (tile >= ntiles) break;
         const uint64_t i = tile * kScanThreads + tid;
         const uint64_t c = i < n ? cnt[i] : 0;
-        const uint64_t sl = (c + kWave - 1) / kWave;
+        // with a sparse list: groups of 1..rowMax entries get no work items;
+        // their ids go to the tile's kScanThreads slots of the list instead
+        // (in order), the tile's count to sparseCnt[tile]
+        const bool sp = sparse && c > 0 && c <= rowMax;
+        const uint64_t sl = sp ? 0 : (c + kWave - 1) / kWave;
+        const uint64_t spm = __ballot(sp);
         uint64_t xc = c, xs = sl;
         for (int o = 1; o < kWave; o <<= 1) {
             const uint64_t yc = __shfl_up(xc, o, kWave), ys = __shfl_up(xs, o, kWave);
@@ -805,8 +1032,19 @@ __global__ __launch_bounds__(kScanThreads) void k_group_scan(const uint64_t* __r
         if (lane == kWave - 1) {
             s_c[wid] = xc;
             s_s[wid] = xs;
+            s_p[wid] = static_cast<uint32_t>(__popcll(spm));
         }
         __syncthreads();
+        if (sparse) {
+            uint32_t p = 0, tot = 0;
+#pragma unroll
+            for (int w = 0; w < kScanThreads / kWave; ++w) {
+                if (w < wid) p += s_p[w];
+                tot += s_p[w];
+            }
+            if (sp) sparse[tile * kScanThreads + p + __popcll(spm & ((1ull << lane) - 1))] = i;
+            if (tid == 0) sparseCnt[tile] = tot;
+        }
         if (wid == 0) {
             uint64_t tc = lane < kScanThreads / kWave ? s_c[lane] : 0;
             uint64_t ts = lane < kScanThreads / kWave ? s_s[lane] : 0;
@@ -1362,6 +1600,17 @@ auto range_kernel(int D) -> void (*)(RangeStepArgs) {
     }
 }
 
+template <int STAGE>
+auto rows_kernel(int D) -> void (*)(RangeStepArgs) {
+    if (D == 4) return k_range_rows<4, true, STAGE>;
+    if (D == 8) return k_range_rows<8, true, STAGE>;
+    if (D == 16) return k_range_rows<16, true, STAGE>;
+    if (D < 4) return k_range_rows<4, false, STAGE>;
+    if (D < 8) return k_range_rows<8, false, STAGE>;
+    if (D < 16) return k_range_rows<16, false, STAGE>;
+    return k_range_rows<32, false, STAGE>;
+}
+
 template <int MODE, int STAGE>
 void launch_range_dm(tblaeq_ctx* ctx, const RangeStepArgs& A, dim3 g, dim3 b) {
     void (*k)(RangeStepArgs) = range_kernel<MODE, STAGE>(A.D);
@@ -1380,6 +1629,9 @@ void warm_range_kernels(tblaeq_batch* b) {
     resident_blocks(reinterpret_cast<const void*>(range_kernel<kFinal, 256>(b->D)), t);
     resident_blocks(reinterpret_cast<const void*>(range_kernel<kFinal, 512>(b->D)), t);
     resident_blocks(reinterpret_cast<const void*>(range_kernel<kFinal, 1024>(b->D)), t);
+    resident_blocks(reinterpret_cast<const void*>(rows_kernel<256>(b->D)), t);
+    resident_blocks(reinterpret_cast<const void*>(rows_kernel<512>(b->D)), t);
+    resident_blocks(reinterpret_cast<const void*>(rows_kernel<1024>(b->D)), t);
     resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<8, true>), kRsThreads);
     resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<8, false>), kRsThreads);
     resident_blocks(reinterpret_cast<const void*>(k_rs_onesweep<9, true>), kRsThreads);
@@ -1399,6 +1651,32 @@ int launch_range_step(tblaeq_batch* bt, const RangeStepArgs& A, uint64_t itemBou
     else if (bt->hitStage == 256) launch_range_dm<MODE, 256>(ctx, A, g, b);
     else if (bt->hitStage == 1024) launch_range_dm<MODE, 1024>(ctx, A, g, b);
     else launch_range_dm<MODE, 512>(ctx, A, g, b);
+    TB_HIP(hipGetLastError());
+    return TBLAEQ_OK;
+}
+
+template <int STAGE>
+static void launch_rows_dm(tblaeq_ctx* ctx, const RangeStepArgs& A, dim3 g, dim3 b) {
+    void (*k)(RangeStepArgs) = rows_kernel<STAGE>(A.D);
+    const uint64_t cap = static_cast<uint64_t>(ctx->cuCount) *
+                         static_cast<uint64_t>(resident_blocks(reinterpret_cast<const void*>(k), static_cast<int>(b.x)));
+    if (g.x > cap) g.x = static_cast<unsigned>(cap);
+    hipLaunchKernelGGL(k, g, b, 0, ctx->stream, A);
+}
+
+// Row-major exact filter of the clusters kept by at most A.rowMax queries
+// (items from queue slot 1, zeroed with the control block).
+static int launch_range_rows(tblaeq_batch* bt, const RangeStepArgs& A, uint64_t itemBound) {
+    tblaeq_ctx* ctx = bt->ctx;
+    uint64_t blocks = (itemBound + kStepWaves - 1) / kStepWaves;
+    const uint64_t cap = static_cast<uint64_t>(ctx->cuCount) * bt->rangeBlocksPerCu;
+    if (blocks > cap) blocks = cap;
+    if (blocks == 0) blocks = 1;
+    const dim3 g(static_cast<unsigned>(blocks)), b(kStepWaves * kWave);
+    TimedScope ts(ctx, kKBatch0 + 12);
+    if (bt->hitStage == 256) launch_rows_dm<256>(ctx, A, g, b);
+    else if (bt->hitStage == 1024) launch_rows_dm<1024>(ctx, A, g, b);
+    else launch_rows_dm<512>(ctx, A, g, b);
     TB_HIP(hipGetLastError());
     return TBLAEQ_OK;
 }
@@ -1569,7 +1847,8 @@ int launch_grouping(tblaeq_batch* b, uint64_t mesh) {
     {
         TimedScope ts(ctx, kKBatch0 + 2);
         hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream,
-                           b->cnt, mesh, b->off, b->items, b->itemCap, ctl + 1, ctl + 3, ctx->status, ctx->ctrl);
+                           b->cnt, mesh, b->off, b->items, b->itemCap, ctl + 1, ctl + 3, ctx->status, ctx->ctrl, 0ull,
+                           nullptr, nullptr);
     }
     {
         TimedScope ts(ctx, kKBatch0 + 3);
@@ -1593,7 +1872,8 @@ int launch_group_scan(tblaeq_batch* b, uint64_t mesh, uint64_t* off, uint64_t* i
     if (rc) return rc;
     TimedScope ts(ctx, kKBatch0 + 2);
     hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream, b->cnt,
-                       mesh, off, items, b->itemCap, nItems, b->ctl + 3, ctx->status, ctx->ctrl);
+                       mesh, off, items, b->itemCap, nItems, b->ctl + 3, ctx->status, ctx->ctrl, 0ull, nullptr,
+                       nullptr);
     TB_HIP(hipGetLastError());
     return TBLAEQ_OK;
 }
@@ -1603,7 +1883,7 @@ int scan_counts(tblaeq_ctx* ctx, const uint64_t* cnt, uint64_t n, uint64_t* off,
     int rc = scan_prepare(ctx, tiles);
     if (rc) return rc;
     hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream, cnt, n,
-                       off, nullptr, 0ull, scratch2, scratch2 + 1, ctx->status, ctx->ctrl);
+                       off, nullptr, 0ull, scratch2, scratch2 + 1, ctx->status, ctx->ctrl, 0ull, nullptr, nullptr);
     TB_HIP(hipGetLastError());
     return TBLAEQ_OK;
 }
@@ -1615,7 +1895,7 @@ int launch_count_scan(tblaeq_batch* b, const uint64_t* cnt, uint64_t n, uint64_t
     if (rc) return rc;
     TimedScope ts(ctx, kKBatch0 + 2);
     hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream, cnt, n,
-                       off, nullptr, 0ull, b->ctl + 6, b->ctl + 7, ctx->status, ctx->ctrl);
+                       off, nullptr, 0ull, b->ctl + 6, b->ctl + 7, ctx->status, ctx->ctrl, 0ull, nullptr, nullptr);
     TB_HIP(hipGetLastError());
     return TBLAEQ_OK;
 }
@@ -1769,6 +2049,11 @@ extern "C" int tblaeq_batch_create(tblaeq_ctx* ctx, const tblaeq_level* levels, 
         if (rc) return fail(rc);
         b->leafBox = balloc<float>(b, b->nLeaves * 2 * D, &rc);
         if (rc) return fail(rc);
+        {
+            const uint64_t m = std::max<uint64_t>(cols, b->count[L - 1]);
+            b->sparse = balloc<uint64_t>(b, sparse_slots(m) + (m + kScanThreads - 1) / kScanThreads, &rc);
+        }
+        if (rc) return fail(rc);
         if (hipMemcpy(dperm, perm.data(), nf * 8, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(b->leafOff, hlo.data(), (cols + 1) * 8, hipMemcpyHostToDevice) != hipSuccess)
             return fail(hip_fail(hipGetLastError(), "batch create: kd order upload"));
@@ -1915,6 +2200,7 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     uint64_t* itemS[2] = {b->items, b->items2};
     uint32_t* qS[2] = {b->qlist, b->qlist2};
     uint64_t* nS[2] = {ctl + 1, ctl + 4};
+    const uint64_t rowMax = std::min<uint64_t>(b->rangeRowMax, static_cast<uint64_t>(rows_max_queries(dm_of(b->D))));
     int cur = 0;
     int rc;
     for (int l = 0; l < L; ++l) {
@@ -1952,9 +2238,13 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
         if ((rc = scan_prepare(ctx, tiles))) return rc;
         {
             TimedScope ts(ctx, kKBatch0 + 2);
+            // the last prune's groups feed the exact filter: clusters kept by at
+            // most rowMax queries go to the row-major filter's list instead
+            const bool split = l == L - 1 && rowMax > 0;
             hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream,
                                b->cnt, mesh, offS[nxt], itemS[nxt], b->itemCap, nS[nxt], ctl + 3, ctx->status,
-                               ctx->ctrl);
+                               ctx->ctrl, split ? rowMax : 0ull, split ? b->sparse : nullptr,
+                               split ? b->sparse + sparse_slots(mesh) : nullptr);
         }
         A.survStat = nullptr;
         A.fineStat = nullptr;
@@ -2000,7 +2290,12 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     }
     A.queue = b->queue;
     A.masks = nullptr;
+    A.rowMax = rowMax;
+    A.sparse = b->sparse;
+    A.sparseTiles = (b->count[L - 1] + kScanThreads - 1) / kScanThreads;
+    A.sparseCnt = b->sparse + sparse_slots(b->count[L - 1]);
     if ((rc = launch_range_step<kFinal>(b, A, b->itemCap))) return rc;
+    if (A.rowMax && (rc = launch_range_rows(b, A, b->count[L - 1]))) return rc;
     // sort the (query, id) keys; the last pass writes the ids to the caller's
     // buffer and the first position of every query to d_out_off
     uint64_t* sorted = nullptr;

@@ -1599,6 +1599,7 @@ const ParamSpec kParams[] = {
     {TBLAEQ_PARAM_STEP_BLOCKS_PER_CU, &tblaeq_batch::stepBlocksPerCu, 1, 64},
     {TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU, &tblaeq_batch::radixBlocksPerCu, 1, 32},
     {TBLAEQ_PARAM_HIT_STAGE, &tblaeq_batch::hitStage, 256, 1024},
+    {TBLAEQ_PARAM_RANGE_ROW_QUERIES, &tblaeq_batch::rangeRowMax, 0, 64},
 };
 const ParamSpec* find_param(int id) {
     for (const auto& p : kParams)

@@ -86,13 +86,14 @@ const char* param_name(int id) {
     case TBLAEQ_PARAM_STEP_BLOCKS_PER_CU: return "step_blocks_per_cu";
     case TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU: return "radix_blocks_per_cu";
     case TBLAEQ_PARAM_HIT_STAGE: return "hit_stage";
+    case TBLAEQ_PARAM_RANGE_ROW_QUERIES: return "range_row_queries";
     default: return nullptr;
     }
 }
 const int kAllParams[] = {TBLAEQ_PARAM_KNN_SEED,           TBLAEQ_PARAM_KNN_WG_PER_CU,
                           TBLAEQ_PARAM_KNN_MODE,           TBLAEQ_PARAM_RANGE_BLOCKS_PER_CU,
                           TBLAEQ_PARAM_STEP_BLOCKS_PER_CU, TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU,
-                          TBLAEQ_PARAM_HIT_STAGE};
+                          TBLAEQ_PARAM_HIT_STAGE,          TBLAEQ_PARAM_RANGE_ROW_QUERIES};
 
 }  // namespace
 }  // namespace tblaeq
@@ -173,6 +174,7 @@ extern "C" int tblaeq_batch_autotune(tblaeq_batch* b, const double* d_boxes, uin
     };
     const std::vector<Candidate> rangeC = {{TBLAEQ_PARAM_RANGE_BLOCKS_PER_CU, {4, 8, 16, 32}},
                                            {TBLAEQ_PARAM_HIT_STAGE, {256, 512, 1024}},
+                                           {TBLAEQ_PARAM_RANGE_ROW_QUERIES, {0, 4, 8, 16, 32, 64}},
                                            {TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU, {2, 4, 8, 16}}};
     const std::vector<Candidate> knnC = {{TBLAEQ_PARAM_KNN_SEED, {16, 32, 64, 128}},
                                          {TBLAEQ_PARAM_KNN_WG_PER_CU, {2, 4, 8}},

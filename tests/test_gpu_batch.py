@@ -70,15 +70,18 @@ def test_batch_range_integer_grid():
     assert sum(len(x) for x in res) > 0
 
 
-@pytest.mark.parametrize("blocks,stage,radix", [(1, 256, 1), (2, 1024, 32), (64, 512, 2), (7, 256, 5),
-                                                (16, 1024, 4)])
-def test_batch_range_launch_params_vs_oracle(blocks, stage, radix):
-    """Range launch parameters (prune and filter grid, LDS hit stage, radix grid) change only
-    how the work is spread: every setting's answers equal the oracle's."""
+@pytest.mark.parametrize("blocks,stage,radix,rowq", [(1, 256, 1, 0), (2, 1024, 32, 64), (64, 512, 2, 1),
+                                                     (7, 256, 5, 8), (16, 1024, 4, 16)])
+def test_batch_range_launch_params_vs_oracle(blocks, stage, radix, rowq):
+    """Range launch parameters (prune and filter grid, LDS hit stage, radix grid, the query count
+    up to which a cluster is filtered row-major) change only how the work is spread: every
+    setting's answers equal the oracle's."""
     h, o = _make(dict(CFG, N=30000, seed=19), max_q=256)
     h.batch_set_param(tblaeq.PARAM_RANGE_BLOCKS_PER_CU, blocks)
     h.batch_set_param(tblaeq.PARAM_HIT_STAGE, stage)
     h.batch_set_param(tblaeq.PARAM_RADIX_BLOCKS_PER_CU, radix)
+    h.batch_set_param(tblaeq.PARAM_RANGE_ROW_QUERIES, rowq)
+    assert h.batch_get_param(tblaeq.PARAM_RANGE_ROW_QUERIES) == rowq
     data = h.data()
     boxes, _ = W.range_boxes(data, 160, 23, 1e-4, 0.2)
     _, pts = OC.points_outside_radius(o, limit=40)

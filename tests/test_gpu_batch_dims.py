@@ -28,8 +28,19 @@ def idx(request):
     h.close()
 
 
-def test_range_dims(idx):
+@pytest.mark.parametrize("rowq", [0, 16, 64])
+def test_range_dims(idx, rowq):
+    """rowq: clusters kept by at most this many queries go to the row-major filter (0: none, 64:
+    every cluster of one slice)."""
     h, o = idx
+    h.batch_set_param(tblaeq.PARAM_RANGE_ROW_QUERIES, rowq)
+    try:
+        _range_dims(h, o)
+    finally:
+        h.batch_set_param(tblaeq.PARAM_RANGE_ROW_QUERIES, 16)
+
+
+def _range_dims(h, o):
     data = h.data()
     boxes, _ = W.range_boxes(data, 200, 40 + h.D, 1e-4, 5e-2)
     _, pts = OC.points_outside_radius(o, limit=64)
