@@ -276,7 +276,10 @@ int tblaeq_batch_knn_graph(tblaeq_batch* b, const double* d_points, const uint64
  *   TBLAEQ_PARAM_RANGE_ROW_QUERIES  last-level clusters kept by at most this
  *                                   many queries are filtered row-major (lane =
  *                                   point, one query at a time), the others
- *                                   query-major, 0..64 (16; 0: all query-major) */
+ *                                   query-major, 0..64 (16; 0: all query-major;
+ *                                   values above 256 / dm take effect as 256 /
+ *                                   dm, dm = 4, 8, 16 or 32 the instantiated
+ *                                   dimension at or above D) */
 enum {
     TBLAEQ_PARAM_KNN_SEED = 1,
     TBLAEQ_PARAM_KNN_WG_PER_CU = 2,
