@@ -35,7 +35,7 @@ FP64_PEAK_TFLOPS = 78.6
 # SIMDs x 16 lanes x 2.4 GHz; half the FP64 FMA flop rate of 78.6 TFLOP/s
 VALU_PEAK_TOPS = 39.3
 PARAMS = (("knn_seed", 1), ("knn_wg_per_cu", 2), ("knn_mode", 3), ("range_blocks_per_cu", 4),
-          ("step_blocks_per_cu", 5), ("radix_blocks_per_cu", 6), ("hit_stage", 7), ("range_row_queries", 8))
+          ("step_blocks_per_cu", 5), ("radix_blocks_per_cu", 6), ("hit_stage", 7), ("range_row_queries", 8), ("knn_row_queries", 9))
 
 
 def final_traffic(cfg):
@@ -745,7 +745,7 @@ def main():
         tp_r, tp_k = [], []
         kinds = (("range_prune", 8), ("range_final", 9), ("group_scan", 10), ("group_scatter", 11),
                  ("radix_sort", 12), ("knn_levels", 13), ("knn_last_passes", 17), ("knn_select", 18),
-                 ("knn_refine", 15), ("knn_merge", 16), ("range_rows", 20))
+                 ("knn_refine", 15), ("knn_merge", 16), ("range_rows", 20), ("knn_rows", 21))
         kacc = {n: [0, 0.0] for n, _ in kinds}
         work_r = work_k = (0, 0)
         leaf_tests = 0

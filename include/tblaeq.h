@@ -239,7 +239,9 @@ int tblaeq_batch_range_graph(tblaeq_batch* b, const double* d_boxes, uint64_t nq
  * level) as the per-query path.  d_status[0] = candidate records refined
  * cluster-major, d_status[1] = overflow flags (non-zero: a capacity of
  * tblaeq_batch_reserve was exceeded, results invalid; split the batch),
- * d_status[2] = distance evaluations of the cluster-major refine,
+ * d_status[2] = distance evaluations of the cluster-major refine (rows of
+ * kd leaves whose box some query of the work item cannot rule out, times the
+ * item's queries),
  * d_status[3] = distance and bound evaluations of the query-major expansions
  * (seed clusters, last-level bound) (d_status holds 4 words).
  * Requires Kmax + empty clusters + 1 <= 256 and at most 8 levels (else
@@ -279,7 +281,13 @@ int tblaeq_batch_knn_graph(tblaeq_batch* b, const double* d_points, const uint64
  *                                   query-major, 0..64 (16; 0: all query-major;
  *                                   values above 256 / dm take effect as 256 /
  *                                   dm, dm = 4, 8, 16 or 32 the instantiated
- *                                   dimension at or above D) */
+ *                                   dimension at or above D)
+ *   TBLAEQ_PARAM_KNN_ROW_QUERIES    last-level clusters refined by at most this
+ *                                   many KNN queries are refined row-major
+ *                                   (lane = (cluster, query) pair for the leaf
+ *                                   tests, then lane = point of a kept leaf),
+ *                                   the others query-major, 0..4 (4; 0: all
+ *                                   query-major; only with more than one level) */
 enum {
     TBLAEQ_PARAM_KNN_SEED = 1,
     TBLAEQ_PARAM_KNN_WG_PER_CU = 2,
@@ -288,7 +296,8 @@ enum {
     TBLAEQ_PARAM_STEP_BLOCKS_PER_CU = 5,
     TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU = 6,
     TBLAEQ_PARAM_HIT_STAGE = 7,
-    TBLAEQ_PARAM_RANGE_ROW_QUERIES = 8
+    TBLAEQ_PARAM_RANGE_ROW_QUERIES = 8,
+    TBLAEQ_PARAM_KNN_ROW_QUERIES = 9
 };
 int tblaeq_batch_set_param(tblaeq_batch* b, int param, uint64_t value);
 int tblaeq_batch_get_param(const tblaeq_batch* b, int param, uint64_t* value);

@@ -40,6 +40,7 @@ struct tblaeq_batch {
     float* aabb = nullptr;       // [count[L-1] * 2D] bounding box of each last-level position's points
     uint64_t* leafOff = nullptr; // [cols of P[L-1] + 1] first kd leaf of every last-level cluster, by id
     float* leafBox = nullptr;    // [leaves * 2D] outward-rounded float box lo[D] | hi[D] of every kd leaf
+    uint64_t maxLeaves = 0;      // most kd leaves of one last-level cluster
     uint64_t nLeaves = 0;
     float* fbox = nullptr;       // [maxQ * 2 * dm] outward float boxes of the current range batch
     uint64_t* sparse = nullptr;  // last-level clusters of the row-major exact filter, per scan tile, and counts
@@ -93,6 +94,7 @@ struct tblaeq_batch {
     uint64_t radixBlocksPerCu = 4;    // hit radix sort
     uint64_t hitStage = 512;          // LDS hits staged per wave by the final filter (256, 512, 1024)
     uint64_t rangeRowMax = 16;        // final filter: clusters kept by at most this many queries go row-major
+    uint64_t knnRowMax = 4;           // KNN refine: clusters refined by at most this many queries go row-major
     uint64_t paramVersion = 0;        // bumped by every parameter change (captured graphs re-capture)
     uint64_t gParamVersion = 0;
 };
@@ -127,7 +129,29 @@ constexpr int kMaskQueue0 = 8;  // queue slots kMaskQueue0 + l: keep-mask rows r
 // b->cnt zeroed and resets the pair count ctl[0].
 int launch_grouping(tblaeq_batch* b, uint64_t mesh);
 // Group scan of b->cnt over mesh clusters into off / items (count in *nItems).
-int launch_group_scan(tblaeq_batch* b, uint64_t mesh, uint64_t* off, uint64_t* items, uint64_t* nItems);
+// rowMax > 0: clusters of 1..rowMax pairs get no work items; they go to the
+// row-major list in b->sparse instead (sparse_list).
+int launch_group_scan(tblaeq_batch* b, uint64_t mesh, uint64_t* off, uint64_t* items, uint64_t* nItems,
+                      uint64_t rowMax = 0);
+// Row-major cluster list of a level of n clusters, in b->sparse: kScanThreads
+// slots per group-scan tile (the tile's listed clusters first, in order), then
+// every tile's count, then the non-empty units of kRowUnit slots that the
+// row-major kernels take as work (unit u: slots u * kRowUnit ..), their number
+// in queue slot kSparseUnitQueue (zeroed with the control block).
+constexpr int kRowUnit = 16;
+constexpr int kSparseUnitQueue = 2;
+constexpr uint64_t sparse_slots(uint64_t n) { return (n + kScanThreads - 1) / kScanThreads * kScanThreads; }
+constexpr uint64_t sparse_words(uint64_t n) {
+    return sparse_slots(n) + sparse_slots(n) / kScanThreads + sparse_slots(n) / kRowUnit;
+}
+struct SparseList {
+    uint64_t* slots = nullptr;
+    uint64_t* counts = nullptr;
+    uint64_t* units = nullptr;
+    uint64_t* unitCount = nullptr;
+};
+// The row-major list of a level of n clusters in b->sparse.
+SparseList sparse_list(tblaeq_batch* b, uint64_t n);
 // Exclusive scan of cnt[0..n) into off[0..n] (off[n] = total).
 int launch_count_scan(tblaeq_batch* b, const uint64_t* cnt, uint64_t n, uint64_t* off);
 // status[0] = ctl[2], status[1] = ctl[3].

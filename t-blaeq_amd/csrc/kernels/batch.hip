@@ -141,6 +141,8 @@ struct RangeStepArgs {
     uint64_t rowMax;
     const uint64_t* sparse;     // [sparseTiles * kScanThreads] (tile t's clusters from t * kScanThreads)
     const uint64_t* sparseCnt;  // [sparseTiles] clusters in each tile's slots
+    const uint64_t* sparseUnits;      // non-empty units of kRowUnit slots
+    const uint64_t* sparseUnitCount;  // their number
     uint64_t sparseTiles;
 };
 
@@ -825,10 +827,6 @@ __device__ __forceinline__ uint64_t rows_in_box(uint64_t m, const double* x, con
 // Queries per cluster the row-major filter holds (their FP64 boxes in LDS,
 // 4 KB per wave).
 constexpr int rows_max_queries(int DM) { return 256 / DM; }
-// Slots of the row-major filter's cluster list for a level of n clusters (one
-// group-scan tile of kScanThreads clusters per kScanThreads slots); the
-// tiles' counts follow the slots.
-constexpr uint64_t sparse_slots(uint64_t n) { return (n + kScanThreads - 1) / kScanThreads * kScanThreads; }
 
 template <int DM, bool EXACT, int STAGE>
 __global__ __launch_bounds__(kStepWaves * kWave) void k_range_rows(RangeStepArgs A) {
@@ -840,9 +838,8 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_rows(RangeStepArgs
     const uint64_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kStepWaves;
     const int D = EXACT ? DM : A.D;
-    // work units of kRowsUnit list slots (a tile's slots split in kScanThreads / kRowsUnit)
-    constexpr uint64_t kRowsUnit = 16;
-    const uint64_t nItems = A.sparseTiles * (kScanThreads / kRowsUnit);
+    // work: the non-empty units of kRowUnit list slots
+    const uint64_t nItems = *((ConstU64)A.sparseUnitCount);
     uint64_t* stage = s_hit[wave];
     uint64_t* lm = s_lm[wave];
     double(*bx)[2 * DM] = s_box[wave];
@@ -851,118 +848,120 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_rows(RangeStepArgs
     const uint64_t chunk = queue_chunk(nItems, nw);
     uint64_t ib, ie;
     while (next_items(A.queue + 1, chunk, nItems, lane, &ib, &ie))
-    for (uint64_t item = ib * kRowsUnit, itemEnd = ie * kRowsUnit; item < itemEnd; ++item) {
-        if ((item & (kScanThreads - 1)) >= ((ConstU64)A.sparseCnt)[item / kScanThreads]) {
-            item |= kRowsUnit - 1;  // the rest of this unit's slots are empty
-            continue;
-        }
-        const uint64_t c = ((ConstU64)A.sparse)[item];
-        const uint64_t g0 = ((ConstU64)A.off)[c];
-        const uint64_t gn = ((ConstU64)A.off)[c + 1] - g0;
-        if (gn == 0 || gn > A.rowMax) continue;
-        // queries whose list entry was stored (an overflowed list drops the rest;
-        // the batch is then rerun)
-        const uint32_t nQ = static_cast<uint32_t>(g0 >= A.qcap ? 0 : (A.qcap - g0 < gn ? A.qcap - g0 : gn));
-        const uint64_t jb = ((ConstU64)A.colResPrev)[c], je = ((ConstU64)A.colResPrev)[c + 1];
-        if (nQ == 0 || jb >= je) continue;
-        // lane k < nQ: query k of the cluster; its FP64 box into LDS
-        const uint32_t qv = static_cast<uint32_t>(lane) < nQ ? A.qlist[g0 + lane] : 0u;
-        if (static_cast<uint32_t>(lane) < nQ) {
-            const double* b = A.boxes + static_cast<uint64_t>(qv) * 2 * D;
-#pragma unroll
-            for (int d = 0; d < DM; ++d) {
-                bx[lane][d] = (EXACT || d < D) ? b[d] : 0.0;
-                bx[lane][DM + d] = (EXACT || d < D) ? b[D + d] : 0.0;
-            }
-        }
-        const uint64_t leaf0 = ((ConstU64)A.leafOff)[c];
-        const uint32_t nLeaves = static_cast<uint32_t>((je - jb + kLeafRows - 1) / kLeafRows);
-        for (uint32_t lc = 0; lc < nLeaves; lc += kWave) {
-            const uint32_t nl = nLeaves - lc < static_cast<uint32_t>(kWave) ? nLeaves - lc : kWave;
-            // (query, leaf) pairs, lane = pair: the query's float box against the
-            // leaf's (outward-rounded both, so a miss is exact); lane k ends with
-            // query k's mask of the chunk's leaves that meet it
-            lm[lane] = 0;
-            wave_lds_sync();
-            const uint32_t pairs = nl * nQ;
-            for (uint32_t p0 = 0; p0 < pairs; p0 += kWave) {
-                const uint32_t p = p0 + lane;
-                const bool pv = p < pairs;
-                const uint32_t k = pv ? p / nl : 0, l = pv ? p - k * nl : 0;
-                const uint64_t q = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(static_cast<int>(k) << 2,
-                                                                                      static_cast<int>(qv)));
-                const uint64_t lb = (leaf0 + lc + l) * 2 * D;
-                float llo[DM], lhi[DM], qlo[DM], qhi[DM];
+    for (uint64_t u = ib; u < ie; ++u) {
+        const uint64_t s0 = ((ConstU64)A.sparseUnits)[u] * kRowUnit;
+        const uint64_t tileEnd = s0 / kScanThreads * kScanThreads + ((ConstU64)A.sparseCnt)[s0 / kScanThreads];
+        const uint64_t s1 = s0 + kRowUnit < tileEnd ? s0 + kRowUnit : tileEnd;
+        for (uint64_t item = s0; item < s1; ++item) {
+            const uint64_t c = ((ConstU64)A.sparse)[item];
+            const uint64_t g0 = ((ConstU64)A.off)[c];
+            const uint64_t gn = ((ConstU64)A.off)[c + 1] - g0;
+            if (gn == 0 || gn > A.rowMax) continue;
+            // queries whose list entry was stored (an overflowed list drops the rest;
+            // the batch is then rerun)
+            const uint32_t nQ = static_cast<uint32_t>(g0 >= A.qcap ? 0 : (A.qcap - g0 < gn ? A.qcap - g0 : gn));
+            const uint64_t jb = ((ConstU64)A.colResPrev)[c], je = ((ConstU64)A.colResPrev)[c + 1];
+            if (nQ == 0 || jb >= je) continue;
+            // lane k < nQ: query k of the cluster; its FP64 box into LDS
+            const uint32_t qv = static_cast<uint32_t>(lane) < nQ ? A.qlist[g0 + lane] : 0u;
+            if (static_cast<uint32_t>(lane) < nQ) {
+                const double* b = A.boxes + static_cast<uint64_t>(qv) * 2 * D;
 #pragma unroll
                 for (int d = 0; d < DM; ++d) {
-                    llo[d] = d < D ? A.leafBox[lb + d] : 0.0f;
-                    lhi[d] = d < D ? A.leafBox[lb + D + d] : 0.0f;
-                    qlo[d] = A.fboxes[q * 2 * DM + d];
-                    qhi[d] = A.fboxes[q * 2 * DM + DM + d];
+                    bx[lane][d] = (EXACT || d < D) ? b[d] : 0.0;
+                    bx[lane][DM + d] = (EXACT || d < D) ? b[D + d] : 0.0;
                 }
-                bool meet = pv;
-#pragma unroll
-                for (int d = 0; d < DM; ++d) meet = meet & (qlo[d] <= lhi[d]) & (qhi[d] >= llo[d]);
-                if (meet) atomicOr(reinterpret_cast<unsigned long long*>(&lm[k]), 1ull << l);
             }
-            wLeafTests += pairs;
-            wave_lds_sync();
-            const uint64_t lmv = static_cast<uint32_t>(lane) < nQ ? lm[lane] : 0ull;
-            uint64_t uni = lmv;
-            for (int o = 1; o < kWave; o <<= 1) uni |= __shfl_xor(uni, o, kWave);
-            uni = uniform_u64(uni);
-            // runs of 64 rows (four leaves) that meet some query
-            for (uint32_t g = 0; 4 * g < nl; ++g) {
-                const uint32_t gm = static_cast<uint32_t>(uni >> (4 * g)) & 15u;
-                if (gm == 0) continue;
-                const uint64_t r0 = jb + (static_cast<uint64_t>(lc) + 4 * g) * kLeafRows;
-                const bool rv = r0 + lane < je;
-                const uint64_t row = rv ? r0 + lane : r0;
-                double x[DM];
-#pragma unroll
-                for (int d = 0; d < DM; ++d) x[d] = (EXACT || d < D) ? A.coords[row * D + d] : 0.0;
-                const uint64_t id = A.ids[row];
-                const uint64_t rvm = __ballot(rv);
-                uint64_t leafRows = 0;
-#pragma unroll
-                for (int b = 0; b < 4; ++b)
-                    if ((gm >> b) & 1u) leafRows |= 0xFFFFull << (16 * b);
-                wRows += __popcll(rvm & leafRows);
-                // queries meeting one of the run's leaves, each one's box read from
-                // LDS (the same address in every lane) and tested
-                const uint32_t qmv = static_cast<uint32_t>(lmv >> (4 * g)) & 15u;
-                for (uint64_t act = __ballot(qmv != 0); act; act &= act - 1) {
-                    const int k = __builtin_ctzll(act);
-                    const uint32_t qm = __builtin_amdgcn_readlane(qmv, k);
-                    const uint64_t q = __builtin_amdgcn_readlane(qv, k);
-                    uint64_t m = 0;
-#pragma unroll
-                    for (int b = 0; b < 4; ++b)
-                        if ((qm >> b) & 1u) m |= 0xFFFFull << (16 * b);
-                    m &= rvm;
-                    double lo[DM], hi[DM];
+            const uint64_t leaf0 = ((ConstU64)A.leafOff)[c];
+            const uint32_t nLeaves = static_cast<uint32_t>((je - jb + kLeafRows - 1) / kLeafRows);
+            for (uint32_t lc = 0; lc < nLeaves; lc += kWave) {
+                const uint32_t nl = nLeaves - lc < static_cast<uint32_t>(kWave) ? nLeaves - lc : kWave;
+                // (query, leaf) pairs, lane = pair: the query's float box against the
+                // leaf's (outward-rounded both, so a miss is exact); lane k ends with
+                // query k's mask of the chunk's leaves that meet it
+                lm[lane] = 0;
+                wave_lds_sync();
+                const uint32_t pairs = nl * nQ;
+                for (uint32_t p0 = 0; p0 < pairs; p0 += kWave) {
+                    const uint32_t p = p0 + lane;
+                    const bool pv = p < pairs;
+                    const uint32_t k = pv ? p / nl : 0, l = pv ? p - k * nl : 0;
+                    const uint64_t q = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(static_cast<int>(k) << 2,
+                                                                                          static_cast<int>(qv)));
+                    const uint64_t lb = (leaf0 + lc + l) * 2 * D;
+                    float llo[DM], lhi[DM], qlo[DM], qhi[DM];
 #pragma unroll
                     for (int d = 0; d < DM; ++d) {
-                        lo[d] = bx[k][d];
-                        hi[d] = bx[k][DM + d];
+                        llo[d] = d < D ? A.leafBox[lb + d] : 0.0f;
+                        lhi[d] = d < D ? A.leafBox[lb + D + d] : 0.0f;
+                        qlo[d] = A.fboxes[q * 2 * DM + d];
+                        qhi[d] = A.fboxes[q * 2 * DM + DM + d];
                     }
-                    if (staged > static_cast<uint32_t>(STAGE - kWave)) {
-                        flush_hits(A, stage, staged, lane);
-                        staged = 0;
-                    }
-                    const uint64_t hit = rows_in_box<DM>(m, x, lo, hi);
-                    wTests += __popcll(m);
-                    if (hit) {
-                        if ((hit >> lane) & 1ull) {
-                            const uint32_t r = __builtin_amdgcn_mbcnt_hi(
-                                static_cast<uint32_t>(hit >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(hit), 0u));
-                            stage[staged + r] = (q << A.idBits) | id;
+                    bool meet = pv;
+#pragma unroll
+                    for (int d = 0; d < DM; ++d) meet = meet & (qlo[d] <= lhi[d]) & (qhi[d] >= llo[d]);
+                    if (meet) atomicOr(reinterpret_cast<unsigned long long*>(&lm[k]), 1ull << l);
+                }
+                wLeafTests += pairs;
+                wave_lds_sync();
+                const uint64_t lmv = static_cast<uint32_t>(lane) < nQ ? lm[lane] : 0ull;
+                uint64_t uni = lmv;
+                for (int o = 1; o < kWave; o <<= 1) uni |= __shfl_xor(uni, o, kWave);
+                uni = uniform_u64(uni);
+                // runs of 64 rows (four leaves) that meet some query
+                for (uint32_t g = 0; 4 * g < nl; ++g) {
+                    const uint32_t gm = static_cast<uint32_t>(uni >> (4 * g)) & 15u;
+                    if (gm == 0) continue;
+                    const uint64_t r0 = jb + (static_cast<uint64_t>(lc) + 4 * g) * kLeafRows;
+                    const bool rv = r0 + lane < je;
+                    const uint64_t row = rv ? r0 + lane : r0;
+                    double x[DM];
+#pragma unroll
+                    for (int d = 0; d < DM; ++d) x[d] = (EXACT || d < D) ? A.coords[row * D + d] : 0.0;
+                    const uint64_t id = A.ids[row];
+                    const uint64_t rvm = __ballot(rv);
+                    uint64_t leafRows = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        if ((gm >> b) & 1u) leafRows |= 0xFFFFull << (16 * b);
+                    wRows += __popcll(rvm & leafRows);
+                    // queries meeting one of the run's leaves, each one's box read from
+                    // LDS (the same address in every lane) and tested
+                    const uint32_t qmv = static_cast<uint32_t>(lmv >> (4 * g)) & 15u;
+                    for (uint64_t act = __ballot(qmv != 0); act; act &= act - 1) {
+                        const int k = __builtin_ctzll(act);
+                        const uint32_t qm = __builtin_amdgcn_readlane(qmv, k);
+                        const uint64_t q = __builtin_amdgcn_readlane(qv, k);
+                        uint64_t m = 0;
+#pragma unroll
+                        for (int b = 0; b < 4; ++b)
+                            if ((qm >> b) & 1u) m |= 0xFFFFull << (16 * b);
+                        m &= rvm;
+                        double lo[DM], hi[DM];
+#pragma unroll
+                        for (int d = 0; d < DM; ++d) {
+                            lo[d] = bx[k][d];
+                            hi[d] = bx[k][DM + d];
                         }
-                        staged += static_cast<uint32_t>(__popcll(hit));
+                        if (staged > static_cast<uint32_t>(STAGE - kWave)) {
+                            flush_hits(A, stage, staged, lane);
+                            staged = 0;
+                        }
+                        const uint64_t hit = rows_in_box<DM>(m, x, lo, hi);
+                        wTests += __popcll(m);
+                        if (hit) {
+                            if ((hit >> lane) & 1ull) {
+                                const uint32_t r =
+                                    __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(hit >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(hit), 0u));
+                                stage[staged + r] = (q << A.idBits) | id;
+                            }
+                            staged += static_cast<uint32_t>(__popcll(hit));
+                        }
                     }
                 }
+                wave_lds_sync();  // the leaf masks are rewritten by the next chunk
             }
-            wave_lds_sync();  // the leaf masks are rewritten by the next chunk
         }
     }
     flush_hits(A, stage, staged, lane);
@@ -999,8 +998,7 @@ __global__ __launch_bounds__(kScanThreads) void k_group_scan(const uint64_t* __r
                                                              uint64_t* __restrict__ off, uint64_t* __restrict__ items,
                                                              uint64_t itemCap, uint64_t* nItems, uint64_t* overflow,
                                                              uint64_t* status, ScanCtrl* ctrl, uint64_t rowMax,
-                                                             uint64_t* __restrict__ sparse,
-                                                             uint64_t* __restrict__ sparseCnt) {
+                                                             SparseList list) {
     __shared__ uint64_t s_tile;
     __shared__ uint64_t s_c[kScanThreads / kWave], s_s[kScanThreads / kWave];
     __shared__ uint32_t s_p[kScanThreads / kWave];
@@ -1015,10 +1013,10 @@ __global__ __launch_bounds__(kScanThreads) void k_group_scan(const uint64_t* __r
         if (tile >= ntiles) break;
         const uint64_t i = tile * kScanThreads + tid;
         const uint64_t c = i < n ? cnt[i] : 0;
-        // with a sparse list: groups of 1..rowMax entries get no work items;
-        // their ids go to the tile's kScanThreads slots of the list instead
-        // (in order), the tile's count to sparseCnt[tile]
-        const bool sp = sparse && c > 0 && c <= rowMax;
+        // with a row-major list: groups of 1..rowMax entries get no work
+        // items; their ids go to the tile's kScanThreads slots of the list
+        // instead (in order), the tile's count and its units to the list
+        const bool sp = list.slots && c > 0 && c <= rowMax;
         const uint64_t sl = sp ? 0 : (c + kWave - 1) / kWave;
         const uint64_t spm = __ballot(sp);
         uint64_t xc = c, xs = sl;
@@ -1035,15 +1033,23 @@ __global__ __launch_bounds__(kScanThreads) void k_group_scan(const uint64_t* __r
             s_p[wid] = static_cast<uint32_t>(__popcll(spm));
         }
         __syncthreads();
-        if (sparse) {
+        if (list.slots) {
             uint32_t p = 0, tot = 0;
 #pragma unroll
             for (int w = 0; w < kScanThreads / kWave; ++w) {
                 if (w < wid) p += s_p[w];
                 tot += s_p[w];
             }
-            if (sp) sparse[tile * kScanThreads + p + __popcll(spm & ((1ull << lane) - 1))] = i;
-            if (tid == 0) sparseCnt[tile] = tot;
+            if (sp) list.slots[tile * kScanThreads + p + __popcll(spm & ((1ull << lane) - 1))] = i;
+            if (tid == 0) {
+                list.counts[tile] = tot;
+                const uint32_t nu = (tot + kRowUnit - 1) / kRowUnit;
+                if (nu) {
+                    const uint64_t base =
+                        atomicAdd(reinterpret_cast<unsigned long long*>(list.unitCount), static_cast<uint64_t>(nu));
+                    for (uint32_t k = 0; k < nu; ++k) list.units[base + k] = tile * (kScanThreads / kRowUnit) + k;
+                }
+            }
         }
         if (wid == 0) {
             uint64_t tc = lane < kScanThreads / kWave ? s_c[lane] : 0;
@@ -1848,7 +1854,7 @@ int launch_grouping(tblaeq_batch* b, uint64_t mesh) {
         TimedScope ts(ctx, kKBatch0 + 2);
         hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream,
                            b->cnt, mesh, b->off, b->items, b->itemCap, ctl + 1, ctl + 3, ctx->status, ctx->ctrl, 0ull,
-                           nullptr, nullptr);
+                           SparseList{});
     }
     {
         TimedScope ts(ctx, kKBatch0 + 3);
@@ -1865,15 +1871,26 @@ int launch_grouping(tblaeq_batch* b, uint64_t mesh) {
     return TBLAEQ_OK;
 }
 
-int launch_group_scan(tblaeq_batch* b, uint64_t mesh, uint64_t* off, uint64_t* items, uint64_t* nItems) {
+SparseList sparse_list(tblaeq_batch* b, uint64_t n) {
+    SparseList l;
+    const uint64_t s = sparse_slots(n);
+    l.slots = b->sparse;
+    l.counts = b->sparse + s;
+    l.units = b->sparse + s + s / kScanThreads;
+    l.unitCount = b->queue + kSparseUnitQueue;
+    return l;
+}
+
+int launch_group_scan(tblaeq_batch* b, uint64_t mesh, uint64_t* off, uint64_t* items, uint64_t* nItems,
+                      uint64_t rowMax) {
     tblaeq_ctx* ctx = b->ctx;
     const uint64_t tiles = (mesh + kScanThreads - 1) / kScanThreads;
     int rc = scan_prepare(ctx, tiles);
     if (rc) return rc;
     TimedScope ts(ctx, kKBatch0 + 2);
     hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream, b->cnt,
-                       mesh, off, items, b->itemCap, nItems, b->ctl + 3, ctx->status, ctx->ctrl, 0ull, nullptr,
-                       nullptr);
+                       mesh, off, items, b->itemCap, nItems, b->ctl + 3, ctx->status, ctx->ctrl, rowMax,
+                       rowMax ? sparse_list(b, mesh) : SparseList{});
     TB_HIP(hipGetLastError());
     return TBLAEQ_OK;
 }
@@ -1883,7 +1900,7 @@ int scan_counts(tblaeq_ctx* ctx, const uint64_t* cnt, uint64_t n, uint64_t* off,
     int rc = scan_prepare(ctx, tiles);
     if (rc) return rc;
     hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream, cnt, n,
-                       off, nullptr, 0ull, scratch2, scratch2 + 1, ctx->status, ctx->ctrl, 0ull, nullptr, nullptr);
+                       off, nullptr, 0ull, scratch2, scratch2 + 1, ctx->status, ctx->ctrl, 0ull, SparseList{});
     TB_HIP(hipGetLastError());
     return TBLAEQ_OK;
 }
@@ -1895,7 +1912,7 @@ int launch_count_scan(tblaeq_batch* b, const uint64_t* cnt, uint64_t n, uint64_t
     if (rc) return rc;
     TimedScope ts(ctx, kKBatch0 + 2);
     hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream, cnt, n,
-                       off, nullptr, 0ull, b->ctl + 6, b->ctl + 7, ctx->status, ctx->ctrl, 0ull, nullptr, nullptr);
+                       off, nullptr, 0ull, b->ctl + 6, b->ctl + 7, ctx->status, ctx->ctrl, 0ull, SparseList{});
     TB_HIP(hipGetLastError());
     return TBLAEQ_OK;
 }
@@ -2032,7 +2049,11 @@ extern "C" int tblaeq_batch_create(tblaeq_ctx* ctx, const tblaeq_level* levels, 
             return fail(hip_fail(hipGetLastError(), "batch create: kd order copy"));
         kd_leaf_order(hx, static_cast<int>(D), hcr, perm);
         std::vector<uint64_t> hlo(cols + 1, 0);
-        for (uint64_t c = 0; c < cols; ++c) hlo[c + 1] = hlo[c] + (hcr[c + 1] - hcr[c] + kLeafRows - 1) / kLeafRows;
+        for (uint64_t c = 0; c < cols; ++c) {
+            const uint64_t nl = (hcr[c + 1] - hcr[c] + kLeafRows - 1) / kLeafRows;
+            hlo[c + 1] = hlo[c] + nl;
+            b->maxLeaves = std::max(b->maxLeaves, nl);
+        }
         b->nLeaves = hlo[cols];
         uint64_t* dperm = balloc<uint64_t>(b, nf, &rc);
         if (rc) return fail(rc);
@@ -2051,7 +2072,7 @@ extern "C" int tblaeq_batch_create(tblaeq_ctx* ctx, const tblaeq_level* levels, 
         if (rc) return fail(rc);
         {
             const uint64_t m = std::max<uint64_t>(cols, b->count[L - 1]);
-            b->sparse = balloc<uint64_t>(b, sparse_slots(m) + (m + kScanThreads - 1) / kScanThreads, &rc);
+            b->sparse = balloc<uint64_t>(b, sparse_words(m), &rc);
         }
         if (rc) return fail(rc);
         if (hipMemcpy(dperm, perm.data(), nf * 8, hipMemcpyHostToDevice) != hipSuccess ||
@@ -2243,8 +2264,7 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
             const bool split = l == L - 1 && rowMax > 0;
             hipLaunchKernelGGL(k_group_scan, dim3(scan_launch_grid(ctx, tiles)), dim3(kScanThreads), 0, ctx->stream,
                                b->cnt, mesh, offS[nxt], itemS[nxt], b->itemCap, nS[nxt], ctl + 3, ctx->status,
-                               ctx->ctrl, split ? rowMax : 0ull, split ? b->sparse : nullptr,
-                               split ? b->sparse + sparse_slots(mesh) : nullptr);
+                               ctx->ctrl, split ? rowMax : 0ull, split ? sparse_list(b, mesh) : SparseList{});
         }
         A.survStat = nullptr;
         A.fineStat = nullptr;
@@ -2291,9 +2311,14 @@ extern "C" int tblaeq_batch_range(tblaeq_batch* b, const double* d_boxes, uint64
     A.queue = b->queue;
     A.masks = nullptr;
     A.rowMax = rowMax;
-    A.sparse = b->sparse;
-    A.sparseTiles = (b->count[L - 1] + kScanThreads - 1) / kScanThreads;
-    A.sparseCnt = b->sparse + sparse_slots(b->count[L - 1]);
+    {
+        const SparseList sl = sparse_list(b, b->count[L - 1]);
+        A.sparse = sl.slots;
+        A.sparseCnt = sl.counts;
+        A.sparseUnits = sl.units;
+        A.sparseUnitCount = sl.unitCount;
+        A.sparseTiles = (b->count[L - 1] + kScanThreads - 1) / kScanThreads;
+    }
     if ((rc = launch_range_step<kFinal>(b, A, b->itemCap))) return rc;
     if (A.rowMax && (rc = launch_range_rows(b, A, b->count[L - 1]))) return rc;
     // sort the (query, id) keys; the last pass writes the ids to the caller's

@@ -1272,6 +1272,8 @@ struct KnnClusterArgs {
     const uint32_t* qlist;
     uint64_t qcap;
     const uint64_t* colRes;   // P[L-1] column pointers
+    const uint64_t* leafOff;  // first kd leaf of every last-level cluster
+    const float* leafBox;     // outward-rounded float box lo[D] | hi[D] of every kd leaf
     const double* coords;     // finest positions
     const uint64_t* ids;
     const double* points;     // nq x D
@@ -1283,6 +1285,13 @@ struct KnnClusterArgs {
     uint64_t* cntQ;           // records per query
     uint64_t* overflow;
     uint64_t* work;           // distance evaluations (child rows x queries)
+    // row-major refine of the clusters refined by at most rowMax queries:
+    // their ids in per-tile slots of the group scan's list (kScanThreads slots
+    // and a count per tile)
+    const uint64_t* sparse;
+    const uint64_t* sparseCnt;
+    uint64_t sparseTiles;
+    uint64_t rowMax;
 };
 
 // One row test of the refine: acc against the lane's threshold (tD as a double;
@@ -1386,6 +1395,246 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_cluster(KnnClusterAr
     }
     if (lane == 0 && A.work) atomicAdd(reinterpret_cast<unsigned long long*>(A.work), wTests);
     (void)wRows;
+}
+
+// ------------------------------------------------------ row-major refine
+// Last-level clusters refined by at most kRowQ queries (most of them when the
+// queries are few against the clusters, e.g. a thousand KNN points over a
+// 500M-point index) leave lane = query mostly idle.  The group scan lists them
+// apart (no work items) and they are refined here: a unit of kRowUnit list
+// slots, lane = (cluster slot, query slot) pair, query points and thresholds
+// in LDS.  The kd leaves of the unit's clusters are then tested with lane =
+// (cluster slot, leaf) -- one box load per lane, against the slot's queries;
+// the (pair, leaf) tuples that a box cannot rule out are queued in LDS and scored kWave / kLeafRows at a time, lane =
+// (tuple, row of the leaf), with the same fma chain and threshold rule, so that
+// the records are the cluster-major refine's.
+constexpr int kRowQ = 4;
+static_assert(kRowUnit * kRowQ == kWave, "one list unit per wave pass");
+constexpr int kRowGroup = kWave / kLeafRows;  // tuples scored per wave pass
+constexpr int kRowRec = kWave;
+// tuple queue: fewer than kWave left before a leaf pass adds up to kWave * kRowQ
+constexpr int kRowTupCap = kWave + kWave * kRowQ;
+
+__device__ __forceinline__ uint64_t bperm_u64(uint64_t v, int src) {
+    const int lo = __builtin_amdgcn_ds_bpermute(src << 2, static_cast<int>(static_cast<uint32_t>(v)));
+    const int hi = __builtin_amdgcn_ds_bpermute(src << 2, static_cast<int>(static_cast<uint32_t>(v >> 32)));
+    return (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo);
+}
+__device__ __forceinline__ double bperm_f64(double v, int src) {
+    return __longlong_as_double(static_cast<long long>(bperm_u64(static_cast<uint64_t>(__double_as_longlong(v)), src)));
+}
+
+template <int DM>
+__global__ __launch_bounds__(kStepWaves * kWave) __attribute__((amdgpu_waves_per_eu(6))) void k_knn_rows(KnnClusterArgs A) {
+    __shared__ double s_qp[kStepWaves][kWave][DM];  // query point of every pair lane
+    __shared__ double s_td[kStepWaves][kWave];      // its threshold as a double (-inf: no pair)
+    __shared__ uint32_t s_tup[kStepWaves][kRowTupCap];
+    __shared__ uint64_t s_rk[kStepWaves][kRowRec];
+    __shared__ uint64_t s_rv[kStepWaves][kRowRec];
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint64_t gw = blockIdx.x * static_cast<uint64_t>(kStepWaves) + wave;
+    const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kStepWaves;
+    const int D = A.D;
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    const uint64_t nUnits = A.sparseTiles * (kScanThreads / kRowUnit);
+    double(*qs)[DM] = s_qp[wave];
+    double* tds = s_td[wave];
+    uint32_t* tup = s_tup[wave];
+    uint64_t* rk = s_rk[wave];
+    uint64_t* rv = s_rv[wave];
+    uint32_t staged = 0;
+    uint64_t wTests = 0;
+    // staged records: one reservation per flush; cntQ counts the stored ones
+    auto flush = [&]() {
+        if (staged == 0) return;
+        wave_lds_sync();
+        uint64_t base = 0;
+        if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(A.recCount), staged);
+        base = uniform_u64(__shfl(base, 0, kWave));
+        for (uint32_t o = lane; o < staged; o += kWave)
+            if (base + o < A.recCap) {
+                A.recK[base + o] = rk[o];
+                A.recV[base + o] = rv[o];
+                atomicAdd(reinterpret_cast<unsigned long long*>(&A.cntQ[rv[o] >> kRecIdBits]), 1ull);
+            }
+        if (lane == 0 && base + staged > A.recCap) atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 4ull);
+        wave_lds_sync();
+        staged = 0;
+    };
+    // the list entry of this lane's slot in unit uu and the tile's list end,
+    // loaded one unit ahead (the slot is read whatever the count: it lies in
+    // the list's allocation)
+    // units of kRowUnit list slots, wave-strided over a resident grid: step u
+    // is unit u / tiles of tile u % tiles, so that every wave meets the first
+    // units of tiles (the full ones) as often as the last
+    const uint64_t tiles = A.sparseTiles;
+    auto unit_slot = [&](uint64_t uu) {
+        return (uu % tiles) * kScanThreads + (uu / tiles) * kRowUnit;
+    };
+    uint64_t nextEnd = 0, nextC = 0;
+    auto fetch = [&](uint64_t uu) {
+        if (uu < nUnits) {
+            const uint64_t s0 = unit_slot(uu);
+            nextEnd = s0 / kScanThreads * kScanThreads + A.sparseCnt[s0 / kScanThreads];
+            nextC = A.sparse[s0 + lane / kRowQ];
+        }
+    };
+    fetch(gw);
+    for (uint64_t u = gw; u < nUnits; u += nw) {
+        const uint64_t s0 = unit_slot(u);
+        const uint64_t tileEnd = uniform_u64(nextEnd);
+        const uint64_t cl = nextC;
+        fetch(u + nw);
+        if (s0 >= tileEnd) continue;
+        const uint64_t j = lane % kRowQ;
+        const bool inSlot = s0 + lane / kRowQ < tileEnd;
+        uint64_t g0 = 0, gn = 0, jb = 0, je = 0, leaf0 = 0;
+        if (inSlot) {
+            g0 = A.off[cl];
+            gn = A.off[cl + 1] - g0;
+            jb = A.colRes[cl];
+            je = A.colRes[cl + 1];
+            leaf0 = A.leafOff[cl];
+        }
+        // list slots that were written (an overflowed list is rerun by the
+        // host) and hold a query below nq
+        const bool inList = inSlot && gn <= A.rowMax && j < gn && g0 + j < A.qcap;
+        const uint64_t ql = inList ? A.qlist[g0 + j] : 0;
+        const bool valid = inList && ql < A.nq;
+        const uint64_t q = valid ? ql : 0;
+        const uint64_t tK = valid ? A.thr[2 * q] : 0, tV = valid ? A.thr[2 * q + 1] : 0;
+#pragma unroll
+        for (int d = 0; d < DM; ++d) qs[lane][d] = (d < D) ? A.points[q * D + d] : 0.0;
+        tds[lane] = valid ? okey_inv(tK) : -__builtin_inf();
+        // (slot, leaf) pairs of the unit, slot-major: each slot's leaf count on
+        // its first lane (0 without a query), inclusive prefix over the slots
+        const uint64_t vm = __ballot(valid);
+        uint32_t x = (j == 0 && ((vm >> lane) & 0xFull)) ? static_cast<uint32_t>((je - jb + kLeafRows - 1) / kLeafRows)
+                                                          : 0u;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, kWave);
+            if (lane >= o) x += y;
+        }
+        const uint32_t total = __builtin_amdgcn_readlane(x, kWave - 1);
+        wave_lds_sync();
+        uint32_t queued = 0;
+        // scores tuples [0, n) of the queue (lane = tuple g, row r of its leaf)
+        auto score = [&](uint32_t n) {
+            wave_lds_sync();
+            for (uint32_t b0 = 0; b0 < n; b0 += kRowGroup) {
+                const uint32_t g = b0 + lane / kLeafRows;
+                const uint32_t tp = g < n ? tup[g] : 0u;
+                const int p = static_cast<int>(tp & (kWave - 1));
+                const uint64_t t = tp / kWave;
+                // (every lane takes part in a bpermute: an inactive source lane reads as 0)
+                const uint64_t pjb = bperm_u64(jb, p), pje = bperm_u64(je, p);
+                const uint64_t tKp = bperm_u64(tK, p), tVp = bperm_u64(tV, p);
+                const uint64_t qHi = bperm_u64(q, p) << kRecIdBits;
+                const uint64_t row = pjb + t * kLeafRows + (lane % kLeafRows);
+                const bool act = g < n && row < pje;
+                double xr[DM];
+#pragma unroll
+                for (int d = 0; d < DM; ++d) xr[d] = (act && d < D) ? A.coords[row * D + d] : 0.0;
+                const uint64_t id = act ? A.ids[row] : 0;
+                double acc = 0.0;
+#pragma unroll
+                for (int d = 0; d < DM; ++d) {
+                    if (d < D) {
+                        const double diff = qs[p][d] - xr[d];
+                        acc = __builtin_fma(diff, diff, acc);
+                    }
+                }
+                wTests += __popcll(__ballot(act));
+                const double tDp = tds[p];
+                bool emit = act && acc < tDp;
+                if (act && !(acc < tDp || acc > tDp)) {
+                    const uint64_t key = okey(acc);
+                    emit = key < tKp || (key == tKp && id < tVp);
+                }
+                const uint64_t em = __ballot(emit);
+                if (em) {
+                    if (staged + __popcll(em) > static_cast<uint32_t>(kRowRec)) flush();
+                    if (emit) {
+                        const uint32_t o = staged + __popcll(em & ltmask);
+                        rk[o] = okey(acc);
+                        rv[o] = qHi | id;
+                    }
+                    staged += __popcll(em);
+                }
+            }
+        };
+        for (uint32_t f0 = 0; f0 < total; f0 += kWave) {
+            // lane = (slot s, leaf t): the leaf's box against the slot's queries.
+            // The squared distance to the (outward-rounded) box bounds every
+            // row's fma-chain acc from below up to a relative 2(D+2) ulps of both
+            // chains (and an absolute 1e-300 for underflow), so a query whose
+            // threshold lies below it by more than that margin cannot emit any
+            // of the leaf's rows
+            const uint32_t f = f0 + lane;
+            const bool fv = f < total;
+            // slot of f: the slots whose inclusive prefix is at most f (binary
+            // search over the slots' last lanes)
+            int sl = 0;
+#pragma unroll
+            for (int step = kRowUnit / 2; step; step >>= 1) {
+                const uint32_t v = static_cast<uint32_t>(
+                    __builtin_amdgcn_ds_bpermute(((sl + step) * kRowQ - 1) << 2, static_cast<int>(x)));
+                if (f >= v) sl += step;
+            }
+            const uint32_t ex = static_cast<uint32_t>(
+                __builtin_amdgcn_ds_bpermute((sl * kRowQ - 1) << 2, static_cast<int>(x)));
+            const uint64_t t = f - (sl ? ex : 0u);
+            const uint64_t sLeaf0 = bperm_u64(leaf0, sl * kRowQ);
+            float lo[DM], hi[DM];
+#pragma unroll
+            for (int d = 0; d < DM; ++d) {
+                const float* bx = A.leafBox + (sLeaf0 + t) * 2 * D;
+                lo[d] = (fv && d < D) ? bx[d] : 0.0f;
+                hi[d] = (fv && d < D) ? bx[D + d] : 0.0f;
+            }
+#pragma unroll 1
+            for (int k = 0; k < kRowQ; ++k) {
+                const int pl = sl * kRowQ + k;
+                double lb = 0.0;
+#pragma unroll
+                for (int d = 0; d < DM; ++d) {
+                    if (d < D) {
+                        const double qd = qs[pl][d];
+                        const double g = __builtin_fmax(
+                            __builtin_fmax(static_cast<double>(lo[d]) - qd, qd - static_cast<double>(hi[d])), 0.0);
+                        lb = __builtin_fma(g, g, lb);
+                    }
+                }
+                const bool need = fv && !(lb * (1.0 - 1e-12) - 1e-300 > tds[pl]);
+                const uint64_t m = __ballot(need);
+                if (need) tup[queued + __popcll(m & ltmask)] = static_cast<uint32_t>(t * kWave) | pl;
+                queued += __popcll(m);
+            }
+            if (queued >= static_cast<uint32_t>(kWave)) {
+                // full passes now; the rest (fewer than kRowGroup) to the front
+                const uint32_t done = queued / kRowGroup * kRowGroup;
+                score(done);
+                const uint32_t rest = queued - done;
+                const uint32_t v = static_cast<uint32_t>(lane) < rest ? tup[done + lane] : 0u;
+                wave_lds_sync();
+                if (static_cast<uint32_t>(lane) < rest) tup[lane] = v;
+                queued = rest;
+            }
+        }
+        if (queued) score(queued);
+        wave_lds_sync();
+    }
+    flush();
+    if (lane == 0 && A.work) atomicAdd(reinterpret_cast<unsigned long long*>(A.work), wTests);
+}
+
+static auto knn_rows_kernel(int D) -> void (*)(KnnClusterArgs) {
+    if (D <= 4) return k_knn_rows<4>;
+    if (D <= 8) return k_knn_rows<8>;
+    if (D <= 16) return k_knn_rows<16>;
+    return k_knn_rows<32>;
 }
 
 // Counting-sort scatter of the records into per-query segments; leaves cntQ
@@ -1600,6 +1849,7 @@ const ParamSpec kParams[] = {
     {TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU, &tblaeq_batch::radixBlocksPerCu, 1, 32},
     {TBLAEQ_PARAM_HIT_STAGE, &tblaeq_batch::hitStage, 256, 1024},
     {TBLAEQ_PARAM_RANGE_ROW_QUERIES, &tblaeq_batch::rangeRowMax, 0, 64},
+    {TBLAEQ_PARAM_KNN_ROW_QUERIES, &tblaeq_batch::knnRowMax, 0, static_cast<uint64_t>(kRowQ)},
 };
 const ParamSpec* find_param(int id) {
     for (const auto& p : kParams)
@@ -1684,6 +1934,8 @@ extern "C" int tblaeq_batch_knn_graph(tblaeq_batch* b, const double* d_points, c
         // the workspace must not be allocated inside the capture
         int rc = ws_reserve(ctx, knn_ws_bytes(b, nq));
         if (rc) return rc;
+        // the row-major refine's occupancy, queried outside the capture
+        resident_blocks(reinterpret_cast<const void*>(knn_rows_kernel(b->D)), kStepWaves * kWave);
         const bool timing = ctx->timing;
         ctx->timing = false;  // no event records inside the graph
         const uint64_t before = ctx->scanLaunches;
@@ -1784,6 +2036,8 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
     if (stride == 0) stride = 1;
     // split: levels 0..L-2 query-major, the last level cluster-major
     const bool split = clusterMajor && L >= 2;
+    // (the row-major refine's LDS tuples hold a leaf index below 2^26)
+    const uint64_t rowMax = split && b->maxLeaves < (1ull << 26) ? b->knnRowMax : 0;
     int rc = ws_reserve(ctx, knn_ws_bytes(b, nq));
     if (rc) return rc;
     uint64_t* thr = static_cast<uint64_t*>(ctx->ws);
@@ -1968,7 +2222,7 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
         LA.masks = b->hits;  // free again: the records went to b->pairs / b->recAux
         launchLast(d_level_surv || d_fine_count ? kLStats : kLCount);
         TB_HIP(hipGetLastError());
-        if ((rc = launch_group_scan(b, b->count[L - 1], b->off2, b->items2, ctl + 4))) return rc;
+        if ((rc = launch_group_scan(b, b->count[L - 1], b->off2, b->items2, ctl + 4, rowMax))) return rc;
         LA.survStat = nullptr;
         LA.fineStat = nullptr;
         launchLast(kLWrite);
@@ -1985,6 +2239,12 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
     }
     if (clusterMajor) {
         KnnClusterArgs C{};
+        // clusters of the row-major list: only after a split last level
+        const SparseList sl = sparse_list(b, b->count[L - 1]);
+        C.rowMax = rowMax;
+        C.sparse = sl.slots;
+        C.sparseCnt = sl.counts;
+        C.sparseTiles = (b->count[L - 1] + kScanThreads - 1) / kScanThreads;
         C.D = b->D;
         C.nq = nq;
         C.items = rItems;
@@ -1994,6 +2254,8 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
         C.qlist = rQ;
         C.qcap = b->pairCap;
         C.colRes = b->colRes[L - 1];
+        C.leafOff = b->leafOff;
+        C.leafBox = b->leafBox;
         C.coords = b->coords[L];
         C.ids = b->ids[L];
         C.points = d_points;
@@ -2011,6 +2273,20 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
             else if (C.D <= 8) hipLaunchKernelGGL(k_knn_cluster<8>, sg, st, 0, ctx->stream, C);
             else if (C.D <= 16) hipLaunchKernelGGL(k_knn_cluster<16>, sg, st, 0, ctx->stream, C);
             else hipLaunchKernelGGL(k_knn_cluster<32>, sg, st, 0, ctx->stream, C);
+            TB_HIP(hipGetLastError());
+        }
+        if (rowMax) {
+            TimedScope ts(ctx, kKBatch0 + 13);
+            void (*k)(KnnClusterArgs) = knn_rows_kernel(C.D);
+            // wave-strided units: a resident grid, so that every wave's share is
+            // processed in one round
+            const uint64_t units = C.sparseTiles * (kScanThreads / kRowUnit);
+            uint64_t blocks = std::min<uint64_t>((units + kStepWaves - 1) / kStepWaves,
+                                                 static_cast<uint64_t>(ctx->cuCount) * b->stepBlocksPerCu);
+            blocks = std::min<uint64_t>(blocks, static_cast<uint64_t>(ctx->cuCount) *
+                                                    static_cast<uint64_t>(resident_blocks(
+                                                        reinterpret_cast<const void*>(k), kStepWaves * kWave)));
+            hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(blocks ? blocks : 1)), st, 0, ctx->stream, C);
             TB_HIP(hipGetLastError());
         }
         if ((rc = launch_count_scan(b, cntQ, nq, offQ))) return rc;

@@ -87,13 +87,15 @@ const char* param_name(int id) {
     case TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU: return "radix_blocks_per_cu";
     case TBLAEQ_PARAM_HIT_STAGE: return "hit_stage";
     case TBLAEQ_PARAM_RANGE_ROW_QUERIES: return "range_row_queries";
+    case TBLAEQ_PARAM_KNN_ROW_QUERIES: return "knn_row_queries";
     default: return nullptr;
     }
 }
 const int kAllParams[] = {TBLAEQ_PARAM_KNN_SEED,           TBLAEQ_PARAM_KNN_WG_PER_CU,
                           TBLAEQ_PARAM_KNN_MODE,           TBLAEQ_PARAM_RANGE_BLOCKS_PER_CU,
                           TBLAEQ_PARAM_STEP_BLOCKS_PER_CU, TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU,
-                          TBLAEQ_PARAM_HIT_STAGE,          TBLAEQ_PARAM_RANGE_ROW_QUERIES};
+                          TBLAEQ_PARAM_HIT_STAGE,          TBLAEQ_PARAM_RANGE_ROW_QUERIES,
+                          TBLAEQ_PARAM_KNN_ROW_QUERIES};
 
 }  // namespace
 }  // namespace tblaeq
@@ -178,7 +180,8 @@ extern "C" int tblaeq_batch_autotune(tblaeq_batch* b, const double* d_boxes, uin
                                            {TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU, {2, 4, 8, 16}}};
     const std::vector<Candidate> knnC = {{TBLAEQ_PARAM_KNN_SEED, {16, 32, 64, 128}},
                                          {TBLAEQ_PARAM_KNN_WG_PER_CU, {2, 4, 8}},
-                                         {TBLAEQ_PARAM_STEP_BLOCKS_PER_CU, {4, 8, 16, 32}}};
+                                         {TBLAEQ_PARAM_STEP_BLOCKS_PER_CU, {4, 8, 16, 32}},
+                                         {TBLAEQ_PARAM_KNN_ROW_QUERIES, {0, 1, 2, 4}}};
     for (int kind = 0; kind < 2; ++kind) {
         const bool range = kind == 0;
         if ((range && !n_boxes) || (!range && !n_points)) {

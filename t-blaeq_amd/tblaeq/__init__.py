@@ -162,6 +162,7 @@ PARAM_STEP_BLOCKS_PER_CU = 5
 PARAM_RADIX_BLOCKS_PER_CU = 6
 PARAM_HIT_STAGE = 7
 PARAM_RANGE_ROW_QUERIES = 8
+PARAM_KNN_ROW_QUERIES = 9
 
 
 class TblaeqError(RuntimeError):

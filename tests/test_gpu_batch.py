@@ -169,16 +169,43 @@ def test_batch_matches_plain_c1():
         assert np.array_equal(kres[i][1].view(np.uint64), pd2.view(np.uint64))
 
 
-@pytest.mark.parametrize("seed,wg", [(1, 1), (7, 16), (256, 2)])
-def test_batch_knn_launch_params(seed, wg):
-    """Seed cluster count and grid shape change only the work done, never the answer."""
+@pytest.mark.parametrize("seed,wg,rowq", [(1, 1, 0), (7, 16, 1), (256, 2, 4), (64, 4, 2)])
+def test_batch_knn_launch_params(seed, wg, rowq):
+    """Seed cluster count, grid shape and the row-major refine's query limit (rowq: clusters
+    refined by at most this many queries go row-major) change only the work done, never the
+    answer."""
     h, o = _make(dict(CFG, N=30000, is_int=True, seed=9), max_q=128)
     h.batch_set_param(tblaeq.PARAM_KNN_SEED, seed)
     h.batch_set_param(tblaeq.PARAM_KNN_WG_PER_CU, wg)
+    h.batch_set_param(tblaeq.PARAM_KNN_ROW_QUERIES, rowq)
+    assert h.batch_get_param(tblaeq.PARAM_KNN_ROW_QUERIES) == rowq
     data = h.data()
     pts = np.concatenate([data[:40] + 0.5, W.knn_points(data, 40, 77, h.mesh_sizes, CFG["vmin"], CFG["vmax"])])
     Ks = np.random.default_rng(seed).choice([1, 10, 64, 150], len(pts))
     _check_knn_batch(h, o, pts, Ks)
+
+
+def test_batch_knn_row_refine_takes_sparse_clusters():
+    """Few spread-out queries: most last-level clusters are refined by one to four of them and
+    go to the row-major refine, whose distance evaluations (only the rows of kd leaves that the
+    pair's own threshold cannot rule out) differ from the query-major count; answers equal the
+    oracle's either way."""
+    h, o = _make(dict(CFG, N=60000, seed=31), max_q=64)
+    pts = W.knn_points(h.data(), 24, 33, h.mesh_sizes, CFG["vmin"], CFG["vmax"], noise=0.5)
+    Ks = np.random.default_rng(34).choice([1, 10, 100], len(pts)).astype(np.uint64)
+    work = {}
+    for rowq in (0, 4):
+        h.batch_set_param(tblaeq.PARAM_KNN_ROW_QUERIES, rowq)
+        _check_knn_batch(h, o, pts, Ks)
+        h.batch_set("knn", pts, Ks)
+        h.batch_use_graph(False)
+        try:
+            h.batch_run()
+        finally:
+            h.batch_use_graph(True)
+        work[rowq] = h.batch_work()[0]
+    assert work[0] > 0 and work[4] > 0
+    assert work[4] != work[0]
 
 
 def test_batch_param_rejects_bad_values(idx):
@@ -187,6 +214,8 @@ def test_batch_param_rejects_bad_values(idx):
         h.batch_set_param(tblaeq.PARAM_KNN_SEED, 0)
     with pytest.raises(tblaeq.TblaeqError):
         h.batch_set_param(tblaeq.PARAM_KNN_WG_PER_CU, 17)
+    with pytest.raises(tblaeq.TblaeqError):
+        h.batch_set_param(tblaeq.PARAM_KNN_ROW_QUERIES, 5)
     with pytest.raises(tblaeq.TblaeqError):
         h.batch_set_param(99, 1)
 
@@ -230,7 +259,7 @@ def test_autotune_keeps_results_and_params_roundtrip(tmp_path):
     Ks = np.random.default_rng(2).choice([1, 10, 60], len(pts)).astype(np.uint64)
     ms = h.batch_autotune(boxes, pts, Ks, reps=1)
     assert ms["range_ms"][1] <= ms["range_ms"][0] and ms["knn_ms"][1] <= ms["knn_ms"][0]
-    params = {p: h.batch_get_param(p) for p in range(1, 8)}
+    params = {p: h.batch_get_param(p) for p in range(1, 10)}
     assert params[tblaeq.PARAM_HIT_STAGE] in (256, 512, 1024)
     _check_range_batch(h, o, boxes)
     h.batch_set("range", boxes)
@@ -244,7 +273,7 @@ def test_autotune_keeps_results_and_params_roundtrip(tmp_path):
     h.batch_set_param(tblaeq.PARAM_HIT_STAGE, 256 if params[tblaeq.PARAM_HIT_STAGE] != 256 else 1024)
     h.batch_set_param(tblaeq.PARAM_KNN_SEED, 3)
     h.batch_load_params(path)
-    assert {p: h.batch_get_param(p) for p in range(1, 8)} == params
+    assert {p: h.batch_get_param(p) for p in range(1, 10)} == params
     bad = tmp_path / "bad.txt"
     bad.write_text("hit_stage 300\n")
     with pytest.raises(tblaeq.TblaeqError):

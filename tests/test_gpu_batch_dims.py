@@ -56,11 +56,17 @@ def _range_dims(h, o):
     assert len(res[-3]) == h.N
 
 
-def test_knn_dims(idx):
+@pytest.mark.parametrize("rowq", [0, 1, 4])
+def test_knn_dims(idx, rowq):
+    """rowq: clusters refined by at most this many queries go to the row-major refine (0: none)."""
     h, o = idx
-    pts = W.knn_points(h.data(), 120, 50 + h.D, h.mesh_sizes, 1.0, 100.0, noise=0.2)
-    Ks = np.random.default_rng(h.D).choice([0, 1, 10, 64, 100, 200], len(pts))
-    check_knn_batch(h, o, pts, Ks)
+    h.batch_set_param(tblaeq.PARAM_KNN_ROW_QUERIES, rowq)
+    try:
+        pts = W.knn_points(h.data(), 120, 50 + h.D, h.mesh_sizes, 1.0, 100.0, noise=0.2)
+        Ks = np.random.default_rng(h.D).choice([0, 1, 10, 64, 100, 200], len(pts))
+        check_knn_batch(h, o, pts, Ks)
+    finally:
+        h.batch_set_param(tblaeq.PARAM_KNN_ROW_QUERIES, 4)
 
 
 def test_knn_query_major_dims(idx):
