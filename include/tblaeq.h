@@ -239,9 +239,9 @@ int tblaeq_batch_range_graph(tblaeq_batch* b, const double* d_boxes, uint64_t nq
  * level) as the per-query path.  d_status[0] = candidate records refined
  * cluster-major, d_status[1] = overflow flags (non-zero: a capacity of
  * tblaeq_batch_reserve was exceeded, results invalid; split the batch),
- * d_status[2] = distance evaluations of the cluster-major refine (rows of
- * kd leaves whose box some query of the work item cannot rule out, times the
- * item's queries),
+ * d_status[2] = distance evaluations of the refine (cluster-major: child
+ * rows times the work item's queries; row-major: for each query, the rows of
+ * the kd leaves its box test keeps),
  * d_status[3] = distance and bound evaluations of the query-major expansions
  * (seed clusters, last-level bound) (d_status holds 4 words).
  * Requires Kmax + empty clusters + 1 <= 256 and at most 8 levels (else

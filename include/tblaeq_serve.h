@@ -45,11 +45,13 @@ typedef struct {
     uint64_t max_k;              /* largest K served by the batched engine (default and upper limit:
                                     256 - empty clusters - 1); requests with a larger K (any K is
                                     accepted) run through the per-query traversal on the KNN worker */
-    uint64_t pair_cap;           /* (cluster, query) pairs per level, 0 = sized from free HBM (<= 2^26) */
+    uint64_t pair_cap;           /* (cluster, query) pairs per level, 0 = sized from free HBM (<= 2^28) */
     uint64_t hit_cap;            /* range hits per batch, 0 = sized from free HBM (<= 2^27); range
                                     batches are sized from the running mean of hits per request so
                                     that they fit, and a batch that overflows is split (into parts
-                                    sized from its hit count) and rerun */
+                                    sized from its hit count) and rerun; after an overflow of
+                                    either kind, the worker's batches start at the size that fit
+                                    and grow by a quarter per batch that fits */
     double hbm_fraction;         /* share of free HBM used as the in-flight budget (default 0.5) */
     uint64_t range_request_bytes; /* admission estimate per range request (default 4 MiB) */
     uint64_t knn_request_bytes;   /* admission estimate per KNN request (default 64 KiB) */

@@ -6,7 +6,8 @@
 // soon as its previous batch has completed, bounded by max_batch and by the
 // in-flight HBM budget shared by both workers.  A batch whose scratch would
 // overflow the engine's capacities is split in halves and rerun, exactly as
-// the handler's batched driver does.
+// the handler's batched driver does, and the worker's next batches start at
+// the size that fit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -78,6 +79,9 @@ struct Worker {
     Pinned<uint64_t> hK, hOff, hIds, hN;
     std::thread th;
     double hitsPerRequest = 0.0;  // range worker: running mean of hits per request (batch sizing)
+    // batch size limit after a capacity overflow: half the batch that
+    // overflowed, grown by a quarter after each batch that fits (0: none)
+    size_t sizeLimit = 0;
     // statistics (under the server lock after each batch)
     uint64_t completed = 0, batches = 0, reruns = 0, traversals = 0;
     double busyUs = 0.0;
@@ -177,6 +181,8 @@ void run_batch(tblaeq_server* s, Worker& w, std::vector<Request*>& batch, size_t
             const double need = static_cast<double>(status[0]) * 1.25 / static_cast<double>(s->cfg.hit_cap);
             parts = std::min<uint64_t>(n, std::max<uint64_t>(2, static_cast<uint64_t>(std::ceil(need))));
         }
+        // the next batches start at the size that fits, instead of overflowing again
+        w.sizeLimit = std::max<size_t>(1, n / parts);
         for (uint64_t k = 0; k < parts; ++k) {
             const size_t b0 = a + n * k / parts, b1 = a + n * (k + 1) / parts;
             if (b1 > b0) run_batch(s, w, batch, b0, b1 - b0);
@@ -254,6 +260,7 @@ void worker_loop(tblaeq_server* s, Worker& w) {
             if (w.type == 0 && w.hitsPerRequest > 0.0)
                 limit = static_cast<size_t>(std::clamp(0.7 * static_cast<double>(s->cfg.hit_cap) / w.hitsPerRequest,
                                                        1.0, static_cast<double>(s->cfg.max_batch)));
+            if (w.sizeLimit) limit = std::min(limit, w.sizeLimit);
             while (!q.empty() && batch.size() < limit) {
                 Request* r = q.front();
                 if (!batch.empty() || s->inflight > 0) {
@@ -296,7 +303,14 @@ void worker_loop(tblaeq_server* s, Worker& w) {
                     r->error = tblaeq_last_error();
                 }
             }
-            if (!batched.empty()) run_batch(s, w, batched, 0, batched.size());
+            if (!batched.empty()) {
+                const uint64_t reruns0 = w.reruns;
+                run_batch(s, w, batched, 0, batched.size());
+                if (w.sizeLimit && w.reruns == reruns0 && batched.size() >= w.sizeLimit) {
+                    w.sizeLimit += std::max<size_t>(1, w.sizeLimit / 4);
+                    if (w.sizeLimit >= s->cfg.max_batch) w.sizeLimit = 0;
+                }
+            }
         } catch (...) {
             rc = tblaeq_host::statusOfCurrentException("tblaeq_server batch");
             rcMsg = tblaeq_last_error();
@@ -580,11 +594,12 @@ int tblaeq_server_create(tblaeq_handler* h, const tblaeq_server_config* cfgIn, t
         s->maxK = cfg.max_k ? std::min<uint64_t>(cfg.max_k, kLimit) : kLimit;
         size_t freeB = 0, totalB = 0;
         hipCheck(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
-        // engine scratch from free HBM when not given: 1/256 of it per buffer, at most 16M entries
-        // pairs: 1/256 of free HBM (16 bytes an entry), at most 64M entries;
-        // hits: 1/128 of it, at most 128M (the range worker holds four
-        // hit-sized buffers)
-        if (!cfg.pair_cap) cfg.pair_cap = std::min<uint64_t>(std::max<uint64_t>(freeB / 256 / 16, 1ull << 20), 1ull << 26);
+        // engine scratch from free HBM when not given: pairs 1/64 of it per
+        // engine (16 bytes an entry), at most 256M entries (a KNN batch over a
+        // large index keeps tens of thousands of (cluster, query) pairs per
+        // query at the level above the last); hits: 1/128 of it, at most 128M
+        // (the range worker holds four hit-sized buffers)
+        if (!cfg.pair_cap) cfg.pair_cap = std::min<uint64_t>(std::max<uint64_t>(freeB / 64 / 16, 1ull << 20), 1ull << 28);
         if (!cfg.hit_cap) cfg.hit_cap = std::min<uint64_t>(std::max<uint64_t>(freeB / 128 / 8, 1ull << 20), 1ull << 27);
         const int device = idx.ctx->device;
         for (int k = 0; k < 2; ++k) {

@@ -1425,7 +1425,7 @@ __device__ __forceinline__ double bperm_f64(double v, int src) {
 }
 
 template <int DM>
-__global__ __launch_bounds__(kStepWaves * kWave) __attribute__((amdgpu_waves_per_eu(6))) void k_knn_rows(KnnClusterArgs A) {
+__global__ __launch_bounds__(kStepWaves * kWave) void k_knn_rows(KnnClusterArgs A) {
     __shared__ double s_qp[kStepWaves][kWave][DM];  // query point of every pair lane
     __shared__ double s_td[kStepWaves][kWave];      // its threshold as a double (-inf: no pair)
     __shared__ uint32_t s_tup[kStepWaves][kRowTupCap];

@@ -120,11 +120,16 @@ def test_serve_admission_defers_by_hbm_budget(idx):
 
 
 def test_serve_splits_batches_over_capacity(idx):
+    """Capacities far below a full batch's needs: overflowing batches are split and rerun, the
+    workers' later batches start at the size that fit, and every answer equals the roomy
+    server's."""
     h, o = idx
     reqs = _requests(h, 40, 13)
     srv = tblaeq.Server(h, max_batch=64, pair_cap=3000, hit_cap=4000)
     try:
         mixed = _mixed(srv, reqs, threads=2)
+        again = _mixed(srv, reqs, threads=2)
+        reruns = srv.kind_stats("range")["reruns"] + srv.kind_stats("knn")["reruns"]
     finally:
         srv.close()
     big = tblaeq.Server(h, max_batch=64)
@@ -132,4 +137,6 @@ def test_serve_splits_batches_over_capacity(idx):
         alone = _alone(big, reqs)
     finally:
         big.close()
+    assert reruns > 0
     assert _same(alone, mixed)
+    assert _same(alone, again)
