@@ -70,6 +70,9 @@ int hip_fail(hipError_t e, const char* what);
 int scan_prepare(tblaeq_ctx* ctx, uint64_t tiles);
 // Grows the context scratch to at least `bytes`.
 int ws_reserve(tblaeq_ctx* ctx, size_t bytes);
+// out[i] = the low 32 bits of in[i], i < n, on stream st (ids that fit in 32
+// bits, narrowed before they cross to the host).
+hipError_t narrow_ids(hipStream_t st, const uint64_t* in, uint32_t* out, uint64_t n);
 
 // Brackets one kernel launch with HIP events on the context stream when timing is on.
 struct TimedScope {

@@ -3,8 +3,10 @@
 // One worker thread per request kind owns a batched engine with its own HIP
 // stream, device query / result buffers and pinned staging.  Requests wait in
 // a FIFO per kind; a worker admits the waiting requests into its next batch as
-// soon as its previous batch has completed, bounded by max_batch and by the
-// in-flight HBM budget shared by both workers.  A batch whose scratch would
+// soon as its previous batch has completed on the GPU (the range worker's
+// results of one batch reach the host and their requests while the next one
+// runs), bounded by max_batch and by the in-flight HBM budget shared by both
+// workers.  A batch whose scratch would
 // overflow the engine's capacities is split in halves and rerun, exactly as
 // the handler's batched driver does, and the worker's next batches start at
 // the size that fit.
@@ -69,19 +71,40 @@ struct Pinned {
     }
 };
 
+// Range results of one batch on their way to the host.  The range worker
+// keeps two: a batch's ids (narrowed to 32 bits when every id of the index
+// fits) are copied right after the batch on the worker's stream and scattered
+// into its requests on the host while the next batch runs on the GPU.  (The
+// copies stay on the batch's stream: on a stream of their own they may share
+// a hardware queue with the next batch and wait for it.)
+struct Flight {
+    DeviceBuffer<uint64_t> dOff, dIds;
+    DeviceBuffer<uint32_t> dIds32;
+    Pinned<uint64_t> hOff, hIds;
+    Pinned<uint32_t> hIds32;
+    hipEvent_t copied = nullptr;
+    // the batch it carries
+    std::vector<Request*> batch, batched;
+    uint64_t hits = 0, est = 0, travs = 0;
+};
+
 struct Worker {
     int type = 0;
     tblaeq_ctx* ctx = nullptr;
     tblaeq_batch* b = nullptr;
     DeviceBuffer<double> dQ, dD2;
-    DeviceBuffer<uint64_t> dK, dOff, dIds, dN, dStatus;
+    DeviceBuffer<uint64_t> dK, dIds, dN, dStatus;
     Pinned<double> hQ, hD2;
-    Pinned<uint64_t> hK, hOff, hIds, hN;
+    Pinned<uint64_t> hK, hIds, hN, hStatus;
     std::thread th;
     double hitsPerRequest = 0.0;  // range worker: running mean of hits per request (batch sizing)
     // batch size limit after a capacity overflow: half the batch that
     // overflowed, grown by a quarter after each batch that fits (0: none)
     size_t sizeLimit = 0;
+    // range worker: result flights, ids sent as 32 bits
+    Flight fl[2];
+    int cur = 0;
+    bool narrow = false;
     // statistics (under the server lock after each batch)
     uint64_t completed = 0, batches = 0, reruns = 0, traversals = 0;
     double busyUs = 0.0;
@@ -89,7 +112,8 @@ struct Worker {
 
 // Copies out[i] = src[off[i] .. off[i+1]) for n requests, on several threads
 // when there are many ids.
-void scatter_results(const uint64_t* src, const uint64_t* off, size_t n, std::vector<Request*>& batch, size_t a) {
+template <class Id>
+void scatter_results(const Id* src, const uint64_t* off, size_t n, std::vector<Request*>& batch, size_t a) {
     const uint64_t total = off[n] - off[0];
     const unsigned threads = total < (4ull << 20) ? 1u : 8u;
     auto work = [&](size_t b, size_t e) {
@@ -139,9 +163,10 @@ struct tblaeq_server {
 
 namespace {
 
-// Runs requests [a, a+n) of `batch` on the worker's engine, splitting on overflow.
-void run_batch(tblaeq_server* s, Worker& w, std::vector<Request*>& batch, size_t a, size_t n) {
-    const size_t D = s->D;
+// Stages requests [a, a+n) of `batch` and launches the engine batch on the
+// worker's stream (range results into flight f's device buffers); the status
+// words follow into w.hStatus.  Returns Kmax for KNN.
+uint64_t launch_batch(tblaeq_server* s, Worker& w, Flight* f, std::vector<Request*>& batch, size_t a, size_t n) {
     hipStream_t st = static_cast<hipStream_t>(tblaeq_ctx_stream(w.ctx));
     uint64_t Kmax = 0;
     for (size_t i = 0; i < n; ++i) {
@@ -152,12 +177,13 @@ void run_batch(tblaeq_server* s, Worker& w, std::vector<Request*>& batch, size_t
             Kmax = std::max(Kmax, r->K);
         }
     }
+    const size_t D = s->D;
     const size_t qw = w.type == 0 ? 2 * D : D;
     hipCheck(hipMemcpyAsync(w.dQ.data(), w.hQ.p, n * qw * sizeof(double), hipMemcpyHostToDevice, st), "H2D");
     int rc;
     if (w.type == 0) {
-        rc = tblaeq_batch_range(w.b, w.dQ.data(), n, w.dOff.data(), w.dIds.data(), w.dIds.size(), w.dStatus.data(),
-                                nullptr, nullptr);
+        rc = tblaeq_batch_range(w.b, w.dQ.data(), n, f->dOff.data(), f->dIds.data(), f->dIds.size(),
+                                w.dStatus.data(), nullptr, nullptr);
     } else {
         if (Kmax == 0) Kmax = 1;
         hipCheck(hipMemcpyAsync(w.dK.data(), w.hK.p, n * 8, hipMemcpyHostToDevice, st), "H2D");
@@ -165,39 +191,71 @@ void run_batch(tblaeq_server* s, Worker& w, std::vector<Request*>& batch, size_t
                               w.dStatus.data(), nullptr, nullptr);
     }
     tbCheck(rc, w.type == 0 ? "serve: range batch" : "serve: KNN batch");
-    uint64_t status[4];
-    hipCheck(hipMemcpyAsync(status, w.dStatus.data(), 32, hipMemcpyDeviceToHost, st), "D2H");
+    hipCheck(hipMemcpyAsync(w.hStatus.p, w.dStatus.data(), 32, hipMemcpyDeviceToHost, st), "D2H");
+    return Kmax;
+}
+
+// Queues the copies of flight f's range results (n requests, f->hits ids) on
+// the worker's stream, after the engine batch that wrote them.
+void send_range(Worker& w, Flight& f, size_t n) {
+    hipStream_t st = static_cast<hipStream_t>(tblaeq_ctx_stream(w.ctx));
+    if (w.narrow && f.hits) hipCheck(tblaeq::narrow_ids(st, f.dIds.data(), f.dIds32.data(), f.hits), "narrow ids");
+    hipCheck(hipMemcpyAsync(f.hOff.p, f.dOff.data(), (n + 1) * 8, hipMemcpyDeviceToHost, st), "D2H");
+    if (w.narrow)
+        hipCheck(hipMemcpyAsync(f.hIds32.p, f.dIds32.data(), f.hits * 4, hipMemcpyDeviceToHost, st), "D2H");
+    else
+        hipCheck(hipMemcpyAsync(f.hIds.p, f.dIds.data(), f.hits * 8, hipMemcpyDeviceToHost, st), "D2H");
+    hipCheck(hipEventRecord(f.copied, st), "event");
+}
+
+// Waits for flight f's copies and fills requests [a, a+n) of `batch`.
+void receive_range(Worker& w, Flight& f, std::vector<Request*>& batch, size_t a, size_t n) {
+    hipCheck(hipEventSynchronize(f.copied), "event sync");
+    if (w.narrow) scatter_results(f.hIds32.p, f.hOff.p, n, batch, a);
+    else scatter_results(f.hIds.p, f.hOff.p, n, batch, a);
+    // running mean of hits per request sizes the next batches
+    const double per = static_cast<double>(f.hits) / static_cast<double>(n);
+    w.hitsPerRequest = w.hitsPerRequest > 0.0 ? 0.8 * w.hitsPerRequest + 0.2 * per : per;
+}
+
+// After a batch overflowed the engine's capacities (status in w.hStatus):
+// reruns it in parts, synchronously.
+void run_batch(tblaeq_server* s, Worker& w, Flight* f, std::vector<Request*>& batch, size_t a, size_t n);
+void split_batch(tblaeq_server* s, Worker& w, Flight* f, std::vector<Request*>& batch, size_t a, size_t n) {
+    if (n == 1) {
+        batch[a]->status = TBLAEQ_ERR_CAPACITY;
+        return;
+    }
+    // only the hit list overflowed: status[0] still counts every hit, so
+    // split once into parts that each fit with a margin; otherwise halve
+    ++w.reruns;
+    uint64_t parts = 2;
+    if (w.type == 0 && w.hStatus.p[1] == 4) {
+        const double need = static_cast<double>(w.hStatus.p[0]) * 1.25 / static_cast<double>(s->cfg.hit_cap);
+        parts = std::min<uint64_t>(n, std::max<uint64_t>(2, static_cast<uint64_t>(std::ceil(need))));
+    }
+    // the next batches start at the size that fits, instead of overflowing again
+    w.sizeLimit = std::max<size_t>(1, n / parts);
+    for (uint64_t k = 0; k < parts; ++k) {
+        const size_t b0 = a + n * k / parts, b1 = a + n * (k + 1) / parts;
+        if (b1 > b0) run_batch(s, w, f, batch, b0, b1 - b0);
+    }
+}
+
+// Runs requests [a, a+n) of `batch` on the worker's engine to completion,
+// splitting on overflow (range results through flight f).
+void run_batch(tblaeq_server* s, Worker& w, Flight* f, std::vector<Request*>& batch, size_t a, size_t n) {
+    hipStream_t st = static_cast<hipStream_t>(tblaeq_ctx_stream(w.ctx));
+    const uint64_t Kmax = launch_batch(s, w, f, batch, a, n);
     hipCheck(hipStreamSynchronize(st), "sync");
-    if (status[1] != 0) {
-        if (n == 1) {
-            batch[a]->status = TBLAEQ_ERR_CAPACITY;
-            return;
-        }
-        // only the hit list overflowed: status[0] still counts every hit, so
-        // split once into parts that each fit with a margin; otherwise halve
-        ++w.reruns;
-        uint64_t parts = 2;
-        if (w.type == 0 && status[1] == 4) {
-            const double need = static_cast<double>(status[0]) * 1.25 / static_cast<double>(s->cfg.hit_cap);
-            parts = std::min<uint64_t>(n, std::max<uint64_t>(2, static_cast<uint64_t>(std::ceil(need))));
-        }
-        // the next batches start at the size that fits, instead of overflowing again
-        w.sizeLimit = std::max<size_t>(1, n / parts);
-        for (uint64_t k = 0; k < parts; ++k) {
-            const size_t b0 = a + n * k / parts, b1 = a + n * (k + 1) / parts;
-            if (b1 > b0) run_batch(s, w, batch, b0, b1 - b0);
-        }
+    if (w.hStatus.p[1] != 0) {
+        split_batch(s, w, f, batch, a, n);
         return;
     }
     if (w.type == 0) {
-        const uint64_t hits = status[0];
-        hipCheck(hipMemcpyAsync(w.hOff.p, w.dOff.data(), (n + 1) * 8, hipMemcpyDeviceToHost, st), "D2H");
-        hipCheck(hipMemcpyAsync(w.hIds.p, w.dIds.data(), hits * 8, hipMemcpyDeviceToHost, st), "D2H");
-        hipCheck(hipStreamSynchronize(st), "sync");
-        scatter_results(w.hIds.p, w.hOff.p, n, batch, a);
-        // running mean of hits per request sizes the next batches
-        const double per = static_cast<double>(hits) / static_cast<double>(n);
-        w.hitsPerRequest = w.hitsPerRequest > 0.0 ? 0.8 * w.hitsPerRequest + 0.2 * per : per;
+        f->hits = w.hStatus.p[0];
+        send_range(w, *f, n);
+        receive_range(w, *f, batch, a, n);
     } else {
         hipCheck(hipMemcpyAsync(w.hN.p, w.dN.data(), n * 8, hipMemcpyDeviceToHost, st), "D2H");
         hipCheck(hipMemcpyAsync(w.hIds.p, w.dIds.data(), n * Kmax * 8, hipMemcpyDeviceToHost, st), "D2H");
@@ -236,55 +294,146 @@ void run_traversal(tblaeq_server* s, Request* r) {
     r->policy = qr.memoryPolicy;
 }
 
+// Device buffers, events and streams of a worker, then its context (the
+// engine has been destroyed).
+void release_worker(Worker& w) {
+    w.dQ = DeviceBuffer<double>();
+    w.dD2 = DeviceBuffer<double>();
+    w.dK = DeviceBuffer<uint64_t>();
+    w.dIds = DeviceBuffer<uint64_t>();
+    w.dN = DeviceBuffer<uint64_t>();
+    w.dStatus = DeviceBuffer<uint64_t>();
+    for (Flight& f : w.fl) {
+        f.dOff = DeviceBuffer<uint64_t>();
+        f.dIds = DeviceBuffer<uint64_t>();
+        f.dIds32 = DeviceBuffer<uint32_t>();
+        if (f.copied) (void)hipEventDestroy(f.copied);
+        f.copied = nullptr;
+    }
+    if (w.ctx) tblaeq_ctx_destroy(w.ctx);
+    w.ctx = nullptr;
+}
+
+// Marks a batch's requests done (rc for those without their own status) and
+// updates the statistics and the in-flight budget.
+void complete_batch(tblaeq_server* s, Worker& w, std::vector<Request*>& batch, uint64_t est, uint64_t travs, int rc,
+                    const std::string& rcMsg) {
+    const auto now = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (Request* r : batch) {
+            if (rc && r->status == TBLAEQ_OK) {
+                r->status = rc;
+                r->error = rcMsg;
+            }
+            r->finished = now;
+            r->done = true;
+            s->latUs.push_back(std::chrono::duration<double, std::micro>(now - r->submitted).count());
+        }
+        for (Request* r : batch)
+            if (r->orphan) s->reqs.erase(r->ticket);
+        s->inflight -= est;
+        s->batches += 1;
+        s->batchedReqs += batch.size();
+        w.completed += batch.size();
+        w.batches += 1;
+        w.traversals += travs;
+    }
+    s->cvDone.notify_all();
+    s->cvWork.notify_all();
+}
+
+// Completes the range batch in flight f: its ids into its requests.
+void finish_flight(tblaeq_server* s, Worker& w, Flight& f) {
+    int rc = TBLAEQ_OK;
+    std::string rcMsg;
+    try {
+        receive_range(w, f, f.batched, 0, f.batched.size());
+    } catch (...) {
+        rc = tblaeq_host::statusOfCurrentException("tblaeq_server batch");
+        rcMsg = tblaeq_last_error();
+    }
+    complete_batch(s, w, f.batch, f.est, f.travs, rc, rcMsg);
+    f.batch.clear();
+    f.batched.clear();
+}
+
 void worker_loop(tblaeq_server* s, Worker& w) {
     if (hipSetDevice(w.ctx->device) != hipSuccess) return;
     const double maxWait = s->cfg.max_wait_us;
+    hipStream_t st = static_cast<hipStream_t>(tblaeq_ctx_stream(w.ctx));
+    // range: the flight whose results are still on their way (its requests
+    // complete while the next batch runs)
+    Flight* pending = nullptr;
+    auto addBusy = [&](Clock::time_point t0) {
+        const double us = std::chrono::duration<double, std::micro>(Clock::now() - t0).count();
+        std::lock_guard<std::mutex> lk(s->mu);
+        w.busyUs += us;
+    };
     for (;;) {
         std::vector<Request*> batch;
         uint64_t est = 0;
+        bool drain = false;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cvWork.wait(lk, [&] { return s->stop || !s->queue[w.type].empty(); });
-            if (s->queue[w.type].empty()) {
-                if (s->stop) return;
-                continue;
-            }
-            if (maxWait > 0 && s->queue[w.type].size() < s->cfg.max_batch && !s->stop) {
-                s->cvWork.wait_for(lk, std::chrono::duration<double, std::micro>(maxWait),
-                                   [&] { return s->stop || s->queue[w.type].size() >= s->cfg.max_batch; });
-            }
-            auto& q = s->queue[w.type];
-            // range batches are sized so that their expected hits fit the
-            // hit list (a batch that overflows is split and rerun)
-            size_t limit = s->cfg.max_batch;
-            if (w.type == 0 && w.hitsPerRequest > 0.0)
-                limit = static_cast<size_t>(std::clamp(0.7 * static_cast<double>(s->cfg.hit_cap) / w.hitsPerRequest,
-                                                       1.0, static_cast<double>(s->cfg.max_batch)));
-            if (w.sizeLimit) limit = std::min(limit, w.sizeLimit);
-            while (!q.empty() && batch.size() < limit) {
-                Request* r = q.front();
-                if (!batch.empty() || s->inflight > 0) {
-                    if (s->inflight + est + r->est > s->budget) {
-                        ++s->deferred;
-                        break;  // wait for memory to free up
+            if (pending && (s->queue[w.type].empty() || maxWait > 0 || s->stop)) {
+                drain = true;  // nothing to overlap with: complete the flight first
+            } else {
+                s->cvWork.wait(lk, [&] { return s->stop || !s->queue[w.type].empty(); });
+                if (s->queue[w.type].empty()) {
+                    if (s->stop) return;
+                    continue;
+                }
+                if (maxWait > 0 && s->queue[w.type].size() < s->cfg.max_batch && !s->stop) {
+                    s->cvWork.wait_for(lk, std::chrono::duration<double, std::micro>(maxWait),
+                                       [&] { return s->stop || s->queue[w.type].size() >= s->cfg.max_batch; });
+                }
+                auto& q = s->queue[w.type];
+                // range batches are sized so that their expected hits fit the
+                // hit list (a batch that overflows is split and rerun)
+                size_t limit = s->cfg.max_batch;
+                if (w.type == 0 && w.hitsPerRequest > 0.0)
+                    limit = static_cast<size_t>(std::clamp(
+                        0.7 * static_cast<double>(s->cfg.hit_cap) / w.hitsPerRequest, 1.0,
+                        static_cast<double>(s->cfg.max_batch)));
+                if (w.sizeLimit) limit = std::min(limit, w.sizeLimit);
+                while (!q.empty() && batch.size() < limit) {
+                    Request* r = q.front();
+                    if (!batch.empty() || s->inflight > 0) {
+                        if (s->inflight + est + r->est > s->budget) {
+                            ++s->deferred;
+                            break;  // wait for memory to free up
+                        }
+                    }
+                    q.pop_front();
+                    est += r->est;
+                    batch.push_back(r);
+                }
+                if (batch.empty()) {
+                    if (pending) {
+                        drain = true;  // the budget is held by this worker's own flight
+                    } else {
+                        // the budget is held by the other worker's batch: wait for it to complete
+                        s->cvDone.wait(lk, [&] { return s->inflight == 0; });
+                        continue;
                     }
                 }
-                q.pop_front();
-                est += r->est;
-                batch.push_back(r);
+                s->inflight += est;
+                s->peakInflight = std::max(s->peakInflight, s->inflight);
             }
-            if (batch.empty()) {
-                // the budget is held by the other worker's batch: wait for it to complete
-                s->cvDone.wait(lk, [&] { return s->inflight == 0; });
-                continue;
-            }
-            s->inflight += est;
-            s->peakInflight = std::max(s->peakInflight, s->inflight);
+        }
+        if (drain) {
+            const auto t0 = Clock::now();
+            finish_flight(s, w, *pending);
+            pending = nullptr;
+            addBusy(t0);
+            continue;
         }
         int rc = TBLAEQ_OK;
         std::string rcMsg;
         const auto busy0 = Clock::now();
         uint64_t travs = 0;
+        bool inFlight = false;
         try {
             // traversal requests, and KNN above the batched kernels' K limit,
             // run the per-query traversal; the rest form the engine's batch
@@ -305,8 +454,33 @@ void worker_loop(tblaeq_server* s, Worker& w) {
             }
             if (!batched.empty()) {
                 const uint64_t reruns0 = w.reruns;
-                run_batch(s, w, batched, 0, batched.size());
-                if (w.sizeLimit && w.reruns == reruns0 && batched.size() >= w.sizeLimit) {
+                if (w.type == 0) {
+                    // launch, complete the previous flight while the GPU runs,
+                    // then send this batch's results on their way
+                    Flight& f = w.fl[w.cur];
+                    launch_batch(s, w, &f, batched, 0, batched.size());
+                    if (pending) {
+                        finish_flight(s, w, *pending);
+                        pending = nullptr;
+                    }
+                    hipCheck(hipStreamSynchronize(st), "sync");
+                    if (w.hStatus.p[1] != 0) {
+                        split_batch(s, w, &f, batched, 0, batched.size());
+                    } else {
+                        f.hits = w.hStatus.p[0];
+                        send_range(w, f, batched.size());
+                        f.batch = batch;
+                        f.batched = std::move(batched);
+                        f.est = est;
+                        f.travs = travs;
+                        pending = &f;
+                        w.cur ^= 1;
+                        inFlight = true;
+                    }
+                } else {
+                    run_batch(s, w, nullptr, batched, 0, batched.size());
+                }
+                if (w.sizeLimit && w.reruns == reruns0 && batch.size() >= w.sizeLimit) {
                     w.sizeLimit += std::max<size_t>(1, w.sizeLimit / 4);
                     if (w.sizeLimit >= s->cfg.max_batch) w.sizeLimit = 0;
                 }
@@ -314,31 +488,13 @@ void worker_loop(tblaeq_server* s, Worker& w) {
         } catch (...) {
             rc = tblaeq_host::statusOfCurrentException("tblaeq_server batch");
             rcMsg = tblaeq_last_error();
-        }
-        const auto now = Clock::now();
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            for (Request* r : batch) {
-                if (rc && r->status == TBLAEQ_OK) {
-                    r->status = rc;
-                    r->error = rcMsg;
-                }
-                r->finished = now;
-                r->done = true;
-                s->latUs.push_back(std::chrono::duration<double, std::micro>(now - r->submitted).count());
+            if (pending) {
+                finish_flight(s, w, *pending);
+                pending = nullptr;
             }
-            for (Request* r : batch)
-                if (r->orphan) s->reqs.erase(r->ticket);
-            s->inflight -= est;
-            s->batches += 1;
-            s->batchedReqs += batch.size();
-            w.completed += batch.size();
-            w.batches += 1;
-            w.traversals += travs;
-            w.busyUs += std::chrono::duration<double, std::micro>(now - busy0).count();
         }
-        s->cvDone.notify_all();
-        s->cvWork.notify_all();
+        if (!inFlight) complete_batch(s, w, batch, est, travs, rc, rcMsg);
+        addBusy(busy0);
     }
 }
 
@@ -617,11 +773,23 @@ int tblaeq_server_create(tblaeq_handler* h, const tblaeq_server_config* cfgIn, t
             w.dQ.resize(cfg.max_batch * qw);
             w.hQ.alloc(cfg.max_batch * qw);
             w.dStatus.resize(4);
+            w.hStatus.alloc(4);
             if (k == 0) {
-                w.dOff.resize(cfg.max_batch + 1);
-                w.dIds.resize(cfg.hit_cap);
-                w.hOff.alloc(cfg.max_batch + 1);
-                w.hIds.alloc(cfg.hit_cap);
+                // two result flights; ids cross to the host as 32 bits when the
+                // index's ids fit
+                w.narrow = idx.N <= (1ull << 32);
+                for (Flight& f : w.fl) {
+                    f.dOff.resize(cfg.max_batch + 1);
+                    f.dIds.resize(cfg.hit_cap);
+                    f.hOff.alloc(cfg.max_batch + 1);
+                    if (w.narrow) {
+                        f.dIds32.resize(cfg.hit_cap);
+                        f.hIds32.alloc(cfg.hit_cap);
+                    } else {
+                        f.hIds.alloc(cfg.hit_cap);
+                    }
+                    hipCheck(hipEventCreateWithFlags(&f.copied, hipEventDisableTiming), "event");
+                }
             } else {
                 const size_t rows = cfg.max_batch * std::max<uint64_t>(s->maxK, 1);
                 w.dK.resize(cfg.max_batch);
@@ -640,7 +808,8 @@ int tblaeq_server_create(tblaeq_handler* h, const tblaeq_server_config* cfgIn, t
             static const int kParamIds[] = {TBLAEQ_PARAM_KNN_SEED,           TBLAEQ_PARAM_KNN_WG_PER_CU,
                                             TBLAEQ_PARAM_KNN_MODE,           TBLAEQ_PARAM_RANGE_BLOCKS_PER_CU,
                                             TBLAEQ_PARAM_STEP_BLOCKS_PER_CU, TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU,
-                                            TBLAEQ_PARAM_HIT_STAGE,          TBLAEQ_PARAM_RANGE_ROW_QUERIES};
+                                            TBLAEQ_PARAM_HIT_STAGE,          TBLAEQ_PARAM_RANGE_ROW_QUERIES,
+                                            TBLAEQ_PARAM_KNN_ROW_QUERIES};
             for (int id : kParamIds) {
                 uint64_t v = 0;
                 tbCheck(tblaeq_batch_get_param(h->slots[0]->b, id, &v), "serve: launch parameter");
@@ -656,7 +825,7 @@ int tblaeq_server_create(tblaeq_handler* h, const tblaeq_server_config* cfgIn, t
         const int rc = tblaeq_host::statusOfCurrentException("tblaeq_server_create");
         for (auto& w : s->w) {
             if (w.b) tblaeq_batch_destroy(w.b);
-            if (w.ctx) tblaeq_ctx_destroy(w.ctx);
+            release_worker(w);
         }
         return rc;
     }
@@ -678,18 +847,7 @@ int tblaeq_server_destroy(tblaeq_server* s) {
         if (w.b) tblaeq_batch_destroy(w.b);
         w.b = nullptr;
     }
-    for (auto& w : s->w) {
-        // buffers go before the contexts
-        w.dQ = DeviceBuffer<double>();
-        w.dD2 = DeviceBuffer<double>();
-        w.dK = DeviceBuffer<uint64_t>();
-        w.dOff = DeviceBuffer<uint64_t>();
-        w.dIds = DeviceBuffer<uint64_t>();
-        w.dN = DeviceBuffer<uint64_t>();
-        w.dStatus = DeviceBuffer<uint64_t>();
-        if (w.ctx) tblaeq_ctx_destroy(w.ctx);
-        w.ctx = nullptr;
-    }
+    for (auto& w : s->w) release_worker(w);
     delete s;
     return TBLAEQ_OK;
 }

@@ -1917,6 +1917,22 @@ int launch_count_scan(tblaeq_batch* b, const uint64_t* cnt, uint64_t n, uint64_t
     return TBLAEQ_OK;
 }
 
+__global__ void k_narrow_ids(const uint64_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t n) {
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        out[i] = static_cast<uint32_t>(in[i]);
+}
+
+hipError_t narrow_ids(hipStream_t st, const uint64_t* in, uint32_t* out, uint64_t n) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
+    const uint64_t g = std::min<uint64_t>((n + 255) / 256, static_cast<uint64_t>(cus) * 8);
+    hipLaunchKernelGGL(k_narrow_ids, dim3(static_cast<unsigned>(g ? g : 1)), dim3(256), 0, st, in, out, n);
+    return hipGetLastError();
+}
+
 __global__ void k_fill_u64(uint64_t* p, uint64_t v, uint64_t n) {
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
