@@ -243,7 +243,45 @@ def run_plain_mix(h, boxes, pts, Ks, n, seed):
             "p99_us": round(float(np.percentile(lat, 99)), 1)}
 
 
-def run_c5(n, rates, seconds, plain_n, log):
+BATCH_KINDS = (("range_prune", 8), ("range_final", 9), ("range_rows", 20), ("group_scan", 10),
+               ("group_scatter", 11), ("radix_sort", 12), ("knn_levels", 13), ("knn_last_passes", 17),
+               ("knn_select", 18), ("knn_refine", 15), ("knn_rows", 21), ("knn_merge", 16))
+
+
+def batch_kernels(h, boxes, pts, Ks, reps, log):
+    """Per-kernel times (ms, HIP events on the library's stream, plain launches, mean of `reps`
+    runs after one warm-up) of one range batch of `boxes` and one KNN batch of `pts` with K `Ks`,
+    on the handler's batched engine with its current launch parameters."""
+    import tblaeq
+
+    timing = tblaeq.Timing(h.ctx())
+    h.batch_use_graph(False)
+    out = {}
+    try:
+        for kind, args in (("range", (boxes,)), ("knn", (pts, Ks))):
+            h.batch_set(kind, *args)
+            res = h.batch_run()
+            acc = {name: 0.0 for name, _ in BATCH_KINDS}
+            wall = []
+            for _ in range(reps):
+                timing.enable(True)
+                timing.reset()
+                t0 = time.perf_counter()
+                h.batch_run()
+                wall.append(time.perf_counter() - t0)
+                for name, k in BATCH_KINDS:
+                    acc[name] += timing.get(k)[1]
+            timing.enable(False)
+            out[kind] = {"queries": len(args[0]), "batch_ms": round(float(np.mean(wall)) * 1e3, 3),
+                         "results": int(res), "work": h.batch_work(),
+                         "kernels_ms": {k: round(v / reps, 3) for k, v in acc.items() if v > 0}}
+            log("batch kernels", kind, out[kind])
+    finally:
+        h.batch_use_graph(True)
+    return out
+
+
+def run_c5(n, rates, seconds, plain_n, log, tune=True):
     """The serving configuration: N FP64 points (500M by default), D = 8, height 5, ratios
     100,50,20,10, every level resident under policy L0, built by the parallel synthetic builder
     (the generator's shape from per-chunk streams).  The serving mix comes from a 1M-row sample of
@@ -275,17 +313,24 @@ def run_c5(n, rates, seconds, plain_n, log):
         ns = 512  # sample of the mix: half the serving batches' size, within the sample engine's capacities
         h.batch_init(ns, 1 << 27, 1 << 27)
         sel = np.nonzero(Ks[:ns] <= 200)[0]  # within the batched KNN kernels' limit
-        tuned = h.batch_autotune(boxes[:ns], pts[sel], Ks[sel].astype(np.uint64), reps=3)
+        tuned = h.batch_autotune(boxes[:ns], pts[sel], Ks[sel].astype(np.uint64), reps=3) if tune else None
         params = {name: h.batch_get_param(pid) for name, pid in PARAMS}
         log("C5: autotune", tuned, params)
+        # one 1024-request range batch and one KNN batch of the mix (K capped at 200, the batched
+        # kernels' range), per kernel, with the tuned parameters
+        h.batch_init(len(boxes), 1 << 27, 1 << 27)  # a new engine: the tuned values are set again
+        for name, pid in PARAMS:
+            h.batch_set_param(pid, params[name])
+        kernels = batch_kernels(h, boxes, pts, np.minimum(Ks, 200).astype(np.uint64), 3, log)
         runs = run_serving(h, boxes, pts, Ks, rates, seconds, log, tag="C5 serving")
         out = {"workload": "synthetic hierarchy N=%d D=8 height 5 ratios 100,50,20,10 sigma/2.2 seed %d (parallel "
                            "builder), policy L0; open-loop Poisson stream, 70%% range (selectivity log-uniform "
                            "1e-6..1e-3), 30%% KNN (K log-uniform 1..256)" % (n, cfg["seed"]),
-               "build_s": round(build_s, 1), "prepare_s": round(prep_s, 1), "plain": plain, "runs": runs,
+               "build_s": round(build_s, 1), "prepare_s": round(prep_s, 1), "plain": plain,
+               "batch_kernels": kernels, "runs": runs,
                "launch_params": params,
-               "autotune_range_ms_before_after": [round(float(x), 3) for x in tuned["range_ms"]],
-               "autotune_knn_ms_before_after": [round(float(x), 3) for x in tuned["knn_ms"]],
+               "autotune_range_ms_before_after": tuned and [round(float(x), 3) for x in tuned["range_ms"]],
+               "autotune_knn_ms_before_after": tuned and [round(float(x), 3) for x in tuned["knn_ms"]],
                "note": "the server runs the batched engine with the seed method (no learned radii)"}
         log("C5", out)
         return out
@@ -639,7 +684,7 @@ def main():
         print(json.dumps({"build_c4": c4}, default=_json_default), flush=True)
         return 0
     if args.c5_only:
-        c5 = run_c5(args.c5_n or 500_000_000, c5_rates, args.c5_seconds, 64, log)
+        c5 = run_c5(args.c5_n or 500_000_000, c5_rates, args.c5_seconds, 64, log, tune=not args.no_tune)
         print(json.dumps({"serving_c5": c5}), flush=True)
         return 0
 
