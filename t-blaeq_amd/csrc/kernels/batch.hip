@@ -834,6 +834,7 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_rows(RangeStepArgs
     __shared__ uint64_t s_hit[kStepWaves][STAGE];
     __shared__ uint64_t s_lm[kStepWaves][kWave];
     __shared__ double s_box[kStepWaves][kMaxQ][2 * DM];
+    __shared__ uint64_t s_hdr[kStepWaves][kRowUnit][5];  // a unit's clusters: list range, rows, first leaf
     const int lane = threadIdx.x & (kWave - 1);
     const uint64_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kStepWaves;
@@ -843,6 +844,7 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_rows(RangeStepArgs
     uint64_t* stage = s_hit[wave];
     uint64_t* lm = s_lm[wave];
     double(*bx)[2 * DM] = s_box[wave];
+    uint64_t(*hdr)[5] = s_hdr[wave];
     uint32_t staged = 0;
     uint64_t wTests = 0, wRows = 0, wLeafTests = 0;
     const uint64_t chunk = queue_chunk(nItems, nw);
@@ -851,28 +853,57 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_rows(RangeStepArgs
     for (uint64_t u = ib; u < ie; ++u) {
         const uint64_t s0 = ((ConstU64)A.sparseUnits)[u] * kRowUnit;
         const uint64_t tileEnd = s0 / kScanThreads * kScanThreads + ((ConstU64)A.sparseCnt)[s0 / kScanThreads];
-        const uint64_t s1 = s0 + kRowUnit < tileEnd ? s0 + kRowUnit : tileEnd;
-        for (uint64_t item = s0; item < s1; ++item) {
-            const uint64_t c = ((ConstU64)A.sparse)[item];
-            const uint64_t g0 = ((ConstU64)A.off)[c];
-            const uint64_t gn = ((ConstU64)A.off)[c + 1] - g0;
+        const int ns = static_cast<int>(s0 + kRowUnit < tileEnd ? kRowUnit : tileEnd - s0);
+        // the unit's clusters and their list and row ranges, lane = slot, in a
+        // few vector loads (rather than a chain of scalar loads per cluster)
+        if (lane < ns) {
+            const uint64_t hc = A.sparse[s0 + lane];
+            const uint64_t h0 = A.off[hc], h1 = A.off[hc + 1], h2 = A.colResPrev[hc], h3 = A.colResPrev[hc + 1],
+                           h4 = A.leafOff[hc];
+            hdr[lane][0] = h0;
+            hdr[lane][1] = h1;
+            hdr[lane][2] = h2;
+            hdr[lane][3] = h3;
+            hdr[lane][4] = h4;
+        }
+        wave_lds_sync();
+        // queries whose list entry was stored (an overflowed list drops the rest;
+        // the batch is then rerun)
+        auto stored = [&](uint64_t g0, uint64_t gn) {
+            return static_cast<uint32_t>(g0 >= A.qcap ? 0 : (A.qcap - g0 < gn ? A.qcap - g0 : gn));
+        };
+        // lane k < nQ: query k of the cluster, loaded one cluster ahead
+        uint32_t qNext = 0;
+        {
+            const uint64_t g0 = uniform_u64(hdr[0][0]), gn = uniform_u64(hdr[0][1]) - g0;
+            if (static_cast<uint32_t>(lane) < stored(g0, gn) && gn <= A.rowMax) qNext = A.qlist[g0 + lane];
+        }
+        for (int k = 0; k < ns; ++k) {
+            const uint64_t g0 = uniform_u64(hdr[k][0]);
+            const uint64_t gn = uniform_u64(hdr[k][1]) - g0;
+            const uint64_t jb = uniform_u64(hdr[k][2]), je = uniform_u64(hdr[k][3]);
+            const uint32_t qv = qNext;
+            if (k + 1 < ns) {
+                const uint64_t g0n = uniform_u64(hdr[k + 1][0]), gnn = uniform_u64(hdr[k + 1][1]) - g0n;
+                qNext = static_cast<uint32_t>(lane) < stored(g0n, gnn) && gnn <= A.rowMax ? A.qlist[g0n + lane] : 0u;
+            }
             if (gn == 0 || gn > A.rowMax) continue;
-            // queries whose list entry was stored (an overflowed list drops the rest;
-            // the batch is then rerun)
-            const uint32_t nQ = static_cast<uint32_t>(g0 >= A.qcap ? 0 : (A.qcap - g0 < gn ? A.qcap - g0 : gn));
-            const uint64_t jb = ((ConstU64)A.colResPrev)[c], je = ((ConstU64)A.colResPrev)[c + 1];
+            const uint32_t nQ = stored(g0, gn);
             if (nQ == 0 || jb >= je) continue;
-            // lane k < nQ: query k of the cluster; its FP64 box into LDS
-            const uint32_t qv = static_cast<uint32_t>(lane) < nQ ? A.qlist[g0 + lane] : 0u;
-            if (static_cast<uint32_t>(lane) < nQ) {
+            // the queries' FP64 boxes, loaded now and written to LDS after the
+            // first leaf tests, so that both loads are in flight together
+            // (lanes past nQ read query qv = 0's box and keep it)
+            double qb[2 * DM];
+            {
                 const double* b = A.boxes + static_cast<uint64_t>(qv) * 2 * D;
 #pragma unroll
                 for (int d = 0; d < DM; ++d) {
-                    bx[lane][d] = (EXACT || d < D) ? b[d] : 0.0;
-                    bx[lane][DM + d] = (EXACT || d < D) ? b[D + d] : 0.0;
+                    qb[d] = (EXACT || d < D) ? b[d] : 0.0;
+                    qb[DM + d] = (EXACT || d < D) ? b[D + d] : 0.0;
                 }
             }
-            const uint64_t leaf0 = ((ConstU64)A.leafOff)[c];
+            bool boxesStored = false;
+            const uint64_t leaf0 = uniform_u64(hdr[k][4]);
             const uint32_t nLeaves = static_cast<uint32_t>((je - jb + kLeafRows - 1) / kLeafRows);
             for (uint32_t lc = 0; lc < nLeaves; lc += kWave) {
                 const uint32_t nl = nLeaves - lc < static_cast<uint32_t>(kWave) ? nLeaves - lc : kWave;
@@ -903,6 +934,13 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_range_rows(RangeStepArgs
                     if (meet) atomicOr(reinterpret_cast<unsigned long long*>(&lm[k]), 1ull << l);
                 }
                 wLeafTests += pairs;
+                if (!boxesStored) {
+                    if (static_cast<uint32_t>(lane) < nQ) {
+#pragma unroll
+                        for (int d = 0; d < 2 * DM; ++d) bx[lane][d] = qb[d];
+                    }
+                    boxesStored = true;
+                }
                 wave_lds_sync();
                 const uint64_t lmv = static_cast<uint32_t>(lane) < nQ ? lm[lane] : 0ull;
                 uint64_t uni = lmv;
