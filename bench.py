@@ -35,7 +35,7 @@ FP64_PEAK_TFLOPS = 78.6
 # SIMDs x 16 lanes x 2.4 GHz; half the FP64 FMA flop rate of 78.6 TFLOP/s
 VALU_PEAK_TOPS = 39.3
 PARAMS = (("knn_seed", 1), ("knn_wg_per_cu", 2), ("knn_mode", 3), ("range_blocks_per_cu", 4),
-          ("step_blocks_per_cu", 5), ("radix_blocks_per_cu", 6), ("hit_stage", 7), ("range_row_queries", 8), ("knn_row_queries", 9))
+          ("step_blocks_per_cu", 5), ("radix_blocks_per_cu", 6), ("hit_stage", 7), ("range_row_queries", 8), ("knn_row_queries", 9), ("knn_leaf_refine", 10))
 
 
 def final_traffic(cfg):
@@ -543,7 +543,10 @@ def run_c3(n_q, n_traces, epochs, plain_sample, log):
         data = h.data()
         pts = W.knn_points(data, n_q, cfg["seed"] + 12, h.mesh_sizes, cfg["vmin"], cfg["vmax"], noise=0.01)
         Ks = np.where(np.arange(n_q) % 2 == 0, 10, 100).astype(np.uint64)
-        h.batch_init(max(n_q, 4096), 1 << 27, 1 << 27)
+        # at D = 16 a query keeps about a third of the last level's clusters: 2^29 candidate
+        # records and pairs (about 20 GB of engine scratch) hold a whole 8192-point batch, where
+        # 2^27 overflowed and split it into reruns (158 ms against 54 ms per batch)
+        h.batch_init(max(n_q, 4096), 1 << 29, 1 << 29)
         t0 = time.perf_counter()
         X, T = learn.traces(h, n_traces, cfg["seed"] + 13, cfg["vmin"], cfg["vmax"])
         t1 = time.perf_counter()

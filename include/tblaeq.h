@@ -287,7 +287,13 @@ int tblaeq_batch_knn_graph(tblaeq_batch* b, const double* d_points, const uint64
  *                                   (lane = (cluster, query) pair for the leaf
  *                                   tests, then lane = point of a kept leaf),
  *                                   the others query-major, 0..4 (4; 0: all
- *                                   query-major; only with more than one level) */
+ *                                   query-major; only with more than one level)
+ *   TBLAEQ_PARAM_KNN_LEAF_REFINE    1: the other clusters' work items (a cluster
+ *                                   and up to 64 queries) are refined through
+ *                                   per (query, kd leaf) box tests too, then the
+ *                                   kept leaves' rows, four leaves a pass; 0 (the
+ *                                   default): every row against every query of
+ *                                   the item (pays where the boxes prune little) */
 enum {
     TBLAEQ_PARAM_KNN_SEED = 1,
     TBLAEQ_PARAM_KNN_WG_PER_CU = 2,
@@ -297,7 +303,8 @@ enum {
     TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU = 6,
     TBLAEQ_PARAM_HIT_STAGE = 7,
     TBLAEQ_PARAM_RANGE_ROW_QUERIES = 8,
-    TBLAEQ_PARAM_KNN_ROW_QUERIES = 9
+    TBLAEQ_PARAM_KNN_ROW_QUERIES = 9,
+    TBLAEQ_PARAM_KNN_LEAF_REFINE = 10
 };
 int tblaeq_batch_set_param(tblaeq_batch* b, int param, uint64_t value);
 int tblaeq_batch_get_param(const tblaeq_batch* b, int param, uint64_t* value);

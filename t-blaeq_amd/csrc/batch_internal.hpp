@@ -95,6 +95,7 @@ struct tblaeq_batch {
     uint64_t hitStage = 512;          // LDS hits staged per wave by the final filter (256, 512, 1024)
     uint64_t rangeRowMax = 16;        // final filter: clusters kept by at most this many queries go row-major
     uint64_t knnRowMax = 4;           // KNN refine: clusters refined by at most this many queries go row-major
+    uint64_t knnLeafRefine = 0;       // KNN refine: work items through kd-leaf tests (1) or all rows (0)
     uint64_t paramVersion = 0;        // bumped by every parameter change (captured graphs re-capture)
     uint64_t gParamVersion = 0;
 };

@@ -809,7 +809,7 @@ int tblaeq_server_create(tblaeq_handler* h, const tblaeq_server_config* cfgIn, t
                                             TBLAEQ_PARAM_KNN_MODE,           TBLAEQ_PARAM_RANGE_BLOCKS_PER_CU,
                                             TBLAEQ_PARAM_STEP_BLOCKS_PER_CU, TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU,
                                             TBLAEQ_PARAM_HIT_STAGE,          TBLAEQ_PARAM_RANGE_ROW_QUERIES,
-                                            TBLAEQ_PARAM_KNN_ROW_QUERIES};
+                                            TBLAEQ_PARAM_KNN_ROW_QUERIES,    TBLAEQ_PARAM_KNN_LEAF_REFINE};
             for (int id : kParamIds) {
                 uint64_t v = 0;
                 tbCheck(tblaeq_batch_get_param(h->slots[0]->b, id, &v), "serve: launch parameter");

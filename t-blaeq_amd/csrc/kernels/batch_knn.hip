@@ -1424,7 +1424,7 @@ __device__ __forceinline__ double bperm_f64(double v, int src) {
     return __longlong_as_double(static_cast<long long>(bperm_u64(static_cast<uint64_t>(__double_as_longlong(v)), src)));
 }
 
-template <int DM>
+template <int DM, bool DENSE>
 __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_rows(KnnClusterArgs A) {
     __shared__ double s_qp[kStepWaves][kWave][DM];  // query point of every pair lane
     __shared__ double s_td[kStepWaves][kWave];      // its threshold as a double (-inf: no pair)
@@ -1437,7 +1437,13 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_rows(KnnClusterArgs 
     const uint64_t nw = static_cast<uint64_t>(gridDim.x) * kStepWaves;
     const int D = A.D;
     const uint64_t ltmask = (1ull << lane) - 1ull;
-    const uint64_t nUnits = A.sparseTiles * (kScanThreads / kRowUnit);
+    // sparse: units of kRowUnit list slots; dense: the work items (cluster,
+    // slice of 64 queries), each taken as kRowUnit slots of one cluster
+    uint64_t nUnits = A.sparseTiles * (kScanThreads / kRowUnit);
+    if (DENSE) {
+        nUnits = *A.nItems;
+        if (nUnits > A.itemCap) nUnits = A.itemCap;
+    }
     double(*qs)[DM] = s_qp[wave];
     double* tds = s_td[wave];
     uint32_t* tup = s_tup[wave];
@@ -1452,12 +1458,24 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_rows(KnnClusterArgs 
         uint64_t base = 0;
         if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(A.recCount), staged);
         base = uniform_u64(__shfl(base, 0, kWave));
-        for (uint32_t o = lane; o < staged; o += kWave)
-            if (base + o < A.recCap) {
+        for (uint32_t o0 = 0; o0 < staged; o0 += kWave) {
+            const uint32_t o = o0 + lane;
+            const bool stored = o < staged && base + o < A.recCap;
+            const uint64_t qr = stored ? rv[o] >> kRecIdBits : ~0ull;
+            if (stored) {
                 A.recK[base + o] = rk[o];
                 A.recV[base + o] = rv[o];
-                atomicAdd(reinterpret_cast<unsigned long long*>(&A.cntQ[rv[o] >> kRecIdBits]), 1ull);
             }
+            // one count update per distinct query of the pass, not per record
+            for (uint64_t left = __ballot(stored); left;) {
+                const uint64_t qv = readlane_u64(qr, __builtin_ctzll(left));
+                const uint64_t same = __ballot(qr == qv);
+                if (lane == __builtin_ctzll(left))
+                    atomicAdd(reinterpret_cast<unsigned long long*>(&A.cntQ[qv]),
+                              static_cast<unsigned long long>(__popcll(same)));
+                left &= ~same;
+            }
+        }
         if (lane == 0 && base + staged > A.recCap) atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 4ull);
         wave_lds_sync();
         staged = 0;
@@ -1474,7 +1492,7 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_rows(KnnClusterArgs 
     };
     uint64_t nextEnd = 0, nextC = 0;
     auto fetch = [&](uint64_t uu) {
-        if (uu < nUnits) {
+        if (!DENSE && uu < nUnits) {
             const uint64_t s0 = unit_slot(uu);
             nextEnd = s0 / kScanThreads * kScanThreads + A.sparseCnt[s0 / kScanThreads];
             nextC = A.sparse[s0 + lane / kRowQ];
@@ -1482,13 +1500,22 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_rows(KnnClusterArgs 
     };
     fetch(gw);
     for (uint64_t u = gw; u < nUnits; u += nw) {
-        const uint64_t s0 = unit_slot(u);
-        const uint64_t tileEnd = uniform_u64(nextEnd);
-        const uint64_t cl = nextC;
-        fetch(u + nw);
-        if (s0 >= tileEnd) continue;
-        const uint64_t j = lane % kRowQ;
-        const bool inSlot = s0 + lane / kRowQ < tileEnd;
+        uint64_t j = lane % kRowQ;
+        bool inSlot = true;
+        uint64_t cl = 0;
+        if (DENSE) {
+            // lane = query lane of the item's slice (the list position)
+            const uint64_t it = A.items[u];
+            cl = it >> kSliceBits;
+            j = (it & ((1ull << kSliceBits) - 1)) * kWave + lane;
+        } else {
+            const uint64_t s0 = unit_slot(u);
+            const uint64_t tileEnd = uniform_u64(nextEnd);
+            cl = nextC;
+            fetch(u + nw);
+            if (s0 >= tileEnd) continue;
+            inSlot = s0 + lane / kRowQ < tileEnd;
+        }
         uint64_t g0 = 0, gn = 0, jb = 0, je = 0, leaf0 = 0;
         if (inSlot) {
             g0 = A.off[cl];
@@ -1499,7 +1526,7 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_rows(KnnClusterArgs 
         }
         // list slots that were written (an overflowed list is rerun by the
         // host) and hold a query below nq
-        const bool inList = inSlot && gn <= A.rowMax && j < gn && g0 + j < A.qcap;
+        const bool inList = inSlot && (DENSE || gn <= A.rowMax) && j < gn && g0 + j < A.qcap;
         const uint64_t ql = inList ? A.qlist[g0 + j] : 0;
         const bool valid = inList && ql < A.nq;
         const uint64_t q = valid ? ql : 0;
@@ -1510,8 +1537,9 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_rows(KnnClusterArgs 
         // (slot, leaf) pairs of the unit, slot-major: each slot's leaf count on
         // its first lane (0 without a query), inclusive prefix over the slots
         const uint64_t vm = __ballot(valid);
-        uint32_t x = (j == 0 && ((vm >> lane) & 0xFull)) ? static_cast<uint32_t>((je - jb + kLeafRows - 1) / kLeafRows)
-                                                          : 0u;
+        uint32_t x = (lane % kRowQ == 0 && ((vm >> lane) & 0xFull))
+                         ? static_cast<uint32_t>((je - jb + kLeafRows - 1) / kLeafRows)
+                         : 0u;
 #pragma unroll
         for (int o = 1; o < kWave; o <<= 1) {
             const uint32_t y = __shfl_up(x, o, kWave);
@@ -1630,11 +1658,12 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_rows(KnnClusterArgs 
     if (lane == 0 && A.work) atomicAdd(reinterpret_cast<unsigned long long*>(A.work), wTests);
 }
 
+template <bool DENSE>
 static auto knn_rows_kernel(int D) -> void (*)(KnnClusterArgs) {
-    if (D <= 4) return k_knn_rows<4>;
-    if (D <= 8) return k_knn_rows<8>;
-    if (D <= 16) return k_knn_rows<16>;
-    return k_knn_rows<32>;
+    if (D <= 4) return k_knn_rows<4, DENSE>;
+    if (D <= 8) return k_knn_rows<8, DENSE>;
+    if (D <= 16) return k_knn_rows<16, DENSE>;
+    return k_knn_rows<32, DENSE>;
 }
 
 // Counting-sort scatter of the records into per-query segments; leaves cntQ
@@ -1850,6 +1879,7 @@ const ParamSpec kParams[] = {
     {TBLAEQ_PARAM_HIT_STAGE, &tblaeq_batch::hitStage, 256, 1024},
     {TBLAEQ_PARAM_RANGE_ROW_QUERIES, &tblaeq_batch::rangeRowMax, 0, 64},
     {TBLAEQ_PARAM_KNN_ROW_QUERIES, &tblaeq_batch::knnRowMax, 0, static_cast<uint64_t>(kRowQ)},
+    {TBLAEQ_PARAM_KNN_LEAF_REFINE, &tblaeq_batch::knnLeafRefine, 0, 1},
 };
 const ParamSpec* find_param(int id) {
     for (const auto& p : kParams)
@@ -1935,7 +1965,8 @@ extern "C" int tblaeq_batch_knn_graph(tblaeq_batch* b, const double* d_points, c
         int rc = ws_reserve(ctx, knn_ws_bytes(b, nq));
         if (rc) return rc;
         // the row-major refine's occupancy, queried outside the capture
-        resident_blocks(reinterpret_cast<const void*>(knn_rows_kernel(b->D)), kStepWaves * kWave);
+        resident_blocks(reinterpret_cast<const void*>(knn_rows_kernel<false>(b->D)), kStepWaves * kWave);
+        resident_blocks(reinterpret_cast<const void*>(knn_rows_kernel<true>(b->D)), kStepWaves * kWave);
         const bool timing = ctx->timing;
         ctx->timing = false;  // no event records inside the graph
         const uint64_t before = ctx->scanLaunches;
@@ -2267,7 +2298,18 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
         C.cntQ = cntQ;
         C.overflow = ctl + 3;
         C.work = ctl + 8;
-        {
+        if (b->knnLeafRefine && b->maxLeaves < (1ull << 26)) {
+            // the work items refined like the row-major list: kd-leaf box tests
+            // per (query, leaf), then the kept leaves' rows
+            TimedScope ts(ctx, kKBatch0 + 7);
+            void (*k)(KnnClusterArgs) = knn_rows_kernel<true>(C.D);
+            const uint64_t blocks = std::min<uint64_t>(
+                static_cast<uint64_t>(sg.x),
+                static_cast<uint64_t>(ctx->cuCount) *
+                    static_cast<uint64_t>(resident_blocks(reinterpret_cast<const void*>(k), kStepWaves * kWave)));
+            hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(blocks ? blocks : 1)), st, 0, ctx->stream, C);
+            TB_HIP(hipGetLastError());
+        } else {
             TimedScope ts(ctx, kKBatch0 + 7);
             if (C.D <= 4) hipLaunchKernelGGL(k_knn_cluster<4>, sg, st, 0, ctx->stream, C);
             else if (C.D <= 8) hipLaunchKernelGGL(k_knn_cluster<8>, sg, st, 0, ctx->stream, C);
@@ -2277,7 +2319,7 @@ static int knn_impl(tblaeq_batch* b, const double* d_points, const uint64_t* d_K
         }
         if (rowMax) {
             TimedScope ts(ctx, kKBatch0 + 13);
-            void (*k)(KnnClusterArgs) = knn_rows_kernel(C.D);
+            void (*k)(KnnClusterArgs) = knn_rows_kernel<false>(C.D);
             // wave-strided units: a resident grid, so that every wave's share is
             // processed in one round
             const uint64_t units = C.sparseTiles * (kScanThreads / kRowUnit);

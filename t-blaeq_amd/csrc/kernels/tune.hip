@@ -88,6 +88,7 @@ const char* param_name(int id) {
     case TBLAEQ_PARAM_HIT_STAGE: return "hit_stage";
     case TBLAEQ_PARAM_RANGE_ROW_QUERIES: return "range_row_queries";
     case TBLAEQ_PARAM_KNN_ROW_QUERIES: return "knn_row_queries";
+    case TBLAEQ_PARAM_KNN_LEAF_REFINE: return "knn_leaf_refine";
     default: return nullptr;
     }
 }
@@ -95,7 +96,7 @@ const int kAllParams[] = {TBLAEQ_PARAM_KNN_SEED,           TBLAEQ_PARAM_KNN_WG_P
                           TBLAEQ_PARAM_KNN_MODE,           TBLAEQ_PARAM_RANGE_BLOCKS_PER_CU,
                           TBLAEQ_PARAM_STEP_BLOCKS_PER_CU, TBLAEQ_PARAM_RADIX_BLOCKS_PER_CU,
                           TBLAEQ_PARAM_HIT_STAGE,          TBLAEQ_PARAM_RANGE_ROW_QUERIES,
-                          TBLAEQ_PARAM_KNN_ROW_QUERIES};
+                          TBLAEQ_PARAM_KNN_ROW_QUERIES,    TBLAEQ_PARAM_KNN_LEAF_REFINE};
 
 }  // namespace
 }  // namespace tblaeq
@@ -181,7 +182,8 @@ extern "C" int tblaeq_batch_autotune(tblaeq_batch* b, const double* d_boxes, uin
     const std::vector<Candidate> knnC = {{TBLAEQ_PARAM_KNN_SEED, {16, 32, 64, 128}},
                                          {TBLAEQ_PARAM_KNN_WG_PER_CU, {2, 4, 8}},
                                          {TBLAEQ_PARAM_STEP_BLOCKS_PER_CU, {4, 8, 16, 32}},
-                                         {TBLAEQ_PARAM_KNN_ROW_QUERIES, {0, 1, 2, 4}}};
+                                         {TBLAEQ_PARAM_KNN_ROW_QUERIES, {0, 1, 2, 4}},
+                                         {TBLAEQ_PARAM_KNN_LEAF_REFINE, {0, 1}}};
     for (int kind = 0; kind < 2; ++kind) {
         const bool range = kind == 0;
         if ((range && !n_boxes) || (!range && !n_points)) {

@@ -163,6 +163,7 @@ PARAM_RADIX_BLOCKS_PER_CU = 6
 PARAM_HIT_STAGE = 7
 PARAM_RANGE_ROW_QUERIES = 8
 PARAM_KNN_ROW_QUERIES = 9
+PARAM_KNN_LEAF_REFINE = 10
 
 
 class TblaeqError(RuntimeError):

@@ -169,16 +169,18 @@ def test_batch_matches_plain_c1():
         assert np.array_equal(kres[i][1].view(np.uint64), pd2.view(np.uint64))
 
 
-@pytest.mark.parametrize("seed,wg,rowq", [(1, 1, 0), (7, 16, 1), (256, 2, 4), (64, 4, 2)])
-def test_batch_knn_launch_params(seed, wg, rowq):
-    """Seed cluster count, grid shape and the row-major refine's query limit (rowq: clusters
-    refined by at most this many queries go row-major) change only the work done, never the
-    answer."""
+@pytest.mark.parametrize("seed,wg,rowq,leaf", [(1, 1, 0, 1), (7, 16, 1, 0), (256, 2, 4, 1), (64, 4, 2, 0)])
+def test_batch_knn_launch_params(seed, wg, rowq, leaf):
+    """Seed cluster count, grid shape, the row-major refine's query limit (rowq: clusters
+    refined by at most this many queries go row-major) and the leaf-tested refine of the other
+    clusters change only the work done, never the answer."""
     h, o = _make(dict(CFG, N=30000, is_int=True, seed=9), max_q=128)
     h.batch_set_param(tblaeq.PARAM_KNN_SEED, seed)
     h.batch_set_param(tblaeq.PARAM_KNN_WG_PER_CU, wg)
     h.batch_set_param(tblaeq.PARAM_KNN_ROW_QUERIES, rowq)
+    h.batch_set_param(tblaeq.PARAM_KNN_LEAF_REFINE, leaf)
     assert h.batch_get_param(tblaeq.PARAM_KNN_ROW_QUERIES) == rowq
+    assert h.batch_get_param(tblaeq.PARAM_KNN_LEAF_REFINE) == leaf
     data = h.data()
     pts = np.concatenate([data[:40] + 0.5, W.knn_points(data, 40, 77, h.mesh_sizes, CFG["vmin"], CFG["vmax"])])
     Ks = np.random.default_rng(seed).choice([1, 10, 64, 150], len(pts))
@@ -259,7 +261,7 @@ def test_autotune_keeps_results_and_params_roundtrip(tmp_path):
     Ks = np.random.default_rng(2).choice([1, 10, 60], len(pts)).astype(np.uint64)
     ms = h.batch_autotune(boxes, pts, Ks, reps=1)
     assert ms["range_ms"][1] <= ms["range_ms"][0] and ms["knn_ms"][1] <= ms["knn_ms"][0]
-    params = {p: h.batch_get_param(p) for p in range(1, 10)}
+    params = {p: h.batch_get_param(p) for p in range(1, 11)}
     assert params[tblaeq.PARAM_HIT_STAGE] in (256, 512, 1024)
     _check_range_batch(h, o, boxes)
     h.batch_set("range", boxes)
@@ -273,7 +275,7 @@ def test_autotune_keeps_results_and_params_roundtrip(tmp_path):
     h.batch_set_param(tblaeq.PARAM_HIT_STAGE, 256 if params[tblaeq.PARAM_HIT_STAGE] != 256 else 1024)
     h.batch_set_param(tblaeq.PARAM_KNN_SEED, 3)
     h.batch_load_params(path)
-    assert {p: h.batch_get_param(p) for p in range(1, 10)} == params
+    assert {p: h.batch_get_param(p) for p in range(1, 11)} == params
     bad = tmp_path / "bad.txt"
     bad.write_text("hit_stage 300\n")
     with pytest.raises(tblaeq.TblaeqError):

@@ -56,17 +56,20 @@ def _range_dims(h, o):
     assert len(res[-3]) == h.N
 
 
-@pytest.mark.parametrize("rowq", [0, 1, 4])
-def test_knn_dims(idx, rowq):
-    """rowq: clusters refined by at most this many queries go to the row-major refine (0: none)."""
+@pytest.mark.parametrize("rowq,leaf", [(0, 0), (1, 0), (4, 1), (0, 1)])
+def test_knn_dims(idx, rowq, leaf):
+    """rowq: clusters refined by at most this many queries go to the row-major refine (0: none);
+    leaf: the other clusters' work items go through per (query, kd leaf) box tests too."""
     h, o = idx
     h.batch_set_param(tblaeq.PARAM_KNN_ROW_QUERIES, rowq)
+    h.batch_set_param(tblaeq.PARAM_KNN_LEAF_REFINE, leaf)
     try:
         pts = W.knn_points(h.data(), 120, 50 + h.D, h.mesh_sizes, 1.0, 100.0, noise=0.2)
         Ks = np.random.default_rng(h.D).choice([0, 1, 10, 64, 100, 200], len(pts))
         check_knn_batch(h, o, pts, Ks)
     finally:
         h.batch_set_param(tblaeq.PARAM_KNN_ROW_QUERIES, 4)
+        h.batch_set_param(tblaeq.PARAM_KNN_LEAF_REFINE, 0)
 
 
 def test_knn_query_major_dims(idx):
