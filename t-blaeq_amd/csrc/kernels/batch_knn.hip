@@ -1294,42 +1294,17 @@ struct KnnClusterArgs {
     uint64_t rowMax;
 };
 
-// One row test of the refine: acc against the lane's threshold (tD as a double;
-// keys only for ties and NaNs); emitting lanes append (okey(acc), q, *idp)
-// records with one reservation per row.
-__device__ __forceinline__ void refine_emit(const KnnClusterArgs& A, double acc, double tD, uint64_t tK, uint64_t tV,
-                                            const uint64_t* idp, uint64_t q, int lane, uint64_t ltmask,
-                                            uint64_t& mine) {
-    bool emit = acc < tD;
-    const bool tie = !(emit || acc > tD);
-    if (__ballot(tie)) {
-        if (tie) {
-            const uint64_t key = okey(acc);
-            emit = key < tK || (key == tK && *idp < tV);
-        }
-    }
-    const uint64_t m = __ballot(emit);
-    if (m) {
-        uint64_t base = 0;
-        if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(A.recCount), __popcll(m));
-        base = __shfl(base, 0, kWave);
-        if (emit) {
-            const uint64_t pos = base + __popcll(m & ltmask);
-            if (pos < A.recCap) {
-                A.recK[pos] = okey(acc);
-                A.recV[pos] = (q << kRecIdBits) | *idp;
-                ++mine;
-            }
-        }
-        if (lane == 0 && base + __popcll(m) > A.recCap)
-            atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 4ull);
-    }
-}
+// Records of the cluster-major refine staged per wave (one reservation per
+// flush of up to kClusterRec records rather than one per row that emits:
+// with learned radii most rows near a query emit).
+constexpr int kClusterRec = 64;
 
 template <int DM>
 __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_cluster(KnnClusterArgs A) {
     __shared__ double s_x[kStepWaves][kWave * DM];
     __shared__ uint64_t s_id[kStepWaves][kWave];
+    __shared__ uint64_t s_rk[kStepWaves][kClusterRec];
+    __shared__ uint64_t s_rv[kStepWaves][kClusterRec];
     const int lane = threadIdx.x & (kWave - 1);
     const uint64_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint64_t gw = blockIdx.x * static_cast<uint64_t>(kStepWaves) + wave;
@@ -1339,6 +1314,30 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_cluster(KnnClusterAr
     if (nItems > A.itemCap) nItems = A.itemCap;
     const uint64_t ltmask = (1ull << lane) - 1ull;
     double* sx = s_x[wave];
+    uint64_t* rk = s_rk[wave];
+    uint64_t* rv = s_rv[wave];
+    uint32_t staged = 0;
+    auto flush = [&]() {
+        if (staged == 0) return;
+        wave_lds_sync();
+        uint64_t base = 0;
+        if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(A.recCount), staged);
+        base = uniform_u64(__shfl(base, 0, kWave));
+        if (static_cast<uint32_t>(lane) < staged) {
+            if (base + lane < A.recCap) {
+                A.recK[base + lane] = rk[lane];
+                A.recV[base + lane] = rv[lane];
+            } else {
+                // not stored: take it back out of its query's count (the items add
+                // their records to the counts as they emit them), so that the
+                // counts, and the offsets scanned from them, cover stored records only
+                atomicAdd(reinterpret_cast<unsigned long long*>(&A.cntQ[rv[lane] >> kRecIdBits]), ~0ull);
+            }
+        }
+        if (lane == 0 && base + staged > A.recCap) atomicOr(reinterpret_cast<unsigned long long*>(A.overflow), 4ull);
+        wave_lds_sync();
+        staged = 0;
+    };
     uint64_t wTests = 0, wRows = 0;
     if (D != DM)
         for (int t = lane; t < kWave * DM; t += kWave) sx[t] = 0.0;
@@ -1385,14 +1384,36 @@ __global__ __launch_bounds__(kStepWaves * kWave) void k_knn_cluster(KnnClusterAr
                     const double diff = qp[d] - x[d];
                     acc = __builtin_fma(diff, diff, acc);
                 }
-                refine_emit(A, acc, tD, tK, tV, &s_id[wave][k], q, lane, ltmask, mine);
+                // a record when (acc, id) precedes the lane's threshold (tD as a
+                // double; the keys only for ties and NaNs)
+                bool emit = acc < tD;
+                const bool tie = !(emit || acc > tD);
+                if (__ballot(tie)) {
+                    if (tie) {
+                        const uint64_t key = okey(acc);
+                        emit = key < tK || (key == tK && s_id[wave][k] < tV);
+                    }
+                }
+                const uint64_t em = __ballot(emit);
+                if (em) {
+                    if (staged + __popcll(em) > static_cast<uint32_t>(kClusterRec)) flush();
+                    if (emit) {
+                        const uint32_t o = staged + __popcll(em & ltmask);
+                        rk[o] = okey(acc);
+                        rv[o] = (q << kRecIdBits) | s_id[wave][k];
+                        ++mine;
+                    }
+                    staged += __popcll(em);
+                }
             }
             wave_lds_sync();
         }
+        // (records past the capacity are taken back out by the flush that drops them)
         if (valid && mine) atomicAdd(reinterpret_cast<unsigned long long*>(&A.cntQ[q]), mine);
         wRows += je - jb;
         wTests += (je - jb) * static_cast<uint64_t>(__popcll(__ballot(valid)));
     }
+    flush();
     if (lane == 0 && A.work) atomicAdd(reinterpret_cast<unsigned long long*>(A.work), wTests);
     (void)wRows;
 }
@@ -1686,8 +1707,10 @@ __global__ __launch_bounds__(kAggThreads) void k_rec_scatter(const uint64_t* __r
             const uint64_t v = recV[i];
             const uint64_t q = v >> kRecIdBits;
             const uint64_t pos = atomicAdd(reinterpret_cast<unsigned long long*>(&cntQ[q]), ~0ull) - 1;
-            gK[offQ[q] + pos] = recK[i];
-            gV[offQ[q] + pos] = v & idMask;
+            if (offQ[q] + pos < cap) {  // the counts cover stored records only; never write past them
+                gK[offQ[q] + pos] = recK[i];
+                gV[offQ[q] + pos] = v & idMask;
+            }
         }
         return;
     }
@@ -1717,8 +1740,10 @@ __global__ __launch_bounds__(kAggThreads) void k_rec_scatter(const uint64_t* __r
             if (q < nq) {
                 const uint64_t i = c * kAggChunk + static_cast<uint64_t>(r) * kAggThreads + tid;
                 const uint64_t pos = offQ[q] + lc[q] + rank[r];
-                gK[pos] = recK[i];
-                gV[pos] = v[r] & idMask;
+                if (pos < cap) {
+                    gK[pos] = recK[i];
+                    gV[pos] = v[r] & idMask;
+                }
             }
         }
         __syncthreads();
