@@ -419,7 +419,7 @@ class Handler(_Closing):
             kmax = 0
         else:
             k = np.ascontiguousarray(np.broadcast_to(np.asarray(Ks, np.uint64), (q.shape[0],)))
-            kmax = int(k.max())
+            kmax = int(k.max()) if k.size else 0
             check(lib().tblaeq_handler_batch_set_queries(self.h, 1, _p(q, f64p), _p(k, u64p), q.shape[0]),
                   "batch_set")
         self.__dict__.setdefault("_slots", {})[self.__dict__.get("_slot", 0)] = (qtype, q.shape[0], kmax)

@@ -42,9 +42,21 @@ def test_batch_range_edges(idx):
         np.r_[np.full(D, -1e9), np.full(D, 1e9)],         # everything
         np.r_[np.full(D, 60.0), np.full(D, 40.0)],        # inverted
         np.r_[h.data()[7], h.data()[7]],                  # degenerate on a point
+        np.r_[np.full(D, -np.inf), np.full(D, np.inf)],   # everything, infinite bounds
+        np.r_[np.nan, np.full(D - 1, -1e9), np.full(D, 1e9)],          # NaN lower bound
+        np.r_[np.full(D, -1e9), 1e9, np.nan, np.full(D - 2, 1e9)],     # NaN upper bound
     ])
     res = _check_range_batch(h, o, boxes)
     assert len(res[1]) == h.N and len(res[0]) == 0
+    assert len(res[4]) == h.N and len(res[5]) == 0 and len(res[6]) == 0
+
+
+def test_batch_rejects_empty_batch(idx):
+    h, _ = idx
+    with pytest.raises(tblaeq.TblaeqError):
+        h.batch_set("range", np.zeros((0, 2 * h.D)))
+    with pytest.raises(tblaeq.TblaeqError):
+        h.batch_set("knn", np.zeros((0, h.D)), np.zeros(0, np.uint64))
 
 
 def test_batch_range_points_outside_stored_radius(idx):
