@@ -50,10 +50,12 @@ def _range_dims(h, o):
         np.r_[np.full(D, -1e9), np.full(D, 1e9)],   # everything
         np.r_[np.full(D, 60.0), np.full(D, 40.0)],  # inverted
         np.r_[data[3], data[3]],                    # degenerate on a data point
+        np.r_[np.full(D, -np.inf), np.full(D, np.inf)],                    # everything, infinite
+        np.r_[np.full(D - 1, -1e9), np.nan, np.full(D, 1e9)],              # NaN lower bound
     ])
     res = check_range_batch(h, o, np.concatenate([boxes, OC.degenerate_boxes(pts), edges]))
     assert sum(len(x) for x in res[:200]) > 0
-    assert len(res[-3]) == h.N
+    assert len(res[-5]) == h.N and len(res[-2]) == h.N and len(res[-1]) == 0
 
 
 @pytest.mark.parametrize("rowq,leaf", [(0, 0), (1, 0), (4, 1), (0, 1)])
