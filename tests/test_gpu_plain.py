@@ -105,6 +105,11 @@ def test_range_edges(small):
     # degenerate box on a data point
     p = h.data()[123]
     _check_range(h, o, p, p)
+    # infinite bounds: every point; a NaN bound: candidates as the reference's
+    # conditional distance keeps them, no refined hit
+    assert _check_range(h, o, np.full(D, -np.inf), np.full(D, np.inf)) == h.N
+    _check_range(h, o, np.r_[np.nan, np.full(D - 1, -1e9)], np.full(D, 1e9))
+    _check_range(h, o, np.full(D, -1e9), np.r_[np.full(D - 1, 1e9), np.nan])
 
 
 @pytest.mark.parametrize("K", [0, 1, 10, 100, 1000])
